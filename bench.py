@@ -1,0 +1,194 @@
+"""Benchmark: MLS-MPM substeps of the 4.2M-particle spheres-and-slab scene on one GPU.
+
+A step is one substep (sort, P2G, grid update, G2P) over all particles.  The
+scene: three Neo-Hookean spheres and a weakly compressible fluid slab, 256^3
+grid with 4^3 blocks, h = 1/256, dt = 5e-5, box walls, f32.  Prints one JSON
+line.  The CPU baseline is the oracle (the C++ restatement of the reference
+CPU step, OpenMP) timed on a bounded sample of the same scene.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+# Algorithmic bytes per unit of work of each kernel (the dataflow with the
+# stress evaluated in G2P and the fused affine momentum A stored):
+#   P2G  reads x, v, A, m                        64 B / particle; writes (m, p) 16 B / active node
+#   grid reads and writes (m, p | v)             32 B / active node
+#   G2P  reads x, F, m, V0, mu, lambda           64 B / particle; writes x, v, F, A 96 B / particle;
+#        reads v                                 16 B / active node
+#   substep total: 224 B / particle + 64 B / active node.
+BYTES = {
+    "p2g": (64, 16),
+    "grid_update": (0, 32),
+    "g2p": (160, 16),
+}
+
+
+def log(msg: str):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(scene, substeps: int, threads: int) -> dict:
+    from oracle import pyoracle
+    from squishy_volumes_amd import scenes
+
+    pyoracle.set_threads(threads)
+    o = pyoracle.Oracle(scene.consts, scene.time_step, scene.io_state, scene.options, 4)
+    o.step_to(scenes.advance_time(o.time, scene.time_step, 1), scene.frame_input)  # warm-up substep
+    t0 = time.perf_counter()
+    o.step_to(scenes.advance_time(o.time, scene.time_step, substeps), scene.frame_input)
+    dt = time.perf_counter() - t0
+    n = scene.io_state.particles.count
+    o.close()
+    return {
+        "value": n * substeps / dt,
+        "unit": "particle-substeps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{scene.name}: all {n} particles, {substeps} substeps after 1 warm-up substep, "
+        f"float oracle (C++ restatement of the reference CPU step), {threads} OpenMP threads, {dt:.2f} s",
+    }
+
+
+def traffic_for(kernel: str, workload: str):
+    f = ROOT / "profiles" / f"traffic_{kernel}_{workload}.json"
+    if f.exists():
+        try:
+            return json.loads(f.read_text()).get("bytes_per_launch")
+        except Exception:
+            return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-substeps", type=int, default=6)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--p2g-waves", type=int, default=0)
+    ap.add_argument("--g2p-threads", type=int, default=0)
+    ap.add_argument("--persistent-blocks", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(
+            "bench.py runs on exactly one GPU: the engine is single-device (one simulation per GPU, "
+            "no collectives); multi-GPU sharding is a design note only.",
+            file=sys.stderr,
+        )
+        sys.exit(2)
+
+    import torch
+
+    from squishy_volumes_amd import scenes
+    from squishy_volumes_amd.gpu_state import GpuState
+
+    scene = scenes.spheres_and_slab()
+    n = scene.io_state.particles.count
+    log(f"bench: {scene.name}, {n} particles, warmup {args.warmup}, steps {args.steps}")
+    tuning = {}
+    if args.p2g_waves:
+        tuning["p2g_waves_per_block"] = args.p2g_waves
+    if args.g2p_threads:
+        tuning["g2p_threads"] = args.g2p_threads
+    if args.persistent_blocks:
+        tuning["persistent_blocks"] = args.persistent_blocks
+    gpu = GpuState.from_io_state(0, scene.consts, scene.time_step, scene.io_state, scene.options, tuning)
+    fi = scene.frame_input
+
+    t = gpu.time
+    if args.warmup:
+        gpu.step_to(scenes.advance_time(t, scene.time_step, args.warmup), fi)
+    log("bench: warm-up done")
+
+    # Timed region: exactly `steps` substeps, per-kernel HIP events on the engine's stream.
+    gpu.kernel_times(reset=True)
+    gpu.set_profiling(True)
+    target = scenes.advance_time(gpu.time, scene.time_step, args.steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    status = gpu.step_to(target, fi)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    gpu.set_profiling(False)
+    assert status.substeps == args.steps, (status.substeps, args.steps)
+    assert status.result == 0, status.result
+    stages = gpu.kernel_times(reset=True)
+    log(f"bench: {args.steps} substeps in {elapsed:.3f} s")
+
+    nodes = int(status.active_nodes)
+    value = n * args.steps / elapsed
+    stage_ms = {k: v[0] / max(v[1], 1) for k, v in stages.items()}
+    dominant = max(BYTES, key=lambda k: stages.get(k, (0, 1))[0])
+    per_particle, per_node = BYTES[dominant]
+    bytes_per_launch = per_particle * n + per_node * nodes
+    avg_s = stage_ms[dominant] / 1e3
+    achieved = bytes_per_launch / avg_s / 1e9
+    substep_bytes = 224 * n + 64 * nodes
+    result = {
+        "metric": "particle-substeps/s",
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded spheres-and-slab scene)",
+        "config": {
+            "workload": "spheres_and_slab_4.2M",
+            "particles": n,
+            "grid": "256^3 dense block grid, 4^3 blocks, h=1/256",
+            "dt": scene.time_step,
+            "materials": "3 Neo-Hookean spheres (E log-uniform 1e4..1e6, nu 0.3) + inviscid fluid slab (K 1e5, gamma 7)",
+            "boundary": "box walls (separating slip)",
+            "active_nodes": nodes,
+            "active_blocks": int(status.active_blocks),
+        },
+        "roofline": {
+            "kernel": dominant,
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic_for(dominant, "spheres_and_slab"),
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+            "avg_launch_ms": stage_ms[dominant],
+        },
+        "substep_roofline": {
+            "bytes_per_substep": substep_bytes,
+            "achieved": substep_bytes * args.steps / elapsed / 1e9,
+            "frac": substep_bytes * args.steps / elapsed / 1e9 / HBM_PEAK_GBS,
+        },
+        "stage_ms_per_substep": stage_ms,
+        "svd_fp64_fallbacks": int(status.svd_fp64_fallbacks),
+    }
+    gpu.close()
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        log(f"bench: CPU baseline, {args.cpu_substeps} substeps on {threads} threads")
+        result["cpu_baseline"] = cpu_baseline(scene, args.cpu_substeps, threads)
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

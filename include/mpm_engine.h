@@ -1,0 +1,290 @@
+/*
+ * mpm_engine.h - C ABI of the MI355X MLS-MPM substep engine.
+ *
+ * The engine replaces the wgpu backend of squishy_volumes behind the backend
+ * contract that the compute thread dispatches to
+ * (rust/crates/core/src/compute_thread.rs:100-163):
+ *
+ *   GpuState::from_io_state     rust/crates/gpu/src/gpu_state.rs:38-190
+ *   GpuState::produce_next_state rust/crates/gpu/src/gpu_state.rs:331-675
+ *   CpuState::to_io_state        rust/crates/cpu/src/cpu_state.rs:71-136
+ *
+ * Every pointer in the engine API (mpm_create .. mpm_read_grid) is a HOST
+ * pointer; the engine owns all device memory.  The mpm_dev_* entry points
+ * further below run one kernel stage on DEVICE pointers and exist for the
+ * per-kernel parity tests and the per-kernel benchmark.
+ *
+ * Threading: one engine is used from one thread at a time (the reference calls
+ * its backend only from the simulation's compute thread,
+ * rust/crates/core/src/compute_thread.rs:78).  No C++ exception crosses the
+ * ABI; every entry point returns an mpm_result code.
+ */
+#ifndef SQUISHY_VOLUMES_AMD_MPM_ENGINE_H
+#define SQUISHY_VOLUMES_AMD_MPM_ENGINE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- particle flags ------------------------------------------------------
+ * Bits 0-6 have the values of ParticleFlags
+ * (rust/crates/file_frame/src/particles.rs:23-33).  Bits 8-9 select the two
+ * constitutive models that the reference does not have (fixed-corotated and
+ * snow plasticity); a solid without them is Neo-Hookean as in the reference. */
+enum {
+    MPM_IS_SOLID = 1u << 0,
+    MPM_IS_FLUID = 1u << 1,
+    MPM_USE_VISCOSITY = 1u << 2,
+    MPM_USE_SAND_ALPHA = 1u << 3,
+    MPM_HAS_GOAL = 1u << 4,
+    MPM_TOMBSTONED = 1u << 5,
+    MPM_FAILED = 1u << 6,
+    MPM_FIXED_COROTATED = 1u << 8,
+    MPM_SNOW = 1u << 9
+};
+
+/* ---- device status word --------------------------------------------------
+ * Low 16 bits: error bits, high 16 bits: id of the reporting kernel, as in
+ * GpuStatus (rust/crates/gpu/src/status.rs:18-77).  Bits 1-8 keep the
+ * reference values; 16.. are errors this engine adds. */
+enum {
+    MPM_STATUS_TABLE_TRIES_EXCEEDED = 1u,      /* never set: the grid is not hashed */
+    MPM_STATUS_TABLE_ENTRY_MISSING = 2u,       /* never set: the grid is not hashed */
+    MPM_STATUS_INDIRECT_LIMIT_EXCEEDED = 4u,   /* a block held more particles than a work list can */
+    MPM_STATUS_PARTICLE_CLOSE_TO_INVERTED = 8u,/* det F < 1e-7 before the stress evaluation */
+    MPM_STATUS_ENERGY_ERROR = 16u,             /* det F <= 0 after the F update (CPU EnergyError) */
+    MPM_STATUS_OUT_OF_GRID = 32u               /* a live particle's stencil left the allocated grid */
+};
+
+/* ---- result codes -------------------------------------------------------- */
+typedef enum mpm_result {
+    MPM_OK = 0,
+    /* Inner errors: the state is valid and readable, the error is reported after
+     * it (the inner Result of produce_next_state,
+     * rust/core/src/compute_thread.rs:165-169). */
+    MPM_ERR_ENERGY = 1,            /* a particle's det F went <= 0; it is flagged FAILED */
+    MPM_ERR_CLOSE_TO_INVERTED = 2, /* GpuShaderError::ParticleCloseToInverted */
+    /* Fatal errors (the outer Err). */
+    MPM_ERR_INVALID_ARGUMENT = -1,
+    MPM_ERR_HIP = -2,
+    MPM_ERR_NO_DEVICE = -3,
+    MPM_ERR_OUT_OF_MEMORY = -4,
+    MPM_ERR_GRID_TOO_LARGE = -5,
+    MPM_ERR_CANCELED = -6,         /* Error::Canceled (rust/crates/cpu/src/errors.rs:11) */
+    MPM_ERR_WRONG_FRAME = -7,      /* FrameInputError::WrongFrameLoaded (xpu/src/frame_input.rs:13) */
+    MPM_ERR_ZERO_TIME_STEP = -8,   /* Error::ZeroTimeStep */
+    MPM_ERR_OUT_OF_GRID = -9,
+    MPM_ERR_CAPACITY = -10
+} mpm_result;
+
+/* ---- engine options (mpm_consts.options) ---------------------------------
+ * Grid-side forces and boundary conditions have no reference counterpart: the
+ * reference adds gravity to particle velocities before P2G
+ * (rust/crates/cpu/src/phase/external_force.rs:44-46) and has no grid BC. */
+enum {
+    MPM_OPT_GRID_GRAVITY = 1u << 0,  /* add dt*g on the grid instead of on particles */
+    MPM_OPT_STICKY_PLANE = 1u << 1,  /* nodes with z < plane_z get v = 0 */
+    MPM_OPT_BOX_WALLS = 1u << 2      /* separating slip on the faces of [wall_min, wall_max] */
+};
+
+/* Per-simulation constants: InputConsts (rust/crates/file_input/src/header.rs:9-69)
+ * after scaling, plus the fixed time step of ComputeSettings
+ * (rust/crates/core/src/simulation.rs:242-250). */
+typedef struct mpm_consts {
+    float grid_node_size;        /* scaled_grid_node_size() */
+    float time_step;             /* fixed dt (adaptive time steps are not supported) */
+    float domain_min[3];         /* scaled_domain_min(); particles are culled outside */
+    float domain_max[3];
+    uint32_t frames_per_second;
+    uint32_t options;            /* MPM_OPT_* */
+    float plane_z;               /* MPM_OPT_STICKY_PLANE */
+    float wall_min[3];           /* MPM_OPT_BOX_WALLS */
+    float wall_max[3];
+    /* snow plasticity (MPM_SNOW particles): singular values are clamped to
+     * [1 - snow_compression, 1 + snow_stretch]; Lame parameters scale with
+     * exp(snow_hardening * (1 - Jp)). */
+    float snow_compression;
+    float snow_stretch;
+    float snow_hardening;
+    int32_t device;              /* HIP device ordinal; GpuState's adapter name */
+} mpm_consts;
+
+/* Per-particle parameters, the flattened form of ParticleParameters
+ * (rust/crates/file_frame/src/particles.rs:61-80) as the reference's device
+ * struct ParticleParametersDevice (rust/crates/gpu/src/particle_parameters.rs:15-24). */
+typedef struct mpm_particle_parameters {
+    float mass;
+    float initial_volume;
+    float viscosity_dynamic;     /* used iff MPM_USE_VISCOSITY */
+    float viscosity_bulk;
+    float mu;                    /* solids */
+    float lambda;
+    float sand_alpha;            /* used iff MPM_USE_SAND_ALPHA */
+    float bulk_modulus;          /* fluids */
+    int32_t exponent;
+} mpm_particle_parameters;
+
+/* A view of Particles (rust/crates/file_frame/src/particles.rs:92-109).  All
+ * arrays hold `count` entries in the caller's (original) particle order.
+ * Vectors are xyz triples, matrices are column-major 3x3 ([[f32;3];3] with the
+ * outer index the column), as in IoState.  plastic_jp is an extension (snow
+ * plasticity); NULL means 1 for every particle on input and "not wanted" on
+ * output. */
+typedef struct mpm_particles_view {
+    uint64_t count;
+    double time;                         /* IoState::time */
+    uint32_t *flags;
+    mpm_particle_parameters *parameters;
+    float *elastic_energies;
+    uint32_t *collider_bits;
+    float *positions;                    /* count*3 */
+    float *position_gradients;           /* count*9 */
+    float *velocities;                   /* count*3 */
+    float *velocity_gradients;           /* count*9 */
+    float *initial_positions;            /* count*3 */
+    float *plastic_jp;                   /* count, optional */
+} mpm_particles_view;
+
+/* Per-frame inputs: the parts of FrameInput (rust/crates/xpu/src/frame_input.rs:33-278)
+ * the substep reads without mesh colliders.  `a` is the frame that contains
+ * the current time, `b` the next one (or absent at the end of the input). */
+typedef struct mpm_frame_input {
+    uint64_t frame;                      /* index of frame a */
+    float gravity_a[3];
+    float gravity_b[3];
+    int32_t has_b;
+    const uint32_t *particle_flags_a;    /* count, original order; NULL: no goals */
+    const uint32_t *particle_flags_b;
+    const float *goal_positions_a;       /* count*3 */
+    const float *goal_positions_b;
+} mpm_frame_input;
+
+/* Report of one mpm_step_to call. */
+typedef struct mpm_status {
+    uint32_t device_status;              /* the device status word (bits above) */
+    int32_t result;                      /* the mpm_result of the call */
+    uint64_t substeps;                   /* substeps run by this call */
+    double time;                         /* simulation time after the call */
+    uint64_t active_blocks;              /* occupied 4^3 blocks in the last substep */
+    uint64_t active_nodes;               /* grid nodes with mass > 0 in the last substep */
+    uint64_t svd_fp64_fallbacks;         /* SVDs recomputed in FP64 during the call */
+} mpm_status;
+
+/* Grid nodes, as GridNodes (rust/crates/file_frame/src/grid_nodes.rs:9-15):
+ * the active nodes (mass > 0) after the last grid update. */
+typedef struct mpm_grid_view {
+    uint64_t capacity;                   /* entries the arrays can take */
+    uint64_t count;                      /* entries written */
+    int32_t *node_ids;                   /* count*3 */
+    uint32_t *collider_bits;
+    float *masses;
+    float *velocities;                   /* count*3 */
+} mpm_grid_view;
+
+typedef struct mpm_engine mpm_engine;
+
+/* Engine tuning (the autotuner's knobs).  Zero fields take defaults. */
+typedef struct mpm_tuning {
+    int32_t p2g_waves_per_block;         /* 1, 2 or 4 */
+    int32_t g2p_threads;                 /* 64 .. 1024, a multiple of 64 */
+    int32_t persistent_blocks;           /* work-groups of the block-loop kernels */
+    int32_t sort_items_per_thread;       /* keys per thread and radix tile */
+} mpm_tuning;
+
+/* ---- engine API (host pointers) ------------------------------------------ */
+
+/* GpuState::from_io_state (rust/crates/gpu/src/gpu_state.rs:38-190).  Copies
+ * the particles to the device.  Fails with MPM_ERR_GRID_TOO_LARGE when the
+ * domain's dense block table does not fit. */
+int mpm_create(const mpm_consts *consts, const mpm_particles_view *particles,
+               const mpm_tuning *tuning, mpm_engine **out_engine);
+
+/* GpuState::produce_next_state (rust/crates/gpu/src/gpu_state.rs:337-675):
+ * runs substeps of the fixed time step while time < target_time.  `cancel`
+ * (may be NULL) is polled between substeps like Harness::check
+ * (rust/crates/xpu/src/harness.rs:65-76). */
+int mpm_step_to(mpm_engine *engine, double target_time, const mpm_frame_input *input,
+                const volatile int32_t *cancel, mpm_status *status);
+
+/* CpuState::to_io_state (rust/crates/cpu/src/cpu_state.rs:71-136): writes the
+ * particle state in original order into caller-owned arrays (count must match;
+ * any array may be NULL to skip it). */
+int mpm_read_particles(mpm_engine *engine, mpm_particles_view *out);
+
+/* The grid half of to_io_state (store_grid = true). */
+int mpm_read_grid(mpm_engine *engine, mpm_grid_view *out);
+
+/* Drop of GpuState. */
+void mpm_destroy(mpm_engine *engine);
+
+/* Text of an mpm_result. */
+const char *mpm_error_string(int result);
+
+/* Simulation time of the engine. */
+double mpm_time(const mpm_engine *engine);
+
+/* Per-kernel timing of the substeps run since the last call with reset=1:
+ * names[i] / total_ms[i] / launches[i] for i < *count (at most `capacity`).
+ * Timing uses HIP events on the engine's stream and is enabled by
+ * mpm_set_profiling. */
+int mpm_set_profiling(mpm_engine *engine, int32_t enabled);
+int mpm_kernel_times(mpm_engine *engine, int32_t reset, int32_t capacity, const char **names,
+                     double *total_ms, uint64_t *launches, int32_t *count);
+
+/* The engine's HIP stream (hipStream_t) so that callers can order work. */
+void *mpm_stream(mpm_engine *engine);
+
+/* Number of HIP devices (GpuContext::available_gpus, rust/crates/gpu/src/context.rs). */
+int mpm_device_count(void);
+
+/* ---- per-kernel entry points (device pointers, stream may be NULL) -------- */
+
+/* Stable LSD radix sort of `n` u32 keys by their low `key_bits` bits, carrying
+ * u32 values.  values_in == NULL sorts the identity permutation.  The result
+ * equals a stable CPU sort by key (sort_on_cpu, rust/crates/gpu/src/util.rs:65-69;
+ * RadixSort::record_all_rounds, rust/crates/gpu/src/radix_sort/mod.rs:209-238).
+ * The temp buffer takes mpm_dev_radix_sort_temp_bytes(n) bytes. */
+size_t mpm_dev_radix_sort_temp_bytes(uint64_t n);
+int mpm_dev_radix_sort(const uint32_t *keys_in, const uint32_t *values_in, uint32_t *keys_out,
+                       uint32_t *values_out, uint64_t n, int32_t key_bits, void *temp,
+                       void *stream);
+
+/* Exclusive prefix sum of u32 (prefix_sum_on_cpu, rust/crates/gpu/src/util.rs:54-63);
+ * *total receives the sum (device pointer). */
+size_t mpm_dev_scan_temp_bytes(uint64_t n);
+int mpm_dev_exclusive_scan(const uint32_t *in, uint32_t *out, uint32_t *total, uint64_t n,
+                           void *temp, void *stream);
+
+/* Copy kernel of known size (float4 loads/stores), used to calibrate the HBM
+ * byte counters of the profiler. */
+int mpm_dev_copy(const void *src, void *dst, uint64_t bytes, void *stream);
+
+/* Per-particle constitutive kernel for the parity tests: for `n` particles
+ * (SoA device arrays, F column-major 9 per particle) writes the Kirchhoff
+ * stress tau = P F^T (9 per particle, column-major), the elastic energy density
+ * and, for SVD-based models, the signed singular values (3 per particle). */
+int mpm_dev_stress(const uint32_t *flags, const mpm_particle_parameters *params,
+                   const float *F, const float *jp, float snow_hardening, float *tau,
+                   float *energy, uint64_t n, void *stream);
+
+/* 3x3 signed SVD F = U diag(s) V^T with U, V rotations (n matrices, column-major). */
+int mpm_dev_svd(const float *F, float *U, float *s, float *V, uint64_t n, void *stream);
+
+/* One substep stage at a time on an engine, for the per-kernel parity tests:
+ * stage 0 = sort, 1 = P2G, 2 = grid update, 3 = G2P (first substep of a call:
+ * P2G evaluates the stress from F and C).  Host-side grid/particle readback
+ * goes through mpm_read_grid / mpm_read_particles. */
+int mpm_run_stage(mpm_engine *engine, int32_t stage, const mpm_frame_input *input);
+
+/* Raw P2G result (mass, momentum before normalisation) of the last P2G stage
+ * for the active nodes, same layout as mpm_read_grid (velocities hold momentum). */
+int mpm_read_grid_momentum(mpm_engine *engine, mpm_grid_view *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,767 @@
+// mpm_oracle.hpp - CPU restatement of the squishy_volumes CPU substep.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP engine: it
+// is compiled into oracle/liboracle.so, which only the tests, the smoke check
+// and the CPU-baseline leg of the benchmark load.  The engine never links it.
+//
+// It restates, phase by phase, CpuState::produce_next_state
+// (rust/crates/cpu/src/cpu_state.rs:146-197) with the phase order of
+// rust/crates/cpu/src/phase/mod.rs:27-75, templated on the real type so that
+// the same code runs in float (the reference's precision) and double (the
+// FP64 run the engine is scored against).  Each function cites the reference
+// lines it follows.  Extensions without a reference counterpart (fixed-
+// corotated and snow materials, grid gravity, grid boundary conditions) are
+// marked as such and pinned by closed forms in the tests.
+//
+// Deviations from the reference that only change summation order:
+//  * the sort is stable (the reference uses par_sort_unstable_by_key);
+//  * grid nodes are indexed through a dense box instead of an FxHashMap, and a
+//    node's contributors are visited in particle order (the reference pushes
+//    them under a Mutex in scheduling order).
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <stdexcept>
+#include <vector>
+
+#include <omp.h>
+
+#include "../include/mpm_engine.h"
+
+namespace mpm_oracle {
+
+// ---------------------------------------------------------------- math types
+template <class R>
+struct V3 {
+    R x = 0, y = 0, z = 0;
+    V3() = default;
+    V3(R a, R b, R c) : x(a), y(b), z(c) {}
+    R &operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+    R operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+    V3 operator+(const V3 &o) const { return {x + o.x, y + o.y, z + o.z}; }
+    V3 operator-(const V3 &o) const { return {x - o.x, y - o.y, z - o.z}; }
+    V3 operator*(R s) const { return {x * s, y * s, z * s}; }
+    V3 operator/(R s) const { return {x / s, y / s, z / s}; }
+    V3 &operator+=(const V3 &o) { x += o.x; y += o.y; z += o.z; return *this; }
+    V3 &operator-=(const V3 &o) { x -= o.x; y -= o.y; z -= o.z; return *this; }
+    V3 &operator*=(R s) { x *= s; y *= s; z *= s; return *this; }
+};
+template <class R> V3<R> operator*(R s, const V3<R> &v) { return v * s; }
+template <class R> V3<R> cross(const V3<R> &a, const V3<R> &b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class R> R dot(const V3<R> &a, const V3<R> &b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// 3x3 matrix, m[row][col].
+template <class R>
+struct M3 {
+    R m[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    static M3 identity() { M3 r; r.m[0][0] = r.m[1][1] = r.m[2][2] = 1; return r; }
+    static M3 diag(R a, R b, R c) { M3 r; r.m[0][0] = a; r.m[1][1] = b; r.m[2][2] = c; return r; }
+    V3<R> col(int j) const { return {m[0][j], m[1][j], m[2][j]}; }
+    void set_col(int j, const V3<R> &v) { m[0][j] = v.x; m[1][j] = v.y; m[2][j] = v.z; }
+    M3 operator+(const M3 &o) const { M3 r; for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) r.m[i][j] = m[i][j] + o.m[i][j]; return r; }
+    M3 operator-(const M3 &o) const { M3 r; for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) r.m[i][j] = m[i][j] - o.m[i][j]; return r; }
+    M3 operator*(R s) const { M3 r; for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) r.m[i][j] = m[i][j] * s; return r; }
+    M3 &operator+=(const M3 &o) { for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) m[i][j] += o.m[i][j]; return *this; }
+    M3 &operator*=(R s) { for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) m[i][j] *= s; return *this; }
+    M3 operator*(const M3 &o) const {
+        M3 r;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                R s = 0;
+                for (int k = 0; k < 3; ++k) s += m[i][k] * o.m[k][j];
+                r.m[i][j] = s;
+            }
+        return r;
+    }
+    V3<R> operator*(const V3<R> &v) const {
+        return {m[0][0] * v.x + m[0][1] * v.y + m[0][2] * v.z, m[1][0] * v.x + m[1][1] * v.y + m[1][2] * v.z,
+                m[2][0] * v.x + m[2][1] * v.y + m[2][2] * v.z};
+    }
+    M3 transpose() const { M3 r; for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) r.m[i][j] = m[j][i]; return r; }
+    R trace() const { return m[0][0] + m[1][1] + m[2][2]; }
+    R determinant() const {
+        return m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+               m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+    }
+    R norm_squared() const { R s = 0; for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) s += m[i][j] * m[i][j]; return s; }
+};
+template <class R> M3<R> outer(const V3<R> &a, const V3<R> &b) {
+    M3<R> r;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) r.m[i][j] = a[i] * b[j];
+    return r;
+}
+
+// ------------------------------------------------- weights (kernels.rs:17-92)
+// Quadratic B-spline, rust/crates/cpu/src/kernels.rs:17-26.
+template <class R> R kernel_quadratic(R x) {
+    x = std::abs(x);
+    if (x < R(0.5)) return R(0.75) - x * x;
+    if (x < R(1.5)) return R(0.5) * (R(1.5) - x) * (R(1.5) - x);
+    return 0;
+}
+
+// position_to_shift_quadratic, rust/crates/cpu/src/kernels.rs:46-49.
+template <class R> void position_to_shift_quadratic(const V3<R> &p, R h, int s[3]) {
+    for (int a = 0; a < 3; ++a) s[a] = (int)std::floor(p[a] / h - R(0.5));
+}
+
+// ---------------------------------------------- constitutive math (elastic.rs)
+// Lame parameters, rust/crates/util/src/elastic.rs:52-64.
+template <class R> R lame_mu(R E, R nu) { return E / R(2) / (R(1) + nu); }
+template <class R> R lame_lambda(R E, R nu) { return E * nu / (R(1) + nu) / (R(1) - R(2) * nu); }
+
+// partial_invariant_3_by_position_gradient, elastic.rs:110-118:
+// columns f1 x f2, f2 x f0, f0 x f1.
+template <class R> M3<R> cofactor(const M3<R> &F) {
+    M3<R> c;
+    c.set_col(0, cross(F.col(1), F.col(2)));
+    c.set_col(1, cross(F.col(2), F.col(0)));
+    c.set_col(2, cross(F.col(0), F.col(1)));
+    return c;
+}
+
+// first_piola_stress_neo_hookean, elastic.rs:299-311 with the partials of
+// :282-291: P = (mu/2) * 2F + ((lambda ln J - mu) / J) * cof(F).
+template <class R> M3<R> piola_neo_hookean(R mu, R lambda, const M3<R> &F) {
+    R J = F.determinant();
+    R d3 = (lambda * std::log(J) - mu) / J;
+    return F * R(2) * (mu / R(2)) + cofactor(F) * d3;
+}
+
+// try_elastic_energy_neo_hookean, elastic.rs:259-272 (+ :246-257).
+template <class R> bool energy_neo_hookean(R mu, R lambda, const M3<R> &F, R &out) {
+    R J = F.determinant();
+    if (!(J > 0)) return false;
+    R lnJ = std::log(J);
+    out = mu / R(2) * (F.norm_squared() - R(3)) - mu * lnJ + lambda / R(2) * lnJ * lnJ;
+    return true;
+}
+
+template <class R> R powi(R x, int n) {
+    // f32::powi / f64::powi: integer power by repeated multiplication.
+    bool neg = n < 0;
+    unsigned e = neg ? (unsigned)(-n) : (unsigned)n;
+    R r = 1, b = x;
+    while (e) { if (e & 1u) r *= b; b *= b; e >>= 1; }
+    return neg ? R(1) / r : r;
+}
+
+// first_piola_stress_inviscid, elastic.rs:582-592 with :537-544.
+template <class R> M3<R> piola_inviscid(R K, int exponent, const M3<R> &F) {
+    R J = F.determinant();
+    R d3 = K * (R(1) - R(1) / powi(J, exponent));
+    return cofactor(F) * d3;
+}
+
+// elastic_energy_inviscid_by_invariant, elastic.rs:524-533.
+template <class R> R energy_inviscid(R K, int exponent, const M3<R> &F) {
+    R J = F.determinant();
+    R at_rest = K * (R(1) - R(1) / (R(1) - R(exponent)));
+    return K * (J - powi(J, 1 - exponent) / (R(1) - R(exponent))) - at_rest;
+}
+
+// cauchy_stress_general_viscosity, elastic.rs:659-677.
+template <class R> M3<R> cauchy_viscosity(R dynamic, R bulk, const M3<R> &C) {
+    M3<R> strain = (C + C.transpose()) * R(0.5);
+    M3<R> r = strain * (R(2) * dynamic);
+    R div = C.trace();
+    for (int i = 0; i < 3; ++i) r.m[i][i] += bulk * div;
+    return r;
+}
+
+// ---------------------------------------------------------------- 3x3 SVD
+// Signed SVD F = U diag(s) V^T with U, V proper rotations; s[0] >= s[1] >= |s[2]|,
+// s[2] < 0 iff det F < 0.  Cyclic Jacobi on F^T F, then a Givens QR of F V.
+// Written for the oracle from the textbook method (Golub & Van Loan, 8.5);
+// it is deliberately not the engine's algorithm.
+template <class R> void jacobi_rotate(M3<R> &A, M3<R> &V, int p, int q) {
+    R apq = A.m[p][q];
+    if (apq == 0) return;
+    R theta = (A.m[q][q] - A.m[p][p]) / (R(2) * apq);
+    R t = (theta >= 0 ? R(1) : R(-1)) / (std::abs(theta) + std::sqrt(theta * theta + R(1)));
+    R c = R(1) / std::sqrt(t * t + R(1));
+    R s = t * c;
+    // A <- G^T A G with G the rotation in the (p, q) plane.
+    for (int k = 0; k < 3; ++k) {
+        R akp = A.m[k][p], akq = A.m[k][q];
+        A.m[k][p] = c * akp - s * akq;
+        A.m[k][q] = s * akp + c * akq;
+    }
+    for (int k = 0; k < 3; ++k) {
+        R apk = A.m[p][k], aqk = A.m[q][k];
+        A.m[p][k] = c * apk - s * aqk;
+        A.m[q][k] = s * apk + c * aqk;
+    }
+    for (int k = 0; k < 3; ++k) {
+        R vkp = V.m[k][p], vkq = V.m[k][q];
+        V.m[k][p] = c * vkp - s * vkq;
+        V.m[k][q] = s * vkp + c * vkq;
+    }
+}
+
+template <class R> void svd3(const M3<R> &F, M3<R> &U, V3<R> &s, M3<R> &V) {
+    M3<R> A = F.transpose() * F;
+    V = M3<R>::identity();
+    const int sweeps = sizeof(R) == 8 ? 30 : 12;
+    for (int it = 0; it < sweeps; ++it) {
+        R off = A.m[0][1] * A.m[0][1] + A.m[0][2] * A.m[0][2] + A.m[1][2] * A.m[1][2];
+        R diag = A.m[0][0] * A.m[0][0] + A.m[1][1] * A.m[1][1] + A.m[2][2] * A.m[2][2];
+        if (off <= std::numeric_limits<R>::min() || off <= diag * std::numeric_limits<R>::epsilon() *
+                                                                std::numeric_limits<R>::epsilon())
+            break;
+        jacobi_rotate(A, V, 0, 1);
+        jacobi_rotate(A, V, 0, 2);
+        jacobi_rotate(A, V, 1, 2);
+    }
+    if (V.determinant() < 0) V.set_col(2, V.col(2) * R(-1));
+    M3<R> B = F * V;
+    // Sort columns of B by decreasing norm, keeping V a rotation.
+    R n[3];
+    for (int j = 0; j < 3; ++j) n[j] = dot(B.col(j), B.col(j));
+    auto swap_cols = [&](int a, int b) {
+        V3<R> t = B.col(a); B.set_col(a, B.col(b)); B.set_col(b, t * R(-1));
+        t = V.col(a); V.set_col(a, V.col(b)); V.set_col(b, t * R(-1));
+        std::swap(n[a], n[b]);
+    };
+    if (n[0] < n[1]) swap_cols(0, 1);
+    if (n[0] < n[2]) swap_cols(0, 2);
+    if (n[1] < n[2]) swap_cols(1, 2);
+    // Givens QR: B = Q R, Q a rotation, R upper triangular.
+    U = M3<R>::identity();
+    auto givens = [&](int i, int j) {  // zero B(j, i) using rows i and j
+        R a = B.m[i][i], b = B.m[j][i];
+        R r = std::sqrt(a * a + b * b);
+        R c = r > 0 ? a / r : R(1), sn = r > 0 ? b / r : R(0);
+        for (int k = 0; k < 3; ++k) {
+            R bi = B.m[i][k], bj = B.m[j][k];
+            B.m[i][k] = c * bi + sn * bj;
+            B.m[j][k] = -sn * bi + c * bj;
+        }
+        for (int k = 0; k < 3; ++k) {  // U <- U G^T
+            R ui = U.m[k][i], uj = U.m[k][j];
+            U.m[k][i] = c * ui + sn * uj;
+            U.m[k][j] = -sn * ui + c * uj;
+        }
+    };
+    givens(0, 1);
+    givens(0, 2);
+    givens(1, 2);
+    s = {B.m[0][0], B.m[1][1], B.m[2][2]};
+    // The first two diagonal entries are made non-negative by rotating pairs.
+    for (int j = 0; j < 2; ++j)
+        if (s[j] < 0) {
+            s[j] = -s[j];
+            s[2] = -s[2];
+            U.set_col(j, U.col(j) * R(-1));
+            U.set_col(2, U.col(2) * R(-1));
+        }
+}
+
+// ---- extensions: fixed-corotated and snow (no reference counterpart) --------
+// Fixed-corotated (Stomakhin et al. 2012): Psi = mu |F - R|^2 + lambda/2 (J-1)^2,
+// tau = P F^T = 2 mu (F - R) F^T + lambda (J - 1) J I.
+template <class R> M3<R> kirchhoff_fixed_corotated(R mu, R lambda, const M3<R> &F, R &energy) {
+    M3<R> U, V;
+    V3<R> s;
+    svd3(F, U, s, V);
+    M3<R> Rot = U * V.transpose();
+    R J = s[0] * s[1] * s[2];
+    M3<R> tau = (F - Rot) * F.transpose() * (R(2) * mu);
+    for (int i = 0; i < 3; ++i) tau.m[i][i] += lambda * (J - R(1)) * J;
+    energy = mu * ((s[0] - 1) * (s[0] - 1) + (s[1] - 1) * (s[1] - 1) + (s[2] - 1) * (s[2] - 1)) +
+             lambda / R(2) * (J - 1) * (J - 1);
+    return tau;
+}
+template <class R> M3<R> piola_fixed_corotated(R mu, R lambda, const M3<R> &F) {
+    M3<R> U, V;
+    V3<R> s;
+    svd3(F, U, s, V);
+    M3<R> Rot = U * V.transpose();
+    R J = F.determinant();
+    return (F - Rot) * (R(2) * mu) + cofactor(F) * (lambda * (J - R(1)));
+}
+
+// Kirchhoff stress tau = P F^T of a particle, the quantity P2G consumes
+// (rust/crates/cpu/src/phase/scatter_momentum.rs:52-84).
+template <class R>
+M3<R> first_piola(uint32_t flags, const mpm_particle_parameters &p, R jp, R hardening, const M3<R> &F) {
+    if (flags & MPM_IS_FLUID) return piola_inviscid<R>(p.bulk_modulus, p.exponent, F);
+    if (flags & (MPM_FIXED_COROTATED | MPM_SNOW)) {
+        R h = (flags & MPM_SNOW) ? std::exp(hardening * (R(1) - jp)) : R(1);
+        return piola_fixed_corotated<R>(R(p.mu) * h, R(p.lambda) * h, F);
+    }
+    return piola_neo_hookean<R>(p.mu, p.lambda, F);
+}
+
+// ----------------------------------------------------------------- the state
+template <class R>
+struct GridKey {
+    int id[3];
+    uint32_t collider_bits;
+};
+
+// collider_bits::compatible, rust/crates/util/src/collider_bits.rs:34-38.
+inline bool collider_compatible(uint32_t a, uint32_t b) {
+    uint32_t mask = (a & b) >> 16;
+    uint32_t diff = a ^ b;
+    return (mask & diff) == 0;
+}
+
+struct FrameInputCopy {
+    uint64_t frame = 0;
+    float gravity_a[3] = {0, 0, 0}, gravity_b[3] = {0, 0, 0};
+    bool has_b = false;
+    std::vector<uint32_t> flags_a, flags_b;
+    std::vector<float> goals_a, goals_b;
+};
+
+template <class R>
+struct State {
+    // CpuState (rust/crates/cpu/src/cpu_state.rs:15-23) and Particles
+    // (rust/crates/cpu/src/particles.rs:13-31).
+    double time = 0;
+    mpm_consts consts{};
+    std::vector<uint32_t> sort_map, reverse_sort_map, flags, collider_bits;
+    std::vector<mpm_particle_parameters> parameters;
+    std::vector<V3<R>> initial_positions, positions, velocities;
+    std::vector<M3<R>> position_gradients, velocity_gradients;
+    std::vector<R> elastic_energies, plastic_jp;
+
+    // GridNodes (rust/crates/cpu/src/grid_nodes.rs:22-33), in CSR form.
+    std::vector<GridKey<R>> keys;
+    std::vector<uint32_t> contrib_offsets, contributors;
+    std::vector<R> masses;
+    std::vector<V3<R>> node_velocities;       // momentum after scatter, velocity after meld
+    std::vector<int32_t> lookup;              // dense node id -> node index (the FxHashMap)
+    int lookup_lo[3] = {0, 0, 0}, lookup_dim[3] = {0, 0, 0};
+
+    // InterpolatedInput (rust/crates/cpu/src/interpolated_input.rs:12-22).
+    V3<R> gravity;
+    std::vector<V3<R>> goal_positions;
+    bool have_goals = false;
+    FrameInputCopy input;
+
+    int32_t lookup_node(const int id[3]) const {
+        int64_t idx = 0;
+        for (int a = 2; a >= 0; --a) {
+            int c = id[a] - lookup_lo[a];
+            if (c < 0 || c >= lookup_dim[a]) return -1;
+            idx = idx * lookup_dim[a] + c;
+        }
+        return lookup[idx];
+    }
+
+    R h() const { return R(consts.grid_node_size); }
+    R dt() const { return R(consts.time_step); }
+
+    // ---- InterpolateInput, rust/crates/cpu/src/phase/interpolate_input.rs:18-35
+    // with FrameInput::frame_factor, rust/crates/xpu/src/frame_input.rs:266-278.
+    int interpolate_input() {
+        double frame_time = time * (double)consts.frames_per_second;
+        uint64_t frame_low = (uint64_t)std::floor(frame_time);
+        if (frame_low != input.frame) return MPM_ERR_WRONG_FRAME;
+        float factor_b = (float)std::fmod(frame_time, 1.0);
+        float factor_a = 1.f - factor_b;
+        const float *gb = input.has_b ? input.gravity_b : input.gravity_a;
+        // The reference interpolates in f32 (Vector3<f32>).
+        gravity = V3<R>(R(factor_a * input.gravity_a[0] + factor_b * gb[0]),
+                        R(factor_a * input.gravity_a[1] + factor_b * gb[1]),
+                        R(factor_a * input.gravity_a[2] + factor_b * gb[2]));
+        have_goals = !input.flags_a.empty();
+        if (have_goals) {
+            size_t n = sort_map.size();
+            goal_positions.resize(n);
+            const std::vector<float> &b = input.has_b ? input.goals_b : input.goals_a;
+#pragma omp parallel for schedule(static)
+            for (int64_t i = 0; i < (int64_t)n; ++i)
+                goal_positions[i] = V3<R>(R(factor_a * input.goals_a[3 * i] + factor_b * b[3 * i]),
+                                          R(factor_a * input.goals_a[3 * i + 1] + factor_b * b[3 * i + 1]),
+                                          R(factor_a * input.goals_a[3 * i + 2] + factor_b * b[3 * i + 2]));
+        }
+        return MPM_OK;
+    }
+
+    // ---- Sort, rust/crates/cpu/src/phase/sort.rs:17-114.
+    void sort() {
+        const size_t n = positions.size();
+        const R hh = h();
+        std::vector<int> key(3 * n);
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < (int64_t)n; ++i)
+            for (int a = 0; a < 3; ++a) key[3 * i + a] = (int)std::floor(positions[i][a] / hh - R(0.5));
+        std::vector<uint32_t> perm(n);
+        std::iota(perm.begin(), perm.end(), 0u);
+        std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) {
+            for (int c = 0; c < 3; ++c) {
+                if (key[3 * a + c] != key[3 * b + c]) return key[3 * a + c] < key[3 * b + c];
+            }
+            return false;
+        });
+        auto apply = [&](auto &v) {
+            auto lookup_copy = v;
+#pragma omp parallel for schedule(static)
+            for (int64_t i = 0; i < (int64_t)n; ++i) v[i] = lookup_copy[perm[i]];
+        };
+        // The arrays that move with the particles (sort.rs:57-69); the elastic
+        // energies are not moved because advance_particles overwrites them.
+        apply(flags);
+        apply(positions);
+        apply(initial_positions);
+        apply(sort_map);
+        apply(parameters);
+        apply(position_gradients);
+        apply(velocities);
+        apply(velocity_gradients);
+        apply(collider_bits);
+        apply(plastic_jp);
+        reverse_sort_map.assign(n, 0);
+        for (size_t cur = 0; cur < n; ++cur) reverse_sort_map[sort_map[cur]] = (uint32_t)cur;
+    }
+
+    // ---- Collide without meshes: rust/crates/cpu/src/phase/collide.rs:53-58
+    // sets collider_bits = 0 for every live particle when no triangle is near.
+    void collide() {
+        const size_t n = positions.size();
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < (int64_t)n; ++i)
+            if (!(flags[i] & MPM_TOMBSTONED)) collider_bits[i] = 0;
+    }
+
+    // ---- ExternalForce, rust/crates/cpu/src/phase/external_force.rs:17-50.
+    void external_force() {
+        const size_t n = positions.size();
+        const R t = dt();
+        const bool grid_gravity = consts.options & MPM_OPT_GRID_GRAVITY;
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < (int64_t)n; ++i) {
+            if (flags[i] & MPM_TOMBSTONED) continue;
+            uint32_t orig = sort_map[i];
+            const uint32_t *fb = input.has_b ? input.flags_b.data() : input.flags_a.data();
+            if (have_goals && (input.flags_a[orig] & MPM_HAS_GOAL) && (fb[orig] & MPM_HAS_GOAL)) {
+                velocities[i] = (goal_positions[orig] - positions[i]) / t;
+            } else if (!grid_gravity) {
+                velocities[i] += gravity * t;
+            }
+        }
+    }
+
+    // ---- UpdateGridNodes, rust/crates/cpu/src/phase/update_grid_nodes.rs:31-156.
+    void update_grid_nodes() {
+        const size_t n = positions.size();
+        const R hh = h();
+        int lo[3] = {std::numeric_limits<int>::max(), std::numeric_limits<int>::max(), std::numeric_limits<int>::max()};
+        int hi[3] = {std::numeric_limits<int>::min(), std::numeric_limits<int>::min(), std::numeric_limits<int>::min()};
+        std::vector<int> shift(3 * n);
+        for (size_t i = 0; i < n; ++i) {
+            if (flags[i] & MPM_TOMBSTONED) continue;
+            position_to_shift_quadratic(positions[i], hh, &shift[3 * i]);
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::min(lo[a], shift[3 * i + a]);
+                hi[a] = std::max(hi[a], shift[3 * i + a] + 2);
+            }
+        }
+        keys.clear();
+        contrib_offsets.assign(1, 0);
+        contributors.clear();
+        if (lo[0] > hi[0]) {
+            lookup.clear();
+            for (int a = 0; a < 3; ++a) lookup_dim[a] = 0;
+            return;
+        }
+        int64_t total = 1;
+        for (int a = 0; a < 3; ++a) {
+            lookup_lo[a] = lo[a];
+            lookup_dim[a] = hi[a] - lo[a] + 1;
+            total *= lookup_dim[a];
+        }
+        if (total > (int64_t)1 << 31) throw std::runtime_error("oracle grid box too large");
+        std::vector<uint32_t> count((size_t)total, 0);
+        auto flat = [&](const int id[3]) {
+            return ((int64_t)(id[2] - lo[2]) * lookup_dim[1] + (id[1] - lo[1])) * lookup_dim[0] + (id[0] - lo[0]);
+        };
+        for (size_t i = 0; i < n; ++i) {
+            if (flags[i] & MPM_TOMBSTONED) continue;
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b)
+                    for (int c = 0; c < 3; ++c) {
+                        int id[3] = {shift[3 * i] + a, shift[3 * i + 1] + b, shift[3 * i + 2] + c};
+                        ++count[flat(id)];
+                    }
+        }
+        lookup.assign((size_t)total, -1);
+        // Node order: node id order (the reference's order is the hash map's).
+        for (int64_t f = 0; f < total; ++f) {
+            if (!count[f]) continue;
+            GridKey<R> k;
+            k.id[0] = (int)(f % lookup_dim[0]) + lo[0];
+            k.id[1] = (int)((f / lookup_dim[0]) % lookup_dim[1]) + lo[1];
+            k.id[2] = (int)(f / ((int64_t)lookup_dim[0] * lookup_dim[1])) + lo[2];
+            k.collider_bits = 0;
+            lookup[f] = (int32_t)keys.size();
+            keys.push_back(k);
+            contrib_offsets.push_back(contrib_offsets.back() + count[f]);
+        }
+        contributors.assign(contrib_offsets.back(), 0);
+        std::vector<uint32_t> fill(keys.size(), 0);
+        for (size_t i = 0; i < n; ++i) {
+            if (flags[i] & MPM_TOMBSTONED) continue;
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b)
+                    for (int c = 0; c < 3; ++c) {
+                        int id[3] = {shift[3 * i] + a, shift[3 * i + 1] + b, shift[3 * i + 2] + c};
+                        int32_t node = lookup[flat(id)];
+                        contributors[contrib_offsets[node] + fill[node]++] = (uint32_t)i;
+                    }
+        }
+    }
+
+    // ---- ScatterMomentum, rust/crates/cpu/src/phase/scatter_momentum.rs:22-93.
+    void scatter_momentum() {
+        const R hh = h();
+        const R scaling = dt() * R(4) / (hh * hh);
+        const size_t nn = keys.size();
+        masses.assign(nn, R(0));
+        node_velocities.assign(nn, V3<R>());
+        const R hardening = R(consts.snow_hardening);
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int64_t node = 0; node < (int64_t)nn; ++node) {
+            const GridKey<R> &key = keys[node];
+            R mass = 0;
+            V3<R> velocity;
+            for (uint32_t c = contrib_offsets[node]; c < contrib_offsets[node + 1]; ++c) {
+                uint32_t p = contributors[c];
+                V3<R> normalized = positions[p] / hh;
+                V3<R> to_node_normalized = V3<R>(R(key.id[0]), R(key.id[1]), R(key.id[2])) - normalized;
+                R weight = kernel_quadratic(to_node_normalized.x) * kernel_quadratic(to_node_normalized.y) *
+                           kernel_quadratic(to_node_normalized.z);
+                V3<R> to_node = to_node_normalized * hh;
+                const mpm_particle_parameters &par = parameters[p];
+                R m = R(par.mass);
+                V3<R> imparted = (velocities[p] + velocity_gradients[p] * to_node) * m;
+                const M3<R> &F = position_gradients[p];
+                M3<R> stress = first_piola<R>(flags[p], par, plastic_jp[p], hardening, F);
+                if (flags[p] & MPM_USE_VISCOSITY) {
+                    M3<R> cauchy = cauchy_viscosity<R>(par.viscosity_dynamic, par.viscosity_bulk, velocity_gradients[p]);
+                    imparted -= cauchy * (to_node * (scaling * F.determinant() * R(par.initial_volume)));
+                }
+                imparted -= stress * (F.transpose() * (to_node * (scaling * R(par.initial_volume))));
+                imparted *= weight;
+                mass += weight * m;
+                velocity += imparted;
+            }
+            masses[node] = mass;
+            node_velocities[node] = velocity;
+        }
+    }
+
+    // ---- MeldGrid, rust/crates/cpu/src/phase/meld_grid.rs:16-69.  Every key
+    // carries collider bits 0 here, so a node's only twin is itself.  The grid
+    // extensions (grid gravity and boundary conditions) follow the division.
+    void meld_grid() {
+        const size_t nn = keys.size();
+        const R hh = h(), t = dt();
+        const bool grid_gravity = consts.options & MPM_OPT_GRID_GRAVITY;
+#pragma omp parallel for schedule(static)
+        for (int64_t node = 0; node < (int64_t)nn; ++node) {
+            R mass = masses[node];
+            V3<R> v = node_velocities[node];
+            if (mass > 0) v = v / mass;
+            else v = V3<R>();
+            if (mass > 0) apply_grid_extensions(keys[node].id, v, hh, t, grid_gravity);
+            node_velocities[node] = v;
+        }
+    }
+
+    // Extension (no reference counterpart): grid gravity, sticky plane, box walls.
+    void apply_grid_extensions(const int id[3], V3<R> &v, R hh, R t, bool grid_gravity) const {
+        if (grid_gravity) v += gravity * t;
+        V3<R> xi(R(id[0]) * hh, R(id[1]) * hh, R(id[2]) * hh);
+        if ((consts.options & MPM_OPT_STICKY_PLANE) && xi.z < R(consts.plane_z)) v = V3<R>();
+        if (consts.options & MPM_OPT_BOX_WALLS) {
+            for (int a = 0; a < 3; ++a) {
+                if (xi[a] < R(consts.wall_min[a]) && v[a] < 0) v[a] = 0;
+                if (xi[a] > R(consts.wall_max[a]) && v[a] > 0) v[a] = 0;
+            }
+        }
+    }
+
+    // ---- CollectVelocity, rust/crates/cpu/src/phase/collect_velocity.rs:19-75.
+    void collect_velocity() {
+        const size_t n = positions.size();
+        const R hh = h();
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < (int64_t)n; ++i) {
+            if (flags[i] & MPM_TOMBSTONED) continue;
+            V3<R> velocity;
+            M3<R> C;
+            const V3<R> &position = positions[i];
+            V3<R> normalized = position / hh;
+            V3<R> shift(std::floor(normalized.x - R(0.5)), std::floor(normalized.y - R(0.5)),
+                        std::floor(normalized.z - R(0.5)));
+            V3<R> shifted = normalized - shift;
+            R w[3][3];
+            for (int a = 0; a < 3; ++a)
+                for (int k = 0; k < 3; ++k) w[a][k] = kernel_quadratic(shifted[a] - R(k));
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b)
+                    for (int c = 0; c < 3; ++c) {
+                        R weight = w[0][a] * w[1][b] * w[2][c];
+                        int id[3] = {(int)shift.x + a, (int)shift.y + b, (int)shift.z + c};
+                        V3<R> node_pos(R(id[0]) * hh, R(id[1]) * hh, R(id[2]) * hh);
+                        V3<R> to_node = node_pos - position;
+                        int32_t node = lookup_node(id);
+                        if (node < 0) throw std::runtime_error("missing node");
+                        V3<R> gv = node_velocities[node];
+                        velocity += gv * weight;
+                        C += outer(gv * weight, to_node);
+                    }
+            C *= R(4) / hh / hh;
+            velocities[i] = velocity;
+            velocity_gradients[i] = C;
+        }
+    }
+
+    // ---- AdvanceParticles, rust/crates/cpu/src/phase/advance_particles.rs:17-93.
+    // Returns false (EnergyError) when a Neo-Hookean particle ends with det F <= 0.
+    bool advance_particles() {
+        const size_t n = positions.size();
+        const R t = dt();
+        bool ok = true;
+        const R snow_lo = R(1) - R(consts.snow_compression), snow_hi = R(1) + R(consts.snow_stretch);
+        const R hardening = R(consts.snow_hardening);
+#pragma omp parallel for schedule(static) reduction(&& : ok)
+        for (int64_t i = 0; i < (int64_t)n; ++i) {
+            if (flags[i] & MPM_TOMBSTONED) continue;
+            positions[i] += velocities[i] * t;
+            M3<R> &F = position_gradients[i];
+            F += velocity_gradients[i] * F * t;
+            const mpm_particle_parameters &p = parameters[i];
+            if (flags[i] & MPM_IS_FLUID) {
+                // advance_particles.rs:76-86: singular values <- (s1 s2 s3)^(1/3).
+                M3<R> U, V;
+                V3<R> s;
+                svd3(F, U, s, V);
+                R g = std::cbrt(s[0] * s[1] * s[2]);
+                F = U * M3<R>::diag(g, g, g) * V.transpose();
+                elastic_energies[i] = energy_inviscid<R>(p.bulk_modulus, p.exponent, F);
+                continue;
+            }
+            if (flags[i] & MPM_SNOW) {
+                // Extension: snow return map (Stomakhin et al. 2013).
+                M3<R> U, V;
+                V3<R> s;
+                svd3(F, U, s, V);
+                R jp = plastic_jp[i];
+                for (int a = 0; a < 3; ++a) {
+                    R c = std::min(std::max(s[a], snow_lo), snow_hi);
+                    jp *= s[a] / c;
+                    s[a] = c;
+                }
+                plastic_jp[i] = jp;
+                F = U * M3<R>::diag(s[0], s[1], s[2]) * V.transpose();
+                R hfac = std::exp(hardening * (R(1) - jp));
+                R e;
+                kirchhoff_fixed_corotated<R>(R(p.mu) * hfac, R(p.lambda) * hfac, F, e);
+                elastic_energies[i] = e;
+                continue;
+            }
+            if ((flags[i] & MPM_USE_SAND_ALPHA)) {
+                // advance_particles.rs:50-71: Drucker-Prager return map in log strain.
+                M3<R> U, V;
+                V3<R> s;
+                svd3(F, U, s, V);
+                R mu = p.mu, lambda = p.lambda, alpha = p.sand_alpha;
+                V3<R> e(std::log(s[0]), std::log(s[1]), std::log(s[2]));
+                R e_tr = e.x + e.y + e.z;
+                V3<R> e_hat = e - V3<R>(e_tr / 3, e_tr / 3, e_tr / 3);
+                R e_hat_norm = std::sqrt(dot(e_hat, e_hat));
+                if (e_tr < 0 && e_hat_norm > 0) {
+                    R delta_gamma = e_hat_norm + (R(3) * lambda + R(2) * mu) / R(2) / mu * e_tr * alpha;
+                    if (delta_gamma > 0) {
+                        V3<R> H = e - e_hat * (delta_gamma / e_hat_norm);
+                        F = U * M3<R>::diag(std::exp(H.x), std::exp(H.y), std::exp(H.z)) * V.transpose();
+                    }
+                } else {
+                    F = U * V.transpose();
+                }
+            }
+            if (flags[i] & MPM_FIXED_COROTATED) {
+                R e;
+                kirchhoff_fixed_corotated<R>(R(p.mu), R(p.lambda), F, e);
+                elastic_energies[i] = e;
+                continue;
+            }
+            R e;
+            if (energy_neo_hookean<R>(p.mu, p.lambda, F, e)) {
+                elastic_energies[i] = e;
+            } else {
+                flags[i] |= MPM_FAILED;
+                ok = false;
+            }
+        }
+        return ok;
+    }
+
+    // ---- CullParticles, rust/crates/cpu/src/phase/cull_particles.rs:17-41
+    // (strict inequalities: a particle on the boundary is culled).
+    void cull_particles() {
+        const size_t n = positions.size();
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < (int64_t)n; ++i) {
+            if (flags[i] & MPM_TOMBSTONED) continue;
+            bool within = true;
+            for (int a = 0; a < 3; ++a) {
+                R p = positions[i][a];
+                within = within && (p > R(consts.domain_min[a]) && p < R(consts.domain_max[a]));
+            }
+            if (!within) flags[i] |= MPM_TOMBSTONED;
+        }
+    }
+
+    // Phases up to and including ScatterMomentum (the P2G half of a substep).
+    int run_to_scatter() {
+        int r = interpolate_input();
+        if (r) return r;
+        sort();
+        collide();
+        external_force();
+        update_grid_nodes();
+        scatter_momentum();
+        return MPM_OK;
+    }
+
+    // The rest of a substep.  Returns MPM_ERR_ENERGY when advance_particles failed.
+    int run_from_meld() {
+        meld_grid();
+        collect_velocity();
+        if (!advance_particles()) return MPM_ERR_ENERGY;
+        cull_particles();
+        time += (double)consts.time_step;  // cpu_state.rs:187-190, f32 dt added in f64
+        return MPM_OK;
+    }
+
+    // produce_next_state, rust/crates/cpu/src/cpu_state.rs:146-197 with fixed dt.
+    int step_to(double target_time, const volatile int32_t *cancel, uint64_t &substeps) {
+        substeps = 0;
+        if (consts.time_step == 0.f) return MPM_ERR_ZERO_TIME_STEP;
+        while (time < target_time) {
+            if (cancel && *cancel) return MPM_ERR_CANCELED;
+            int r = run_to_scatter();
+            if (r) return r;
+            r = run_from_meld();
+            if (r) return r;
+            ++substeps;
+        }
+        return MPM_OK;
+    }
+};
+
+}  // namespace mpm_oracle

@@ -1,0 +1,739 @@
+// engine.hip - host side of the MLS-MPM engine and its C ABI.
+//
+// mpm_create / mpm_step_to / mpm_read_particles stand where GpuState::
+// from_io_state, GpuState::produce_next_state and the state download of
+// rust/crates/gpu/src/gpu_state.rs:38-675 stand.  The substep loop, the time
+// accumulation and the frame-factor check follow CpuState::produce_next_state
+// (rust/crates/cpu/src/cpu_state.rs:146-197) and FrameInput::frame_factor
+// (rust/crates/xpu/src/frame_input.rs:266-278).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "device_math.hpp"
+#include "internal.hpp"
+
+using namespace svamd;
+
+namespace {
+
+enum Stage { ST_KEYS = 0, ST_SORT, ST_BLOCKS, ST_P2G, ST_GRID, ST_G2P, ST_COPY, ST_COUNT };
+const char *kStageNames[ST_COUNT] = {"keys", "radix_sort", "block_list", "p2g", "grid_update", "g2p", "copy_dead"};
+
+thread_local std::string g_last_error;
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t _e = (expr);                                                                    \
+        if (_e != hipSuccess) {                                                                    \
+            g_last_error = std::string(#expr) + ": " + hipGetErrorString(_e);                      \
+            return MPM_ERR_HIP;                                                                    \
+        }                                                                                          \
+    } while (0)
+
+struct EventRecord {
+    int stage;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct mpm_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    mpm_consts consts{};
+    uint64_t n = 0;
+    GridGeom geom{};
+    int key_bits = 0;
+    TransferLaunch launch{};
+
+    ParticleArrays P[2]{};
+    int cur = 0;
+    AuxArrays X{};
+
+    uint32_t *keys = nullptr, *keys_sorted = nullptr, *order = nullptr;
+    void *sort_temp = nullptr;
+    uint32_t *blk_start = nullptr, *blk_key = nullptr;
+    uint32_t blk_capacity = 0;
+    void *blk_temp = nullptr;
+    float4 *grid = nullptr, *vgrid = nullptr;
+    uint32_t *blk_flag = nullptr, *blk_epoch = nullptr;
+    DeviceScalars *ds = nullptr;
+    DeviceScalars *ds_host = nullptr;  // pinned
+
+    double time = 0;
+    uint32_t epoch = 0;
+    uint32_t grid_epoch = 0;   // epoch stamped by the last grid update
+    bool grid_valid = false;   // a grid update has run
+    bool next_from_c = true;
+    bool stepped = false;
+    bool goals_allocated = false;
+
+    std::vector<mpm_particle_parameters> parameters;
+    std::vector<float> initial_positions;
+    std::vector<uint32_t> collider_bits;
+    std::vector<uint8_t> dead_at_create;
+
+    bool profiling = false;
+    std::vector<EventRecord> records;
+    std::vector<hipEvent_t> event_pool;
+    double stage_ms[ST_COUNT] = {0};
+    uint64_t stage_launches[ST_COUNT] = {0};
+
+    std::vector<void *> allocations;
+
+    ~mpm_engine() {
+        if (device >= 0) (void)hipSetDevice(device);
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void *p : allocations) (void)hipFree(p);
+        if (ds_host) (void)hipHostFree(ds_host);
+        for (auto &r : records) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+        for (auto e : event_pool) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    template <class T>
+    hipError_t alloc(T *&ptr, size_t count) {
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, std::max<size_t>(count * sizeof(T), 256));
+        if (e == hipSuccess) allocations.push_back(p);
+        ptr = static_cast<T *>(p);
+        return e;
+    }
+
+    hipEvent_t take_event() {
+        if (!event_pool.empty()) {
+            hipEvent_t e = event_pool.back();
+            event_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+
+    void begin(int stage, EventRecord &r) {
+        if (!profiling) return;
+        r.stage = stage;
+        r.a = take_event();
+        r.b = take_event();
+        (void)hipEventRecord(r.a, stream);
+    }
+    void end(EventRecord &r) {
+        if (!profiling) return;
+        (void)hipEventRecord(r.b, stream);
+        records.push_back(r);
+    }
+    void harvest() {
+        for (auto &r : records) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+                stage_ms[r.stage] += ms;
+                stage_launches[r.stage] += 1;
+            }
+            event_pool.push_back(r.a);
+            event_pool.push_back(r.b);
+        }
+        records.clear();
+    }
+};
+
+namespace {
+
+bool finite3(const float *v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
+
+int compute_geometry(const mpm_consts &c, GridGeom &g, int &key_bits) {
+    const double h = c.grid_node_size;
+    int nbmax = 0;
+    for (int a = 0; a < 3; ++a) {
+        double lo = std::floor((double)c.domain_min[a] / h - 0.5);
+        double hi = std::floor((double)c.domain_max[a] / h - 0.5);
+        if (!(hi - lo < 4.0 * 1023)) return MPM_ERR_GRID_TOO_LARGE;
+        long long origin = 4 * (long long)std::floor((lo - 2.0) / 4.0);
+        long long cmax = (long long)hi + 2 - origin;
+        long long nb = (cmax >> 2) + 2;
+        if (origin < -(1ll << 30) || origin > (1ll << 30)) return MPM_ERR_GRID_TOO_LARGE;
+        g.origin[a] = (int)origin;
+        g.nb[a] = (int)nb;
+        nbmax = std::max<int>(nbmax, (int)nb);
+    }
+    uint32_t k = 1;
+    while ((1u << k) < (uint32_t)nbmax) ++k;
+    if (k > 10) return MPM_ERR_GRID_TOO_LARGE;
+    g.axis_bits = k;
+    key_bits = (int)(3 * k);
+    g.tomb_key = (key_bits == 32) ? 0xffffffffu : ((1u << key_bits) - 1u);
+    g.h = c.grid_node_size;
+    uint64_t total = (uint64_t)g.nb[0] * g.nb[1] * g.nb[2];
+    // Two float4 node grids: accumulation and velocity.
+    if (total * 64ull * 16ull * 2ull > (96ull << 30)) return MPM_ERR_GRID_TOO_LARGE;
+    g.num_blocks_total = (uint32_t)total;
+    return MPM_OK;
+}
+
+hipError_t alloc_particles(mpm_engine *e, ParticleArrays &P) {
+    const size_t n = e->n;
+    hipError_t r;
+    for (int a = 0; a < 3; ++a) {
+        if ((r = e->alloc(P.x[a], n)) != hipSuccess) return r;
+        if ((r = e->alloc(P.v[a], n)) != hipSuccess) return r;
+    }
+    for (int k = 0; k < 9; ++k) {
+        if ((r = e->alloc(P.F[k], n)) != hipSuccess) return r;
+        if ((r = e->alloc(P.A[k], n)) != hipSuccess) return r;
+    }
+    if ((r = e->alloc(P.m, n)) != hipSuccess) return r;
+    if ((r = e->alloc(P.vol, n)) != hipSuccess) return r;
+    if ((r = e->alloc(P.p0, n)) != hipSuccess) return r;
+    if ((r = e->alloc(P.p1, n)) != hipSuccess) return r;
+    if ((r = e->alloc(P.flags, n)) != hipSuccess) return r;
+    if ((r = e->alloc(P.id, n)) != hipSuccess) return r;
+    return hipSuccess;
+}
+
+SubstepParams make_params(const mpm_engine *e, const float gravity[3], float factor_b, bool goals, bool has_b) {
+    SubstepParams S{};
+    const mpm_consts &c = e->consts;
+    S.dt = c.time_step;
+    S.h = c.grid_node_size;
+    for (int a = 0; a < 3; ++a) {
+        S.gravity[a] = gravity[a];
+        S.wall_min[a] = c.wall_min[a];
+        S.wall_max[a] = c.wall_max[a];
+        S.domain_min[a] = c.domain_min[a];
+        S.domain_max[a] = c.domain_max[a];
+    }
+    S.factor_b = factor_b;
+    S.has_goals = goals ? 1 : 0;
+    S.has_b = has_b ? 1 : 0;
+    S.options = c.options;
+    S.plane_z = c.plane_z;
+    S.snow_lo = 1.0f - c.snow_compression;
+    S.snow_hi = 1.0f + c.snow_stretch;
+    S.snow_hardening = c.snow_hardening;
+    S.epoch = e->epoch;
+    return S;
+}
+
+// FrameInput::frame_factor and the gravity interpolation of
+// rust/crates/cpu/src/phase/interpolate_input.rs:24-28, evaluated in the same
+// f64 -> f32 order as the reference.
+int frame_params(const mpm_engine *e, const mpm_frame_input *in, double time, float gravity[3], float &factor_b) {
+    double frame_time = time * (double)e->consts.frames_per_second;
+    uint64_t frame_low = (uint64_t)std::floor(frame_time);
+    uint64_t frame = in ? in->frame : 0;
+    if (frame_low != frame) return MPM_ERR_WRONG_FRAME;
+    factor_b = (float)std::fmod(frame_time, 1.0);
+    float factor_a = 1.0f - factor_b;
+    const float *ga = in ? in->gravity_a : nullptr;
+    const float *gb = in ? (in->has_b ? in->gravity_b : in->gravity_a) : nullptr;
+    for (int a = 0; a < 3; ++a) gravity[a] = ga ? factor_a * ga[a] + factor_b * gb[a] : 0.0f;
+    return MPM_OK;
+}
+
+int upload_goals(mpm_engine *e, const mpm_frame_input *in, bool &goals) {
+    goals = in && in->particle_flags_a && in->goal_positions_a && (!in->has_b || (in->particle_flags_b && in->goal_positions_b));
+    if (!goals) return MPM_OK;
+    const size_t n = e->n;
+    if (!e->goals_allocated) {
+        for (int a = 0; a < 3; ++a) {
+            HIP_TRY(e->alloc(e->X.goal_a[a], n));
+            HIP_TRY(e->alloc(e->X.goal_b[a], n));
+        }
+        HIP_TRY(e->alloc(e->X.goal_flags_a, n));
+        HIP_TRY(e->alloc(e->X.goal_flags_b, n));
+        e->goals_allocated = true;
+    }
+    std::vector<float> tmp(n);
+    for (int which = 0; which < 2; ++which) {
+        const float *src = which == 0 ? in->goal_positions_a : (in->has_b ? in->goal_positions_b : in->goal_positions_a);
+        for (int a = 0; a < 3; ++a) {
+            for (size_t i = 0; i < n; ++i) tmp[i] = src[3 * i + a];
+            HIP_TRY(hipMemcpy(which == 0 ? e->X.goal_a[a] : e->X.goal_b[a], tmp.data(), n * 4, hipMemcpyHostToDevice));
+        }
+    }
+    HIP_TRY(hipMemcpy(e->X.goal_flags_a, in->particle_flags_a, n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->X.goal_flags_b, in->has_b ? in->particle_flags_b : in->particle_flags_a, n * 4,
+                      hipMemcpyHostToDevice));
+    return MPM_OK;
+}
+
+// Sort, block list: everything before P2G.
+int run_sort(mpm_engine *e) {
+    EventRecord r{};
+    ParticleArrays &P = e->P[e->cur];
+    HIP_TRY(launch_reset_scalars(e->ds, e->stream));
+    e->begin(ST_KEYS, r);
+    HIP_TRY(launch_keys(P, e->geom, e->n, e->keys, e->ds, e->stream));
+    e->end(r);
+    e->begin(ST_SORT, r);
+    HIP_TRY(radix_sort(e->keys, nullptr, e->keys_sorted, e->order, nullptr, nullptr, e->n, e->key_bits, e->sort_temp,
+                       e->stream));
+    e->end(r);
+    e->begin(ST_BLOCKS, r);
+    HIP_TRY(launch_block_list(e->keys_sorted, e->n, e->geom.tomb_key, e->blk_start, e->blk_key, e->blk_capacity,
+                              e->ds, e->blk_temp, e->stream));
+    e->end(r);
+    return MPM_OK;
+}
+
+int run_p2g(mpm_engine *e, const SubstepParams &S) {
+    EventRecord r{};
+    e->begin(ST_P2G, r);
+    HIP_TRY(launch_p2g(e->launch, e->next_from_c, e->P[e->cur], e->X, e->order, e->blk_start, e->blk_key, e->geom, S,
+                       e->grid, e->blk_flag, e->ds, e->stream));
+    e->end(r);
+    e->next_from_c = false;
+    return MPM_OK;
+}
+
+int run_grid(mpm_engine *e, const SubstepParams &S) {
+    EventRecord r{};
+    e->begin(ST_GRID, r);
+    HIP_TRY(launch_grid_update(e->launch, e->blk_key, e->geom, S, e->grid, e->vgrid, e->blk_flag, e->blk_epoch, e->ds,
+                               e->stream));
+    e->end(r);
+    e->grid_epoch = S.epoch;
+    e->grid_valid = true;
+    return MPM_OK;
+}
+
+int run_g2p(mpm_engine *e, const SubstepParams &S, bool last) {
+    EventRecord r{};
+    ParticleArrays &Pin = e->P[e->cur];
+    ParticleArrays &Pout = e->P[1 - e->cur];
+    e->begin(ST_G2P, r);
+    HIP_TRY(launch_g2p(e->launch, last, Pin, Pout, e->X, e->order, e->blk_start, e->blk_key, e->geom, S, e->vgrid,
+                       e->ds, e->stream));
+    e->end(r);
+    e->begin(ST_COPY, r);
+    HIP_TRY(launch_copy_dead(Pin, Pout, e->order, e->n, e->ds, e->stream));
+    e->end(r);
+    e->cur = 1 - e->cur;
+    if (last) e->next_from_c = true;
+    e->epoch += 1;
+    return MPM_OK;
+}
+
+int fetch_scalars(mpm_engine *e) {
+    HIP_TRY(hipMemcpyAsync(e->ds_host, e->ds, sizeof(DeviceScalars), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return MPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mpm_error_string(int result) {
+    switch (result) {
+        case MPM_OK: return "ok";
+        case MPM_ERR_ENERGY: return "a particle's deformation gradient lost its positive determinant (energy error)";
+        case MPM_ERR_CLOSE_TO_INVERTED: return "a particle is too close to inversion";
+        case MPM_ERR_INVALID_ARGUMENT: return "invalid argument";
+        case MPM_ERR_HIP: return g_last_error.empty() ? "HIP error" : g_last_error.c_str();
+        case MPM_ERR_NO_DEVICE: return "no HIP device";
+        case MPM_ERR_OUT_OF_MEMORY: return "out of device memory";
+        case MPM_ERR_GRID_TOO_LARGE: return "the domain's dense block grid is too large";
+        case MPM_ERR_CANCELED: return "the computation was canceled";
+        case MPM_ERR_WRONG_FRAME: return "the frame input does not contain the current time";
+        case MPM_ERR_ZERO_TIME_STEP: return "the time step is zero";
+        case MPM_ERR_OUT_OF_GRID: return "a particle left the allocated grid";
+        case MPM_ERR_CAPACITY: return "capacity exceeded";
+        default: return "unknown error";
+    }
+}
+
+int mpm_device_count(void) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+int mpm_create(const mpm_consts *consts, const mpm_particles_view *pv, const mpm_tuning *tuning,
+               mpm_engine **out_engine) {
+    if (!consts || !pv || !out_engine) return MPM_ERR_INVALID_ARGUMENT;
+    *out_engine = nullptr;
+    if (!(consts->grid_node_size > 0.f) || !(consts->time_step >= 0.f) || consts->frames_per_second == 0)
+        return MPM_ERR_INVALID_ARGUMENT;
+    if (!finite3(consts->domain_min) || !finite3(consts->domain_max)) return MPM_ERR_INVALID_ARGUMENT;
+    for (int a = 0; a < 3; ++a)
+        if (!(consts->domain_min[a] < consts->domain_max[a])) return MPM_ERR_INVALID_ARGUMENT;
+    if (pv->count > 0 && (!pv->flags || !pv->parameters || !pv->positions || !pv->position_gradients ||
+                          !pv->velocities || !pv->velocity_gradients))
+        return MPM_ERR_INVALID_ARGUMENT;
+    if (pv->count >= (1ull << 31)) return MPM_ERR_CAPACITY;
+    int ndev = mpm_device_count();
+    if (ndev <= 0) return MPM_ERR_NO_DEVICE;
+    if (consts->device < 0 || consts->device >= ndev) return MPM_ERR_INVALID_ARGUMENT;
+
+    mpm_engine *e = new (std::nothrow) mpm_engine();
+    if (!e) return MPM_ERR_OUT_OF_MEMORY;
+    e->device = consts->device;
+    e->consts = *consts;
+    e->n = pv->count;
+    e->time = pv->time;
+    int rc = compute_geometry(*consts, e->geom, e->key_bits);
+    if (rc != MPM_OK) {
+        delete e;
+        return rc;
+    }
+    auto fail = [&](int code) {
+        delete e;
+        return code;
+    };
+    if (hipSetDevice(e->device) != hipSuccess) return fail(MPM_ERR_HIP);
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) return fail(MPM_ERR_HIP);
+
+    hipDeviceProp_t prop{};
+    if (hipGetDeviceProperties(&prop, e->device) != hipSuccess) return fail(MPM_ERR_HIP);
+    e->launch.persistent_blocks = prop.multiProcessorCount * 8;
+    e->launch.p2g_waves = 1;
+    e->launch.g2p_threads = 256;
+    if (tuning) {
+        if (tuning->persistent_blocks > 0) e->launch.persistent_blocks = tuning->persistent_blocks;
+        if (tuning->p2g_waves_per_block == 1 || tuning->p2g_waves_per_block == 2 || tuning->p2g_waves_per_block == 4)
+            e->launch.p2g_waves = tuning->p2g_waves_per_block;
+        if (tuning->g2p_threads >= 64 && tuning->g2p_threads <= 1024 && tuning->g2p_threads % 64 == 0)
+            e->launch.g2p_threads = tuning->g2p_threads;
+    }
+    // The G2P kernel is compiled for at most 256 threads per work-group.
+    e->launch.g2p_threads = std::min(e->launch.g2p_threads, 256);
+
+    const size_t n = e->n;
+    const uint64_t max_blocks = std::max<uint64_t>(std::min<uint64_t>(n, e->geom.num_blocks_total), 1);
+    e->blk_capacity = (uint32_t)max_blocks;
+    const uint64_t nodes = (uint64_t)e->geom.num_blocks_total * 64ull;
+    if (alloc_particles(e, e->P[0]) != hipSuccess || alloc_particles(e, e->P[1]) != hipSuccess)
+        return fail(MPM_ERR_OUT_OF_MEMORY);
+    if (e->alloc(e->X.visc_dyn, n) || e->alloc(e->X.visc_bulk, n) || e->alloc(e->X.sand_alpha, n) ||
+        e->alloc(e->X.jp, n) || e->alloc(e->X.energy, n))
+        return fail(MPM_ERR_OUT_OF_MEMORY);
+    char *st = nullptr, *bt = nullptr;
+    if (e->alloc(e->keys, n) || e->alloc(e->keys_sorted, n) || e->alloc(e->order, n) ||
+        e->alloc(st, radix_sort_temp_bytes(n)) || e->alloc(e->blk_start, max_blocks) ||
+        e->alloc(e->blk_key, max_blocks) || e->alloc(bt, block_list_temp_bytes(n)) || e->alloc(e->grid, nodes) ||
+        e->alloc(e->vgrid, nodes) || e->alloc(e->blk_flag, e->geom.num_blocks_total) ||
+        e->alloc(e->blk_epoch, e->geom.num_blocks_total) || e->alloc(e->ds, 1))
+        return fail(MPM_ERR_OUT_OF_MEMORY);
+    e->sort_temp = st;
+    e->blk_temp = bt;
+    if (hipHostMalloc((void **)&e->ds_host, sizeof(DeviceScalars), hipHostMallocDefault) != hipSuccess)
+        return fail(MPM_ERR_OUT_OF_MEMORY);
+    if (hipMemsetAsync(e->grid, 0, nodes * 16, e->stream) != hipSuccess ||
+        hipMemsetAsync(e->vgrid, 0, nodes * 16, e->stream) != hipSuccess ||
+        hipMemsetAsync(e->blk_flag, 0, (size_t)e->geom.num_blocks_total * 4, e->stream) != hipSuccess ||
+        hipMemsetAsync(e->blk_epoch, 0xff, (size_t)e->geom.num_blocks_total * 4, e->stream) != hipSuccess ||
+        hipMemsetAsync(e->ds, 0, sizeof(DeviceScalars), e->stream) != hipSuccess)
+        return fail(MPM_ERR_HIP);
+
+    // Host AoS -> device SoA.
+    e->parameters.assign(pv->parameters, pv->parameters + n);
+    e->initial_positions.assign(3 * n, 0.f);
+    if (pv->initial_positions) std::copy(pv->initial_positions, pv->initial_positions + 3 * n, e->initial_positions.begin());
+    e->collider_bits.assign(n, 0u);
+    if (pv->collider_bits) std::copy(pv->collider_bits, pv->collider_bits + n, e->collider_bits.begin());
+    e->dead_at_create.assign(n, 0);
+    std::vector<float> col(n);
+    std::vector<uint32_t> ucol(n);
+    auto up = [&](void *dst, const void *src, size_t bytes) {
+        return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+    };
+    ParticleArrays &P = e->P[0];
+    bool ok = true;
+    for (int a = 0; a < 3 && ok; ++a) {
+        for (size_t i = 0; i < n; ++i) col[i] = pv->positions[3 * i + a];
+        ok = ok && up(P.x[a], col.data(), n * 4);
+        for (size_t i = 0; i < n; ++i) col[i] = pv->velocities[3 * i + a];
+        ok = ok && up(P.v[a], col.data(), n * 4);
+    }
+    for (int k = 0; k < 9 && ok; ++k) {
+        for (size_t i = 0; i < n; ++i) col[i] = pv->position_gradients[9 * i + k];
+        ok = ok && up(P.F[k], col.data(), n * 4);
+        for (size_t i = 0; i < n; ++i) col[i] = pv->velocity_gradients[9 * i + k];
+        ok = ok && up(P.A[k], col.data(), n * 4);
+    }
+    const mpm_particle_parameters *pp = pv->parameters;
+    auto upf = [&](float *dst, auto getter) {
+        for (size_t i = 0; i < n; ++i) col[i] = getter(i);
+        return up(dst, col.data(), n * 4);
+    };
+    ok = ok && upf(P.m, [&](size_t i) { return pp[i].mass; });
+    ok = ok && upf(P.vol, [&](size_t i) { return pp[i].initial_volume; });
+    ok = ok && upf(P.p0, [&](size_t i) { return (pv->flags[i] & MPM_IS_FLUID) ? pp[i].bulk_modulus : pp[i].mu; });
+    ok = ok && upf(P.p1, [&](size_t i) {
+        return (pv->flags[i] & MPM_IS_FLUID) ? (float)pp[i].exponent : pp[i].lambda;
+    });
+    ok = ok && upf(e->X.visc_dyn, [&](size_t i) { return pp[i].viscosity_dynamic; });
+    ok = ok && upf(e->X.visc_bulk, [&](size_t i) { return pp[i].viscosity_bulk; });
+    ok = ok && upf(e->X.sand_alpha, [&](size_t i) { return pp[i].sand_alpha; });
+    ok = ok && upf(e->X.jp, [&](size_t i) { return pv->plastic_jp ? pv->plastic_jp[i] : 1.0f; });
+    ok = ok && upf(e->X.energy, [&](size_t i) { return pv->elastic_energies ? pv->elastic_energies[i] : 0.0f; });
+    for (size_t i = 0; i < n; ++i) {
+        ucol[i] = pv->flags[i];
+        e->dead_at_create[i] = (pv->flags[i] & MPM_TOMBSTONED) ? 1 : 0;
+    }
+    ok = ok && up(P.flags, ucol.data(), n * 4);
+    for (size_t i = 0; i < n; ++i) ucol[i] = (uint32_t)i;
+    ok = ok && up(P.id, ucol.data(), n * 4);
+    if (!ok) return fail(MPM_ERR_HIP);
+    if (hipStreamSynchronize(e->stream) != hipSuccess) return fail(MPM_ERR_HIP);
+    *out_engine = e;
+    return MPM_OK;
+}
+
+void mpm_destroy(mpm_engine *engine) { delete engine; }
+
+double mpm_time(const mpm_engine *engine) { return engine ? engine->time : 0.0; }
+
+void *mpm_stream(mpm_engine *engine) { return engine ? (void *)engine->stream : nullptr; }
+
+int mpm_step_to(mpm_engine *e, double target_time, const mpm_frame_input *input, const volatile int32_t *cancel,
+                mpm_status *status) {
+    if (!e) return MPM_ERR_INVALID_ARGUMENT;
+    if (status) std::memset(status, 0, sizeof(*status));
+    HIP_TRY(hipSetDevice(e->device));
+    if (e->consts.time_step == 0.0f) return MPM_ERR_ZERO_TIME_STEP;
+
+    // The substep times of the call, accumulated as the reference does
+    // (f32 time step added to an f64 time, cpu_state.rs:187-190).
+    std::vector<double> times;
+    for (double t = e->time; t < target_time; t += (double)e->consts.time_step) times.push_back(t);
+    std::vector<SubstepParams> params;
+    params.reserve(times.size());
+    bool goals = false;
+    int rc = upload_goals(e, input, goals);
+    if (rc != MPM_OK) return rc;
+    for (double t : times) {
+        float g[3], fb;
+        rc = frame_params(e, input, t, g, fb);
+        if (rc != MPM_OK) return rc;
+        params.push_back(make_params(e, g, fb, goals, input && input->has_b));
+    }
+    HIP_TRY(hipMemsetAsync(&e->ds->status, 0, 4, e->stream));
+    HIP_TRY(hipMemsetAsync(&e->ds->svd_fp64, 0, 4, e->stream));
+
+    uint64_t done = 0;
+    int result = MPM_OK;
+    for (size_t s = 0; s < params.size(); ++s) {
+        if (cancel && *cancel) {
+            result = MPM_ERR_CANCELED;
+            break;
+        }
+        SubstepParams S = params[s];
+        S.epoch = e->epoch;
+        const bool last = s + 1 == params.size();
+        if ((rc = run_sort(e)) != MPM_OK) return rc;
+        if ((rc = run_p2g(e, S)) != MPM_OK) return rc;
+        if ((rc = run_grid(e, S)) != MPM_OK) return rc;
+        if ((rc = run_g2p(e, S, last)) != MPM_OK) return rc;
+        e->time = times[s] + (double)e->consts.time_step;
+        ++done;
+    }
+    if (done > 0) e->stepped = true;
+    if ((rc = fetch_scalars(e)) != MPM_OK) return rc;
+    if (e->profiling) e->harvest();
+    const uint32_t st = e->ds_host->status & 0xffffu;
+    if (result == MPM_OK) {
+        if (st & MPM_STATUS_INDIRECT_LIMIT_EXCEEDED) result = MPM_ERR_CAPACITY;
+        else if (st & MPM_STATUS_ENERGY_ERROR) result = MPM_ERR_ENERGY;
+        else if (st & MPM_STATUS_OUT_OF_GRID) result = MPM_ERR_OUT_OF_GRID;
+    }
+    if (status) {
+        status->device_status = e->ds_host->status;
+        status->result = result;
+        status->substeps = done;
+        status->time = e->time;
+        status->active_blocks = e->ds_host->num_blocks;
+        status->active_nodes = e->ds_host->active_nodes;
+        status->svd_fp64_fallbacks = e->ds_host->svd_fp64;
+    }
+    return result;
+}
+
+int mpm_read_particles(mpm_engine *e, mpm_particles_view *out) {
+    if (!e || !out) return MPM_ERR_INVALID_ARGUMENT;
+    if (out->count != e->n) return MPM_ERR_INVALID_ARGUMENT;
+    HIP_TRY(hipSetDevice(e->device));
+    const size_t n = e->n;
+    out->time = e->time;
+    if (n == 0) return MPM_OK;
+    float *x, *v, *F, *C;
+    uint32_t *fl;
+    HIP_TRY(hipMalloc((void **)&x, n * 3 * 4));
+    HIP_TRY(hipMalloc((void **)&v, n * 3 * 4));
+    HIP_TRY(hipMalloc((void **)&F, n * 9 * 4));
+    HIP_TRY(hipMalloc((void **)&C, n * 9 * 4));
+    HIP_TRY(hipMalloc((void **)&fl, n * 4));
+    int rc = MPM_OK;
+    if (launch_unpermute(e->P[e->cur], n, x, v, F, C, fl, e->stream) != hipSuccess ||
+        hipStreamSynchronize(e->stream) != hipSuccess) {
+        rc = MPM_ERR_HIP;
+    } else {
+        auto down = [&](void *dst, const void *src, size_t bytes) {
+            if (dst && rc == MPM_OK && hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) != hipSuccess) rc = MPM_ERR_HIP;
+        };
+        down(out->positions, x, n * 12);
+        down(out->velocities, v, n * 12);
+        down(out->position_gradients, F, n * 36);
+        down(out->velocity_gradients, C, n * 36);
+        down(out->flags, fl, n * 4);
+        down(out->elastic_energies, e->X.energy, n * 4);
+        down(out->plastic_jp, e->X.jp, n * 4);
+    }
+    (void)hipFree(x);
+    (void)hipFree(v);
+    (void)hipFree(F);
+    (void)hipFree(C);
+    (void)hipFree(fl);
+    if (rc != MPM_OK) return rc;
+    if (out->parameters) std::copy(e->parameters.begin(), e->parameters.end(), out->parameters);
+    if (out->initial_positions) std::copy(e->initial_positions.begin(), e->initial_positions.end(), out->initial_positions);
+    if (out->collider_bits) {
+        // Without mesh colliders every live particle's collider bits are set
+        // to 0 in each substep (rust/crates/cpu/src/phase/collide.rs:53-58).
+        for (size_t i = 0; i < n; ++i)
+            out->collider_bits[i] = (e->stepped && !e->dead_at_create[i]) ? 0u : e->collider_bits[i];
+    }
+    return MPM_OK;
+}
+
+static int read_grid_impl(mpm_engine *e, mpm_grid_view *out, bool momentum) {
+    if (!e || !out) return MPM_ERR_INVALID_ARGUMENT;
+    HIP_TRY(hipSetDevice(e->device));
+    const uint64_t nblk = e->geom.num_blocks_total;
+    std::vector<float4> g(nblk * 64);
+    std::vector<uint32_t> ep(nblk);
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    HIP_TRY(hipMemcpy(g.data(), momentum ? e->grid : e->vgrid, nblk * 64 * 16, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(ep.data(), e->blk_epoch, nblk * 4, hipMemcpyDeviceToHost));
+    uint64_t count = 0;
+    const int *nb = e->geom.nb;
+    for (uint64_t B = 0; B < nblk; ++B) {
+        if (!momentum && (!e->grid_valid || ep[B] != e->grid_epoch)) continue;
+        const int bx = (int)(B % nb[0]), by = (int)((B / nb[0]) % nb[1]), bz = (int)(B / ((uint64_t)nb[0] * nb[1]));
+        for (int s = 0; s < 64; ++s) {
+            const float4 q = g[B * 64 + s];
+            if (!(q.x != 0.f)) continue;
+            if (count < out->capacity) {
+                const int id[3] = {e->geom.origin[0] + 4 * bx + (s & 3), e->geom.origin[1] + 4 * by + ((s >> 2) & 3),
+                                   e->geom.origin[2] + 4 * bz + (s >> 4)};
+                if (out->node_ids)
+                    for (int a = 0; a < 3; ++a) out->node_ids[3 * count + a] = id[a];
+                if (out->collider_bits) out->collider_bits[count] = 0u;
+                if (out->masses) out->masses[count] = q.x;
+                if (out->velocities) {
+                    out->velocities[3 * count] = q.y;
+                    out->velocities[3 * count + 1] = q.z;
+                    out->velocities[3 * count + 2] = q.w;
+                }
+            }
+            ++count;
+        }
+    }
+    out->count = count;
+    return count > out->capacity ? MPM_ERR_CAPACITY : MPM_OK;
+}
+
+int mpm_read_grid(mpm_engine *e, mpm_grid_view *out) { return read_grid_impl(e, out, false); }
+int mpm_read_grid_momentum(mpm_engine *e, mpm_grid_view *out) { return read_grid_impl(e, out, true); }
+
+int mpm_run_stage(mpm_engine *e, int32_t stage, const mpm_frame_input *input) {
+    if (!e) return MPM_ERR_INVALID_ARGUMENT;
+    HIP_TRY(hipSetDevice(e->device));
+    float g[3], fb;
+    int rc = frame_params(e, input, e->time, g, fb);
+    if (rc != MPM_OK) return rc;
+    bool goals = false;
+    if ((rc = upload_goals(e, input, goals)) != MPM_OK) return rc;
+    SubstepParams S = make_params(e, g, fb, goals, input && input->has_b);
+    switch (stage) {
+        case 0: rc = run_sort(e); break;
+        case 1: rc = run_p2g(e, S); break;
+        case 2: rc = run_grid(e, S); break;
+        case 3:
+            rc = run_g2p(e, S, true);
+            if (rc == MPM_OK) {
+                e->time += (double)e->consts.time_step;
+                e->stepped = true;
+            }
+            break;
+        default: return MPM_ERR_INVALID_ARGUMENT;
+    }
+    if (rc != MPM_OK) return rc;
+    return fetch_scalars(e);
+}
+
+int mpm_set_profiling(mpm_engine *e, int32_t enabled) {
+    if (!e) return MPM_ERR_INVALID_ARGUMENT;
+    e->profiling = enabled != 0;
+    return MPM_OK;
+}
+
+int mpm_kernel_times(mpm_engine *e, int32_t reset, int32_t capacity, const char **names, double *total_ms,
+                     uint64_t *launches, int32_t *count) {
+    if (!e) return MPM_ERR_INVALID_ARGUMENT;
+    int c = std::min<int>(capacity, ST_COUNT);
+    for (int i = 0; i < c; ++i) {
+        if (names) names[i] = kStageNames[i];
+        if (total_ms) total_ms[i] = e->stage_ms[i];
+        if (launches) launches[i] = e->stage_launches[i];
+    }
+    if (count) *count = c;
+    if (reset) {
+        for (int i = 0; i < ST_COUNT; ++i) {
+            e->stage_ms[i] = 0;
+            e->stage_launches[i] = 0;
+        }
+    }
+    return MPM_OK;
+}
+
+// ---- per-kernel entry points
+size_t mpm_dev_radix_sort_temp_bytes(uint64_t n) { return radix_sort_temp_bytes(n); }
+
+int mpm_dev_radix_sort(const uint32_t *keys_in, const uint32_t *values_in, uint32_t *keys_out, uint32_t *values_out,
+                       uint64_t n, int32_t key_bits, void *temp, void *stream) {
+    if (n > 0 && (!keys_in || !keys_out || !values_out || !temp)) return MPM_ERR_INVALID_ARGUMENT;
+    if (key_bits < 0 || key_bits > 32 || n >= (1ull << 32)) return MPM_ERR_INVALID_ARGUMENT;
+    if (keys_in == keys_out || (values_in && values_in == values_out)) return MPM_ERR_INVALID_ARGUMENT;
+    HIP_TRY(radix_sort(keys_in, values_in, keys_out, values_out, nullptr, nullptr, n, key_bits, temp,
+                       static_cast<hipStream_t>(stream)));
+    return MPM_OK;
+}
+
+size_t mpm_dev_scan_temp_bytes(uint64_t n) { return scan_temp_bytes(n); }
+
+int mpm_dev_exclusive_scan(const uint32_t *in, uint32_t *out, uint32_t *total, uint64_t n, void *temp, void *stream) {
+    if (!temp || (n > 0 && (!in || !out))) return MPM_ERR_INVALID_ARGUMENT;
+    HIP_TRY(exclusive_scan(in, out, total, n, temp, static_cast<hipStream_t>(stream)));
+    return MPM_OK;
+}
+
+int mpm_dev_copy(const void *src, void *dst, uint64_t bytes, void *stream) {
+    if (bytes % 16 != 0) return MPM_ERR_INVALID_ARGUMENT;
+    HIP_TRY(launch_copy(src, dst, bytes, static_cast<hipStream_t>(stream)));
+    return MPM_OK;
+}
+
+int mpm_dev_stress(const uint32_t *flags, const mpm_particle_parameters *params, const float *F, const float *jp,
+                   float snow_hardening, float *tau, float *energy, uint64_t n, void *stream) {
+    if (n > 0 && (!flags || !params || !F || !tau)) return MPM_ERR_INVALID_ARGUMENT;
+    HIP_TRY(launch_stress(flags, params, F, jp, snow_hardening, tau, energy, n, static_cast<hipStream_t>(stream)));
+    return MPM_OK;
+}
+
+int mpm_dev_svd(const float *F, float *U, float *s, float *V, uint64_t n, void *stream) {
+    if (n > 0 && (!F || !U || !s || !V)) return MPM_ERR_INVALID_ARGUMENT;
+    HIP_TRY(launch_svd(F, U, s, V, n, static_cast<hipStream_t>(stream)));
+    return MPM_OK;
+}
+
+}  // extern "C"

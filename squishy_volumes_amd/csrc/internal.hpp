@@ -1,0 +1,135 @@
+// internal.hpp - device data layout and kernel launchers shared by the
+// engine's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/mpm_engine.h"
+
+namespace svamd {
+
+constexpr uint32_t TOMB_KEY_FLAG = 0xffffffffu;
+
+// One set of permuted particle arrays, structure of arrays.  F and A are nine
+// component arrays each, component k = 3*col + row.  A holds the fused affine
+// momentum m C - dt 4/h^2 V0 (tau + J sigma_visc) between substeps and the
+// APIC matrix C at the end of a step_to call.
+struct ParticleArrays {
+    float *x[3];
+    float *v[3];
+    float *F[9];
+    float *A[9];
+    float *m;
+    float *vol;
+    float *p0;  // mu (solids) or bulk modulus (fluids)
+    float *p1;  // lambda (solids) or exponent (fluids)
+    uint32_t *flags;
+    uint32_t *id;  // original particle index (sort_map of the reference)
+};
+
+// Arrays indexed by original particle index: never permuted.
+struct AuxArrays {
+    float *visc_dyn;
+    float *visc_bulk;
+    float *sand_alpha;
+    float *jp;
+    float *energy;
+    float *goal_a[3];
+    float *goal_b[3];
+    uint32_t *goal_flags_a;
+    uint32_t *goal_flags_b;
+};
+
+// Dense block grid.  Local node (x, y, z) = global node - origin lives in
+// block (x>>2, y>>2, z>>2), linear block index bx + nb0*(by + nb1*bz), and at
+// slot (x&3) + 4*(y&3) + 16*(z&3) of the block's 64 nodes.
+struct GridGeom {
+    int origin[3];
+    int nb[3];
+    float h;
+    uint32_t axis_bits;  // Morton bits per axis; 2^axis_bits > nb[a]
+    uint32_t tomb_key;   // all-ones in 3*axis_bits bits: never a valid block
+    uint32_t num_blocks_total;
+};
+
+// Device-side counters, one instance in device memory.
+struct DeviceScalars {
+    uint32_t num_blocks;    // occupied blocks this substep
+    uint32_t live_count;    // particles with a valid key (sorted to the front)
+    uint32_t status;        // MPM_STATUS_* bits | kernel id << 16
+    uint32_t active_nodes;  // nodes with m > 0 after the grid update
+    uint32_t svd_fp64;      // FP64 SVD fallbacks taken
+    uint32_t pad[3];
+};
+
+struct SubstepParams {
+    float dt;
+    float h;
+    float gravity[3];  // interpolated gravity of the substep
+    float factor_b;    // frame factor (goal interpolation)
+    int has_goals;
+    int has_b;
+    uint32_t options;  // MPM_OPT_*
+    float plane_z;
+    float wall_min[3];
+    float wall_max[3];
+    float domain_min[3];
+    float domain_max[3];
+    float snow_lo, snow_hi, snow_hardening;
+    uint32_t epoch;    // substep counter, stamps grid blocks
+};
+
+enum KernelId : uint32_t {
+    KID_KEYS = 1,
+    KID_SORT = 2,
+    KID_BLOCKS = 3,
+    KID_P2G = 4,
+    KID_GRID = 5,
+    KID_G2P = 6,
+    KID_COPY = 7
+};
+
+// ---- sort.hip
+size_t radix_sort_temp_bytes(uint64_t n);
+hipError_t radix_sort(const uint32_t *keys_in, const uint32_t *vals_in, uint32_t *keys_out, uint32_t *vals_out,
+                      uint32_t *keys_tmp, uint32_t *vals_tmp, uint64_t n, int key_bits, void *temp,
+                      hipStream_t stream);
+size_t scan_temp_bytes(uint64_t n);
+hipError_t exclusive_scan(const uint32_t *in, uint32_t *out, uint32_t *total, uint64_t n, void *temp,
+                          hipStream_t stream);
+hipError_t launch_keys(const ParticleArrays &P, const GridGeom &G, uint64_t n, uint32_t *keys, DeviceScalars *ds,
+                       hipStream_t stream);
+size_t block_list_temp_bytes(uint64_t n);
+hipError_t launch_block_list(const uint32_t *keys, uint64_t n, uint32_t tomb_key, uint32_t *blk_start,
+                             uint32_t *blk_key, uint32_t capacity, DeviceScalars *ds, void *temp, hipStream_t stream);
+hipError_t launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t stream);
+hipError_t launch_reset_scalars(DeviceScalars *ds, hipStream_t stream);
+
+// ---- transfer.hip
+struct TransferLaunch {
+    int persistent_blocks;
+    int p2g_waves;
+    int g2p_threads;
+};
+hipError_t launch_p2g(const TransferLaunch &L, bool from_c, const ParticleArrays &P, const AuxArrays &X,
+                      const uint32_t *order, const uint32_t *blk_start, const uint32_t *blk_key,
+                      const GridGeom &G, const SubstepParams &S, float4 *grid, uint32_t *blk_flag,
+                      DeviceScalars *ds, hipStream_t stream);
+hipError_t launch_grid_update(const TransferLaunch &L, const uint32_t *blk_key, const GridGeom &G,
+                              const SubstepParams &S, float4 *grid, float4 *vgrid, uint32_t *blk_flag,
+                              uint32_t *blk_epoch, DeviceScalars *ds, hipStream_t stream);
+hipError_t launch_g2p(const TransferLaunch &L, bool last, const ParticleArrays &Pin, const ParticleArrays &Pout,
+                      const AuxArrays &X, const uint32_t *order, const uint32_t *blk_start, const uint32_t *blk_key,
+                      const GridGeom &G, const SubstepParams &S, const float4 *vgrid, DeviceScalars *ds,
+                      hipStream_t stream);
+hipError_t launch_copy_dead(const ParticleArrays &Pin, const ParticleArrays &Pout, const uint32_t *order,
+                            uint64_t n, DeviceScalars *ds, hipStream_t stream);
+hipError_t launch_unpermute(const ParticleArrays &P, uint64_t n, float *x, float *v, float *F, float *C,
+                            uint32_t *flags, hipStream_t stream);
+hipError_t launch_stress(const uint32_t *flags, const mpm_particle_parameters *params, const float *F,
+                         const float *jp, float snow_hardening, float *tau, float *energy, uint64_t n,
+                         hipStream_t stream);
+hipError_t launch_svd(const float *F, float *U, float *s, float *V, uint64_t n, hipStream_t stream);
+
+}  // namespace svamd

@@ -1,0 +1,432 @@
+// sort.hip - Morton block keys, stable LSD radix sort, device-wide scans and
+// the occupied-block list.
+//
+// The radix sort is the reduce-then-scan form: per pass, a histogram kernel
+// counts the digits of each 4096-key tile, a device scan turns the digit-major
+// tile counts into output offsets, and a scatter kernel ranks every key inside
+// its tile stably.  Ranking within a wave64 uses ballots over the digit bits
+// (a match of equal digits), so equal keys keep their input order and the
+// result equals a stable CPU sort by key (the reference's sort_on_cpu,
+// rust/crates/gpu/src/util.rs:65-69, and its radix sort,
+// rust/crates/gpu/src/radix_sort/mod.rs:148-238).
+#include <algorithm>
+
+#include "device_math.hpp"
+#include "internal.hpp"
+
+namespace svamd {
+
+namespace {
+
+constexpr int SORT_THREADS = 256;
+constexpr int SORT_ITEMS = 16;
+constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;  // 4096 keys
+constexpr int RADIX_BITS = 8;
+constexpr int RADIX = 1 << RADIX_BITS;
+constexpr int WAVES = SORT_THREADS / 64;
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+constexpr int TOP_THREADS = 1024;
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Lanes of the calling wave (among the active ones) whose digit equals d.
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, int bits) {
+    uint64_t peers = __ballot(1);
+    for (int b = 0; b < bits; ++b) {
+        uint32_t bit = (d >> b) & 1u;
+        uint64_t m = __ballot(bit);
+        peers &= bit ? m : ~m;
+    }
+    return peers;
+}
+
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
+    uint32_t lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    return x;
+}
+
+// Exclusive scan of one value per thread across the work-group; returns the
+// exclusive prefix and writes the group total to *total (all threads).
+template <int THREADS>
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t x, uint32_t *wave_sums, uint32_t &total) {
+    constexpr int NW = THREADS / 64;
+    uint32_t lane = lane_id();
+    uint32_t wave = threadIdx.x / 64;
+    uint32_t incl = wave_inclusive_scan(x);
+    if (lane == 63) wave_sums[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        uint32_t w = threadIdx.x < (uint32_t)NW ? wave_sums[threadIdx.x] : 0u;
+        uint32_t wi = wave_inclusive_scan(w);
+        if (threadIdx.x < (uint32_t)NW) wave_sums[threadIdx.x] = wi - w;
+        if (threadIdx.x == (uint32_t)NW - 1) wave_sums[NW] = wi;
+    }
+    __syncthreads();
+    uint32_t r = wave_sums[wave] + incl - x;
+    total = wave_sums[NW];
+    __syncthreads();
+    return r;
+}
+
+// ------------------------------------------------------------------ scans
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const uint32_t *__restrict__ in, uint64_t m,
+                                                               uint32_t *__restrict__ sums) {
+    __shared__ uint32_t ws[SCAN_THREADS / 64 + 1];
+    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        uint64_t i = base + (uint64_t)k * SCAN_THREADS + threadIdx.x;
+        if (i < m) s += in[i];
+    }
+    uint32_t total;
+    block_exclusive_scan<SCAN_THREADS>(s, ws, total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// Exclusive scan of sums[0, nt) in place by one work-group; *total = sum.
+__global__ __launch_bounds__(TOP_THREADS) void k_scan_top(uint32_t *__restrict__ sums, uint32_t nt,
+                                                           uint32_t *__restrict__ total) {
+    __shared__ uint32_t ws[TOP_THREADS / 64 + 1];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nt; base += TOP_THREADS) {
+        uint32_t i = base + threadIdx.x;
+        uint32_t x = i < nt ? sums[i] : 0u;
+        uint32_t t;
+        uint32_t ex = block_exclusive_scan<TOP_THREADS>(x, ws, t);
+        if (i < nt) sums[i] = carry + ex;
+        carry += t;
+    }
+    if (threadIdx.x == 0 && total) *total = carry;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const uint32_t *__restrict__ in,
+                                                             uint32_t *__restrict__ out, uint64_t m,
+                                                             const uint32_t *__restrict__ sums) {
+    __shared__ uint32_t ws[SCAN_THREADS / 64 + 1];
+    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        uint64_t i = base + k;
+        v[k] = i < m ? in[i] : 0u;
+        s += v[k];
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive_scan<SCAN_THREADS>(s, ws, total) + sums[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        uint64_t i = base + k;
+        if (i < m) out[i] = run;
+        run += v[k];
+    }
+}
+
+// ------------------------------------------------------------- radix sort
+__global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint32_t *__restrict__ keys, uint64_t n, int shift,
+                                                              int bits, uint32_t *__restrict__ spine, uint32_t T) {
+    __shared__ uint32_t h[RADIX];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    uint64_t base = (uint64_t)blockIdx.x * SORT_TILE;
+    const uint32_t mask = (1u << bits) - 1u;
+#pragma unroll 4
+    for (int k = 0; k < SORT_ITEMS; ++k) {
+        uint64_t i = base + (uint64_t)k * SORT_THREADS + threadIdx.x;
+        if (i < n) {
+            uint32_t d = (keys[i] >> shift) & mask;
+            uint64_t peers = match_digit(d, bits);
+            uint32_t leader = (uint32_t)__ffsll((unsigned long long)peers) - 1u;
+            if (lane_id() == leader) atomicAdd(&h[d], (uint32_t)__popcll(peers));
+        }
+    }
+    __syncthreads();
+    spine[(uint64_t)threadIdx.x * T + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint32_t *__restrict__ keys_in,
+                                                                 const uint32_t *__restrict__ vals_in,
+                                                                 uint32_t *__restrict__ keys_out,
+                                                                 uint32_t *__restrict__ vals_out, uint64_t n,
+                                                                 int shift, int bits,
+                                                                 const uint32_t *__restrict__ offsets, uint32_t T) {
+    __shared__ uint32_t digit_base[RADIX];
+    __shared__ uint32_t wave_cnt[WAVES][RADIX];
+    const uint32_t wave = threadIdx.x / 64;
+    const uint32_t lane = lane_id();
+    const uint32_t mask = (1u << bits) - 1u;
+    digit_base[threadIdx.x] = offsets[(uint64_t)threadIdx.x * T + blockIdx.x];
+    uint64_t base = (uint64_t)blockIdx.x * SORT_TILE;
+    for (int k = 0; k < SORT_ITEMS; ++k) {
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) wave_cnt[w][threadIdx.x] = 0;
+        __syncthreads();
+        uint64_t i = base + (uint64_t)k * SORT_THREADS + threadIdx.x;
+        bool valid = i < n;
+        uint32_t key = 0, val = 0, d = 0, rank = 0;
+        if (valid) {
+            key = keys_in[i];
+            val = vals_in ? vals_in[i] : (uint32_t)i;
+            d = (key >> shift) & mask;
+            uint64_t peers = match_digit(d, bits);
+            rank = (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
+            uint32_t leader = (uint32_t)__ffsll((unsigned long long)peers) - 1u;
+            if (lane == leader) wave_cnt[wave][d] = (uint32_t)__popcll(peers);
+        }
+        __syncthreads();
+        {
+            // Per digit: the waves' counts become their output offsets.
+            uint32_t run = digit_base[threadIdx.x];
+#pragma unroll
+            for (int w = 0; w < WAVES; ++w) {
+                uint32_t c = wave_cnt[w][threadIdx.x];
+                wave_cnt[w][threadIdx.x] = run;
+                run += c;
+            }
+            digit_base[threadIdx.x] = run;
+        }
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = wave_cnt[wave][d] + rank;
+            if (pos < n) {  // always true for a consistent spine
+                keys_out[pos] = key;
+                vals_out[pos] = val;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_identity_copy(const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ vals_in,
+                                uint32_t *__restrict__ keys_out, uint32_t *__restrict__ vals_out, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        keys_out[i] = keys_in[i];
+        vals_out[i] = vals_in ? vals_in[i] : (uint32_t)i;
+    }
+}
+
+// ------------------------------------------------------------------ keys
+// Block key of a particle: Morton code of the 4^3 block holding its base cell
+// floor(x/h - 1/2) (position_to_shift_quadratic, rust/crates/cpu/src/kernels.rs:46-49).
+__global__ __launch_bounds__(256) void k_keys(ParticleArrays P, GridGeom G, uint64_t n, uint32_t *__restrict__ keys,
+                                              DeviceScalars *ds) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t flags = P.flags[i];
+    if (flags & F_TOMB) {
+        keys[i] = G.tomb_key;
+        return;
+    }
+    uint32_t b[3];
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float xn = P.x[a][i] / G.h - 0.5f;
+        ok = ok && isfinite(xn);
+        float fl = floorf(xn);
+        ok = ok && fl > -1.0e9f && fl < 1.0e9f;
+        int c = (int)fl - G.origin[a];
+        ok = ok && c >= 0 && (c >> 2) <= G.nb[a] - 2;
+        b[a] = (uint32_t)(c >> 2);
+    }
+    if (!ok) {
+        atomicOr(&ds->status, MPM_STATUS_OUT_OF_GRID | (KID_KEYS << 16));
+        keys[i] = G.tomb_key;
+        return;
+    }
+    keys[i] = morton3(b[0], b[1], b[2]);
+}
+
+// ------------------------------------------------------------ block list
+// Tiles of 4096 sorted keys, 16 consecutive keys per thread.  A head is the
+// first key of a run of equal valid keys; the heads, in key order, are the
+// occupied blocks.
+__global__ __launch_bounds__(SCAN_THREADS) void k_heads_count(const uint32_t *__restrict__ keys, uint64_t n,
+                                                               uint32_t tomb, uint32_t *__restrict__ tile_counts,
+                                                               DeviceScalars *ds) {
+    __shared__ uint32_t ws[SCAN_THREADS / 64 + 1];
+    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t cnt = 0;
+    uint32_t prev = base > 0 && base - 1 < n ? keys[base - 1] : tomb ^ 1u;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        uint64_t i = base + k;
+        if (i < n) {
+            uint32_t key = keys[i];
+            bool valid = key != tomb;
+            cnt += (valid && (i == 0 || prev != key)) ? 1u : 0u;
+            if (valid && (i + 1 == n || keys[i + 1] == tomb)) ds->live_count = (uint32_t)(i + 1);
+            prev = key;
+        }
+    }
+    uint32_t total;
+    block_exclusive_scan<SCAN_THREADS>(cnt, ws, total);
+    if (threadIdx.x == 0) tile_counts[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_heads_write(const uint32_t *__restrict__ keys, uint64_t n,
+                                                               uint32_t tomb, const uint32_t *__restrict__ tile_offsets,
+                                                               uint32_t *__restrict__ blk_start,
+                                                               uint32_t *__restrict__ blk_key, uint32_t capacity,
+                                                               DeviceScalars *ds) {
+    __shared__ uint32_t ws[SCAN_THREADS / 64 + 1];
+    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t head_bits = 0, cnt = 0;
+    uint32_t prev = base > 0 && base - 1 < n ? keys[base - 1] : tomb ^ 1u;
+    uint32_t kv[SCAN_ITEMS];
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        uint64_t i = base + k;
+        kv[k] = i < n ? keys[i] : tomb;
+        bool head = i < n && kv[k] != tomb && (i == 0 || prev != kv[k]);
+        head_bits |= (head ? 1u : 0u) << k;
+        cnt += head ? 1u : 0u;
+        prev = kv[k];
+    }
+    uint32_t total;
+    uint32_t slot = block_exclusive_scan<SCAN_THREADS>(cnt, ws, total) + tile_offsets[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        if (head_bits & (1u << k)) {
+            if (slot < capacity) {
+                blk_start[slot] = (uint32_t)(base + k);
+                blk_key[slot] = kv[k];
+            } else {
+                atomicOr(&ds->status, MPM_STATUS_INDIRECT_LIMIT_EXCEEDED | (KID_BLOCKS << 16));
+            }
+            ++slot;
+        }
+    }
+}
+
+__global__ void k_copy_bytes(const float4 *__restrict__ src, float4 *__restrict__ dst, uint64_t n4) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+__global__ void k_reset_scalars(DeviceScalars *ds) {
+    if (threadIdx.x == 0) {
+        ds->num_blocks = 0;
+        ds->live_count = 0;
+        ds->active_nodes = 0;
+    }
+}
+
+inline uint32_t tiles_of(uint64_t n, int tile) { return (uint32_t)((n + tile - 1) / tile); }
+
+}  // namespace
+
+size_t scan_temp_bytes(uint64_t n) {
+    size_t nt = tiles_of(n, SCAN_TILE);
+    return std::max<size_t>(nt, 1) * sizeof(uint32_t);
+}
+
+hipError_t exclusive_scan(const uint32_t *in, uint32_t *out, uint32_t *total, uint64_t n, void *temp,
+                          hipStream_t stream) {
+    uint32_t nt = tiles_of(n, SCAN_TILE);
+    uint32_t *sums = static_cast<uint32_t *>(temp);
+    if (n == 0) {
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(TOP_THREADS), 0, stream, sums, 0u, total);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nt), dim3(SCAN_THREADS), 0, stream, in, n, sums);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(TOP_THREADS), 0, stream, sums, nt, total);
+    hipLaunchKernelGGL(k_scan_down, dim3(nt), dim3(SCAN_THREADS), 0, stream, in, out, n, sums);
+    return hipGetLastError();
+}
+
+// Layout of the radix-sort temp buffer: keys ping-pong, values ping-pong,
+// the digit-major spine and the spine scan's tile sums.
+size_t radix_sort_temp_bytes(uint64_t n) {
+    size_t T = std::max<uint32_t>(tiles_of(n, SORT_TILE), 1u);
+    size_t spine = (size_t)RADIX * T;
+    auto align = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    return align(n * 4) * 2 + align(spine * 4) + align(scan_temp_bytes(spine));
+}
+
+hipError_t radix_sort(const uint32_t *keys_in, const uint32_t *vals_in, uint32_t *keys_out, uint32_t *vals_out,
+                      uint32_t *keys_tmp, uint32_t *vals_tmp, uint64_t n, int key_bits, void *temp,
+                      hipStream_t stream) {
+    auto align = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    uint32_t T = std::max<uint32_t>(tiles_of(n, SORT_TILE), 1u);
+    size_t spine_len = (size_t)RADIX * T;
+    char *t = static_cast<char *>(temp);
+    if (!keys_tmp) keys_tmp = reinterpret_cast<uint32_t *>(t);
+    t += align(n * 4);
+    if (!vals_tmp) vals_tmp = reinterpret_cast<uint32_t *>(t);
+    t += align(n * 4);
+    uint32_t *spine = reinterpret_cast<uint32_t *>(t);
+    t += align(spine_len * 4);
+    void *scan_tmp = t;
+
+    int passes = (key_bits + RADIX_BITS - 1) / RADIX_BITS;
+    if (n == 0) return hipSuccess;
+    if (passes <= 0) {
+        hipLaunchKernelGGL(k_identity_copy, dim3(std::min<uint64_t>((n + 255) / 256, 4096)), dim3(256), 0, stream,
+                           keys_in, vals_in, keys_out, vals_out, n);
+        return hipGetLastError();
+    }
+    const uint32_t *ksrc = keys_in;
+    const uint32_t *vsrc = vals_in;
+    for (int p = 0; p < passes; ++p) {
+        bool to_out = ((passes - 1 - p) % 2) == 0;
+        uint32_t *kdst = to_out ? keys_out : keys_tmp;
+        uint32_t *vdst = to_out ? vals_out : vals_tmp;
+        int shift = p * RADIX_BITS;
+        int bits = std::min(RADIX_BITS, key_bits - shift);
+        hipLaunchKernelGGL(k_radix_hist, dim3(T), dim3(SORT_THREADS), 0, stream, ksrc, n, shift, bits, spine, T);
+        hipError_t e = exclusive_scan(spine, spine, nullptr, spine_len, scan_tmp, stream);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_radix_scatter, dim3(T), dim3(SORT_THREADS), 0, stream, ksrc, vsrc, kdst, vdst, n, shift,
+                           bits, spine, T);
+        ksrc = kdst;
+        vsrc = vdst;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_keys(const ParticleArrays &P, const GridGeom &G, uint64_t n, uint32_t *keys, DeviceScalars *ds,
+                       hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_keys, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, stream, P, G, n, keys, ds);
+    return hipGetLastError();
+}
+
+size_t block_list_temp_bytes(uint64_t n) { return (size_t)std::max<uint32_t>(tiles_of(n, SCAN_TILE), 1u) * 4; }
+
+hipError_t launch_block_list(const uint32_t *keys, uint64_t n, uint32_t tomb_key, uint32_t *blk_start,
+                             uint32_t *blk_key, uint32_t capacity, DeviceScalars *ds, void *temp, hipStream_t stream) {
+    uint32_t nt = tiles_of(n, SCAN_TILE);
+    uint32_t *tile = static_cast<uint32_t *>(temp);
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_heads_count, dim3(nt), dim3(SCAN_THREADS), 0, stream, keys, n, tomb_key, tile, ds);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(TOP_THREADS), 0, stream, tile, nt, &ds->num_blocks);
+    hipLaunchKernelGGL(k_heads_write, dim3(nt), dim3(SCAN_THREADS), 0, stream, keys, n, tomb_key, tile, blk_start,
+                       blk_key, capacity, ds);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t stream) {
+    uint64_t n4 = bytes / 16;
+    if (n4 == 0) return hipSuccess;
+    uint32_t grid = (uint32_t)std::min<uint64_t>((n4 + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_copy_bytes, dim3(grid), dim3(256), 0, stream, static_cast<const float4 *>(src),
+                       static_cast<float4 *>(dst), n4);
+    return hipGetLastError();
+}
+
+hipError_t launch_reset_scalars(DeviceScalars *ds, hipStream_t stream) {
+    hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(64), 0, stream, ds);
+    return hipGetLastError();
+}
+
+}  // namespace svamd

@@ -1,0 +1,757 @@
+// transfer.hip - P2G, grid update and G2P of the MLS-MPM substep (gfx950).
+//
+// Dataflow of one substep (after the block sort):
+//   P2G   per occupied 4^3 block: the block's particles are grouped by cell in
+//         LDS, lane l of a wave owns cell l and accumulates its particles'
+//         3x3x3 stencils in registers, the 64 lanes add their stencils into a
+//         6^3-node LDS arena (one add per lane and node, no two lanes on the
+//         same bank), and the arena is flushed to the dense grid with global
+//         float atomics.  Mirrors scatter_momentum
+//         (rust/crates/cpu/src/phase/scatter_momentum.rs:22-93).
+//   grid  v = p / m per active node (meld_grid without collider twins,
+//         rust/crates/cpu/src/phase/meld_grid.rs:61-66) plus the optional grid
+//         gravity and boundary conditions; clears the accumulation grid.
+//   G2P   per occupied block: the 6^3 node velocities are staged in LDS, each
+//         thread gathers v and C of one particle (collect_velocity,
+//         rust/crates/cpu/src/phase/collect_velocity.rs:19-75), advances x and
+//         F (advance_particles, rust/crates/cpu/src/phase/advance_particles.rs:17-93),
+//         culls (rust/crates/cpu/src/phase/cull_particles.rs:17-41) and writes
+//         the fused affine momentum A = m C - dt 4/h^2 V0 (tau + J sigma_visc)
+//         for the next P2G, permuting the particle arrays into block order.
+#include <algorithm>
+
+#include "device_math.hpp"
+#include "internal.hpp"
+
+namespace svamd {
+
+namespace {
+
+constexpr int ARENA_ROWS = 8;               // (x>>2, y>>2, z>>2) of a 6^3 arena
+constexpr int ARENA = ARENA_ROWS * 64;      // slots per component
+constexpr int P2G_CHUNK = 1024;             // particles grouped per pass in LDS
+constexpr int G2P_NODES = 216;              // 6^3
+
+// LDS slot of arena node (x, y, z), 0 <= x, y, z < 6: the bank is
+// (x&3) | (y&3)<<2 | (z&3)<<4, so the 64 nodes that the 64 lanes (cells) of
+// a wave touch for one stencil offset fall on 64 distinct banks.
+__device__ __forceinline__ int arena_slot(int x, int y, int z) {
+    return (((x >> 2) | ((y >> 2) << 1) | ((z >> 2) << 2)) << 6) | (x & 3) | ((y & 3) << 2) | ((z & 3) << 4);
+}
+
+__device__ __forceinline__ uint32_t block_linear(const GridGeom &G, uint32_t bx, uint32_t by, uint32_t bz) {
+    return bx + (uint32_t)G.nb[0] * (by + (uint32_t)G.nb[1] * bz);
+}
+
+__device__ __forceinline__ void decode_block(uint32_t key, uint32_t b[3]) {
+    b[0] = compact1by2(key);
+    b[1] = compact1by2(key >> 1);
+    b[2] = compact1by2(key >> 2);
+}
+
+// Dense node index of local node (x, y, z).
+__device__ __forceinline__ uint32_t node_index(const GridGeom &G, uint32_t x, uint32_t y, uint32_t z) {
+    return block_linear(G, x >> 2, y >> 2, z >> 2) * 64u + (x & 3u) + ((y & 3u) << 2) + ((z & 3u) << 4);
+}
+
+// Kirchhoff stress of a particle, tau = P F^T, for the model its flags select.
+// U, s, V must hold the SVD of F for fixed-corotated and snow particles.
+__device__ __forceinline__ void kirchhoff(uint32_t flags, float p0, float p1, float hard, const float F[9], float J,
+                                          const float U[9], const float s[3], const float V[9], float tau[9]) {
+    if (flags & F_FLUID) {
+        tau_inviscid(p0, (int)p1, J, tau);
+    } else if (flags & (F_FCR | F_SNOW)) {
+        tau_fixed_corotated(p0 * hard, p1 * hard, F, U, s, V, tau);
+    } else {
+        tau_neo_hookean(p0, p1, F, J, tau);
+    }
+}
+
+// A = m C - s V0 tau - s J V0 sigma_visc(C)  (scatter_momentum.rs:46-84 regrouped).
+__device__ __forceinline__ void fused_affine(float m, float vol, float scaling, const float C[9], const float tau[9],
+                                             float J, bool visc, float vd, float vb, float A[9]) {
+    float sv = scaling * vol;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) A[k] = m * C[k] - sv * tau[k];
+    if (visc) {
+        float sig[9];
+        cauchy_viscosity(vd, vb, C, sig);
+        float svj = sv * J;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) A[k] -= svj * sig[k];
+    }
+}
+
+__device__ __forceinline__ bool needs_svd_for_stress(uint32_t flags) {
+    return !(flags & F_FLUID) && (flags & (F_FCR | F_SNOW));
+}
+
+// ------------------------------------------------------------------- P2G
+template <int WAVES, bool FROM_C, bool GOALS>
+__global__ __launch_bounds__(64 * WAVES) void k_p2g(ParticleArrays P, AuxArrays X, const uint32_t *__restrict__ order,
+                                                    const uint32_t *__restrict__ blk_start,
+                                                    const uint32_t *__restrict__ blk_key, GridGeom G, SubstepParams S,
+                                                    float4 *__restrict__ grid, uint32_t *__restrict__ blk_flag,
+                                                    DeviceScalars *ds) {
+    constexpr int NT = 64 * WAVES;
+    constexpr int PER_THREAD = P2G_CHUNK / NT;
+    __shared__ float arena[4 * ARENA];
+    __shared__ uint32_t cell_count[64];
+    __shared__ uint32_t cell_start[64];
+    __shared__ uint32_t cell_fill[64];
+    __shared__ uint32_t list[P2G_CHUNK];
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u;
+    const uint32_t wave = tid >> 6;
+    const uint32_t nblocks = ds->num_blocks;
+    const uint32_t live = ds->live_count;
+    const float h = S.h;
+    const float scaling = S.dt * 4.0f / (h * h);
+
+    for (uint32_t bi = blockIdx.x; bi < nblocks; bi += gridDim.x) {
+        const uint32_t start = blk_start[bi];
+        const uint32_t end = bi + 1 < nblocks ? blk_start[bi + 1] : live;
+        uint32_t b[3];
+        decode_block(blk_key[bi], b);
+        int cell0[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) cell0[a] = G.origin[a] + 4 * (int)b[a];
+
+        for (int k = tid; k < 4 * ARENA; k += NT) arena[k] = 0.f;
+
+        for (uint32_t cs = start; cs < end; cs += P2G_CHUNK) {
+            const uint32_t ce = min(end, cs + (uint32_t)P2G_CHUNK);
+            if (tid < 64) {
+                cell_count[tid] = 0;
+                cell_fill[tid] = 0;
+            }
+            __syncthreads();
+            // Pass 1: the cell of every particle of the chunk.
+            uint32_t myp[PER_THREAD], mycell[PER_THREAD];
+#pragma unroll
+            for (int k = 0; k < PER_THREAD; ++k) {
+                uint32_t i = cs + tid + (uint32_t)(k * NT);
+                mycell[k] = 64u;
+                myp[k] = 0u;
+                if (i < ce) {
+                    uint32_t p = order[i];
+                    int c[3];
+                    bool inside = true;
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        c[a] = (int)floorf(P.x[a][p] / h - 0.5f) - cell0[a];
+                        inside = inside && c[a] >= 0 && c[a] < 4;
+                    }
+                    if (inside) {
+                        uint32_t cell = (uint32_t)(c[0] | (c[1] << 2) | (c[2] << 4));
+                        atomicAdd(&cell_count[cell], 1u);
+                        mycell[k] = cell;
+                        myp[k] = p;
+                    } else {
+                        atomicOr(&ds->status, MPM_STATUS_OUT_OF_GRID | (KID_P2G << 16));
+                    }
+                }
+            }
+            __syncthreads();
+            if (tid < 64) {
+                uint32_t c = cell_count[tid];
+                uint32_t incl = c;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    uint32_t y = __shfl_up(incl, o, 64);
+                    if (lane >= (uint32_t)o) incl += y;
+                }
+                cell_start[tid] = incl - c;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < PER_THREAD; ++k) {
+                if (mycell[k] < 64u) {
+                    uint32_t slot = cell_start[mycell[k]] + atomicAdd(&cell_fill[mycell[k]], 1u);
+                    list[slot] = myp[k];
+                }
+            }
+            __syncthreads();
+
+            // Pass 2: lane = cell; the wave's lanes walk their cells' particles.
+            const uint32_t cnt = cell_count[lane];
+            const uint32_t st = cell_start[lane];
+            float acc[27][4];
+#pragma unroll
+            for (int n = 0; n < 27; ++n)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) acc[n][c] = 0.f;
+
+            for (uint32_t r = wave; r < cnt; r += WAVES) {
+                const uint32_t p = list[st + r];
+                float xp[3], fx[3], w[3][3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    xp[a] = P.x[a][p];
+                    float xn = xp[a] / h;
+                    fx[a] = xn - floorf(xn - 0.5f);
+                    bspline_weights(fx[a], w[a]);
+                }
+                const float m = P.m[p];
+                float v[3] = {P.v[0][p], P.v[1][p], P.v[2][p]};
+                // external_force (rust/crates/cpu/src/phase/external_force.rs:36-47).
+                bool goal = false;
+                if constexpr (GOALS) {
+                    uint32_t o = P.id[p];
+                    uint32_t fb = S.has_b ? X.goal_flags_b[o] : X.goal_flags_a[o];
+                    if ((X.goal_flags_a[o] & F_GOAL) && (fb & F_GOAL)) {
+                        goal = true;
+                        float fa = 1.0f - S.factor_b;
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            float gb = S.has_b ? X.goal_b[a][o] : X.goal_a[a][o];
+                            float gp = fa * X.goal_a[a][o] + S.factor_b * gb;
+                            v[a] = (gp - xp[a]) / S.dt;
+                        }
+                    }
+                }
+                if (!goal && !(S.options & MPM_OPT_GRID_GRAVITY)) {
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) v[a] += S.dt * S.gravity[a];
+                }
+                float A[9];
+                if constexpr (FROM_C) {
+                    const uint32_t flags = P.flags[p];
+                    float F[9], C[9];
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) {
+                        F[k] = P.F[k][p];
+                        C[k] = P.A[k][p];
+                    }
+                    const float J = det3(F);
+                    float U[9], s[3] = {1.f, 1.f, 1.f}, V[9];
+                    float hard = 1.0f;
+                    if (needs_svd_for_stress(flags)) {
+                        if (svd3(F, U, s, V)) atomicAdd(&ds->svd_fp64, 1u);
+                        if (flags & F_SNOW) hard = expf(S.snow_hardening * (1.0f - X.jp[P.id[p]]));
+                    }
+                    if (!(J >= 1e-7f) && !(flags & F_FLUID))
+                        atomicOr(&ds->status, MPM_STATUS_PARTICLE_CLOSE_TO_INVERTED | (KID_P2G << 16));
+                    float tau[9];
+                    kirchhoff(flags, P.p0[p], P.p1[p], hard, F, J, U, s, V, tau);
+                    const bool visc = flags & F_VISC;
+                    float vd = 0.f, vb = 0.f;
+                    if (visc) {
+                        vd = X.visc_dyn[P.id[p]];
+                        vb = X.visc_bulk[P.id[p]];
+                    }
+                    fused_affine(m, P.vol[p], scaling, C, tau, J, visc, vd, vb, A);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) A[k] = P.A[k][p];
+                }
+                // Contribution to node base + (i, j, k):
+                //   w_ijk (m v + A d), d = h ((i, j, k) - fx) = q + (h A) (i, j, k).
+                float q[3], c0[3], c1[3], c2[3];
+#pragma unroll
+                for (int r3 = 0; r3 < 3; ++r3) {
+                    q[r3] = m * v[r3] - h * (A[r3] * fx[0] + A[3 + r3] * fx[1] + A[6 + r3] * fx[2]);
+                    c0[r3] = h * A[r3];
+                    c1[r3] = h * A[3 + r3];
+                    c2[r3] = h * A[6 + r3];
+                }
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        const float wij = w[0][i] * w[1][j];
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            const float wt = wij * w[2][k];
+                            const int n = 9 * i + 3 * j + k;
+                            acc[n][0] += wt * m;
+#pragma unroll
+                            for (int r3 = 0; r3 < 3; ++r3)
+                                acc[n][1 + r3] += wt * (q[r3] + (float)i * c0[r3] + (float)j * c1[r3] + (float)k * c2[r3]);
+                        }
+                    }
+                }
+            }
+            // Add the lanes' stencils into the arena.
+            if (cnt > wave) {
+                const int cx = (int)(lane & 3u), cy = (int)((lane >> 2) & 3u), cz = (int)(lane >> 4);
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j)
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            const int n = 9 * i + 3 * j + k;
+                            const int slot = arena_slot(cx + i, cy + j, cz + k);
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) atomicAdd(&arena[c * ARENA + slot], acc[n][c]);
+                        }
+            }
+            __syncthreads();
+        }
+
+        // Flush the 6^3 arena to the grid.
+        for (int nidx = tid; nidx < 216; nidx += NT) {
+            const int ax = nidx % 6, ay = (nidx / 6) % 6, az = nidx / 36;
+            const int slot = arena_slot(ax, ay, az);
+            const float m = arena[slot];
+            if (m != 0.f) {
+                uint32_t node = node_index(G, 4u * b[0] + ax, 4u * b[1] + ay, 4u * b[2] + az);
+                float *g = reinterpret_cast<float *>(grid + node);
+                unsafeAtomicAdd(g + 0, m);
+                unsafeAtomicAdd(g + 1, arena[ARENA + slot]);
+                unsafeAtomicAdd(g + 2, arena[2 * ARENA + slot]);
+                unsafeAtomicAdd(g + 3, arena[3 * ARENA + slot]);
+            }
+        }
+        if (tid < 8) blk_flag[block_linear(G, b[0] + (tid & 1u), b[1] + ((tid >> 1) & 1u), b[2] + (tid >> 2))] = 1u;
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ grid update
+__global__ __launch_bounds__(64) void k_grid_update(const uint32_t *__restrict__ blk_key, GridGeom G, SubstepParams S,
+                                                    float4 *__restrict__ grid, float4 *__restrict__ vgrid,
+                                                    uint32_t *__restrict__ blk_flag, uint32_t *__restrict__ blk_epoch,
+                                                    DeviceScalars *ds) {
+    const uint32_t lane = threadIdx.x;
+    const uint32_t ncand = ds->num_blocks * 8u;
+    for (uint32_t c = blockIdx.x; c < ncand; c += gridDim.x) {
+        uint32_t b[3];
+        decode_block(blk_key[c >> 3], b);
+        const uint32_t d = c & 7u;
+        b[0] += d & 1u;
+        b[1] += (d >> 1) & 1u;
+        b[2] += d >> 2;
+        const uint32_t B = block_linear(G, b[0], b[1], b[2]);
+        uint32_t claimed = 0;
+        if (lane == 0) claimed = atomicExch(&blk_flag[B], 0u);
+        claimed = __shfl(claimed, 0, 64);
+        if (!claimed) continue;
+        const uint32_t node = B * 64u + lane;
+        float4 g = grid[node];
+        const float m = g.x;
+        float v[3] = {0.f, 0.f, 0.f};
+        if (m > 0.f) {
+            v[0] = g.y / m;
+            v[1] = g.z / m;
+            v[2] = g.w / m;
+            // Extensions without a reference counterpart.
+            if (S.options & MPM_OPT_GRID_GRAVITY) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) v[a] += S.gravity[a] * S.dt;
+            }
+            if (S.options & (MPM_OPT_STICKY_PLANE | MPM_OPT_BOX_WALLS)) {
+                const int lx = (int)(4u * b[0] + (lane & 3u)), ly = (int)(4u * b[1] + ((lane >> 2) & 3u)),
+                          lz = (int)(4u * b[2] + (lane >> 4));
+                const float xi[3] = {(float)(lx + G.origin[0]) * S.h, (float)(ly + G.origin[1]) * S.h,
+                                     (float)(lz + G.origin[2]) * S.h};
+                if ((S.options & MPM_OPT_STICKY_PLANE) && xi[2] < S.plane_z) v[0] = v[1] = v[2] = 0.f;
+                if (S.options & MPM_OPT_BOX_WALLS) {
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        if (xi[a] < S.wall_min[a] && v[a] < 0.f) v[a] = 0.f;
+                        if (xi[a] > S.wall_max[a] && v[a] > 0.f) v[a] = 0.f;
+                    }
+                }
+            }
+        }
+        vgrid[node] = make_float4(m, v[0], v[1], v[2]);
+        grid[node] = make_float4(0.f, 0.f, 0.f, 0.f);
+        const uint64_t act = __ballot(m > 0.f);
+        if (lane == 0) {
+            atomicAdd(&ds->active_nodes, (uint32_t)__popcll(act));
+            blk_epoch[B] = S.epoch;
+        }
+    }
+}
+
+// ------------------------------------------------------------------- G2P
+template <bool LAST>
+__global__ __launch_bounds__(256) void k_g2p(ParticleArrays Pin, ParticleArrays Pout, AuxArrays X,
+                                             const uint32_t *__restrict__ order, const uint32_t *__restrict__ blk_start,
+                                             const uint32_t *__restrict__ blk_key, GridGeom G, SubstepParams S,
+                                             const float4 *__restrict__ vgrid, DeviceScalars *ds) {
+    __shared__ float4 vloc[G2P_NODES];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t NT = blockDim.x;
+    const uint32_t nblocks = ds->num_blocks;
+    const uint32_t live = ds->live_count;
+    const float h = S.h;
+    const float dt = S.dt;
+    const float scaling = dt * 4.0f / (h * h);
+    const float apic = 4.0f / h;
+
+    for (uint32_t bi = blockIdx.x; bi < nblocks; bi += gridDim.x) {
+        const uint32_t start = blk_start[bi];
+        const uint32_t end = bi + 1 < nblocks ? blk_start[bi + 1] : live;
+        uint32_t b[3];
+        decode_block(blk_key[bi], b);
+        for (uint32_t nidx = tid; nidx < (uint32_t)G2P_NODES; nidx += NT) {
+            const uint32_t ax = nidx % 6u, ay = (nidx / 6u) % 6u, az = nidx / 36u;
+            vloc[nidx] = vgrid[node_index(G, 4u * b[0] + ax, 4u * b[1] + ay, 4u * b[2] + az)];
+        }
+        __syncthreads();
+        int cell0[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) cell0[a] = G.origin[a] + 4 * (int)b[a];
+
+        for (uint32_t i = start + tid; i < end; i += NT) {
+            const uint32_t p = order[i];
+            float xp[3], fx[3], w[3][3];
+            int c[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                xp[a] = Pin.x[a][p];
+                const float xn = xp[a] / h;
+                const float fl = floorf(xn - 0.5f);
+                fx[a] = xn - fl;
+                c[a] = min(max((int)fl - cell0[a], 0), 3);
+                bspline_weights(fx[a], w[a]);
+            }
+            // collect_velocity: v = sum w v_i, C = 4/h^2 sum w v_i (x_i - x_p)^T
+            float v[3] = {0.f, 0.f, 0.f};
+            float C[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) C[k] = 0.f;
+#pragma unroll
+            for (int i3 = 0; i3 < 3; ++i3) {
+                const float dx = (float)i3 - fx[0];
+#pragma unroll
+                for (int j3 = 0; j3 < 3; ++j3) {
+                    const float dy = (float)j3 - fx[1];
+                    const float wij = w[0][i3] * w[1][j3];
+#pragma unroll
+                    for (int k3 = 0; k3 < 3; ++k3) {
+                        const float dz = (float)k3 - fx[2];
+                        const float wt = wij * w[2][k3];
+                        const int nidx = (c[0] + i3) + 6 * ((c[1] + j3) + 6 * (c[2] + k3));
+                        const float4 gv = vloc[nidx];
+                        const float wv[3] = {wt * gv.y, wt * gv.z, wt * gv.w};
+#pragma unroll
+                        for (int r = 0; r < 3; ++r) {
+                            v[r] += wv[r];
+                            C[r] += wv[r] * dx;
+                            C[3 + r] += wv[r] * dy;
+                            C[6 + r] += wv[r] * dz;
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 9; ++k) C[k] *= apic;
+
+            // advance_particles: x += dt v, F += (C F) dt, plasticity, energy.
+            uint32_t flags = Pin.flags[p];
+            const uint32_t id = Pin.id[p];
+            const float m = Pin.m[p], vol = Pin.vol[p], p0 = Pin.p0[p], p1 = Pin.p1[p];
+            float F[9], CF[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) F[k] = Pin.F[k][p];
+            matmul3(C, F, CF);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) F[k] += CF[k] * dt;
+            float xnew[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) xnew[a] = xp[a] + v[a] * dt;
+
+            float U[9], s[3] = {1.f, 1.f, 1.f}, V[9];
+            float hard = 1.0f;
+            float energy = 0.f;
+            float J;
+            if (flags & F_FLUID) {
+                // advance_particles.rs:76-86: singular values <- (s1 s2 s3)^(1/3).
+                if (svd3(F, U, s, V)) atomicAdd(&ds->svd_fp64, 1u);
+                const float g = cbrtf(s[0] * s[1] * s[2]);
+                const float gd[3] = {g, g, g};
+                recompose(U, gd, V, F);
+                J = det3(F);
+                if (LAST) energy = energy_inviscid(p0, (int)p1, J);
+            } else if (flags & F_SNOW) {
+                if (svd3(F, U, s, V)) atomicAdd(&ds->svd_fp64, 1u);
+                float jp = X.jp[id];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const float cl = fminf(fmaxf(s[a], S.snow_lo), S.snow_hi);
+                    jp *= s[a] / cl;
+                    s[a] = cl;
+                }
+                X.jp[id] = jp;
+                recompose(U, s, V, F);
+                J = s[0] * s[1] * s[2];
+                hard = expf(S.snow_hardening * (1.0f - jp));
+                if (LAST) energy = energy_fixed_corotated(p0 * hard, p1 * hard, s);
+            } else {
+                if (flags & F_SAND) {
+                    // advance_particles.rs:50-71: Drucker-Prager return map in log strain.
+                    if (svd3(F, U, s, V)) atomicAdd(&ds->svd_fp64, 1u);
+                    const float alpha = X.sand_alpha[id];
+                    float e[3] = {logf(s[0]), logf(s[1]), logf(s[2])};
+                    const float etr = e[0] + e[1] + e[2];
+                    float eh[3] = {e[0] - etr / 3.f, e[1] - etr / 3.f, e[2] - etr / 3.f};
+                    const float ehn = sqrtf(eh[0] * eh[0] + eh[1] * eh[1] + eh[2] * eh[2]);
+                    if (etr < 0.f && ehn > 0.f) {
+                        const float dg = ehn + (3.f * p1 + 2.f * p0) / 2.f / p0 * etr * alpha;
+                        if (dg > 0.f) {
+                            float d[3];
+#pragma unroll
+                            for (int a = 0; a < 3; ++a) d[a] = expf(e[a] - dg / ehn * eh[a]);
+                            recompose(U, d, V, F);
+                        }
+                    } else {
+                        const float one[3] = {1.f, 1.f, 1.f};
+                        recompose(U, one, V, F);
+                    }
+                }
+                if (flags & F_FCR) {
+                    if (svd3(F, U, s, V)) atomicAdd(&ds->svd_fp64, 1u);
+                    J = s[0] * s[1] * s[2];
+                    if (LAST) energy = energy_fixed_corotated(p0, p1, s);
+                } else {
+                    J = det3(F);
+                    if (!(J > 0.f)) {
+                        // try_elastic_energy_neo_hookean fails: EnergyError, particle FAILED.
+                        flags |= F_FAILED;
+                        atomicOr(&ds->status, MPM_STATUS_ENERGY_ERROR | (KID_G2P << 16));
+                    } else if (LAST) {
+                        energy = energy_neo_hookean(p0, p1, F, J);
+                    }
+                }
+            }
+
+            // cull_particles: strict inequalities.
+            bool inside = true;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) inside = inside && xnew[a] > S.domain_min[a] && xnew[a] < S.domain_max[a];
+            if (!inside) flags |= F_TOMB;
+
+            float Aout[9];
+            if (LAST || !inside) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) Aout[k] = C[k];
+            } else {
+                float tau[9];
+                kirchhoff(flags, p0, p1, hard, F, J, U, s, V, tau);
+                const bool visc = flags & F_VISC;
+                float vd = 0.f, vb = 0.f;
+                if (visc) {
+                    vd = X.visc_dyn[id];
+                    vb = X.visc_bulk[id];
+                }
+                fused_affine(m, vol, scaling, C, tau, J, visc, vd, vb, Aout);
+            }
+            if (LAST) X.energy[id] = energy;
+
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                Pout.x[a][i] = xnew[a];
+                Pout.v[a][i] = v[a];
+            }
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                Pout.F[k][i] = F[k];
+                Pout.A[k][i] = Aout[k];
+            }
+            Pout.m[i] = m;
+            Pout.vol[i] = vol;
+            Pout.p0[i] = p0;
+            Pout.p1[i] = p1;
+            Pout.flags[i] = flags;
+            Pout.id[i] = id;
+        }
+        __syncthreads();
+    }
+}
+
+// Particles that are not in any block (tombstoned, sorted to the end) are
+// copied to their sorted slot unchanged.
+__global__ __launch_bounds__(256) void k_copy_dead(ParticleArrays Pin, ParticleArrays Pout,
+                                                   const uint32_t *__restrict__ order, uint64_t n, DeviceScalars *ds) {
+    const uint64_t live = ds->live_count;
+    for (uint64_t i = live + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t p = order[i];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            Pout.x[a][i] = Pin.x[a][p];
+            Pout.v[a][i] = Pin.v[a][p];
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            Pout.F[k][i] = Pin.F[k][p];
+            Pout.A[k][i] = Pin.A[k][p];
+        }
+        Pout.m[i] = Pin.m[p];
+        Pout.vol[i] = Pin.vol[p];
+        Pout.p0[i] = Pin.p0[p];
+        Pout.p1[i] = Pin.p1[p];
+        Pout.flags[i] = Pin.flags[p];
+        Pout.id[i] = Pin.id[p];
+    }
+}
+
+// to_io_state's reverse permutation (rust/crates/cpu/src/cpu_state.rs:84-93):
+// out[id] = state[i], vectors as xyz triples, matrices column-major.
+__global__ __launch_bounds__(256) void k_unpermute(ParticleArrays P, uint64_t n, float *__restrict__ x,
+                                                   float *__restrict__ v, float *__restrict__ F,
+                                                   float *__restrict__ C, uint32_t *__restrict__ flags) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t o = P.id[i];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            x[3ull * o + a] = P.x[a][i];
+            v[3ull * o + a] = P.v[a][i];
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            F[9ull * o + k] = P.F[k][i];
+            C[9ull * o + k] = P.A[k][i];
+        }
+        flags[o] = P.flags[i];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_stress(const uint32_t *__restrict__ flags,
+                                                const mpm_particle_parameters *__restrict__ params,
+                                                const float *__restrict__ Fin, const float *__restrict__ jp,
+                                                float hardening, float *__restrict__ tau_out,
+                                                float *__restrict__ energy_out, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t f = flags[i];
+    const mpm_particle_parameters pp = params[i];
+    const bool fluid = f & F_FLUID;
+    const float p0 = fluid ? pp.bulk_modulus : pp.mu;
+    const float p1 = fluid ? (float)pp.exponent : pp.lambda;
+    float F[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) F[k] = Fin[9 * i + k];
+    float U[9], s[3] = {1.f, 1.f, 1.f}, V[9];
+    float hard = 1.f;
+    float J = det3(F);
+    float energy;
+    if (needs_svd_for_stress(f)) {
+        svd3(F, U, s, V);
+        if (f & F_SNOW) hard = expf(hardening * (1.f - (jp ? jp[i] : 1.f)));
+        energy = energy_fixed_corotated(p0 * hard, p1 * hard, s);
+    } else if (fluid) {
+        energy = energy_inviscid(p0, (int)p1, J);
+    } else {
+        energy = J > 0.f ? energy_neo_hookean(p0, p1, F, J) : __builtin_nanf("");
+    }
+    float tau[9];
+    kirchhoff(f, p0, p1, hard, F, J, U, s, V, tau);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) tau_out[9 * i + k] = tau[k];
+    if (energy_out) energy_out[i] = energy;
+}
+
+__global__ __launch_bounds__(256) void k_svd(const float *__restrict__ Fin, float *__restrict__ Uo,
+                                             float *__restrict__ so, float *__restrict__ Vo, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float F[9], U[9], s[3], V[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) F[k] = Fin[9 * i + k];
+    svd3(F, U, s, V);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        Uo[9 * i + k] = U[k];
+        Vo[9 * i + k] = V[k];
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) so[3 * i + a] = s[a];
+}
+
+template <int WAVES>
+hipError_t p2g_dispatch(const TransferLaunch &L, bool from_c, bool goals, const ParticleArrays &P, const AuxArrays &X,
+                        const uint32_t *order, const uint32_t *blk_start, const uint32_t *blk_key, const GridGeom &G,
+                        const SubstepParams &S, float4 *grid, uint32_t *blk_flag, DeviceScalars *ds,
+                        hipStream_t stream) {
+    dim3 grd(L.persistent_blocks), blk(64 * WAVES);
+    if (from_c) {
+        if (goals)
+            hipLaunchKernelGGL((k_p2g<WAVES, true, true>), grd, blk, 0, stream, P, X, order, blk_start, blk_key, G, S,
+                               grid, blk_flag, ds);
+        else
+            hipLaunchKernelGGL((k_p2g<WAVES, true, false>), grd, blk, 0, stream, P, X, order, blk_start, blk_key, G, S,
+                               grid, blk_flag, ds);
+    } else {
+        if (goals)
+            hipLaunchKernelGGL((k_p2g<WAVES, false, true>), grd, blk, 0, stream, P, X, order, blk_start, blk_key, G, S,
+                               grid, blk_flag, ds);
+        else
+            hipLaunchKernelGGL((k_p2g<WAVES, false, false>), grd, blk, 0, stream, P, X, order, blk_start, blk_key, G,
+                               S, grid, blk_flag, ds);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_p2g(const TransferLaunch &L, bool from_c, const ParticleArrays &P, const AuxArrays &X,
+                      const uint32_t *order, const uint32_t *blk_start, const uint32_t *blk_key, const GridGeom &G,
+                      const SubstepParams &S, float4 *grid, uint32_t *blk_flag, DeviceScalars *ds, hipStream_t stream) {
+    const bool goals = S.has_goals != 0;
+    switch (L.p2g_waves) {
+        case 1:
+            return p2g_dispatch<1>(L, from_c, goals, P, X, order, blk_start, blk_key, G, S, grid, blk_flag, ds, stream);
+        case 2:
+            return p2g_dispatch<2>(L, from_c, goals, P, X, order, blk_start, blk_key, G, S, grid, blk_flag, ds, stream);
+        default:
+            return p2g_dispatch<4>(L, from_c, goals, P, X, order, blk_start, blk_key, G, S, grid, blk_flag, ds, stream);
+    }
+}
+
+hipError_t launch_grid_update(const TransferLaunch &L, const uint32_t *blk_key, const GridGeom &G,
+                              const SubstepParams &S, float4 *grid, float4 *vgrid, uint32_t *blk_flag,
+                              uint32_t *blk_epoch, DeviceScalars *ds, hipStream_t stream) {
+    hipLaunchKernelGGL(k_grid_update, dim3(L.persistent_blocks * 4), dim3(64), 0, stream, blk_key, G, S, grid, vgrid,
+                       blk_flag, blk_epoch, ds);
+    return hipGetLastError();
+}
+
+hipError_t launch_g2p(const TransferLaunch &L, bool last, const ParticleArrays &Pin, const ParticleArrays &Pout,
+                      const AuxArrays &X, const uint32_t *order, const uint32_t *blk_start, const uint32_t *blk_key,
+                      const GridGeom &G, const SubstepParams &S, const float4 *vgrid, DeviceScalars *ds,
+                      hipStream_t stream) {
+    dim3 grd(L.persistent_blocks), blk(L.g2p_threads);
+    if (last)
+        hipLaunchKernelGGL(k_g2p<true>, grd, blk, 0, stream, Pin, Pout, X, order, blk_start, blk_key, G, S, vgrid, ds);
+    else
+        hipLaunchKernelGGL(k_g2p<false>, grd, blk, 0, stream, Pin, Pout, X, order, blk_start, blk_key, G, S, vgrid, ds);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_dead(const ParticleArrays &Pin, const ParticleArrays &Pout, const uint32_t *order, uint64_t n,
+                            DeviceScalars *ds, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_copy_dead, dim3(grid), dim3(256), 0, stream, Pin, Pout, order, n, ds);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpermute(const ParticleArrays &P, uint64_t n, float *x, float *v, float *F, float *C,
+                            uint32_t *flags, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_unpermute, dim3(grid), dim3(256), 0, stream, P, n, x, v, F, C, flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_stress(const uint32_t *flags, const mpm_particle_parameters *params, const float *F, const float *jp,
+                         float snow_hardening, float *tau, float *energy, uint64_t n, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_stress, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, stream, flags, params, F, jp,
+                       snow_hardening, tau, energy, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_svd(const float *F, float *U, float *s, float *V, uint64_t n, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_svd, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, stream, F, U, s, V, n);
+    return hipGetLastError();
+}
+
+}  // namespace svamd

@@ -1,0 +1,171 @@
+"""GpuState: the MI355X engine behind the reference's backend contract.
+
+`GpuState.from_io_state` and `GpuState.produce_next_state` keep the names,
+argument meaning and error split of rust/crates/gpu/src/gpu_state.rs:38-190
+and :331-675: a fatal error raises (the outer Err), an error after which the
+state is still valid comes back beside the state (the inner Result,
+rust/crates/core/src/compute_thread.rs:165-169).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import abi
+from .io_state import EngineOptions, FrameInput, GridNodes, InputConsts, IoState, Particles, make_consts
+
+
+class GpuError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{message} (code {code})")
+        self.code = code
+
+
+@dataclass
+class GpuRunParameters:
+    """GpuRunParameters, rust/crates/gpu/src/gpu_state.rs:331-334."""
+
+    target_time: float
+    store_grid: bool = False
+
+
+def _check(lib, rc: int):
+    if rc < 0:
+        raise GpuError(rc, lib.mpm_error_string(rc).decode())
+    return rc
+
+
+def available_gpus() -> list[str]:
+    """GpuContext::available_gpus: HIP device ordinals as names."""
+    lib = abi.load_library()
+    return [f"hip:{i}" for i in range(lib.mpm_device_count())]
+
+
+class GpuState:
+    def __init__(self, lib, handle, n: int, consts: abi.MpmConsts):
+        self._lib = lib
+        self._h = handle
+        self.n = n
+        self._consts = consts
+        self.last_status: Optional[abi.MpmStatus] = None
+
+    # rust/crates/gpu/src/gpu_state.rs:38-190
+    @classmethod
+    def from_io_state(
+        cls,
+        gpu: int | str,
+        consts: InputConsts,
+        time_step: float,
+        io_state: IoState,
+        options: EngineOptions | None = None,
+        tuning: dict | None = None,
+    ) -> "GpuState":
+        lib = abi.load_library()
+        device = int(str(gpu).split(":")[-1]) if not isinstance(gpu, int) else gpu
+        c = make_consts(consts, time_step, options, device)
+        parts = io_state.particles.contiguous()
+        view = parts.view()
+        view.time = io_state.time
+        t = abi.MpmTuning()
+        for k, v in (tuning or {}).items():
+            setattr(t, k, int(v))
+        h = C.c_void_p()
+        _check(lib, lib.mpm_create(C.byref(c), C.byref(view), C.byref(t), C.byref(h)))
+        st = cls(lib, h, parts.count, c)
+        st._parameters = parts.parameters.copy()
+        st._initial_positions = parts.initial_positions.copy()
+        st._with_jp = parts.plastic_jp is not None
+        return st
+
+    def close(self):
+        if self._h:
+            self._lib.mpm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def time(self) -> float:
+        return self._lib.mpm_time(self._h)
+
+    # rust/crates/gpu/src/gpu_state.rs:337-675
+    def produce_next_state(
+        self, frame_input: FrameInput, params: GpuRunParameters, cancel: Optional[C.c_int32] = None
+    ) -> tuple[IoState, Optional[GpuError]]:
+        fi, keep = frame_input.to_c()
+        status = abi.MpmStatus()
+        rc = self._lib.mpm_step_to(
+            self._h, params.target_time, C.byref(fi), C.byref(cancel) if cancel is not None else None, C.byref(status)
+        )
+        self.last_status = status
+        _check(self._lib, rc)
+        state = self.to_io_state(params.store_grid)
+        inner = GpuError(rc, self._lib.mpm_error_string(rc).decode()) if rc > 0 else None
+        return state, inner
+
+    def step_to(self, target_time: float, frame_input: FrameInput) -> abi.MpmStatus:
+        """Advance without downloading the state (benchmarks)."""
+        fi, keep = frame_input.to_c()
+        status = abi.MpmStatus()
+        rc = self._lib.mpm_step_to(self._h, target_time, C.byref(fi), None, C.byref(status))
+        self.last_status = status
+        _check(self._lib, rc)
+        return status
+
+    def to_io_state(self, store_grid: bool = False) -> IoState:
+        p = Particles.empty(self.n, with_jp=self._with_jp)
+        view = p.view()
+        _check(self._lib, self._lib.mpm_read_particles(self._h, C.byref(view)))
+        grid = self.read_grid() if store_grid else None
+        return IoState(time=view.time, particles=p, grid_nodes=grid)
+
+    def _read_grid(self, fn) -> GridNodes:
+        cap = 1 << 16
+        while True:
+            ids = np.zeros((cap, 3), np.int32)
+            bits = np.zeros(cap, np.uint32)
+            m = np.zeros(cap, np.float32)
+            v = np.zeros((cap, 3), np.float32)
+            g = abi.MpmGridView()
+            g.capacity = cap
+            g.node_ids, g.collider_bits, g.masses, g.velocities = (abi.ptr(ids), abi.ptr(bits), abi.ptr(m), abi.ptr(v))
+            rc = fn(self._h, C.byref(g))
+            if rc == abi.ERR_CAPACITY:
+                cap = int(g.count)
+                continue
+            _check(self._lib, rc)
+            k = int(g.count)
+            return GridNodes(ids[:k], bits[:k], m[:k], v[:k])
+
+    def read_grid(self) -> GridNodes:
+        return self._read_grid(self._lib.mpm_read_grid)
+
+    def read_grid_momentum(self) -> GridNodes:
+        return self._read_grid(self._lib.mpm_read_grid_momentum)
+
+    def run_stage(self, stage: int, frame_input: FrameInput):
+        fi, keep = frame_input.to_c()
+        _check(self._lib, self._lib.mpm_run_stage(self._h, stage, C.byref(fi)))
+
+    def set_profiling(self, enabled: bool):
+        _check(self._lib, self._lib.mpm_set_profiling(self._h, 1 if enabled else 0))
+
+    def kernel_times(self, reset: bool = True) -> dict:
+        cap = 16
+        names = (C.c_char_p * cap)()
+        ms = (C.c_double * cap)()
+        launches = (C.c_uint64 * cap)()
+        count = C.c_int32()
+        _check(self._lib, self._lib.mpm_kernel_times(self._h, 1 if reset else 0, cap, names, ms, launches, C.byref(count)))
+        return {names[i].decode(): (ms[i], int(launches[i])) for i in range(count.value)}
+
+    @property
+    def stream(self) -> int:
+        return int(self._lib.mpm_stream(self._h) or 0)

@@ -1,0 +1,174 @@
+"""Seeded synthetic scenes for the parity tests and the benchmark.
+
+Every scene is generated from a seed with numpy's PCG64, so the engine and the
+CPU oracle receive byte-identical inputs.  Particles are sampled at 8 per cell
+as a 2x2x2 stratified jitter inside each cell; V0 = h^3 / 8 and m = rho V0
+(initialize_io_state, rust/crates/core/src/initialization.rs:195-262, with
+size = h / 2 per particle).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import abi
+from .io_state import EngineOptions, FrameInput, InputConsts, IoState, Particles, lame_lambda, lame_mu
+
+
+@dataclass
+class Scene:
+    name: str
+    consts: InputConsts
+    time_step: float
+    options: EngineOptions
+    io_state: IoState
+    frame_input: FrameInput
+    substeps: int
+    info: dict = field(default_factory=dict)
+
+
+def advance_time(time: float, dt: float, substeps: int) -> float:
+    """The time after `substeps` steps, accumulated as the substep loop does
+    (f32 dt added to an f64 time, rust/crates/cpu/src/cpu_state.rs:187-190)."""
+    t = float(time)
+    dtf = float(np.float32(dt))
+    for _ in range(substeps):
+        t += dtf
+    return t
+
+
+def jittered_cells(cells: np.ndarray, h: float, rng: np.random.Generator, ppc: int = 8) -> np.ndarray:
+    """Stratified jitter: ppc = 8 points per cell, one per octant."""
+    assert ppc == 8
+    octs = np.array([[i, j, k] for i in range(2) for j in range(2) for k in range(2)], np.float64)
+    base = cells[:, None, :].astype(np.float64) + 0.5 * octs[None, :, :]
+    jit = rng.random((cells.shape[0], 8, 3)) * 0.5
+    return ((base + jit) * h).reshape(-1, 3)
+
+
+def box_cells(lo: np.ndarray, hi: np.ndarray) -> np.ndarray:
+    """Integer cells c with lo <= c < hi (per axis)."""
+    axes = [np.arange(lo[a], hi[a]) for a in range(3)]
+    g = np.stack(np.meshgrid(*axes, indexing="ij"), -1).reshape(-1, 3)
+    return g
+
+
+def sphere_points(center, radius, h, rng) -> np.ndarray:
+    c = np.asarray(center, np.float64)
+    lo = np.floor((c - radius) / h).astype(np.int64)
+    hi = np.ceil((c + radius) / h).astype(np.int64)
+    cells = box_cells(lo, hi)
+    pts = jittered_cells(cells, h, rng)
+    keep = np.sum((pts - c) ** 2, axis=1) < radius * radius
+    return pts[keep]
+
+
+def particles_from(points: np.ndarray, flags: int, rho: float, h: float, velocity, *, E=None, nu=None,
+                   bulk_modulus=None, exponent=7, with_jp=False) -> Particles:
+    n = points.shape[0]
+    p = Particles.empty(n, with_jp=with_jp)
+    p.positions[:] = points.astype(np.float32)
+    p.initial_positions[:] = p.positions
+    p.position_gradients[:] = np.eye(3, dtype=np.float32)
+    p.velocities[:] = np.asarray(velocity, np.float32)
+    p.flags[:] = flags
+    vol = np.float32(h ** 3 / 8.0)
+    p.parameters["initial_volume"] = vol
+    p.parameters["mass"] = np.float32(vol * np.float32(rho))
+    if flags & abi.IS_FLUID:
+        p.parameters["bulk_modulus"] = bulk_modulus
+        p.parameters["exponent"] = exponent
+    else:
+        p.parameters["mu"] = np.float32(lame_mu(np.float32(E), np.float32(nu)))
+        p.parameters["lambda"] = np.float32(lame_lambda(np.float32(E), np.float32(nu)))
+    return p
+
+
+def concat(parts: list[Particles]) -> Particles:
+    out = Particles.empty(0, with_jp=any(p.plastic_jp is not None for p in parts))
+    for name in out.__dataclass_fields__:
+        arrs = [getattr(p, name) for p in parts]
+        if name == "plastic_jp":
+            if out.plastic_jp is None:
+                continue
+            arrs = [a if a is not None else np.ones(p.count, np.float32) for a, p in zip(arrs, parts)]
+        setattr(out, name, np.ascontiguousarray(np.concatenate(arrs)))
+    return out
+
+
+def cube_scene(*, n_cells: int = 16, h: float = 1.0 / 64, corner_cells=(24, 24, 16), E: float = 5e4, nu: float = 0.3,
+               rho: float = 1000.0, velocity=(0.0, 0.0, -2.0), gravity=(0.0, 0.0, -9.81), dt: float = 1e-4,
+               substeps: int = 1000, model: str = "neo_hookean", sticky_plane_z: Optional[float] = None,
+               domain=((0.0, 0.0, 0.0), (1.0, 1.0, 1.0)), fps: int = 24, seed: int = 0) -> Scene:
+    """One elastic cube of n_cells^3 cells at 8 particles per cell."""
+    rng = np.random.default_rng(seed)
+    lo = np.asarray(corner_cells, np.int64)
+    pts = jittered_cells(box_cells(lo, lo + n_cells), h, rng)
+    flags = abi.IS_SOLID | (abi.FIXED_COROTATED if model == "fixed_corotated" else 0)
+    parts = particles_from(pts, flags, rho, h, velocity, E=E, nu=nu)
+    consts = InputConsts(grid_node_size=h, frames_per_second=fps, domain_min=tuple(domain[0]), domain_max=tuple(domain[1]))
+    opts = EngineOptions(sticky_plane_z=sticky_plane_z)
+    return Scene(
+        name=f"cube{n_cells}_{model}",
+        consts=consts,
+        time_step=dt,
+        options=opts,
+        io_state=IoState(time=0.0, particles=parts),
+        frame_input=FrameInput(frame=0, gravity_a=tuple(gravity)),
+        substeps=substeps,
+        info={"particles": parts.count, "h": h},
+    )
+
+
+def plain_cube_parity(**kw) -> Scene:
+    """The reference-expressible form of the 64^3 cube: Neo-Hookean, no plane,
+    a domain large enough that no particle is culled."""
+    kw.setdefault("domain", ((-0.5, -0.5, -0.5), (1.5, 1.5, 1.5)))
+    return cube_scene(model="neo_hookean", sticky_plane_z=None, **kw)
+
+
+def plain_cube_stated(**kw) -> Scene:
+    """The 64^3 cube as stated: fixed-corotated, sticky ground plane at z = 0.1."""
+    return cube_scene(model="fixed_corotated", sticky_plane_z=0.1, **kw)
+
+
+def spheres_and_slab(*, h: float = 1.0 / 256, target_particles: int = 4_200_000, seed: int = 0,
+                     dt: float = 5e-5, substeps: int = 500, fps: int = 1) -> Scene:
+    """Three Neo-Hookean spheres (r = 0.08, 0.115, 0.15; E log-uniform in
+    [1e4, 1e6]; nu = 0.3) above a weakly compressible fluid slab (K = 1e5),
+    in a unit box with separating-slip walls, 8 particles per cell."""
+    rng = np.random.default_rng(seed)
+    spheres = [((0.30, 0.30, 0.45), 0.15), ((0.70, 0.35, 0.40), 0.115), ((0.50, 0.72, 0.50), 0.08)]
+    Es = np.exp(rng.uniform(np.log(1e4), np.log(1e6), size=3))
+    parts = []
+    for (c, r), E in zip(spheres, Es):
+        pts = sphere_points(c, r, h, rng)
+        parts.append(particles_from(pts, abi.IS_SOLID, 1000.0, h, (0.0, 0.0, -1.0), E=float(E), nu=0.3))
+    n_spheres = sum(p.count for p in parts)
+    wall = 4 * h
+    # A slab three cells thick, centred in x and y, whose side is chosen so
+    # that the scene holds about target_particles.
+    layers = 3
+    side = int(round(np.sqrt(max(target_particles - n_spheres, 8 * layers) / (8 * layers))))
+    side = min(side, int(np.floor(0.92 / h)))
+    lo_xy = int(round(0.5 / h)) - side // 2
+    hi_xy = lo_xy + side
+    z0 = int(np.ceil(wall / h)) + 1
+    cells = box_cells(np.array([lo_xy, lo_xy, z0]), np.array([hi_xy, hi_xy, z0 + layers]))
+    pts = jittered_cells(cells, h, rng)
+    parts.append(particles_from(pts, abi.IS_FLUID, 1000.0, h, (0.0, 0.0, 0.0), bulk_modulus=1e5, exponent=7))
+    allp = concat(parts)
+    consts = InputConsts(grid_node_size=h, frames_per_second=fps, domain_min=(0.0, 0.0, 0.0), domain_max=(1.0, 1.0, 1.0))
+    opts = EngineOptions(box_walls=((wall, wall, wall), (1.0 - wall, 1.0 - wall, 1.0 - wall)))
+    return Scene(
+        name="spheres_and_slab",
+        consts=consts,
+        time_step=dt,
+        options=opts,
+        io_state=IoState(time=0.0, particles=allp),
+        frame_input=FrameInput(frame=0, gravity_a=(0.0, 0.0, -9.81)),
+        substeps=substeps,
+        info={"particles": allp.count, "sphere_particles": n_spheres, "slab_layers": layers, "E": Es.tolist(), "h": h},
+    )

@@ -1,0 +1,185 @@
+"""Per-kernel parity on the GPU, through the C ABI's mpm_dev_* entry points.
+
+Sort: bit-exact against a stable CPU sort, the reference's criterion
+(rust/crates/gpu/src/radix_sort/test.rs:12-53: assert_eq! against sort_on_cpu).
+Scan: exact against a sequential scan (prefix_sum_on_cpu, rust/crates/gpu/src/util.rs:54-63).
+Stress and SVD: against the FP64 oracle and numpy's SVD.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle
+from squishy_volumes_amd import abi
+from tests.helpers import random_position_gradients, reference_fixtures
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def dev(a, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    return t.to("cuda")
+
+
+def run_sort(lib, keys: np.ndarray, bits: int, values: np.ndarray | None = None):
+    n = keys.size
+    k_in = dev(keys.astype(np.uint32).view(np.int32))
+    v_in = dev(values.astype(np.uint32).view(np.int32)) if values is not None else None
+    k_out = torch.empty(max(n, 1), dtype=torch.int32, device="cuda")
+    v_out = torch.empty(max(n, 1), dtype=torch.int32, device="cuda")
+    temp = torch.empty(max(int(lib.mpm_dev_radix_sort_temp_bytes(n)), 16), dtype=torch.uint8, device="cuda")
+    rc = lib.mpm_dev_radix_sort(abi.ptr(k_in), abi.ptr(v_in), abi.ptr(k_out), abi.ptr(v_out), n, bits, abi.ptr(temp), None)
+    assert rc == 0
+    torch.cuda.synchronize()
+    return k_out[:n].cpu().numpy().view(np.uint32), v_out[:n].cpu().numpy().view(np.uint32)
+
+
+def stable_reference(keys, bits, values=None):
+    mask = np.uint64((1 << bits) - 1) if bits < 32 else np.uint64(0xFFFFFFFF)
+    order = np.argsort(keys.astype(np.uint64) & mask, kind="stable")
+    vals = order.astype(np.uint32) if values is None else values[order]
+    return keys[order], vals
+
+
+@pytest.mark.parametrize("case", ["simple", "specific"])
+def test_radix_sort_reference_cases(engine_lib, case):
+    fx = reference_fixtures()
+    keys = np.array(fx["radix_sort_simple_keys"] if case == "simple" else fx["radix_sort_specific_keys"], np.uint32)
+    bits = 2 if case == "simple" else 8
+    k, v = run_sort(engine_lib, keys, bits)
+    rk, rv = stable_reference(keys, bits)
+    assert np.array_equal(v, rv) and np.array_equal(k, rk)
+
+
+@pytest.mark.parametrize(
+    "n,bits,kind",
+    [
+        (0, 8, "random"),
+        (1, 8, "random"),
+        (1000, 32, "random"),
+        (4095, 12, "random"),
+        (4097, 21, "random"),
+        (100_003, 18, "random"),
+        (200_000, 21, "nearly_sorted"),
+        (50_000, 21, "equal"),
+        (4_200_000, 21, "nearly_sorted"),
+        (300_000, 30, "random"),
+    ],
+)
+def test_radix_sort_is_stable_and_exact(engine_lib, n, bits, kind):
+    rng = np.random.default_rng(n + bits)
+    if kind == "random":
+        keys = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+    elif kind == "equal":
+        keys = np.full(n, 12345, np.uint32)
+    else:
+        keys = np.sort(rng.integers(0, 2**bits, n, dtype=np.uint64)).astype(np.uint32)
+        swaps = rng.integers(0, max(n - 1, 1), n // 50)
+        keys[swaps], keys[swaps + 1] = keys[swaps + 1].copy(), keys[swaps].copy()
+    values = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+    for vals in (None, values):
+        k, v = run_sort(engine_lib, keys, bits, vals)
+        rk, rv = stable_reference(keys, bits, vals)
+        assert np.array_equal(v, rv)
+        assert np.array_equal(k, rk)
+
+
+def test_radix_sort_rejects_aliasing(engine_lib):
+    k = torch.zeros(16, dtype=torch.int32, device="cuda")
+    temp = torch.empty(int(engine_lib.mpm_dev_radix_sort_temp_bytes(16)), dtype=torch.uint8, device="cuda")
+    assert engine_lib.mpm_dev_radix_sort(abi.ptr(k), None, abi.ptr(k), abi.ptr(k), 16, 8, abi.ptr(temp), None) == abi.ERR_INVALID_ARGUMENT
+
+
+@pytest.mark.parametrize("n", [0, 1, 4096, 4097, 1_000_003])
+def test_exclusive_scan(engine_lib, n):
+    rng = np.random.default_rng(n)
+    a = rng.integers(0, 1000, n).astype(np.uint32)
+    d_in = dev(a.view(np.int32)) if n else torch.zeros(1, dtype=torch.int32, device="cuda")
+    d_out = torch.empty(max(n, 1), dtype=torch.int32, device="cuda")
+    total = torch.zeros(1, dtype=torch.int32, device="cuda")
+    temp = torch.empty(int(engine_lib.mpm_dev_scan_temp_bytes(n)), dtype=torch.uint8, device="cuda")
+    assert engine_lib.mpm_dev_exclusive_scan(abi.ptr(d_in), abi.ptr(d_out), abi.ptr(total), n, abi.ptr(temp), None) == 0
+    torch.cuda.synchronize()
+    out = d_out[:n].cpu().numpy().view(np.uint32)
+    ref = (np.cumsum(a.astype(np.uint64)) - a).astype(np.uint32)
+    assert np.array_equal(out, ref)
+    assert int(total.cpu().numpy().view(np.uint32)[0]) == int(a.astype(np.uint64).sum() & 0xFFFFFFFF)
+
+
+def test_copy_kernel(engine_lib):
+    src = torch.randn(1 << 20, device="cuda")
+    dst = torch.zeros_like(src)
+    assert engine_lib.mpm_dev_copy(abi.ptr(src), abi.ptr(dst), src.numel() * 4, None) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(src, dst)
+
+
+def _stress_case(flags_value, n=600, **params):
+    F = np.concatenate([random_position_gradients(n - 9, seed=21),
+                        np.array(reference_fixtures()["position_gradients_simple_row_major"], np.float64)])
+    F = (0.6 * np.eye(3) + 0.4 * F) if flags_value & abi.IS_FLUID else F
+    flags = np.full(len(F), flags_value, np.uint32)
+    p = np.zeros(len(F), abi.PARAMS_DTYPE)
+    for k, v in params.items():
+        p[k] = v
+    return F, flags, p
+
+
+@pytest.mark.parametrize(
+    "flags_value,params",
+    [
+        (abi.IS_SOLID, dict(mu=3846.1538, **{"lambda": 5769.2307})),
+        (abi.IS_SOLID | abi.FIXED_COROTATED, dict(mu=3846.1538, **{"lambda": 5769.2307})),
+        (abi.IS_SOLID | abi.SNOW, dict(mu=3846.1538, **{"lambda": 5769.2307})),
+        (abi.IS_FLUID, dict(bulk_modulus=1000.0, exponent=7)),
+    ],
+)
+def test_stress_matches_fp64_oracle(engine_lib, flags_value, params):
+    F, flags, p = _stress_case(flags_value, **params)
+    n = len(F)
+    jp = np.linspace(0.9, 1.1, n).astype(np.float32)
+    F32 = F.astype(np.float32)
+    Fc = np.ascontiguousarray(np.transpose(F32, (0, 2, 1)))
+    d_F, d_flags, d_p, d_jp = dev(Fc), dev(flags.view(np.int32)), dev(p.view(np.uint8)), dev(jp)
+    d_tau = torch.zeros((n, 3, 3), device="cuda")
+    d_e = torch.zeros(n, device="cuda")
+    assert engine_lib.mpm_dev_stress(abi.ptr(d_flags), abi.ptr(d_p), abi.ptr(d_F), abi.ptr(d_jp), 10.0, abi.ptr(d_tau), abi.ptr(d_e), n, None) == 0
+    torch.cuda.synchronize()
+    tau = np.transpose(d_tau.cpu().numpy(), (0, 2, 1)).astype(np.float64)
+    energy = d_e.cpu().numpy().astype(np.float64)
+    rtau, _, re = pyoracle.stress(F32.astype(np.float64), flags, p, jp.astype(np.float64), hardening=10.0)
+    # Per particle: max |tau_gpu - tau_fp64| <= 1e-4 * max |tau_fp64| (float32 arithmetic).
+    scale = np.abs(rtau).reshape(n, -1).max(1) + 1e-3
+    err = np.abs(tau - rtau).reshape(n, -1).max(1)
+    assert np.all(err <= 1e-4 * scale), (err / scale).max()
+    assert np.allclose(energy, re, rtol=1e-3, atol=1e-3 * np.abs(re).max())
+
+
+def test_svd_against_numpy(engine_lib):
+    F = np.concatenate([
+        random_position_gradients(2000, seed=31),
+        -random_position_gradients(100, seed=32),
+        np.einsum("nij,jk->nik", random_position_gradients(100, seed=33), np.diag([1.0, 1.0, 1e-4])),  # FP64 path
+        np.repeat(np.eye(3)[None], 10, 0),
+    ]).astype(np.float32)
+    n = len(F)
+    Fc = np.ascontiguousarray(np.transpose(F, (0, 2, 1)))
+    d_F = dev(Fc)
+    dU, dV = torch.zeros((n, 3, 3), device="cuda"), torch.zeros((n, 3, 3), device="cuda")
+    ds = torch.zeros((n, 3), device="cuda")
+    assert engine_lib.mpm_dev_svd(abi.ptr(d_F), abi.ptr(dU), abi.ptr(ds), abi.ptr(dV), n, None) == 0
+    torch.cuda.synchronize()
+    U = np.transpose(dU.cpu().numpy(), (0, 2, 1)).astype(np.float64)
+    V = np.transpose(dV.cpu().numpy(), (0, 2, 1)).astype(np.float64)
+    s = ds.cpu().numpy().astype(np.float64)
+    F64 = F.astype(np.float64)
+    ref = np.linalg.svd(F64, compute_uv=False)
+    assert np.allclose(np.abs(s), ref, rtol=1e-4, atol=2e-6 * ref[:, :1])
+    assert np.all(np.sign(s[:, 2]) == np.sign(np.linalg.det(F64)))
+    rec = U @ (s[:, :, None] * np.transpose(V, (0, 2, 1)))
+    assert np.abs(rec - F64).max() < 2e-5 * np.abs(F64).max()
+    eye = np.eye(3)[None]
+    assert np.abs(U @ np.transpose(U, (0, 2, 1)) - eye).max() < 1e-5
+    assert np.abs(V @ np.transpose(V, (0, 2, 1)) - eye).max() < 1e-5
