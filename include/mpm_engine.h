@@ -56,7 +56,8 @@ enum {
     MPM_STATUS_INDIRECT_LIMIT_EXCEEDED = 4u,   /* a block held more particles than a work list can */
     MPM_STATUS_PARTICLE_CLOSE_TO_INVERTED = 8u,/* det F < 1e-7 before the stress evaluation */
     MPM_STATUS_ENERGY_ERROR = 16u,             /* det F <= 0 after the F update (CPU EnergyError) */
-    MPM_STATUS_OUT_OF_GRID = 32u               /* a live particle's stencil left the allocated grid */
+    MPM_STATUS_OUT_OF_GRID = 32u,              /* a live particle's stencil left the allocated grid */
+    MPM_STATUS_SVD_FP64 = 64u                  /* information: a near-degenerate F took the FP64 SVD */
 };
 
 /* ---- result codes -------------------------------------------------------- */
@@ -170,7 +171,7 @@ typedef struct mpm_status {
     double time;                         /* simulation time after the call */
     uint64_t active_blocks;              /* occupied 4^3 blocks in the last substep */
     uint64_t active_nodes;               /* grid nodes with mass > 0 in the last substep */
-    uint64_t svd_fp64_fallbacks;         /* SVDs recomputed in FP64 during the call */
+    uint64_t svd_fp64_fallbacks;         /* 1 when an SVD was recomputed in FP64 during the call */
 } mpm_status;
 
 /* Grid nodes, as GridNodes (rust/crates/file_frame/src/grid_nodes.rs:9-15):

@@ -31,6 +31,7 @@ STATUS_INDIRECT_LIMIT_EXCEEDED = 4
 STATUS_PARTICLE_CLOSE_TO_INVERTED = 8
 STATUS_ENERGY_ERROR = 16
 STATUS_OUT_OF_GRID = 32
+STATUS_SVD_FP64 = 64
 
 OPT_GRID_GRAVITY = 1 << 0
 OPT_STICKY_PLANE = 1 << 1

@@ -257,6 +257,11 @@ SV_DEV void recompose(const float U[9], const float d[3], const float V[9], floa
 
 // ------------------------------------------------------- constitutive models
 // Material selector carried in the particle flags.
+// Model sets the substep kernels are specialised on.
+enum : uint32_t { M_NH = 1, M_FLUID = 2, M_FCR = 4, M_SNOW = 8, M_SAND = 16, M_VISC = 32 };
+constexpr uint32_t SET_POLAR = M_NH | M_FLUID | M_FCR | M_VISC;  // no SVD in the hot path
+constexpr uint32_t SET_ALL = M_NH | M_FLUID | M_FCR | M_SNOW | M_SAND | M_VISC;
+
 enum : uint32_t {
     F_SOLID = 1u << 0,
     F_FLUID = 1u << 1,
@@ -291,16 +296,73 @@ SV_DEV void tau_inviscid(float K, int exponent, float J, float tau[9]) {
     tau[0] = tau[4] = tau[8] = p;
 }
 
+// Polar factor R of F (det F > 0) by the scaled Newton iteration
+// X <- (g X + X^-T / g) / 2, g = det(X)^(-1/3) (Higham, "Computing the polar
+// decomposition with applications", SIAM J. Sci. Stat. Comput. 7, 1986).
+// X^-T = cof(X) / det(X) with the cofactor columns of partial_invariant_3.
+// Returns false when det F <= 0 or the iteration has not converged.
+SV_DEV bool polar3(const float F[9], float R[9]) {
+    float X[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) X[k] = F[k];
+    bool ok = false;
+#pragma unroll 1
+    for (int it = 0; it < 10; ++it) {
+        float cof[9];
+        // columns: x1 x x2, x2 x x0, x0 x x1
+        cof[0] = X[4] * X[8] - X[5] * X[7];
+        cof[1] = X[5] * X[6] - X[3] * X[8];
+        cof[2] = X[3] * X[7] - X[4] * X[6];
+        cof[3] = X[7] * X[2] - X[8] * X[1];
+        cof[4] = X[8] * X[0] - X[6] * X[2];
+        cof[5] = X[6] * X[1] - X[7] * X[0];
+        cof[6] = X[1] * X[5] - X[2] * X[4];
+        cof[7] = X[2] * X[3] - X[0] * X[5];
+        cof[8] = X[0] * X[4] - X[1] * X[3];
+        const float det = X[0] * cof[0] + X[1] * cof[1] + X[2] * cof[2];
+        if (!(det > 0.f)) return false;
+        const float g = 1.0f / cbrtf(det);
+        const float a = 0.5f * g, bb = 0.5f / (g * det);
+        float delta = 0.f;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const float nx = a * X[k] + bb * cof[k];
+            delta = fmaxf(delta, fabsf(nx - X[k]));
+            X[k] = nx;
+        }
+        if (delta < 1e-6f) {
+            ok = true;
+            break;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) R[k] = X[k];
+    return ok;
+}
+
 // Fixed-corotated (no reference counterpart): tau = 2 mu (F - R) F^T + lambda (J - 1) J I.
-SV_DEV void tau_fixed_corotated(float mu, float lambda, const float F[9], const float U[9], const float s[3],
-                                const float V[9], float tau[9]) {
-    // (F - R) F^T = U diag(s) V^T V diag(s) U^T - U V^T V diag(s) U^T = U diag(s^2 - s) U^T.
+// From the SVD: (F - R) F^T = U diag(s^2 - s) U^T.
+SV_DEV void tau_fixed_corotated_svd(float mu, float lambda, const float U[9], const float s[3], float tau[9]) {
     float d[3] = {s[0] * s[0] - s[0], s[1] * s[1] - s[1], s[2] * s[2] - s[2]};
     recompose(U, d, U, tau);
     float J = s[0] * s[1] * s[2];
     float vol = lambda * (J - 1.0f) * J;
 #pragma unroll
     for (int k = 0; k < 9; ++k) tau[k] *= 2.0f * mu;
+    tau[0] += vol;
+    tau[4] += vol;
+    tau[8] += vol;
+}
+
+// The same from the polar factor: (F - R) F^T = F F^T - R F^T.
+SV_DEV void tau_fixed_corotated_polar(float mu, float lambda, const float F[9], const float R[9], float J,
+                                      float tau[9]) {
+    float FFt[9], RFt[9];
+    matmul3_bt(F, F, FFt);
+    matmul3_bt(R, F, RFt);
+    const float vol = lambda * (J - 1.0f) * J;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) tau[k] = 2.0f * mu * (FFt[k] - RFt[k]);
     tau[0] += vol;
     tau[4] += vol;
     tau[8] += vol;
@@ -316,6 +378,12 @@ SV_DEV float energy_neo_hookean(float mu, float lambda, const float F[9], float 
 SV_DEV float energy_inviscid(float K, int exponent, float J) {
     float at_rest = K * (1.f - 1.f / (1.f - (float)exponent));
     return K * (J - powi_dev(J, 1 - exponent) / (1.f - (float)exponent)) - at_rest;
+}
+SV_DEV float energy_fixed_corotated_polar(float mu, float lambda, const float F[9], const float R[9], float J) {
+    float d = 0.f;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) d += (F[k] - R[k]) * (F[k] - R[k]);
+    return mu * d + lambda / 2.f * (J - 1.f) * (J - 1.f);
 }
 SV_DEV float energy_fixed_corotated(float mu, float lambda, const float s[3]) {
     float J = s[0] * s[1] * s[2];

@@ -62,6 +62,9 @@ struct mpm_engine {
     void *blk_temp = nullptr;
     float4 *grid = nullptr, *vgrid = nullptr;
     uint32_t *blk_flag = nullptr, *blk_epoch = nullptr;
+    uint32_t *cand = nullptr;     // 8 candidate touched blocks per occupied block
+    uint32_t *partial = nullptr;  // per-wave active-node counts of the grid update
+    bool keys_valid = false;      // e->keys holds the keys of the current particle order
     DeviceScalars *ds = nullptr;
     DeviceScalars *ds_host = nullptr;  // pinned
 
@@ -270,9 +273,11 @@ int run_sort(mpm_engine *e) {
     EventRecord r{};
     ParticleArrays &P = e->P[e->cur];
     HIP_TRY(launch_reset_scalars(e->ds, e->stream));
-    e->begin(ST_KEYS, r);
-    HIP_TRY(launch_keys(P, e->geom, e->n, e->keys, e->ds, e->stream));
-    e->end(r);
+    if (!e->keys_valid) {
+        e->begin(ST_KEYS, r);
+        HIP_TRY(launch_keys(P, e->geom, e->n, e->keys, e->ds, e->stream));
+        e->end(r);
+    }
     e->begin(ST_SORT, r);
     HIP_TRY(radix_sort(e->keys, nullptr, e->keys_sorted, e->order, nullptr, nullptr, e->n, e->key_bits, e->sort_temp,
                        e->stream));
@@ -288,7 +293,7 @@ int run_p2g(mpm_engine *e, const SubstepParams &S) {
     EventRecord r{};
     e->begin(ST_P2G, r);
     HIP_TRY(launch_p2g(e->launch, e->next_from_c, e->P[e->cur], e->X, e->order, e->blk_start, e->blk_key, e->geom, S,
-                       e->grid, e->blk_flag, e->ds, e->stream));
+                       e->grid, e->blk_flag, e->cand, e->ds, e->stream));
     e->end(r);
     e->next_from_c = false;
     return MPM_OK;
@@ -297,8 +302,8 @@ int run_p2g(mpm_engine *e, const SubstepParams &S) {
 int run_grid(mpm_engine *e, const SubstepParams &S) {
     EventRecord r{};
     e->begin(ST_GRID, r);
-    HIP_TRY(launch_grid_update(e->launch, e->blk_key, e->geom, S, e->grid, e->vgrid, e->blk_flag, e->blk_epoch, e->ds,
-                               e->stream));
+    HIP_TRY(launch_grid_update(e->launch, e->cand, e->geom, S, e->grid, e->vgrid, e->blk_flag, e->blk_epoch,
+                               e->partial, e->ds, e->stream));
     e->end(r);
     e->grid_epoch = S.epoch;
     e->grid_valid = true;
@@ -311,11 +316,12 @@ int run_g2p(mpm_engine *e, const SubstepParams &S, bool last) {
     ParticleArrays &Pout = e->P[1 - e->cur];
     e->begin(ST_G2P, r);
     HIP_TRY(launch_g2p(e->launch, last, Pin, Pout, e->X, e->order, e->blk_start, e->blk_key, e->geom, S, e->vgrid,
-                       e->ds, e->stream));
+                       e->keys, e->ds, e->stream));
     e->end(r);
     e->begin(ST_COPY, r);
-    HIP_TRY(launch_copy_dead(Pin, Pout, e->order, e->n, e->ds, e->stream));
+    HIP_TRY(launch_copy_dead(Pin, Pout, e->order, e->n, e->keys, e->geom.tomb_key, e->ds, e->stream));
     e->end(r);
+    e->keys_valid = true;
     e->cur = 1 - e->cur;
     if (last) e->next_from_c = true;
     e->epoch += 1;
@@ -323,6 +329,7 @@ int run_g2p(mpm_engine *e, const SubstepParams &S, bool last) {
 }
 
 int fetch_scalars(mpm_engine *e) {
+    if (e->grid_valid) HIP_TRY(launch_sum_partials(e->launch, e->partial, e->ds, e->stream));
     HIP_TRY(hipMemcpyAsync(e->ds_host, e->ds, sizeof(DeviceScalars), hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     return MPM_OK;
@@ -394,16 +401,21 @@ int mpm_create(const mpm_consts *consts, const mpm_particles_view *pv, const mpm
 
     hipDeviceProp_t prop{};
     if (hipGetDeviceProperties(&prop, e->device) != hipSuccess) return fail(MPM_ERR_HIP);
-    e->launch.persistent_blocks = prop.multiProcessorCount * 8;
-    e->launch.p2g_waves = 1;
+    e->launch.persistent_blocks = prop.multiProcessorCount * 4;
     e->launch.g2p_threads = 256;
     if (tuning) {
         if (tuning->persistent_blocks > 0) e->launch.persistent_blocks = tuning->persistent_blocks;
-        if (tuning->p2g_waves_per_block == 1 || tuning->p2g_waves_per_block == 2 || tuning->p2g_waves_per_block == 4)
-            e->launch.p2g_waves = tuning->p2g_waves_per_block;
         if (tuning->g2p_threads >= 64 && tuning->g2p_threads <= 1024 && tuning->g2p_threads % 64 == 0)
             e->launch.g2p_threads = tuning->g2p_threads;
     }
+    // Kernels are specialised on the constitutive models present: snow and
+    // sand need the full SVD, every other model a polar decomposition at most.
+    e->launch.model_set = SET_POLAR;
+    for (uint64_t i = 0; i < pv->count; ++i)
+        if (pv->flags[i] & (MPM_SNOW | MPM_USE_SAND_ALPHA)) {
+            e->launch.model_set = SET_ALL;
+            break;
+        }
     // The G2P kernel is compiled for at most 256 threads per work-group.
     e->launch.g2p_threads = std::min(e->launch.g2p_threads, 256);
 
@@ -421,7 +433,8 @@ int mpm_create(const mpm_consts *consts, const mpm_particles_view *pv, const mpm
         e->alloc(st, radix_sort_temp_bytes(n)) || e->alloc(e->blk_start, max_blocks) ||
         e->alloc(e->blk_key, max_blocks) || e->alloc(bt, block_list_temp_bytes(n)) || e->alloc(e->grid, nodes) ||
         e->alloc(e->vgrid, nodes) || e->alloc(e->blk_flag, e->geom.num_blocks_total) ||
-        e->alloc(e->blk_epoch, e->geom.num_blocks_total) || e->alloc(e->ds, 1))
+        e->alloc(e->blk_epoch, e->geom.num_blocks_total) || e->alloc(e->ds, 1) ||
+        e->alloc(e->cand, 8 * max_blocks) || e->alloc(e->partial, grid_update_waves(e->launch)))
         return fail(MPM_ERR_OUT_OF_MEMORY);
     e->sort_temp = st;
     e->blk_temp = bt;
@@ -518,7 +531,6 @@ int mpm_step_to(mpm_engine *e, double target_time, const mpm_frame_input *input,
         params.push_back(make_params(e, g, fb, goals, input && input->has_b));
     }
     HIP_TRY(hipMemsetAsync(&e->ds->status, 0, 4, e->stream));
-    HIP_TRY(hipMemsetAsync(&e->ds->svd_fp64, 0, 4, e->stream));
 
     uint64_t done = 0;
     int result = MPM_OK;
@@ -553,7 +565,7 @@ int mpm_step_to(mpm_engine *e, double target_time, const mpm_frame_input *input,
         status->time = e->time;
         status->active_blocks = e->ds_host->num_blocks;
         status->active_nodes = e->ds_host->active_nodes;
-        status->svd_fp64_fallbacks = e->ds_host->svd_fp64;
+        status->svd_fp64_fallbacks = (e->ds_host->status & MPM_STATUS_SVD_FP64) ? 1 : 0;
     }
     return result;
 }

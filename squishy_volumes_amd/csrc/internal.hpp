@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include "../../include/mpm_engine.h"
+#include "device_math.hpp"
 
 namespace svamd {
 
@@ -58,10 +59,33 @@ struct DeviceScalars {
     uint32_t num_blocks;    // occupied blocks this substep
     uint32_t live_count;    // particles with a valid key (sorted to the front)
     uint32_t status;        // MPM_STATUS_* bits | kernel id << 16
-    uint32_t active_nodes;  // nodes with m > 0 after the grid update
-    uint32_t svd_fp64;      // FP64 SVD fallbacks taken
-    uint32_t pad[3];
+    uint32_t active_nodes;  // nodes with m > 0 after the last grid update (summed on request)
+    uint32_t pad[4];
 };
+
+// Block key of a position: Morton code of the 4^3 block holding the base
+// cell floor(x/h - 1/2) (position_to_shift_quadratic,
+// rust/crates/cpu/src/kernels.rs:46-49), or the tombstone key (and an
+// OUT_OF_GRID status) when the block is outside the allocated grid.
+__device__ __forceinline__ uint32_t particle_block_key(const float x[3], const GridGeom &G, DeviceScalars *ds) {
+    uint32_t b[3];
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float xn = x[a] / G.h - 0.5f;
+        ok = ok && isfinite(xn);
+        const float fl = floorf(xn);
+        ok = ok && fl > -1.0e9f && fl < 1.0e9f;
+        const int c = (int)fl - G.origin[a];
+        ok = ok && c >= 0 && (c >> 2) <= G.nb[a] - 2;
+        b[a] = (uint32_t)(c >> 2);
+    }
+    if (!ok) {
+        atomicOr(&ds->status, MPM_STATUS_OUT_OF_GRID | (1u << 16));
+        return G.tomb_key;
+    }
+    return morton3(b[0], b[1], b[2]);
+}
 
 struct SubstepParams {
     float dt;
@@ -109,22 +133,25 @@ hipError_t launch_reset_scalars(DeviceScalars *ds, hipStream_t stream);
 // ---- transfer.hip
 struct TransferLaunch {
     int persistent_blocks;
-    int p2g_waves;
     int g2p_threads;
+    uint32_t model_set;  // SET_POLAR or SET_ALL
 };
 hipError_t launch_p2g(const TransferLaunch &L, bool from_c, const ParticleArrays &P, const AuxArrays &X,
                       const uint32_t *order, const uint32_t *blk_start, const uint32_t *blk_key,
-                      const GridGeom &G, const SubstepParams &S, float4 *grid, uint32_t *blk_flag,
+                      const GridGeom &G, const SubstepParams &S, float4 *grid, uint32_t *blk_flag, uint32_t *cand,
                       DeviceScalars *ds, hipStream_t stream);
-hipError_t launch_grid_update(const TransferLaunch &L, const uint32_t *blk_key, const GridGeom &G,
+uint32_t grid_update_waves(const TransferLaunch &L);
+hipError_t launch_grid_update(const TransferLaunch &L, const uint32_t *cand, const GridGeom &G,
                               const SubstepParams &S, float4 *grid, float4 *vgrid, uint32_t *blk_flag,
-                              uint32_t *blk_epoch, DeviceScalars *ds, hipStream_t stream);
+                              uint32_t *blk_epoch, uint32_t *partial, DeviceScalars *ds, hipStream_t stream);
+hipError_t launch_sum_partials(const TransferLaunch &L, const uint32_t *partial, DeviceScalars *ds,
+                               hipStream_t stream);
 hipError_t launch_g2p(const TransferLaunch &L, bool last, const ParticleArrays &Pin, const ParticleArrays &Pout,
                       const AuxArrays &X, const uint32_t *order, const uint32_t *blk_start, const uint32_t *blk_key,
-                      const GridGeom &G, const SubstepParams &S, const float4 *vgrid, DeviceScalars *ds,
-                      hipStream_t stream);
+                      const GridGeom &G, const SubstepParams &S, const float4 *vgrid, uint32_t *keys_out,
+                      DeviceScalars *ds, hipStream_t stream);
 hipError_t launch_copy_dead(const ParticleArrays &Pin, const ParticleArrays &Pout, const uint32_t *order,
-                            uint64_t n, DeviceScalars *ds, hipStream_t stream);
+                            uint64_t n, uint32_t *keys_out, uint32_t tomb, DeviceScalars *ds, hipStream_t stream);
 hipError_t launch_unpermute(const ParticleArrays &P, uint64_t n, float *x, float *v, float *F, float *C,
                             uint32_t *flags, hipStream_t stream);
 hipError_t launch_stress(const uint32_t *flags, const mpm_particle_parameters *params, const float *F,

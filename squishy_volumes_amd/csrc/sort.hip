@@ -226,47 +226,33 @@ __global__ __launch_bounds__(256) void k_keys(ParticleArrays P, GridGeom G, uint
         keys[i] = G.tomb_key;
         return;
     }
-    uint32_t b[3];
-    bool ok = true;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        float xn = P.x[a][i] / G.h - 0.5f;
-        ok = ok && isfinite(xn);
-        float fl = floorf(xn);
-        ok = ok && fl > -1.0e9f && fl < 1.0e9f;
-        int c = (int)fl - G.origin[a];
-        ok = ok && c >= 0 && (c >> 2) <= G.nb[a] - 2;
-        b[a] = (uint32_t)(c >> 2);
-    }
-    if (!ok) {
-        atomicOr(&ds->status, MPM_STATUS_OUT_OF_GRID | (KID_KEYS << 16));
-        keys[i] = G.tomb_key;
-        return;
-    }
-    keys[i] = morton3(b[0], b[1], b[2]);
+    const float x[3] = {P.x[0][i], P.x[1][i], P.x[2][i]};
+    keys[i] = particle_block_key(x, G, ds);
 }
 
 // ------------------------------------------------------------ block list
-// Tiles of 4096 sorted keys, 16 consecutive keys per thread.  A head is the
-// first key of a run of equal valid keys; the heads, in key order, are the
-// occupied blocks.
+// Tiles of 4096 sorted keys read in 16 coalesced rounds of 256.  A head is
+// the first key of a run of equal valid keys; the heads, in key order, are
+// the occupied blocks.
+__device__ __forceinline__ bool is_head(const uint32_t *__restrict__ keys, uint64_t i, uint64_t n, uint32_t tomb,
+                                        uint32_t &key) {
+    if (i >= n) return false;
+    key = keys[i];
+    return key != tomb && (i == 0 || keys[i - 1] != key);
+}
+
 __global__ __launch_bounds__(SCAN_THREADS) void k_heads_count(const uint32_t *__restrict__ keys, uint64_t n,
                                                                uint32_t tomb, uint32_t *__restrict__ tile_counts,
                                                                DeviceScalars *ds) {
     __shared__ uint32_t ws[SCAN_THREADS / 64 + 1];
-    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + threadIdx.x;
     uint32_t cnt = 0;
-    uint32_t prev = base > 0 && base - 1 < n ? keys[base - 1] : tomb ^ 1u;
-#pragma unroll
+#pragma unroll 4
     for (int k = 0; k < SCAN_ITEMS; ++k) {
-        uint64_t i = base + k;
-        if (i < n) {
-            uint32_t key = keys[i];
-            bool valid = key != tomb;
-            cnt += (valid && (i == 0 || prev != key)) ? 1u : 0u;
-            if (valid && (i + 1 == n || keys[i + 1] == tomb)) ds->live_count = (uint32_t)(i + 1);
-            prev = key;
-        }
+        const uint64_t i = base + (uint64_t)k * SCAN_THREADS;
+        uint32_t key = tomb;
+        cnt += is_head(keys, i, n, tomb, key) ? 1u : 0u;
+        if (i < n && key != tomb && (i + 1 == n || keys[i + 1] == tomb)) ds->live_count = (uint32_t)(i + 1);
     }
     uint32_t total;
     block_exclusive_scan<SCAN_THREADS>(cnt, ws, total);
@@ -278,33 +264,35 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_heads_write(const uint32_t *__
                                                                uint32_t *__restrict__ blk_start,
                                                                uint32_t *__restrict__ blk_key, uint32_t capacity,
                                                                DeviceScalars *ds) {
-    __shared__ uint32_t ws[SCAN_THREADS / 64 + 1];
-    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
-    uint32_t head_bits = 0, cnt = 0;
-    uint32_t prev = base > 0 && base - 1 < n ? keys[base - 1] : tomb ^ 1u;
-    uint32_t kv[SCAN_ITEMS];
-#pragma unroll
+    __shared__ uint32_t ws[SCAN_THREADS / 64];
+    const uint32_t lane = lane_id();
+    const uint32_t wave = threadIdx.x / 64;
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + threadIdx.x;
+    uint32_t running = tile_offsets[blockIdx.x];
     for (int k = 0; k < SCAN_ITEMS; ++k) {
-        uint64_t i = base + k;
-        kv[k] = i < n ? keys[i] : tomb;
-        bool head = i < n && kv[k] != tomb && (i == 0 || prev != kv[k]);
-        head_bits |= (head ? 1u : 0u) << k;
-        cnt += head ? 1u : 0u;
-        prev = kv[k];
-    }
-    uint32_t total;
-    uint32_t slot = block_exclusive_scan<SCAN_THREADS>(cnt, ws, total) + tile_offsets[blockIdx.x];
+        const uint64_t i = base + (uint64_t)k * SCAN_THREADS;
+        uint32_t key = tomb;
+        const bool head = is_head(keys, i, n, tomb, key);
+        const uint64_t m = __ballot(head);
+        if (lane == 0) ws[wave] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t before = 0, round = 0;
 #pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; ++k) {
-        if (head_bits & (1u << k)) {
+        for (int w = 0; w < SCAN_THREADS / 64; ++w) {
+            before += (uint32_t)w < wave ? ws[w] : 0u;
+            round += ws[w];
+        }
+        if (head) {
+            const uint32_t slot = running + before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
             if (slot < capacity) {
-                blk_start[slot] = (uint32_t)(base + k);
-                blk_key[slot] = kv[k];
+                blk_start[slot] = (uint32_t)i;
+                blk_key[slot] = key;
             } else {
                 atomicOr(&ds->status, MPM_STATUS_INDIRECT_LIMIT_EXCEEDED | (KID_BLOCKS << 16));
             }
-            ++slot;
         }
+        running += round;
+        __syncthreads();
     }
 }
 

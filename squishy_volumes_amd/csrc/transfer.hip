@@ -1,23 +1,31 @@
 // transfer.hip - P2G, grid update and G2P of the MLS-MPM substep (gfx950).
 //
 // Dataflow of one substep (after the block sort):
-//   P2G   per occupied 4^3 block: the block's particles are grouped by cell in
-//         LDS, lane l of a wave owns cell l and accumulates its particles'
-//         3x3x3 stencils in registers, the 64 lanes add their stencils into a
-//         6^3-node LDS arena (one add per lane and node, no two lanes on the
-//         same bank), and the arena is flushed to the dense grid with global
+//   P2G   per occupied 4^3 block, 4 waves: the block's particles are loaded
+//         once (coalesced through the sort order), reduced to a 16-float
+//         payload, and grouped by cell in LDS.  Lane l of every wave owns cell
+//         l and accumulates its particles' 3x3x3 stencils in registers; the
+//         waves then add their stencils into a 6^3-node float4 arena one wave
+//         at a time (plain LDS read-add-write: within one wave no two lanes
+//         touch the same node, and the arena layout puts them on distinct
+//         banks).  The arena is flushed to the dense node grid with global
 //         float atomics.  Mirrors scatter_momentum
 //         (rust/crates/cpu/src/phase/scatter_momentum.rs:22-93).
-//   grid  v = p / m per active node (meld_grid without collider twins,
-//         rust/crates/cpu/src/phase/meld_grid.rs:61-66) plus the optional grid
-//         gravity and boundary conditions; clears the accumulation grid.
+//   grid  v = p / m per node of every touched block (meld_grid without
+//         collider twins, rust/crates/cpu/src/phase/meld_grid.rs:61-66) plus the
+//         optional grid gravity and boundary conditions; clears the
+//         accumulation grid.
 //   G2P   per occupied block: the 6^3 node velocities are staged in LDS, each
 //         thread gathers v and C of one particle (collect_velocity,
 //         rust/crates/cpu/src/phase/collect_velocity.rs:19-75), advances x and
 //         F (advance_particles, rust/crates/cpu/src/phase/advance_particles.rs:17-93),
-//         culls (rust/crates/cpu/src/phase/cull_particles.rs:17-41) and writes
+//         culls (rust/crates/cpu/src/phase/cull_particles.rs:17-41), and writes
 //         the fused affine momentum A = m C - dt 4/h^2 V0 (tau + J sigma_visc)
 //         for the next P2G, permuting the particle arrays into block order.
+//
+// The kernels are specialised on the set of constitutive models present: the
+// common set (Neo-Hookean, fluid, fixed-corotated, viscosity) needs only a
+// polar decomposition; snow and sand need the full SVD.
 #include <algorithm>
 
 #include "device_math.hpp"
@@ -27,14 +35,18 @@ namespace svamd {
 
 namespace {
 
-constexpr int ARENA_ROWS = 8;               // (x>>2, y>>2, z>>2) of a 6^3 arena
-constexpr int ARENA = ARENA_ROWS * 64;      // slots per component
-constexpr int P2G_CHUNK = 1024;             // particles grouped per pass in LDS
-constexpr int G2P_NODES = 216;              // 6^3
+constexpr int P2G_THREADS = 256;
+constexpr int P2G_WAVES = P2G_THREADS / 64;
+constexpr int P2G_CAP = 512;  // particles staged per pass
+constexpr int P2G_PER_THREAD = P2G_CAP / P2G_THREADS;
+constexpr int ARENA = 512;    // float4 slots of the 6^3 arena (8 rows x 64 banks)
+constexpr int G2P_NODES = 216;
+constexpr uint32_t NO_BLOCK = 0xffffffffu;
 
-// LDS slot of arena node (x, y, z), 0 <= x, y, z < 6: the bank is
-// (x&3) | (y&3)<<2 | (z&3)<<4, so the 64 nodes that the 64 lanes (cells) of
-// a wave touch for one stencil offset fall on 64 distinct banks.
+// LDS slot of arena node (x, y, z), 0 <= x, y, z < 6.  The low six bits are
+// (x&3) | (y&3)<<2 | (z&3)<<4: for one stencil offset the 64 lanes (cells)
+// of a wave address 64 distinct slots, and any 16 consecutive lanes (one
+// ds_read_b128 phase) address 16 distinct 16-byte bank groups.
 __device__ __forceinline__ int arena_slot(int x, int y, int z) {
     return (((x >> 2) | ((y >> 2) << 1) | ((z >> 2) << 2)) << 6) | (x & 3) | ((y & 3) << 2) | ((z & 3) << 4);
 }
@@ -54,52 +66,93 @@ __device__ __forceinline__ uint32_t node_index(const GridGeom &G, uint32_t x, ui
     return block_linear(G, x >> 2, y >> 2, z >> 2) * 64u + (x & 3u) + ((y & 3u) << 2) + ((z & 3u) << 4);
 }
 
-// Kirchhoff stress of a particle, tau = P F^T, for the model its flags select.
-// U, s, V must hold the SVD of F for fixed-corotated and snow particles.
+__device__ __noinline__ bool svd3_call(const float F[9], float U[9], float s[3], float V[9]) {
+    return svd3(F, U, s, V);
+}
+
+// Kirchhoff stress tau = P F^T of a particle.  R must hold the polar factor
+// of F for fixed-corotated particles; U, s hold the SVD for snow particles.
+template <uint32_t SET>
 __device__ __forceinline__ void kirchhoff(uint32_t flags, float p0, float p1, float hard, const float F[9], float J,
-                                          const float U[9], const float s[3], const float V[9], float tau[9]) {
-    if (flags & F_FLUID) {
+                                          const float R[9], const float U[9], const float s[3], float tau[9]) {
+    if ((SET & M_FLUID) && (flags & F_FLUID)) {
         tau_inviscid(p0, (int)p1, J, tau);
-    } else if (flags & (F_FCR | F_SNOW)) {
-        tau_fixed_corotated(p0 * hard, p1 * hard, F, U, s, V, tau);
+    } else if ((SET & M_SNOW) && (flags & F_SNOW)) {
+        tau_fixed_corotated_svd(p0 * hard, p1 * hard, U, s, tau);
+    } else if ((SET & M_FCR) && (flags & F_FCR)) {
+        tau_fixed_corotated_polar(p0, p1, F, R, J, tau);
     } else {
         tau_neo_hookean(p0, p1, F, J, tau);
     }
 }
 
 // A = m C - s V0 tau - s J V0 sigma_visc(C)  (scatter_momentum.rs:46-84 regrouped).
+template <uint32_t SET>
 __device__ __forceinline__ void fused_affine(float m, float vol, float scaling, const float C[9], const float tau[9],
                                              float J, bool visc, float vd, float vb, float A[9]) {
-    float sv = scaling * vol;
+    const float sv = scaling * vol;
 #pragma unroll
     for (int k = 0; k < 9; ++k) A[k] = m * C[k] - sv * tau[k];
-    if (visc) {
+    if ((SET & M_VISC) && visc) {
         float sig[9];
         cauchy_viscosity(vd, vb, C, sig);
-        float svj = sv * J;
+        const float svj = sv * J;
 #pragma unroll
         for (int k = 0; k < 9; ++k) A[k] -= svj * sig[k];
     }
 }
 
-__device__ __forceinline__ bool needs_svd_for_stress(uint32_t flags) {
-    return !(flags & F_FLUID) && (flags & (F_FCR | F_SNOW));
+// Stress inputs of a particle at P2G time (first substep of a call: the
+// state holds C, not A).
+template <uint32_t SET>
+__device__ __forceinline__ void affine_from_state(const ParticleArrays &P, const AuxArrays &X, uint32_t p, float m,
+                                                  float scaling, float hardening, DeviceScalars *ds, float A[9]) {
+    const uint32_t flags = P.flags[p];
+    float F[9], C[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        F[k] = P.F[k][p];
+        C[k] = P.A[k][p];
+    }
+    float J = det3(F);
+    float R[9], U[9], s[3] = {1.f, 1.f, 1.f};
+    float hard = 1.0f;
+    if ((SET & M_SNOW) && (flags & F_SNOW)) {
+        float V[9];
+        if (svd3_call(F, U, s, V)) atomicOr(&ds->status, MPM_STATUS_SVD_FP64);
+        hard = expf(hardening * (1.0f - X.jp[P.id[p]]));
+    } else if ((SET & M_FCR) && (flags & F_FCR)) {
+        if (!polar3(F, R)) {
+            float V[9];
+            svd3_call(F, U, s, V);
+            matmul3_bt(U, V, R);
+        }
+    }
+    if (!(J >= 1e-7f) && !(flags & F_FLUID))
+        atomicOr(&ds->status, MPM_STATUS_PARTICLE_CLOSE_TO_INVERTED | (KID_P2G << 16));
+    float tau[9];
+    kirchhoff<SET>(flags, P.p0[p], P.p1[p], hard, F, J, R, U, s, tau);
+    const bool visc = flags & F_VISC;
+    float vd = 0.f, vb = 0.f;
+    if ((SET & M_VISC) && visc) {
+        vd = X.visc_dyn[P.id[p]];
+        vb = X.visc_bulk[P.id[p]];
+    }
+    fused_affine<SET>(m, P.vol[p], scaling, C, tau, J, visc, vd, vb, A);
 }
 
 // ------------------------------------------------------------------- P2G
-template <int WAVES, bool FROM_C, bool GOALS>
-__global__ __launch_bounds__(64 * WAVES) void k_p2g(ParticleArrays P, AuxArrays X, const uint32_t *__restrict__ order,
-                                                    const uint32_t *__restrict__ blk_start,
-                                                    const uint32_t *__restrict__ blk_key, GridGeom G, SubstepParams S,
-                                                    float4 *__restrict__ grid, uint32_t *__restrict__ blk_flag,
-                                                    DeviceScalars *ds) {
-    constexpr int NT = 64 * WAVES;
-    constexpr int PER_THREAD = P2G_CHUNK / NT;
-    __shared__ float arena[4 * ARENA];
+template <bool FROM_C, bool GOALS, uint32_t SET>
+__global__ __launch_bounds__(P2G_THREADS) void k_p2g(ParticleArrays P, AuxArrays X, const uint32_t *__restrict__ order,
+                                                     const uint32_t *__restrict__ blk_start,
+                                                     const uint32_t *__restrict__ blk_key, GridGeom G, SubstepParams S,
+                                                     float4 *__restrict__ grid, uint32_t *__restrict__ blk_flag,
+                                                     uint32_t *__restrict__ cand, DeviceScalars *ds) {
+    __shared__ float4 arena[ARENA];
+    __shared__ float4 pay[P2G_CAP][4];
     __shared__ uint32_t cell_count[64];
     __shared__ uint32_t cell_start[64];
     __shared__ uint32_t cell_fill[64];
-    __shared__ uint32_t list[P2G_CHUNK];
 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
@@ -117,96 +170,56 @@ __global__ __launch_bounds__(64 * WAVES) void k_p2g(ParticleArrays P, AuxArrays 
         int cell0[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) cell0[a] = G.origin[a] + 4 * (int)b[a];
+        for (int k = tid; k < ARENA; k += P2G_THREADS) arena[k] = make_float4(0.f, 0.f, 0.f, 0.f);
 
-        for (int k = tid; k < 4 * ARENA; k += NT) arena[k] = 0.f;
-
-        for (uint32_t cs = start; cs < end; cs += P2G_CHUNK) {
-            const uint32_t ce = min(end, cs + (uint32_t)P2G_CHUNK);
+        for (uint32_t cs = start; cs < end; cs += P2G_CAP) {
+            const uint32_t ce = min(end, cs + (uint32_t)P2G_CAP);
             if (tid < 64) {
                 cell_count[tid] = 0;
                 cell_fill[tid] = 0;
             }
             __syncthreads();
-            // Pass 1: the cell of every particle of the chunk.
-            uint32_t myp[PER_THREAD], mycell[PER_THREAD];
+            // Stage: each thread reduces its particles to payloads
+            //   (fx, m | q | h A columns), contribution at stencil node (i,j,k):
+            //   w_ijk (m v + A d) = w_ijk (q + i c0 + j c1 + k c2).
+            float pl[P2G_PER_THREAD][16];
+            uint32_t mycell[P2G_PER_THREAD];
 #pragma unroll
-            for (int k = 0; k < PER_THREAD; ++k) {
-                uint32_t i = cs + tid + (uint32_t)(k * NT);
+            for (int k = 0; k < P2G_PER_THREAD; ++k) {
+                const uint32_t i = cs + tid + (uint32_t)(k * P2G_THREADS);
                 mycell[k] = 64u;
-                myp[k] = 0u;
-                if (i < ce) {
-                    uint32_t p = order[i];
-                    int c[3];
-                    bool inside = true;
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        c[a] = (int)floorf(P.x[a][p] / h - 0.5f) - cell0[a];
-                        inside = inside && c[a] >= 0 && c[a] < 4;
-                    }
-                    if (inside) {
-                        uint32_t cell = (uint32_t)(c[0] | (c[1] << 2) | (c[2] << 4));
-                        atomicAdd(&cell_count[cell], 1u);
-                        mycell[k] = cell;
-                        myp[k] = p;
-                    } else {
-                        atomicOr(&ds->status, MPM_STATUS_OUT_OF_GRID | (KID_P2G << 16));
-                    }
-                }
-            }
-            __syncthreads();
-            if (tid < 64) {
-                uint32_t c = cell_count[tid];
-                uint32_t incl = c;
-#pragma unroll
-                for (int o = 1; o < 64; o <<= 1) {
-                    uint32_t y = __shfl_up(incl, o, 64);
-                    if (lane >= (uint32_t)o) incl += y;
-                }
-                cell_start[tid] = incl - c;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < PER_THREAD; ++k) {
-                if (mycell[k] < 64u) {
-                    uint32_t slot = cell_start[mycell[k]] + atomicAdd(&cell_fill[mycell[k]], 1u);
-                    list[slot] = myp[k];
-                }
-            }
-            __syncthreads();
-
-            // Pass 2: lane = cell; the wave's lanes walk their cells' particles.
-            const uint32_t cnt = cell_count[lane];
-            const uint32_t st = cell_start[lane];
-            float acc[27][4];
-#pragma unroll
-            for (int n = 0; n < 27; ++n)
-#pragma unroll
-                for (int c = 0; c < 4; ++c) acc[n][c] = 0.f;
-
-            for (uint32_t r = wave; r < cnt; r += WAVES) {
-                const uint32_t p = list[st + r];
-                float xp[3], fx[3], w[3][3];
+                if (i >= ce) continue;
+                const uint32_t p = order[i];
+                float xp[3], fx[3];
+                int c[3];
+                bool inside = true;
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
                     xp[a] = P.x[a][p];
-                    float xn = xp[a] / h;
-                    fx[a] = xn - floorf(xn - 0.5f);
-                    bspline_weights(fx[a], w[a]);
+                    const float xn = xp[a] / h;
+                    const float fl = floorf(xn - 0.5f);
+                    fx[a] = xn - fl;
+                    c[a] = (int)fl - cell0[a];
+                    inside = inside && c[a] >= 0 && c[a] < 4;
+                }
+                if (!inside) {
+                    atomicOr(&ds->status, MPM_STATUS_OUT_OF_GRID | (KID_P2G << 16));
+                    continue;
                 }
                 const float m = P.m[p];
                 float v[3] = {P.v[0][p], P.v[1][p], P.v[2][p]};
                 // external_force (rust/crates/cpu/src/phase/external_force.rs:36-47).
                 bool goal = false;
                 if constexpr (GOALS) {
-                    uint32_t o = P.id[p];
-                    uint32_t fb = S.has_b ? X.goal_flags_b[o] : X.goal_flags_a[o];
+                    const uint32_t o = P.id[p];
+                    const uint32_t fb = S.has_b ? X.goal_flags_b[o] : X.goal_flags_a[o];
                     if ((X.goal_flags_a[o] & F_GOAL) && (fb & F_GOAL)) {
                         goal = true;
-                        float fa = 1.0f - S.factor_b;
+                        const float fa = 1.0f - S.factor_b;
 #pragma unroll
                         for (int a = 0; a < 3; ++a) {
-                            float gb = S.has_b ? X.goal_b[a][o] : X.goal_a[a][o];
-                            float gp = fa * X.goal_a[a][o] + S.factor_b * gb;
+                            const float gb = S.has_b ? X.goal_b[a][o] : X.goal_a[a][o];
+                            const float gp = fa * X.goal_a[a][o] + S.factor_b * gb;
                             v[a] = (gp - xp[a]) / S.dt;
                         }
                     }
@@ -217,162 +230,221 @@ __global__ __launch_bounds__(64 * WAVES) void k_p2g(ParticleArrays P, AuxArrays 
                 }
                 float A[9];
                 if constexpr (FROM_C) {
-                    const uint32_t flags = P.flags[p];
-                    float F[9], C[9];
-#pragma unroll
-                    for (int k = 0; k < 9; ++k) {
-                        F[k] = P.F[k][p];
-                        C[k] = P.A[k][p];
-                    }
-                    const float J = det3(F);
-                    float U[9], s[3] = {1.f, 1.f, 1.f}, V[9];
-                    float hard = 1.0f;
-                    if (needs_svd_for_stress(flags)) {
-                        if (svd3(F, U, s, V)) atomicAdd(&ds->svd_fp64, 1u);
-                        if (flags & F_SNOW) hard = expf(S.snow_hardening * (1.0f - X.jp[P.id[p]]));
-                    }
-                    if (!(J >= 1e-7f) && !(flags & F_FLUID))
-                        atomicOr(&ds->status, MPM_STATUS_PARTICLE_CLOSE_TO_INVERTED | (KID_P2G << 16));
-                    float tau[9];
-                    kirchhoff(flags, P.p0[p], P.p1[p], hard, F, J, U, s, V, tau);
-                    const bool visc = flags & F_VISC;
-                    float vd = 0.f, vb = 0.f;
-                    if (visc) {
-                        vd = X.visc_dyn[P.id[p]];
-                        vb = X.visc_bulk[P.id[p]];
-                    }
-                    fused_affine(m, P.vol[p], scaling, C, tau, J, visc, vd, vb, A);
+                    affine_from_state<SET>(P, X, p, m, scaling, S.snow_hardening, ds, A);
                 } else {
 #pragma unroll
-                    for (int k = 0; k < 9; ++k) A[k] = P.A[k][p];
+                    for (int q = 0; q < 9; ++q) A[q] = P.A[q][p];
                 }
-                // Contribution to node base + (i, j, k):
-                //   w_ijk (m v + A d), d = h ((i, j, k) - fx) = q + (h A) (i, j, k).
-                float q[3], c0[3], c1[3], c2[3];
+                float *o = pl[k];
+                o[0] = fx[0];
+                o[1] = fx[1];
+                o[2] = fx[2];
+                o[3] = m;
 #pragma unroll
-                for (int r3 = 0; r3 < 3; ++r3) {
-                    q[r3] = m * v[r3] - h * (A[r3] * fx[0] + A[3 + r3] * fx[1] + A[6 + r3] * fx[2]);
-                    c0[r3] = h * A[r3];
-                    c1[r3] = h * A[3 + r3];
-                    c2[r3] = h * A[6 + r3];
+                for (int r = 0; r < 3; ++r) {
+                    o[4 + r] = m * v[r] - h * (A[r] * fx[0] + A[3 + r] * fx[1] + A[6 + r] * fx[2]);
+                    o[7 + r] = h * A[r];
+                    o[10 + r] = h * A[3 + r];
+                    o[13 + r] = h * A[6 + r];
                 }
+                const uint32_t cell = (uint32_t)(c[0] | (c[1] << 2) | (c[2] << 4));
+                atomicAdd(&cell_count[cell], 1u);
+                mycell[k] = cell;
+            }
+            __syncthreads();
+            if (tid < 64) {
+                const uint32_t cnt = cell_count[tid];
+                uint32_t incl = cnt;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const uint32_t y = __shfl_up(incl, o, 64);
+                    if (lane >= (uint32_t)o) incl += y;
+                }
+                cell_start[tid] = incl - cnt;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < P2G_PER_THREAD; ++k) {
+                if (mycell[k] < 64u) {
+                    const uint32_t slot = cell_start[mycell[k]] + atomicAdd(&cell_fill[mycell[k]], 1u);
+                    const float *o = pl[k];
+                    pay[slot][0] = make_float4(o[0], o[1], o[2], o[3]);
+                    pay[slot][1] = make_float4(o[4], o[5], o[6], o[7]);
+                    pay[slot][2] = make_float4(o[8], o[9], o[10], o[11]);
+                    pay[slot][3] = make_float4(o[12], o[13], o[14], o[15]);
+                }
+            }
+            __syncthreads();
+
+            // Accumulate: lane = cell, wave w takes every P2G_WAVES-th particle.
+            const uint32_t cnt = cell_count[lane];
+            const uint32_t st = cell_start[lane];
+            float acc[27][4];
+#pragma unroll
+            for (int n = 0; n < 27; ++n)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) acc[n][c] = 0.f;
+            for (uint32_t r = wave; r < cnt; r += P2G_WAVES) {
+                const float4 a0 = pay[st + r][0], a1 = pay[st + r][1], a2 = pay[st + r][2], a3 = pay[st + r][3];
+                const float fx[3] = {a0.x, a0.y, a0.z};
+                const float m = a0.w;
+                const float q[3] = {a1.x, a1.y, a1.z};
+                const float c0[3] = {a1.w, a2.x, a2.y};
+                const float c1[3] = {a2.z, a2.w, a3.x};
+                const float c2[3] = {a3.y, a3.z, a3.w};
+                float w[3][3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) bspline_weights(fx[a], w[a]);
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
 #pragma unroll
                     for (int j = 0; j < 3; ++j) {
                         const float wij = w[0][i] * w[1][j];
+                        float t[3];
+#pragma unroll
+                        for (int r3 = 0; r3 < 3; ++r3) t[r3] = q[r3] + (float)i * c0[r3] + (float)j * c1[r3];
 #pragma unroll
                         for (int k = 0; k < 3; ++k) {
                             const float wt = wij * w[2][k];
                             const int n = 9 * i + 3 * j + k;
                             acc[n][0] += wt * m;
 #pragma unroll
-                            for (int r3 = 0; r3 < 3; ++r3)
-                                acc[n][1 + r3] += wt * (q[r3] + (float)i * c0[r3] + (float)j * c1[r3] + (float)k * c2[r3]);
+                            for (int r3 = 0; r3 < 3; ++r3) acc[n][1 + r3] += wt * (t[r3] + (float)k * c2[r3]);
                         }
                     }
                 }
             }
-            // Add the lanes' stencils into the arena.
-            if (cnt > wave) {
-                const int cx = (int)(lane & 3u), cy = (int)((lane >> 2) & 3u), cz = (int)(lane >> 4);
+            // Waves add their stencils one after the other.
+            const int cx = (int)(lane & 3u), cy = (int)((lane >> 2) & 3u), cz = (int)(lane >> 4);
+            for (uint32_t w = 0; w < (uint32_t)P2G_WAVES; ++w) {
+                if (wave == w && cnt > wave) {
 #pragma unroll
-                for (int i = 0; i < 3; ++i)
+                    for (int i = 0; i < 3; ++i)
 #pragma unroll
-                    for (int j = 0; j < 3; ++j)
+                        for (int j = 0; j < 3; ++j)
 #pragma unroll
-                        for (int k = 0; k < 3; ++k) {
-                            const int n = 9 * i + 3 * j + k;
-                            const int slot = arena_slot(cx + i, cy + j, cz + k);
-#pragma unroll
-                            for (int c = 0; c < 4; ++c) atomicAdd(&arena[c * ARENA + slot], acc[n][c]);
-                        }
+                            for (int k = 0; k < 3; ++k) {
+                                const int n = 9 * i + 3 * j + k;
+                                const int slot = arena_slot(cx + i, cy + j, cz + k);
+                                float4 g = arena[slot];
+                                g.x += acc[n][0];
+                                g.y += acc[n][1];
+                                g.z += acc[n][2];
+                                g.w += acc[n][3];
+                                arena[slot] = g;
+                            }
+                }
+                __syncthreads();
             }
-            __syncthreads();
         }
 
-        // Flush the 6^3 arena to the grid.
-        for (int nidx = tid; nidx < 216; nidx += NT) {
+        // Flush the 6^3 arena to the grid; claim the 8 blocks it covers.
+        for (int nidx = tid; nidx < 216; nidx += P2G_THREADS) {
             const int ax = nidx % 6, ay = (nidx / 6) % 6, az = nidx / 36;
-            const int slot = arena_slot(ax, ay, az);
-            const float m = arena[slot];
-            if (m != 0.f) {
-                uint32_t node = node_index(G, 4u * b[0] + ax, 4u * b[1] + ay, 4u * b[2] + az);
+            const float4 g4 = arena[arena_slot(ax, ay, az)];
+            if (g4.x != 0.f) {
+                const uint32_t node = node_index(G, 4u * b[0] + ax, 4u * b[1] + ay, 4u * b[2] + az);
                 float *g = reinterpret_cast<float *>(grid + node);
-                unsafeAtomicAdd(g + 0, m);
-                unsafeAtomicAdd(g + 1, arena[ARENA + slot]);
-                unsafeAtomicAdd(g + 2, arena[2 * ARENA + slot]);
-                unsafeAtomicAdd(g + 3, arena[3 * ARENA + slot]);
+                unsafeAtomicAdd(g + 0, g4.x);
+                unsafeAtomicAdd(g + 1, g4.y);
+                unsafeAtomicAdd(g + 2, g4.z);
+                unsafeAtomicAdd(g + 3, g4.w);
             }
         }
-        if (tid < 8) blk_flag[block_linear(G, b[0] + (tid & 1u), b[1] + ((tid >> 1) & 1u), b[2] + (tid >> 2))] = 1u;
+        if (tid < 8) {
+            const uint32_t T = block_linear(G, b[0] + (tid & 1u), b[1] + ((tid >> 1) & 1u), b[2] + (tid >> 2));
+            const uint32_t old = atomicExch(&blk_flag[T], 1u);
+            cand[8u * bi + tid] = old == 0u ? T : NO_BLOCK;
+        }
         __syncthreads();
     }
 }
 
 // ------------------------------------------------------------ grid update
-__global__ __launch_bounds__(64) void k_grid_update(const uint32_t *__restrict__ blk_key, GridGeom G, SubstepParams S,
-                                                    float4 *__restrict__ grid, float4 *__restrict__ vgrid,
-                                                    uint32_t *__restrict__ blk_flag, uint32_t *__restrict__ blk_epoch,
-                                                    DeviceScalars *ds) {
-    const uint32_t lane = threadIdx.x;
+// One wave per touched block: the candidate slots written by P2G are read 64
+// at a time and the claimed blocks processed in turn.
+__global__ __launch_bounds__(256) void k_grid_update(const uint32_t *__restrict__ cand, GridGeom G, SubstepParams S,
+                                                     float4 *__restrict__ grid, float4 *__restrict__ vgrid,
+                                                     uint32_t *__restrict__ blk_flag, uint32_t *__restrict__ blk_epoch,
+                                                     uint32_t *__restrict__ partial, DeviceScalars *ds) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t gw = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    const uint32_t nw = gridDim.x * (blockDim.x / 64);
     const uint32_t ncand = ds->num_blocks * 8u;
-    for (uint32_t c = blockIdx.x; c < ncand; c += gridDim.x) {
-        uint32_t b[3];
-        decode_block(blk_key[c >> 3], b);
-        const uint32_t d = c & 7u;
-        b[0] += d & 1u;
-        b[1] += (d >> 1) & 1u;
-        b[2] += d >> 2;
-        const uint32_t B = block_linear(G, b[0], b[1], b[2]);
-        uint32_t claimed = 0;
-        if (lane == 0) claimed = atomicExch(&blk_flag[B], 0u);
-        claimed = __shfl(claimed, 0, 64);
-        if (!claimed) continue;
-        const uint32_t node = B * 64u + lane;
-        float4 g = grid[node];
-        const float m = g.x;
-        float v[3] = {0.f, 0.f, 0.f};
-        if (m > 0.f) {
-            v[0] = g.y / m;
-            v[1] = g.z / m;
-            v[2] = g.w / m;
-            // Extensions without a reference counterpart.
-            if (S.options & MPM_OPT_GRID_GRAVITY) {
+    uint32_t active = 0;
+    for (uint32_t chunk = gw; chunk * 64u < ncand; chunk += nw) {
+        const uint32_t c = chunk * 64u + lane;
+        const uint32_t mine = c < ncand ? cand[c] : NO_BLOCK;
+        uint64_t todo = __ballot(mine != NO_BLOCK);
+        while (todo) {
+            const int l = __ffsll((unsigned long long)todo) - 1;
+            todo &= todo - 1ull;
+            const uint32_t B = __shfl(mine, l, 64);
+            const uint32_t node = B * 64u + lane;
+            const float4 g = grid[node];
+            const float m = g.x;
+            float v[3] = {0.f, 0.f, 0.f};
+            if (m > 0.f) {
+                v[0] = g.y / m;
+                v[1] = g.z / m;
+                v[2] = g.w / m;
+                // Extensions without a reference counterpart.
+                if (S.options & MPM_OPT_GRID_GRAVITY) {
 #pragma unroll
-                for (int a = 0; a < 3; ++a) v[a] += S.gravity[a] * S.dt;
-            }
-            if (S.options & (MPM_OPT_STICKY_PLANE | MPM_OPT_BOX_WALLS)) {
-                const int lx = (int)(4u * b[0] + (lane & 3u)), ly = (int)(4u * b[1] + ((lane >> 2) & 3u)),
-                          lz = (int)(4u * b[2] + (lane >> 4));
-                const float xi[3] = {(float)(lx + G.origin[0]) * S.h, (float)(ly + G.origin[1]) * S.h,
-                                     (float)(lz + G.origin[2]) * S.h};
-                if ((S.options & MPM_OPT_STICKY_PLANE) && xi[2] < S.plane_z) v[0] = v[1] = v[2] = 0.f;
-                if (S.options & MPM_OPT_BOX_WALLS) {
+                    for (int a = 0; a < 3; ++a) v[a] += S.gravity[a] * S.dt;
+                }
+                if (S.options & (MPM_OPT_STICKY_PLANE | MPM_OPT_BOX_WALLS)) {
+                    const uint32_t bx = B % (uint32_t)G.nb[0];
+                    const uint32_t by = (B / (uint32_t)G.nb[0]) % (uint32_t)G.nb[1];
+                    const uint32_t bz = B / ((uint32_t)G.nb[0] * (uint32_t)G.nb[1]);
+                    const int lx = (int)(4u * bx + (lane & 3u)), ly = (int)(4u * by + ((lane >> 2) & 3u)),
+                              lz = (int)(4u * bz + (lane >> 4));
+                    const float xi[3] = {(float)(lx + G.origin[0]) * S.h, (float)(ly + G.origin[1]) * S.h,
+                                         (float)(lz + G.origin[2]) * S.h};
+                    if ((S.options & MPM_OPT_STICKY_PLANE) && xi[2] < S.plane_z) v[0] = v[1] = v[2] = 0.f;
+                    if (S.options & MPM_OPT_BOX_WALLS) {
 #pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        if (xi[a] < S.wall_min[a] && v[a] < 0.f) v[a] = 0.f;
-                        if (xi[a] > S.wall_max[a] && v[a] > 0.f) v[a] = 0.f;
+                        for (int a = 0; a < 3; ++a) {
+                            if (xi[a] < S.wall_min[a] && v[a] < 0.f) v[a] = 0.f;
+                            if (xi[a] > S.wall_max[a] && v[a] > 0.f) v[a] = 0.f;
+                        }
                     }
                 }
+                ++active;
+            }
+            vgrid[node] = make_float4(m, v[0], v[1], v[2]);
+            grid[node] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (lane == 0) {
+                blk_flag[B] = 0u;
+                blk_epoch[B] = S.epoch;
             }
         }
-        vgrid[node] = make_float4(m, v[0], v[1], v[2]);
-        grid[node] = make_float4(0.f, 0.f, 0.f, 0.f);
-        const uint64_t act = __ballot(m > 0.f);
-        if (lane == 0) {
-            atomicAdd(&ds->active_nodes, (uint32_t)__popcll(act));
-            blk_epoch[B] = S.epoch;
-        }
     }
+    // Per-wave count of active nodes (summed once per step_to call).
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) active += __shfl_xor(active, o, 64);
+    if (lane == 0) partial[gw] = active;
+}
+
+__global__ __launch_bounds__(256) void k_sum_partials(const uint32_t *__restrict__ partial, uint32_t n,
+                                                      DeviceScalars *ds) {
+    __shared__ uint32_t ws[4];
+    uint32_t s = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s += partial[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63u) == 0) ws[threadIdx.x / 64] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) ds->active_nodes = ws[0] + ws[1] + ws[2] + ws[3];
 }
 
 // ------------------------------------------------------------------- G2P
-template <bool LAST>
+template <bool LAST, uint32_t SET>
 __global__ __launch_bounds__(256) void k_g2p(ParticleArrays Pin, ParticleArrays Pout, AuxArrays X,
                                              const uint32_t *__restrict__ order, const uint32_t *__restrict__ blk_start,
                                              const uint32_t *__restrict__ blk_key, GridGeom G, SubstepParams S,
-                                             const float4 *__restrict__ vgrid, DeviceScalars *ds) {
+                                             const float4 *__restrict__ vgrid, uint32_t *__restrict__ keys_out,
+                                             DeviceScalars *ds) {
     __shared__ float4 vloc[G2P_NODES];
     const uint32_t tid = threadIdx.x;
     const uint32_t NT = blockDim.x;
@@ -426,8 +498,7 @@ __global__ __launch_bounds__(256) void k_g2p(ParticleArrays Pin, ParticleArrays 
                     for (int k3 = 0; k3 < 3; ++k3) {
                         const float dz = (float)k3 - fx[2];
                         const float wt = wij * w[2][k3];
-                        const int nidx = (c[0] + i3) + 6 * ((c[1] + j3) + 6 * (c[2] + k3));
-                        const float4 gv = vloc[nidx];
+                        const float4 gv = vloc[(c[0] + i3) + 6 * ((c[1] + j3) + 6 * (c[2] + k3))];
                         const float wv[3] = {wt * gv.y, wt * gv.z, wt * gv.w};
 #pragma unroll
                         for (int r = 0; r < 3; ++r) {
@@ -456,20 +527,27 @@ __global__ __launch_bounds__(256) void k_g2p(ParticleArrays Pin, ParticleArrays 
 #pragma unroll
             for (int a = 0; a < 3; ++a) xnew[a] = xp[a] + v[a] * dt;
 
-            float U[9], s[3] = {1.f, 1.f, 1.f}, V[9];
+            float R[9], U[9], s[3] = {1.f, 1.f, 1.f};
             float hard = 1.0f;
             float energy = 0.f;
-            float J;
-            if (flags & F_FLUID) {
-                // advance_particles.rs:76-86: singular values <- (s1 s2 s3)^(1/3).
-                if (svd3(F, U, s, V)) atomicAdd(&ds->svd_fp64, 1u);
-                const float g = cbrtf(s[0] * s[1] * s[2]);
-                const float gd[3] = {g, g, g};
-                recompose(U, gd, V, F);
+            float J = det3(F);
+            if ((SET & M_FLUID) && (flags & F_FLUID)) {
+                // advance_particles.rs:76-86: singular values <- (s1 s2 s3)^(1/3),
+                // i.e. F <- J^(1/3) R with R the polar factor of F.
+                if (!(J > 0.f) || !polar3(F, R)) {
+                    float V[9];
+                    if (svd3_call(F, U, s, V)) atomicOr(&ds->status, MPM_STATUS_SVD_FP64);
+                    matmul3_bt(U, V, R);
+                    J = s[0] * s[1] * s[2];
+                }
+                const float g = cbrtf(J);
+#pragma unroll
+                for (int k = 0; k < 9; ++k) F[k] = g * R[k];
                 J = det3(F);
                 if (LAST) energy = energy_inviscid(p0, (int)p1, J);
-            } else if (flags & F_SNOW) {
-                if (svd3(F, U, s, V)) atomicAdd(&ds->svd_fp64, 1u);
+            } else if ((SET & M_SNOW) && (flags & F_SNOW)) {
+                float V[9];
+                if (svd3_call(F, U, s, V)) atomicOr(&ds->status, MPM_STATUS_SVD_FP64);
                 float jp = X.jp[id];
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
@@ -483,13 +561,14 @@ __global__ __launch_bounds__(256) void k_g2p(ParticleArrays Pin, ParticleArrays 
                 hard = expf(S.snow_hardening * (1.0f - jp));
                 if (LAST) energy = energy_fixed_corotated(p0 * hard, p1 * hard, s);
             } else {
-                if (flags & F_SAND) {
+                if ((SET & M_SAND) && (flags & F_SAND)) {
                     // advance_particles.rs:50-71: Drucker-Prager return map in log strain.
-                    if (svd3(F, U, s, V)) atomicAdd(&ds->svd_fp64, 1u);
+                    float V[9];
+                    if (svd3_call(F, U, s, V)) atomicOr(&ds->status, MPM_STATUS_SVD_FP64);
                     const float alpha = X.sand_alpha[id];
-                    float e[3] = {logf(s[0]), logf(s[1]), logf(s[2])};
+                    const float e[3] = {logf(s[0]), logf(s[1]), logf(s[2])};
                     const float etr = e[0] + e[1] + e[2];
-                    float eh[3] = {e[0] - etr / 3.f, e[1] - etr / 3.f, e[2] - etr / 3.f};
+                    const float eh[3] = {e[0] - etr / 3.f, e[1] - etr / 3.f, e[2] - etr / 3.f};
                     const float ehn = sqrtf(eh[0] * eh[0] + eh[1] * eh[1] + eh[2] * eh[2]);
                     if (etr < 0.f && ehn > 0.f) {
                         const float dg = ehn + (3.f * p1 + 2.f * p0) / 2.f / p0 * etr * alpha;
@@ -503,13 +582,16 @@ __global__ __launch_bounds__(256) void k_g2p(ParticleArrays Pin, ParticleArrays 
                         const float one[3] = {1.f, 1.f, 1.f};
                         recompose(U, one, V, F);
                     }
-                }
-                if (flags & F_FCR) {
-                    if (svd3(F, U, s, V)) atomicAdd(&ds->svd_fp64, 1u);
-                    J = s[0] * s[1] * s[2];
-                    if (LAST) energy = energy_fixed_corotated(p0, p1, s);
-                } else {
                     J = det3(F);
+                }
+                if ((SET & M_FCR) && (flags & F_FCR)) {
+                    if (!polar3(F, R)) {
+                        float V[9];
+                        svd3_call(F, U, s, V);
+                        matmul3_bt(U, V, R);
+                    }
+                    if (LAST) energy = energy_fixed_corotated_polar(p0, p1, F, R, J);
+                } else {
                     if (!(J > 0.f)) {
                         // try_elastic_energy_neo_hookean fails: EnergyError, particle FAILED.
                         flags |= F_FAILED;
@@ -532,16 +614,17 @@ __global__ __launch_bounds__(256) void k_g2p(ParticleArrays Pin, ParticleArrays 
                 for (int k = 0; k < 9; ++k) Aout[k] = C[k];
             } else {
                 float tau[9];
-                kirchhoff(flags, p0, p1, hard, F, J, U, s, V, tau);
+                kirchhoff<SET>(flags, p0, p1, hard, F, J, R, U, s, tau);
                 const bool visc = flags & F_VISC;
                 float vd = 0.f, vb = 0.f;
-                if (visc) {
+                if ((SET & M_VISC) && visc) {
                     vd = X.visc_dyn[id];
                     vb = X.visc_bulk[id];
                 }
-                fused_affine(m, vol, scaling, C, tau, J, visc, vd, vb, Aout);
+                fused_affine<SET>(m, vol, scaling, C, tau, J, visc, vd, vb, Aout);
             }
             if (LAST) X.energy[id] = energy;
+            if (keys_out) keys_out[i] = inside ? particle_block_key(xnew, G, ds) : G.tomb_key;
 
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
@@ -567,7 +650,8 @@ __global__ __launch_bounds__(256) void k_g2p(ParticleArrays Pin, ParticleArrays 
 // Particles that are not in any block (tombstoned, sorted to the end) are
 // copied to their sorted slot unchanged.
 __global__ __launch_bounds__(256) void k_copy_dead(ParticleArrays Pin, ParticleArrays Pout,
-                                                   const uint32_t *__restrict__ order, uint64_t n, DeviceScalars *ds) {
+                                                   const uint32_t *__restrict__ order, uint64_t n,
+                                                   uint32_t *__restrict__ keys_out, uint32_t tomb, DeviceScalars *ds) {
     const uint64_t live = ds->live_count;
     for (uint64_t i = live + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x) {
@@ -588,6 +672,7 @@ __global__ __launch_bounds__(256) void k_copy_dead(ParticleArrays Pin, ParticleA
         Pout.p1[i] = Pin.p1[p];
         Pout.flags[i] = Pin.flags[p];
         Pout.id[i] = Pin.id[p];
+        if (keys_out) keys_out[i] = tomb;
     }
 }
 
@@ -617,7 +702,7 @@ __global__ __launch_bounds__(256) void k_stress(const uint32_t *__restrict__ fla
                                                 const float *__restrict__ Fin, const float *__restrict__ jp,
                                                 float hardening, float *__restrict__ tau_out,
                                                 float *__restrict__ energy_out, uint64_t n) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t f = flags[i];
     const mpm_particle_parameters pp = params[i];
@@ -627,21 +712,27 @@ __global__ __launch_bounds__(256) void k_stress(const uint32_t *__restrict__ fla
     float F[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) F[k] = Fin[9 * i + k];
-    float U[9], s[3] = {1.f, 1.f, 1.f}, V[9];
+    float R[9], U[9], s[3] = {1.f, 1.f, 1.f}, V[9];
     float hard = 1.f;
-    float J = det3(F);
+    const float J = det3(F);
     float energy;
-    if (needs_svd_for_stress(f)) {
-        svd3(F, U, s, V);
-        if (f & F_SNOW) hard = expf(hardening * (1.f - (jp ? jp[i] : 1.f)));
+    if (!fluid && (f & F_SNOW)) {
+        svd3_call(F, U, s, V);
+        hard = expf(hardening * (1.f - (jp ? jp[i] : 1.f)));
         energy = energy_fixed_corotated(p0 * hard, p1 * hard, s);
+    } else if (!fluid && (f & F_FCR)) {
+        if (!polar3(F, R)) {
+            svd3_call(F, U, s, V);
+            matmul3_bt(U, V, R);
+        }
+        energy = energy_fixed_corotated_polar(p0, p1, F, R, J);
     } else if (fluid) {
         energy = energy_inviscid(p0, (int)p1, J);
     } else {
         energy = J > 0.f ? energy_neo_hookean(p0, p1, F, J) : __builtin_nanf("");
     }
     float tau[9];
-    kirchhoff(f, p0, p1, hard, F, J, U, s, V, tau);
+    kirchhoff<SET_ALL>(f, p0, p1, hard, F, J, R, U, s, tau);
 #pragma unroll
     for (int k = 0; k < 9; ++k) tau_out[9 * i + k] = tau[k];
     if (energy_out) energy_out[i] = energy;
@@ -649,7 +740,7 @@ __global__ __launch_bounds__(256) void k_stress(const uint32_t *__restrict__ fla
 
 __global__ __launch_bounds__(256) void k_svd(const float *__restrict__ Fin, float *__restrict__ Uo,
                                              float *__restrict__ so, float *__restrict__ Vo, uint64_t n) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float F[9], U[9], s[3], V[9];
 #pragma unroll
@@ -664,27 +755,23 @@ __global__ __launch_bounds__(256) void k_svd(const float *__restrict__ Fin, floa
     for (int a = 0; a < 3; ++a) so[3 * i + a] = s[a];
 }
 
-template <int WAVES>
+template <uint32_t SET>
 hipError_t p2g_dispatch(const TransferLaunch &L, bool from_c, bool goals, const ParticleArrays &P, const AuxArrays &X,
                         const uint32_t *order, const uint32_t *blk_start, const uint32_t *blk_key, const GridGeom &G,
-                        const SubstepParams &S, float4 *grid, uint32_t *blk_flag, DeviceScalars *ds,
+                        const SubstepParams &S, float4 *grid, uint32_t *blk_flag, uint32_t *cand, DeviceScalars *ds,
                         hipStream_t stream) {
-    dim3 grd(L.persistent_blocks), blk(64 * WAVES);
+    dim3 grd(L.persistent_blocks), blk(P2G_THREADS);
+#define SV_P2G(FC, GO)                                                                                              \
+    hipLaunchKernelGGL((k_p2g<FC, GO, SET>), grd, blk, 0, stream, P, X, order, blk_start, blk_key, G, S, grid, blk_flag, \
+                       cand, ds)
     if (from_c) {
-        if (goals)
-            hipLaunchKernelGGL((k_p2g<WAVES, true, true>), grd, blk, 0, stream, P, X, order, blk_start, blk_key, G, S,
-                               grid, blk_flag, ds);
-        else
-            hipLaunchKernelGGL((k_p2g<WAVES, true, false>), grd, blk, 0, stream, P, X, order, blk_start, blk_key, G, S,
-                               grid, blk_flag, ds);
+        if (goals) SV_P2G(true, true);
+        else SV_P2G(true, false);
     } else {
-        if (goals)
-            hipLaunchKernelGGL((k_p2g<WAVES, false, true>), grd, blk, 0, stream, P, X, order, blk_start, blk_key, G, S,
-                               grid, blk_flag, ds);
-        else
-            hipLaunchKernelGGL((k_p2g<WAVES, false, false>), grd, blk, 0, stream, P, X, order, blk_start, blk_key, G,
-                               S, grid, blk_flag, ds);
+        if (goals) SV_P2G(false, true);
+        else SV_P2G(false, false);
     }
+#undef SV_P2G
     return hipGetLastError();
 }
 
@@ -692,50 +779,62 @@ hipError_t p2g_dispatch(const TransferLaunch &L, bool from_c, bool goals, const 
 
 hipError_t launch_p2g(const TransferLaunch &L, bool from_c, const ParticleArrays &P, const AuxArrays &X,
                       const uint32_t *order, const uint32_t *blk_start, const uint32_t *blk_key, const GridGeom &G,
-                      const SubstepParams &S, float4 *grid, uint32_t *blk_flag, DeviceScalars *ds, hipStream_t stream) {
+                      const SubstepParams &S, float4 *grid, uint32_t *blk_flag, uint32_t *cand, DeviceScalars *ds,
+                      hipStream_t stream) {
     const bool goals = S.has_goals != 0;
-    switch (L.p2g_waves) {
-        case 1:
-            return p2g_dispatch<1>(L, from_c, goals, P, X, order, blk_start, blk_key, G, S, grid, blk_flag, ds, stream);
-        case 2:
-            return p2g_dispatch<2>(L, from_c, goals, P, X, order, blk_start, blk_key, G, S, grid, blk_flag, ds, stream);
-        default:
-            return p2g_dispatch<4>(L, from_c, goals, P, X, order, blk_start, blk_key, G, S, grid, blk_flag, ds, stream);
-    }
+    if (L.model_set == SET_POLAR)
+        return p2g_dispatch<SET_POLAR>(L, from_c, goals, P, X, order, blk_start, blk_key, G, S, grid, blk_flag, cand, ds,
+                                       stream);
+    return p2g_dispatch<SET_ALL>(L, from_c, goals, P, X, order, blk_start, blk_key, G, S, grid, blk_flag, cand, ds,
+                                 stream);
 }
 
-hipError_t launch_grid_update(const TransferLaunch &L, const uint32_t *blk_key, const GridGeom &G,
-                              const SubstepParams &S, float4 *grid, float4 *vgrid, uint32_t *blk_flag,
-                              uint32_t *blk_epoch, DeviceScalars *ds, hipStream_t stream) {
-    hipLaunchKernelGGL(k_grid_update, dim3(L.persistent_blocks * 4), dim3(64), 0, stream, blk_key, G, S, grid, vgrid,
-                       blk_flag, blk_epoch, ds);
+uint32_t grid_update_waves(const TransferLaunch &L) { return (uint32_t)L.persistent_blocks * 4u; }
+
+hipError_t launch_grid_update(const TransferLaunch &L, const uint32_t *cand, const GridGeom &G, const SubstepParams &S,
+                              float4 *grid, float4 *vgrid, uint32_t *blk_flag, uint32_t *blk_epoch, uint32_t *partial,
+                              DeviceScalars *ds, hipStream_t stream) {
+    hipLaunchKernelGGL(k_grid_update, dim3(L.persistent_blocks), dim3(256), 0, stream, cand, G, S, grid, vgrid, blk_flag,
+                       blk_epoch, partial, ds);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_partials(const TransferLaunch &L, const uint32_t *partial, DeviceScalars *ds,
+                               hipStream_t stream) {
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, stream, partial, grid_update_waves(L), ds);
     return hipGetLastError();
 }
 
 hipError_t launch_g2p(const TransferLaunch &L, bool last, const ParticleArrays &Pin, const ParticleArrays &Pout,
                       const AuxArrays &X, const uint32_t *order, const uint32_t *blk_start, const uint32_t *blk_key,
-                      const GridGeom &G, const SubstepParams &S, const float4 *vgrid, DeviceScalars *ds,
-                      hipStream_t stream) {
+                      const GridGeom &G, const SubstepParams &S, const float4 *vgrid, uint32_t *keys_out,
+                      DeviceScalars *ds, hipStream_t stream) {
     dim3 grd(L.persistent_blocks), blk(L.g2p_threads);
-    if (last)
-        hipLaunchKernelGGL(k_g2p<true>, grd, blk, 0, stream, Pin, Pout, X, order, blk_start, blk_key, G, S, vgrid, ds);
-    else
-        hipLaunchKernelGGL(k_g2p<false>, grd, blk, 0, stream, Pin, Pout, X, order, blk_start, blk_key, G, S, vgrid, ds);
+#define SV_G2P(LA, SE) \
+    hipLaunchKernelGGL((k_g2p<LA, SE>), grd, blk, 0, stream, Pin, Pout, X, order, blk_start, blk_key, G, S, vgrid, keys_out, ds)
+    if (L.model_set == SET_POLAR) {
+        if (last) SV_G2P(true, SET_POLAR);
+        else SV_G2P(false, SET_POLAR);
+    } else {
+        if (last) SV_G2P(true, SET_ALL);
+        else SV_G2P(false, SET_ALL);
+    }
+#undef SV_G2P
     return hipGetLastError();
 }
 
 hipError_t launch_copy_dead(const ParticleArrays &Pin, const ParticleArrays &Pout, const uint32_t *order, uint64_t n,
-                            DeviceScalars *ds, hipStream_t stream) {
+                            uint32_t *keys_out, uint32_t tomb, DeviceScalars *ds, hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 1024);
-    hipLaunchKernelGGL(k_copy_dead, dim3(grid), dim3(256), 0, stream, Pin, Pout, order, n, ds);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_copy_dead, dim3(grid), dim3(256), 0, stream, Pin, Pout, order, n, keys_out, tomb, ds);
     return hipGetLastError();
 }
 
 hipError_t launch_unpermute(const ParticleArrays &P, uint64_t n, float *x, float *v, float *F, float *C,
                             uint32_t *flags, hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 8192);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 8192);
     hipLaunchKernelGGL(k_unpermute, dim3(grid), dim3(256), 0, stream, P, n, x, v, F, C, flags);
     return hipGetLastError();
 }

@@ -150,10 +150,13 @@ def test_stress_matches_fp64_oracle(engine_lib, flags_value, params):
     tau = np.transpose(d_tau.cpu().numpy(), (0, 2, 1)).astype(np.float64)
     energy = d_e.cpu().numpy().astype(np.float64)
     rtau, _, re = pyoracle.stress(F32.astype(np.float64), flags, p, jp.astype(np.float64), hardening=10.0)
-    # Per particle: max |tau_gpu - tau_fp64| <= 1e-4 * max |tau_fp64| (float32 arithmetic).
+    # Per particle: max |tau_gpu - tau_fp64| <= tol * max |tau_fp64| in float32
+    # arithmetic; tol = 1e-4, and 1e-3 for the inviscid model, whose J^-7
+    # multiplies the float rounding of det F by the exponent.
+    tol = 1e-3 if flags_value & abi.IS_FLUID else 1e-4
     scale = np.abs(rtau).reshape(n, -1).max(1) + 1e-3
     err = np.abs(tau - rtau).reshape(n, -1).max(1)
-    assert np.all(err <= 1e-4 * scale), (err / scale).max()
+    assert np.all(err <= tol * scale), (err / scale).max()
     assert np.allclose(energy, re, rtol=1e-3, atol=1e-3 * np.abs(re).max())
 
 

@@ -101,18 +101,18 @@ def test_stagewise_substep_matches_oracle():
     compare_states(g.to_io_state().particles, o64.particles_f64(), sc.consts.grid_node_size, "stages vs fp64")
 
 
-@pytest.mark.parametrize("waves", [1, 2, 4])
-def test_ten_substeps_match_oracle(waves):
+@pytest.mark.parametrize("threads", [64, 128, 256])
+def test_ten_substeps_match_oracle(threads):
     sc = perturbed_cube()
     T = scenes.advance_time(0.0, sc.time_step, 10)
-    g = engine(sc, tuning={"p2g_waves_per_block": waves})
+    g = engine(sc, tuning={"g2p_threads": threads})
     st, err = g.produce_next_state(sc.frame_input, GpuRunParameters(target_time=T))
     assert err is None and g.last_status.substeps == 10
     o32, o64 = oracles(sc)
     o32.step_to(T, sc.frame_input)
     o64.step_to(T, sc.frame_input)
-    compare_states(st.particles, o64.particles_f64(), sc.consts.grid_node_size, f"10 substeps vs fp64 (waves={waves})")
-    compare_states(st.particles, o32.particles_f64(), sc.consts.grid_node_size, f"10 substeps vs fp32 (waves={waves})")
+    compare_states(st.particles, o64.particles_f64(), sc.consts.grid_node_size, f"10 substeps vs fp64 (g2p threads={threads})")
+    compare_states(st.particles, o32.particles_f64(), sc.consts.grid_node_size, f"10 substeps vs fp32 (g2p threads={threads})")
 
 
 def test_cube_32768_particles_parity_over_frames():
