@@ -401,7 +401,8 @@ int mpm_create(const mpm_consts *consts, const mpm_particles_view *pv, const mpm
 
     hipDeviceProp_t prop{};
     if (hipGetDeviceProperties(&prop, e->device) != hipSuccess) return fail(MPM_ERR_HIP);
-    e->launch.persistent_blocks = prop.multiProcessorCount * 4;
+    e->launch.persistent_blocks = 0;
+    e->launch.cu_count = prop.multiProcessorCount;
     e->launch.g2p_threads = 256;
     if (tuning) {
         if (tuning->persistent_blocks > 0) e->launch.persistent_blocks = tuning->persistent_blocks;

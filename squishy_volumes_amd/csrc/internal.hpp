@@ -132,9 +132,10 @@ hipError_t launch_reset_scalars(DeviceScalars *ds, hipStream_t stream);
 
 // ---- transfer.hip
 struct TransferLaunch {
-    int persistent_blocks;
+    int persistent_blocks;  // work-groups of the block-loop kernels; 0: all resident ones
     int g2p_threads;
-    uint32_t model_set;  // SET_POLAR or SET_ALL
+    uint32_t model_set;     // SET_POLAR or SET_ALL
+    int cu_count;
 };
 hipError_t launch_p2g(const TransferLaunch &L, bool from_c, const ParticleArrays &P, const AuxArrays &X,
                       const uint32_t *order, const uint32_t *blk_start, const uint32_t *blk_key,

@@ -4,12 +4,12 @@
 //   P2G   per occupied 4^3 block, 4 waves: the block's particles are loaded
 //         once (coalesced through the sort order), reduced to a 16-float
 //         payload, and grouped by cell in LDS.  Lane l of every wave owns cell
-//         l and accumulates its particles' 3x3x3 stencils in registers; the
-//         waves then add their stencils into a 6^3-node float4 arena one wave
-//         at a time (plain LDS read-add-write: within one wave no two lanes
+//         l and accumulates its particles' stencil one 3x3 plane at a time in
+//         registers; each wave adds its planes into its own 6^3-node float4
+//         arena with plain LDS read-add-write (within one wave no two lanes
 //         touch the same node, and the arena layout puts them on distinct
-//         banks).  The arena is flushed to the dense node grid with global
-//         float atomics.  Mirrors scatter_momentum
+//         banks).  The summed arenas are flushed to the dense node grid with
+//         global float atomics.  Mirrors scatter_momentum
 //         (rust/crates/cpu/src/phase/scatter_momentum.rs:22-93).
 //   grid  v = p / m per node of every touched block (meld_grid without
 //         collider twins, rust/crates/cpu/src/phase/meld_grid.rs:61-66) plus the
@@ -37,7 +37,7 @@ namespace {
 
 constexpr int P2G_THREADS = 256;
 constexpr int P2G_WAVES = P2G_THREADS / 64;
-constexpr int P2G_CAP = 512;  // particles staged per pass
+constexpr int P2G_CAP = 256;  // particles staged per pass
 constexpr int P2G_PER_THREAD = P2G_CAP / P2G_THREADS;
 constexpr int ARENA = 512;    // float4 slots of the 6^3 arena (8 rows x 64 banks)
 constexpr int G2P_NODES = 216;
@@ -64,6 +64,20 @@ __device__ __forceinline__ void decode_block(uint32_t key, uint32_t b[3]) {
 // Dense node index of local node (x, y, z).
 __device__ __forceinline__ uint32_t node_index(const GridGeom &G, uint32_t x, uint32_t y, uint32_t z) {
     return block_linear(G, x >> 2, y >> 2, z >> 2) * 64u + (x & 3u) + ((y & 3u) << 2) + ((z & 3u) << 4);
+}
+
+// The block-list range of this work-group.  Work-group w runs on XCD w % 8;
+// the Morton-ordered block list is cut into 8 contiguous parts, one per XCD,
+// so that neighbouring blocks, which share halo nodes, share an L2.  The
+// work-groups of an XCD stride through its part.  gridDim.x is a multiple of 8.
+struct BlockRange {
+    uint32_t begin, end, step;
+};
+__device__ __forceinline__ BlockRange xcd_block_range(uint32_t nblocks) {
+    const uint32_t x = blockIdx.x & 7u, r = blockIdx.x >> 3, per = gridDim.x >> 3;
+    const uint32_t chunk = (nblocks + 7u) >> 3;
+    const uint32_t b0 = min(nblocks, x * chunk), b1 = min(nblocks, b0 + chunk);
+    return {b0 + r, b1, per};
 }
 
 __device__ __noinline__ bool svd3_call(const float F[9], float U[9], float s[3], float V[9]) {
@@ -148,7 +162,7 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g(ParticleArrays P, AuxArrays
                                                      const uint32_t *__restrict__ blk_key, GridGeom G, SubstepParams S,
                                                      float4 *__restrict__ grid, uint32_t *__restrict__ blk_flag,
                                                      uint32_t *__restrict__ cand, DeviceScalars *ds) {
-    __shared__ float4 arena[ARENA];
+    __shared__ float4 arena[P2G_WAVES][ARENA];  // one arena per wave: no barriers between their adds
     __shared__ float4 pay[P2G_CAP][4];
     __shared__ uint32_t cell_count[64];
     __shared__ uint32_t cell_start[64];
@@ -162,7 +176,8 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g(ParticleArrays P, AuxArrays
     const float h = S.h;
     const float scaling = S.dt * 4.0f / (h * h);
 
-    for (uint32_t bi = blockIdx.x; bi < nblocks; bi += gridDim.x) {
+    const BlockRange range = xcd_block_range(nblocks);
+    for (uint32_t bi = range.begin; bi < range.end; bi += range.step) {
         const uint32_t start = blk_start[bi];
         const uint32_t end = bi + 1 < nblocks ? blk_start[bi + 1] : live;
         uint32_t b[3];
@@ -170,7 +185,14 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g(ParticleArrays P, AuxArrays
         int cell0[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) cell0[a] = G.origin[a] + 4 * (int)b[a];
-        for (int k = tid; k < ARENA; k += P2G_THREADS) arena[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        // Claim the 8 blocks the arena covers (first claimant lists the block
+        // for the grid update); issued first so the latency overlaps the staging.
+        if (tid < 8) {
+            const uint32_t T = block_linear(G, b[0] + (tid & 1u), b[1] + ((tid >> 1) & 1u), b[2] + (tid >> 2));
+            const uint32_t old = atomicExch(&blk_flag[T], 1u);
+            cand[8u * bi + tid] = old == 0u ? T : NO_BLOCK;
+        }
+        for (int k = lane; k < ARENA; k += 64) arena[wave][k] = make_float4(0.f, 0.f, 0.f, 0.f);
 
         for (uint32_t cs = start; cs < end; cs += P2G_CAP) {
             const uint32_t ce = min(end, cs + (uint32_t)P2G_CAP);
@@ -276,72 +298,82 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g(ParticleArrays P, AuxArrays
             }
             __syncthreads();
 
-            // Accumulate: lane = cell, wave w takes every P2G_WAVES-th particle.
+            // Accumulate: lane = cell, wave w takes every P2G_WAVES-th particle
+            // of the cell; one stencil plane (fixed i) at a time, 36 registers.
             const uint32_t cnt = cell_count[lane];
             const uint32_t st = cell_start[lane];
-            float acc[27][4];
-#pragma unroll
-            for (int n = 0; n < 27; ++n)
-#pragma unroll
-                for (int c = 0; c < 4; ++c) acc[n][c] = 0.f;
-            for (uint32_t r = wave; r < cnt; r += P2G_WAVES) {
-                const float4 a0 = pay[st + r][0], a1 = pay[st + r][1], a2 = pay[st + r][2], a3 = pay[st + r][3];
-                const float fx[3] = {a0.x, a0.y, a0.z};
-                const float m = a0.w;
-                const float q[3] = {a1.x, a1.y, a1.z};
-                const float c0[3] = {a1.w, a2.x, a2.y};
-                const float c1[3] = {a2.z, a2.w, a3.x};
-                const float c2[3] = {a3.y, a3.z, a3.w};
-                float w[3][3];
-#pragma unroll
-                for (int a = 0; a < 3; ++a) bspline_weights(fx[a], w[a]);
-#pragma unroll
+            const int cx = (int)(lane & 3u), cy = (int)((lane >> 2) & 3u), cz = (int)(lane >> 4);
+            if (cnt > wave) {
+#pragma unroll 1
                 for (int i = 0; i < 3; ++i) {
+                    float acc[9][4];
 #pragma unroll
-                    for (int j = 0; j < 3; ++j) {
-                        const float wij = w[0][i] * w[1][j];
-                        float t[3];
+                    for (int n = 0; n < 9; ++n)
 #pragma unroll
-                        for (int r3 = 0; r3 < 3; ++r3) t[r3] = q[r3] + (float)i * c0[r3] + (float)j * c1[r3];
+                        for (int c = 0; c < 4; ++c) acc[n][c] = 0.f;
+                    for (uint32_t r = wave; r < cnt; r += P2G_WAVES) {
+                        const float4 a0 = pay[st + r][0], a1 = pay[st + r][1], a2 = pay[st + r][2], a3 = pay[st + r][3];
+                        const float m = a0.w;
+                        const float c0[3] = {a1.w, a2.x, a2.y};
+                        const float c1[3] = {a2.z, a2.w, a3.x};
+                        const float c2[3] = {a3.y, a3.z, a3.w};
+                        float wx[3], wy[3], wz[3];
+                        bspline_weights(a0.x, wx);
+                        bspline_weights(a0.y, wy);
+                        bspline_weights(a0.z, wz);
+                        const float wi = i == 0 ? wx[0] : (i == 1 ? wx[1] : wx[2]);
+                        const float fi = (float)i;
+                        const float ti[3] = {a1.x + fi * c0[0], a1.y + fi * c0[1], a1.z + fi * c0[2]};
 #pragma unroll
-                        for (int k = 0; k < 3; ++k) {
-                            const float wt = wij * w[2][k];
-                            const int n = 9 * i + 3 * j + k;
-                            acc[n][0] += wt * m;
+                        for (int j = 0; j < 3; ++j) {
+                            const float wij = wi * wy[j];
 #pragma unroll
-                            for (int r3 = 0; r3 < 3; ++r3) acc[n][1 + r3] += wt * (t[r3] + (float)k * c2[r3]);
+                            for (int k = 0; k < 3; ++k) {
+                                const float wt = wij * wz[k];
+                                const int n = 3 * j + k;
+                                acc[n][0] += wt * m;
+#pragma unroll
+                                for (int r3 = 0; r3 < 3; ++r3)
+                                    acc[n][1 + r3] += wt * (ti[r3] + (float)j * c1[r3] + (float)k * c2[r3]);
+                            }
                         }
+                    }
+                    // Add the plane into this wave's arena, one stencil offset per
+                    // read-add-write round: for a fixed offset the 64 lanes hit 64
+                    // distinct nodes, while two different offsets of two lanes can
+                    // hit the same node, so a round's write must land before the
+                    // next round's read.
+#pragma unroll
+                    for (int n = 0; n < 9; ++n) {
+                        const int slot = arena_slot(cx + i, cy + n / 3, cz + n % 3);
+                        float4 g = arena[wave][slot];
+                        g.x += acc[n][0];
+                        g.y += acc[n][1];
+                        g.z += acc[n][2];
+                        g.w += acc[n][3];
+                        arena[wave][slot] = g;
+                        // Keeps the compiler from moving the next round's read
+                        // above this write (LDS ops of one wave execute in order).
+                        __builtin_amdgcn_wave_barrier();
                     }
                 }
             }
-            // Waves add their stencils one after the other.
-            const int cx = (int)(lane & 3u), cy = (int)((lane >> 2) & 3u), cz = (int)(lane >> 4);
-            for (uint32_t w = 0; w < (uint32_t)P2G_WAVES; ++w) {
-                if (wave == w && cnt > wave) {
-#pragma unroll
-                    for (int i = 0; i < 3; ++i)
-#pragma unroll
-                        for (int j = 0; j < 3; ++j)
-#pragma unroll
-                            for (int k = 0; k < 3; ++k) {
-                                const int n = 9 * i + 3 * j + k;
-                                const int slot = arena_slot(cx + i, cy + j, cz + k);
-                                float4 g = arena[slot];
-                                g.x += acc[n][0];
-                                g.y += acc[n][1];
-                                g.z += acc[n][2];
-                                g.w += acc[n][3];
-                                arena[slot] = g;
-                            }
-                }
-                __syncthreads();
-            }
+            __syncthreads();
         }
 
-        // Flush the 6^3 arena to the grid; claim the 8 blocks it covers.
+        // Flush the sum of the wave arenas (6^3 nodes) to the grid.
         for (int nidx = tid; nidx < 216; nidx += P2G_THREADS) {
             const int ax = nidx % 6, ay = (nidx / 6) % 6, az = nidx / 36;
-            const float4 g4 = arena[arena_slot(ax, ay, az)];
+            const int slot = arena_slot(ax, ay, az);
+            float4 g4 = arena[0][slot];
+#pragma unroll
+            for (int w = 1; w < P2G_WAVES; ++w) {
+                const float4 o = arena[w][slot];
+                g4.x += o.x;
+                g4.y += o.y;
+                g4.z += o.z;
+                g4.w += o.w;
+            }
             if (g4.x != 0.f) {
                 const uint32_t node = node_index(G, 4u * b[0] + ax, 4u * b[1] + ay, 4u * b[2] + az);
                 float *g = reinterpret_cast<float *>(grid + node);
@@ -350,11 +382,6 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g(ParticleArrays P, AuxArrays
                 unsafeAtomicAdd(g + 2, g4.z);
                 unsafeAtomicAdd(g + 3, g4.w);
             }
-        }
-        if (tid < 8) {
-            const uint32_t T = block_linear(G, b[0] + (tid & 1u), b[1] + ((tid >> 1) & 1u), b[2] + (tid >> 2));
-            const uint32_t old = atomicExch(&blk_flag[T], 1u);
-            cand[8u * bi + tid] = old == 0u ? T : NO_BLOCK;
         }
         __syncthreads();
     }
@@ -455,7 +482,8 @@ __global__ __launch_bounds__(256) void k_g2p(ParticleArrays Pin, ParticleArrays 
     const float scaling = dt * 4.0f / (h * h);
     const float apic = 4.0f / h;
 
-    for (uint32_t bi = blockIdx.x; bi < nblocks; bi += gridDim.x) {
+    const BlockRange range = xcd_block_range(nblocks);
+    for (uint32_t bi = range.begin; bi < range.end; bi += range.step) {
         const uint32_t start = blk_start[bi];
         const uint32_t end = bi + 1 < nblocks ? blk_start[bi + 1] : live;
         uint32_t b[3];
@@ -487,7 +515,8 @@ __global__ __launch_bounds__(256) void k_g2p(ParticleArrays Pin, ParticleArrays 
             float C[9];
 #pragma unroll
             for (int k = 0; k < 9; ++k) C[k] = 0.f;
-#pragma unroll
+            // Planes one at a time: keeps the LDS reads of a single plane in flight.
+#pragma unroll 1
             for (int i3 = 0; i3 < 3; ++i3) {
                 const float dx = (float)i3 - fx[0];
 #pragma unroll
@@ -755,15 +784,28 @@ __global__ __launch_bounds__(256) void k_svd(const float *__restrict__ Fin, floa
     for (int a = 0; a < 3; ++a) so[3 * i + a] = s[a];
 }
 
+// Work-groups of a persistent launch: all resident at once (the occupancy of
+// the kernel times the CU count), a multiple of 8 for the XCD mapping.
+int resident_groups(const void *kernel, int threads, const TransferLaunch &L) {
+    if (L.persistent_blocks > 0) return std::max(8, L.persistent_blocks & ~7);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    return std::max(8, (per_cu * L.cu_count) & ~7);
+}
+
 template <uint32_t SET>
 hipError_t p2g_dispatch(const TransferLaunch &L, bool from_c, bool goals, const ParticleArrays &P, const AuxArrays &X,
                         const uint32_t *order, const uint32_t *blk_start, const uint32_t *blk_key, const GridGeom &G,
                         const SubstepParams &S, float4 *grid, uint32_t *blk_flag, uint32_t *cand, DeviceScalars *ds,
                         hipStream_t stream) {
-    dim3 grd(L.persistent_blocks), blk(P2G_THREADS);
-#define SV_P2G(FC, GO)                                                                                              \
-    hipLaunchKernelGGL((k_p2g<FC, GO, SET>), grd, blk, 0, stream, P, X, order, blk_start, blk_key, G, S, grid, blk_flag, \
-                       cand, ds)
+    const dim3 blk(P2G_THREADS);
+#define SV_P2G(FC, GO)                                                                                                  \
+    do {                                                                                                                \
+        static const int groups = resident_groups(reinterpret_cast<const void *>(&k_p2g<FC, GO, SET>), P2G_THREADS, L); \
+        hipLaunchKernelGGL((k_p2g<FC, GO, SET>), dim3(groups), blk, 0, stream, P, X, order, blk_start, blk_key, G, S,    \
+                           grid, blk_flag, cand, ds);                                                                   \
+    } while (0)
     if (from_c) {
         if (goals) SV_P2G(true, true);
         else SV_P2G(true, false);
@@ -789,12 +831,15 @@ hipError_t launch_p2g(const TransferLaunch &L, bool from_c, const ParticleArrays
                                  stream);
 }
 
-uint32_t grid_update_waves(const TransferLaunch &L) { return (uint32_t)L.persistent_blocks * 4u; }
+// The grid update runs on a fixed grid so that its per-wave partial counts
+// have a fixed size.
+uint32_t grid_update_groups(const TransferLaunch &L) { return (uint32_t)std::max(8, 4 * L.cu_count); }
+uint32_t grid_update_waves(const TransferLaunch &L) { return grid_update_groups(L) * 4u; }
 
 hipError_t launch_grid_update(const TransferLaunch &L, const uint32_t *cand, const GridGeom &G, const SubstepParams &S,
                               float4 *grid, float4 *vgrid, uint32_t *blk_flag, uint32_t *blk_epoch, uint32_t *partial,
                               DeviceScalars *ds, hipStream_t stream) {
-    hipLaunchKernelGGL(k_grid_update, dim3(L.persistent_blocks), dim3(256), 0, stream, cand, G, S, grid, vgrid, blk_flag,
+    hipLaunchKernelGGL(k_grid_update, dim3(grid_update_groups(L)), dim3(256), 0, stream, cand, G, S, grid, vgrid, blk_flag,
                        blk_epoch, partial, ds);
     return hipGetLastError();
 }
@@ -809,9 +854,13 @@ hipError_t launch_g2p(const TransferLaunch &L, bool last, const ParticleArrays &
                       const AuxArrays &X, const uint32_t *order, const uint32_t *blk_start, const uint32_t *blk_key,
                       const GridGeom &G, const SubstepParams &S, const float4 *vgrid, uint32_t *keys_out,
                       DeviceScalars *ds, hipStream_t stream) {
-    dim3 grd(L.persistent_blocks), blk(L.g2p_threads);
-#define SV_G2P(LA, SE) \
-    hipLaunchKernelGGL((k_g2p<LA, SE>), grd, blk, 0, stream, Pin, Pout, X, order, blk_start, blk_key, G, S, vgrid, keys_out, ds)
+    const dim3 blk(L.g2p_threads);
+#define SV_G2P(LA, SE)                                                                                              \
+    do {                                                                                                            \
+        const int groups = resident_groups(reinterpret_cast<const void *>(&k_g2p<LA, SE>), L.g2p_threads, L);       \
+        hipLaunchKernelGGL((k_g2p<LA, SE>), dim3(groups), blk, 0, stream, Pin, Pout, X, order, blk_start, blk_key, G, \
+                           S, vgrid, keys_out, ds);                                                                 \
+    } while (0)
     if (L.model_set == SET_POLAR) {
         if (last) SV_G2P(true, SET_POLAR);
         else SV_G2P(false, SET_POLAR);
