@@ -181,24 +181,12 @@ int compute_geometry(const mpm_consts &c, GridGeom &g, int &key_bits) {
     return MPM_OK;
 }
 
+// Whole 32-particle tiles, rounded to an even count so a wave of 64 never
+// reads past the allocation.
+size_t particle_tiles(uint64_t n) { return (size_t)((n + 63) / 64) * 2; }
+
 hipError_t alloc_particles(mpm_engine *e, ParticleArrays &P) {
-    const size_t n = e->n;
-    hipError_t r;
-    for (int a = 0; a < 3; ++a) {
-        if ((r = e->alloc(P.x[a], n)) != hipSuccess) return r;
-        if ((r = e->alloc(P.v[a], n)) != hipSuccess) return r;
-    }
-    for (int k = 0; k < 9; ++k) {
-        if ((r = e->alloc(P.F[k], n)) != hipSuccess) return r;
-        if ((r = e->alloc(P.A[k], n)) != hipSuccess) return r;
-    }
-    if ((r = e->alloc(P.m, n)) != hipSuccess) return r;
-    if ((r = e->alloc(P.vol, n)) != hipSuccess) return r;
-    if ((r = e->alloc(P.p0, n)) != hipSuccess) return r;
-    if ((r = e->alloc(P.p1, n)) != hipSuccess) return r;
-    if ((r = e->alloc(P.flags, n)) != hipSuccess) return r;
-    if ((r = e->alloc(P.id, n)) != hipSuccess) return r;
-    return hipSuccess;
+    return e->alloc(P.base, particle_tiles(e->n) * NFIELDS * TILE);
 }
 
 SubstepParams make_params(const mpm_engine *e, const float gravity[3], float factor_b, bool goals, bool has_b) {
@@ -456,47 +444,47 @@ int mpm_create(const mpm_consts *consts, const mpm_particles_view *pv, const mpm
     if (pv->collider_bits) std::copy(pv->collider_bits, pv->collider_bits + n, e->collider_bits.begin());
     e->dead_at_create.assign(n, 0);
     std::vector<float> col(n);
-    std::vector<uint32_t> ucol(n);
     auto up = [&](void *dst, const void *src, size_t bytes) {
         return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
     };
-    ParticleArrays &P = e->P[0];
-    bool ok = true;
-    for (int a = 0; a < 3 && ok; ++a) {
-        for (size_t i = 0; i < n; ++i) col[i] = pv->positions[3 * i + a];
-        ok = ok && up(P.x[a], col.data(), n * 4);
-        for (size_t i = 0; i < n; ++i) col[i] = pv->velocities[3 * i + a];
-        ok = ok && up(P.v[a], col.data(), n * 4);
-    }
-    for (int k = 0; k < 9 && ok; ++k) {
-        for (size_t i = 0; i < n; ++i) col[i] = pv->position_gradients[9 * i + k];
-        ok = ok && up(P.F[k], col.data(), n * 4);
-        for (size_t i = 0; i < n; ++i) col[i] = pv->velocity_gradients[9 * i + k];
-        ok = ok && up(P.A[k], col.data(), n * 4);
-    }
+    // Particle fields are staged in the device's tiled layout and uploaded
+    // in one copy.
+    std::vector<float> tiles(particle_tiles(n) * NFIELDS * TILE, 0.f);
+    auto put = [&](int f, size_t i, float value) { tiles[(i >> 5) * (NFIELDS * TILE) + f * TILE + (i & 31)] = value; };
+    auto putu = [&](int f, size_t i, uint32_t value) {
+        std::memcpy(&tiles[(i >> 5) * (NFIELDS * TILE) + f * TILE + (i & 31)], &value, 4);
+    };
     const mpm_particle_parameters *pp = pv->parameters;
+    for (size_t i = 0; i < n; ++i) {
+        const bool fluid = (pv->flags[i] & MPM_IS_FLUID) != 0;
+        for (int a = 0; a < 3; ++a) {
+            put(FX + a, i, pv->positions[3 * i + a]);
+            put(FV + a, i, pv->velocities[3 * i + a]);
+        }
+        for (int k = 0; k < 9; ++k) {
+            put(FF + k, i, pv->position_gradients[9 * i + k]);
+            put(FA + k, i, pv->velocity_gradients[9 * i + k]);
+        }
+        put(FM, i, pp[i].mass);
+        put(FVOL, i, pp[i].initial_volume);
+        put(FP0, i, fluid ? pp[i].bulk_modulus : pp[i].mu);
+        put(FP1, i, fluid ? (float)pp[i].exponent : pp[i].lambda);
+        putu(FFLAGS, i, pv->flags[i]);
+        putu(FID, i, (uint32_t)i);
+        e->dead_at_create[i] = (pv->flags[i] & MPM_TOMBSTONED) ? 1 : 0;
+    }
+    ParticleArrays &P = e->P[0];
+    bool ok = up(P.base, tiles.data(), tiles.size() * 4);
+    std::vector<float>().swap(tiles);
     auto upf = [&](float *dst, auto getter) {
         for (size_t i = 0; i < n; ++i) col[i] = getter(i);
         return up(dst, col.data(), n * 4);
     };
-    ok = ok && upf(P.m, [&](size_t i) { return pp[i].mass; });
-    ok = ok && upf(P.vol, [&](size_t i) { return pp[i].initial_volume; });
-    ok = ok && upf(P.p0, [&](size_t i) { return (pv->flags[i] & MPM_IS_FLUID) ? pp[i].bulk_modulus : pp[i].mu; });
-    ok = ok && upf(P.p1, [&](size_t i) {
-        return (pv->flags[i] & MPM_IS_FLUID) ? (float)pp[i].exponent : pp[i].lambda;
-    });
     ok = ok && upf(e->X.visc_dyn, [&](size_t i) { return pp[i].viscosity_dynamic; });
     ok = ok && upf(e->X.visc_bulk, [&](size_t i) { return pp[i].viscosity_bulk; });
     ok = ok && upf(e->X.sand_alpha, [&](size_t i) { return pp[i].sand_alpha; });
     ok = ok && upf(e->X.jp, [&](size_t i) { return pv->plastic_jp ? pv->plastic_jp[i] : 1.0f; });
     ok = ok && upf(e->X.energy, [&](size_t i) { return pv->elastic_energies ? pv->elastic_energies[i] : 0.0f; });
-    for (size_t i = 0; i < n; ++i) {
-        ucol[i] = pv->flags[i];
-        e->dead_at_create[i] = (pv->flags[i] & MPM_TOMBSTONED) ? 1 : 0;
-    }
-    ok = ok && up(P.flags, ucol.data(), n * 4);
-    for (size_t i = 0; i < n; ++i) ucol[i] = (uint32_t)i;
-    ok = ok && up(P.id, ucol.data(), n * 4);
     if (!ok) return fail(MPM_ERR_HIP);
     if (hipStreamSynchronize(e->stream) != hipSuccess) return fail(MPM_ERR_HIP);
     *out_engine = e;

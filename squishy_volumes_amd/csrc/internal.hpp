@@ -12,21 +12,39 @@ namespace svamd {
 
 constexpr uint32_t TOMB_KEY_FLAG = 0xffffffffu;
 
-// One set of permuted particle arrays, structure of arrays.  F and A are nine
-// component arrays each, component k = 3*col + row.  A holds the fused affine
-// momentum m C - dt 4/h^2 V0 (tau + J sigma_visc) between substeps and the
-// APIC matrix C at the end of a step_to call.
+// One set of permuted particle arrays in tiles of 32 particles (AoSoA): tile
+// t holds NFIELDS rows of 32 floats, so field f of particle i lives at
+//   base + t * NFIELDS * 32 + f * 32 + (i & 31),   t = i >> 5.
+// One vector address per particle reaches every field through the load's
+// immediate offset (f * 128 bytes < 4 KiB), and a wave of 64 consecutive
+// particles touches two full 128-byte segments per field.  F and A are nine
+// component fields each, component k = 3*col + row.  A holds the fused
+// affine momentum m C - dt 4/h^2 V0 (tau + J sigma_visc) between substeps and
+// the APIC matrix C at the end of a step_to call.  p0/p1 are mu/lambda for
+// solids and the bulk modulus/exponent for fluids; id is the original
+// particle index (the reference's sort_map).
+enum : int { FX = 0, FV = 3, FF = 6, FA = 15, FM = 24, FVOL = 25, FP0 = 26, FP1 = 27, FFLAGS = 28, FID = 29, NFIELDS = 30 };
+constexpr int TILE = 32;
+
 struct ParticleArrays {
-    float *x[3];
-    float *v[3];
-    float *F[9];
-    float *A[9];
-    float *m;
-    float *vol;
-    float *p0;  // mu (solids) or bulk modulus (fluids)
-    float *p1;  // lambda (solids) or exponent (fluids)
-    uint32_t *flags;
-    uint32_t *id;  // original particle index (sort_map of the reference)
+    float *base;
+    __host__ __device__ __forceinline__ float *at(uint64_t i) const {
+        return base + (i >> 5) * (uint64_t)(NFIELDS * TILE) + (i & 31u);
+    }
+    __host__ __device__ __forceinline__ float &x(int a, uint64_t i) const { return at(i)[(FX + a) * TILE]; }
+    __host__ __device__ __forceinline__ float &v(int a, uint64_t i) const { return at(i)[(FV + a) * TILE]; }
+    __host__ __device__ __forceinline__ float &F(int k, uint64_t i) const { return at(i)[(FF + k) * TILE]; }
+    __host__ __device__ __forceinline__ float &A(int k, uint64_t i) const { return at(i)[(FA + k) * TILE]; }
+    __host__ __device__ __forceinline__ float &m(uint64_t i) const { return at(i)[FM * TILE]; }
+    __host__ __device__ __forceinline__ float &vol(uint64_t i) const { return at(i)[FVOL * TILE]; }
+    __host__ __device__ __forceinline__ float &p0(uint64_t i) const { return at(i)[FP0 * TILE]; }
+    __host__ __device__ __forceinline__ float &p1(uint64_t i) const { return at(i)[FP1 * TILE]; }
+    __host__ __device__ __forceinline__ uint32_t &flags(uint64_t i) const {
+        return *reinterpret_cast<uint32_t *>(&at(i)[FFLAGS * TILE]);
+    }
+    __host__ __device__ __forceinline__ uint32_t &id(uint64_t i) const {
+        return *reinterpret_cast<uint32_t *>(&at(i)[FID * TILE]);
+    }
 };
 
 // Arrays indexed by original particle index: never permuted.

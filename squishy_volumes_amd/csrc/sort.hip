@@ -221,12 +221,12 @@ __global__ __launch_bounds__(256) void k_keys(ParticleArrays P, GridGeom G, uint
                                               DeviceScalars *ds) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint32_t flags = P.flags[i];
+    uint32_t flags = P.flags(i);
     if (flags & F_TOMB) {
         keys[i] = G.tomb_key;
         return;
     }
-    const float x[3] = {P.x[0][i], P.x[1][i], P.x[2][i]};
+    const float x[3] = {P.x(0, i), P.x(1, i), P.x(2, i)};
     keys[i] = particle_block_key(x, G, ds);
 }
 

@@ -121,12 +121,12 @@ __device__ __forceinline__ void fused_affine(float m, float vol, float scaling, 
 template <uint32_t SET>
 __device__ __forceinline__ void affine_from_state(const ParticleArrays &P, const AuxArrays &X, uint32_t p, float m,
                                                   float scaling, float hardening, DeviceScalars *ds, float A[9]) {
-    const uint32_t flags = P.flags[p];
+    const uint32_t flags = P.flags(p);
     float F[9], C[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-        F[k] = P.F[k][p];
-        C[k] = P.A[k][p];
+        F[k] = P.F(k, p);
+        C[k] = P.A(k, p);
     }
     float J = det3(F);
     float R[9], U[9], s[3] = {1.f, 1.f, 1.f};
@@ -134,7 +134,7 @@ __device__ __forceinline__ void affine_from_state(const ParticleArrays &P, const
     if ((SET & M_SNOW) && (flags & F_SNOW)) {
         float V[9];
         if (svd3_call(F, U, s, V)) atomicOr(&ds->status, MPM_STATUS_SVD_FP64);
-        hard = expf(hardening * (1.0f - X.jp[P.id[p]]));
+        hard = expf(hardening * (1.0f - X.jp[P.id(p)]));
     } else if ((SET & M_FCR) && (flags & F_FCR)) {
         if (!polar3(F, R)) {
             float V[9];
@@ -145,14 +145,14 @@ __device__ __forceinline__ void affine_from_state(const ParticleArrays &P, const
     if (!(J >= 1e-7f) && !(flags & F_FLUID))
         atomicOr(&ds->status, MPM_STATUS_PARTICLE_CLOSE_TO_INVERTED | (KID_P2G << 16));
     float tau[9];
-    kirchhoff<SET>(flags, P.p0[p], P.p1[p], hard, F, J, R, U, s, tau);
+    kirchhoff<SET>(flags, P.p0(p), P.p1(p), hard, F, J, R, U, s, tau);
     const bool visc = flags & F_VISC;
     float vd = 0.f, vb = 0.f;
     if ((SET & M_VISC) && visc) {
-        vd = X.visc_dyn[P.id[p]];
-        vb = X.visc_bulk[P.id[p]];
+        vd = X.visc_dyn[P.id(p)];
+        vb = X.visc_bulk[P.id(p)];
     }
-    fused_affine<SET>(m, P.vol[p], scaling, C, tau, J, visc, vd, vb, A);
+    fused_affine<SET>(m, P.vol(p), scaling, C, tau, J, visc, vd, vb, A);
 }
 
 // ------------------------------------------------------------------- P2G
@@ -217,7 +217,7 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g(ParticleArrays P, AuxArrays
                 bool inside = true;
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
-                    xp[a] = P.x[a][p];
+                    xp[a] = P.x(a, p);
                     const float xn = xp[a] / h;
                     const float fl = floorf(xn - 0.5f);
                     fx[a] = xn - fl;
@@ -228,12 +228,12 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g(ParticleArrays P, AuxArrays
                     atomicOr(&ds->status, MPM_STATUS_OUT_OF_GRID | (KID_P2G << 16));
                     continue;
                 }
-                const float m = P.m[p];
-                float v[3] = {P.v[0][p], P.v[1][p], P.v[2][p]};
+                const float m = P.m(p);
+                float v[3] = {P.v(0, p), P.v(1, p), P.v(2, p)};
                 // external_force (rust/crates/cpu/src/phase/external_force.rs:36-47).
                 bool goal = false;
                 if constexpr (GOALS) {
-                    const uint32_t o = P.id[p];
+                    const uint32_t o = P.id(p);
                     const uint32_t fb = S.has_b ? X.goal_flags_b[o] : X.goal_flags_a[o];
                     if ((X.goal_flags_a[o] & F_GOAL) && (fb & F_GOAL)) {
                         goal = true;
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g(ParticleArrays P, AuxArrays
                     affine_from_state<SET>(P, X, p, m, scaling, S.snow_hardening, ds, A);
                 } else {
 #pragma unroll
-                    for (int q = 0; q < 9; ++q) A[q] = P.A[q][p];
+                    for (int q = 0; q < 9; ++q) A[q] = P.A(q, p);
                 }
                 float *o = pl[k];
                 o[0] = fx[0];
@@ -503,7 +503,7 @@ __global__ __launch_bounds__(256) void k_g2p(ParticleArrays Pin, ParticleArrays 
             int c[3];
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                xp[a] = Pin.x[a][p];
+                xp[a] = Pin.x(a, p);
                 const float xn = xp[a] / h;
                 const float fl = floorf(xn - 0.5f);
                 fx[a] = xn - fl;
@@ -543,12 +543,12 @@ __global__ __launch_bounds__(256) void k_g2p(ParticleArrays Pin, ParticleArrays 
             for (int k = 0; k < 9; ++k) C[k] *= apic;
 
             // advance_particles: x += dt v, F += (C F) dt, plasticity, energy.
-            uint32_t flags = Pin.flags[p];
-            const uint32_t id = Pin.id[p];
-            const float m = Pin.m[p], vol = Pin.vol[p], p0 = Pin.p0[p], p1 = Pin.p1[p];
+            uint32_t flags = Pin.flags(p);
+            const uint32_t id = Pin.id(p);
+            const float m = Pin.m(p), vol = Pin.vol(p), p0 = Pin.p0(p), p1 = Pin.p1(p);
             float F[9], CF[9];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) F[k] = Pin.F[k][p];
+            for (int k = 0; k < 9; ++k) F[k] = Pin.F(k, p);
             matmul3(C, F, CF);
 #pragma unroll
             for (int k = 0; k < 9; ++k) F[k] += CF[k] * dt;
@@ -657,20 +657,20 @@ __global__ __launch_bounds__(256) void k_g2p(ParticleArrays Pin, ParticleArrays 
 
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                Pout.x[a][i] = xnew[a];
-                Pout.v[a][i] = v[a];
+                Pout.x(a, i) = xnew[a];
+                Pout.v(a, i) = v[a];
             }
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
-                Pout.F[k][i] = F[k];
-                Pout.A[k][i] = Aout[k];
+                Pout.F(k, i) = F[k];
+                Pout.A(k, i) = Aout[k];
             }
-            Pout.m[i] = m;
-            Pout.vol[i] = vol;
-            Pout.p0[i] = p0;
-            Pout.p1[i] = p1;
-            Pout.flags[i] = flags;
-            Pout.id[i] = id;
+            Pout.m(i) = m;
+            Pout.vol(i) = vol;
+            Pout.p0(i) = p0;
+            Pout.p1(i) = p1;
+            Pout.flags(i) = flags;
+            Pout.id(i) = id;
         }
         __syncthreads();
     }
@@ -687,20 +687,20 @@ __global__ __launch_bounds__(256) void k_copy_dead(ParticleArrays Pin, ParticleA
         const uint32_t p = order[i];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            Pout.x[a][i] = Pin.x[a][p];
-            Pout.v[a][i] = Pin.v[a][p];
+            Pout.x(a, i) = Pin.x(a, p);
+            Pout.v(a, i) = Pin.v(a, p);
         }
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
-            Pout.F[k][i] = Pin.F[k][p];
-            Pout.A[k][i] = Pin.A[k][p];
+            Pout.F(k, i) = Pin.F(k, p);
+            Pout.A(k, i) = Pin.A(k, p);
         }
-        Pout.m[i] = Pin.m[p];
-        Pout.vol[i] = Pin.vol[p];
-        Pout.p0[i] = Pin.p0[p];
-        Pout.p1[i] = Pin.p1[p];
-        Pout.flags[i] = Pin.flags[p];
-        Pout.id[i] = Pin.id[p];
+        Pout.m(i) = Pin.m(p);
+        Pout.vol(i) = Pin.vol(p);
+        Pout.p0(i) = Pin.p0(p);
+        Pout.p1(i) = Pin.p1(p);
+        Pout.flags(i) = Pin.flags(p);
+        Pout.id(i) = Pin.id(p);
         if (keys_out) keys_out[i] = tomb;
     }
 }
@@ -711,18 +711,18 @@ __global__ __launch_bounds__(256) void k_unpermute(ParticleArrays P, uint64_t n,
                                                    float *__restrict__ v, float *__restrict__ F,
                                                    float *__restrict__ C, uint32_t *__restrict__ flags) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t o = P.id[i];
+        const uint32_t o = P.id(i);
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            x[3ull * o + a] = P.x[a][i];
-            v[3ull * o + a] = P.v[a][i];
+            x[3ull * o + a] = P.x(a, i);
+            v[3ull * o + a] = P.v(a, i);
         }
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
-            F[9ull * o + k] = P.F[k][i];
-            C[9ull * o + k] = P.A[k][i];
+            F[9ull * o + k] = P.F(k, i);
+            C[9ull * o + k] = P.A(k, i);
         }
-        flags[o] = P.flags[i];
+        flags[o] = P.flags(i);
     }
 }
 
