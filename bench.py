@@ -60,6 +60,26 @@ def cpu_baseline(scene, substeps: int, threads: int) -> dict:
     }
 
 
+def copy_calibration(mib: int):
+    """Three launches of the engine's plain copy kernel over `mib` MiB."""
+    import ctypes as C
+
+    import torch
+
+    from squishy_volumes_amd import abi
+
+    lib = abi.load_library()
+    src = torch.ones(mib << 18, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    stream = torch.cuda.current_stream().cuda_stream
+    for _ in range(3):
+        rc = lib.mpm_dev_copy(abi.ptr(src), abi.ptr(dst), src.numel() * 4, C.c_void_p(stream))
+        if rc != 0:
+            raise RuntimeError(f"mpm_dev_copy failed: {rc}")
+    torch.cuda.synchronize()
+    log(f"bench: copy calibration, 3 x {mib} MiB")
+
+
 def traffic_for(kernel: str, workload: str):
     f = ROOT / "profiles" / f"traffic_{kernel}_{workload}.json"
     if f.exists():
@@ -81,6 +101,13 @@ def main():
     ap.add_argument("--p2g-waves", type=int, default=0)
     ap.add_argument("--g2p-threads", type=int, default=0)
     ap.add_argument("--persistent-blocks", type=int, default=0)
+    ap.add_argument(
+        "--copy-mib",
+        type=int,
+        default=0,
+        help="first launch the plain copy kernel (k_copy_bytes) three times over a buffer of this many MiB, "
+        "outside the timed region, so that hardware counters can be scaled by a copy of known size",
+    )
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -95,6 +122,9 @@ def main():
 
     from squishy_volumes_amd import scenes
     from squishy_volumes_amd.gpu_state import GpuState
+
+    if args.copy_mib:
+        copy_calibration(args.copy_mib)
 
     scene = scenes.spheres_and_slab()
     n = scene.io_state.particles.count
