@@ -21,16 +21,19 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 # Algorithmic bytes per unit of work of each kernel (the dataflow with the
-# stress evaluated in G2P and the fused affine momentum A stored):
+# stress evaluated after the gather and the fused affine momentum A formed on
+# chip):
 #   P2G  reads x, v, A, m                        64 B / particle; writes (m, p) 16 B / active node
 #   grid reads and writes (m, p | v)             32 B / active node
 #   G2P  reads x, F, m, V0, mu, lambda           64 B / particle; writes x, v, F, A 96 B / particle;
 #        reads v                                 16 B / active node
 #   substep total: 224 B / particle + 64 B / active node.
+# The fused gather-scatter kernel (g2p2g) does a G2P and the next P2G.
 BYTES = {
     "p2g": (64, 16),
     "grid_update": (0, 32),
     "g2p": (160, 16),
+    "g2p2g": (224, 32),
 }
 
 
@@ -98,8 +101,7 @@ def main():
     ap.add_argument("--cpu-substeps", type=int, default=6)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--p2g-waves", type=int, default=0)
-    ap.add_argument("--g2p-threads", type=int, default=0)
+    ap.add_argument("--resort-interval", type=int, default=0, help="substeps between particle sorts (0: engine default)")
     ap.add_argument("--persistent-blocks", type=int, default=0)
     ap.add_argument(
         "--copy-mib",
@@ -130,10 +132,8 @@ def main():
     n = scene.io_state.particles.count
     log(f"bench: {scene.name}, {n} particles, warmup {args.warmup}, steps {args.steps}")
     tuning = {}
-    if args.p2g_waves:
-        tuning["p2g_waves_per_block"] = args.p2g_waves
-    if args.g2p_threads:
-        tuning["g2p_threads"] = args.g2p_threads
+    if args.resort_interval:
+        tuning["resort_interval"] = args.resort_interval
     if args.persistent_blocks:
         tuning["persistent_blocks"] = args.persistent_blocks
     gpu = GpuState.from_io_state(0, scene.consts, scene.time_step, scene.io_state, scene.options, tuning)

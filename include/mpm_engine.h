@@ -189,10 +189,11 @@ typedef struct mpm_engine mpm_engine;
 
 /* Engine tuning (the autotuner's knobs).  Zero fields take defaults. */
 typedef struct mpm_tuning {
-    int32_t p2g_waves_per_block;         /* 1, 2 or 4 */
-    int32_t g2p_threads;                 /* 64 .. 1024, a multiple of 64 */
-    int32_t persistent_blocks;           /* work-groups of the block-loop kernels */
-    int32_t sort_items_per_thread;       /* keys per thread and radix tile */
+    int32_t resort_interval;             /* substeps between particle sorts (default 4); between
+                                            sorts a particle keeps its slot and may drift one cell
+                                            out of its block; farther drift takes a slower path */
+    int32_t persistent_blocks;           /* work-groups of the transfer kernel (default: all resident) */
+    int32_t reserved[2];
 } mpm_tuning;
 
 /* ---- engine API (host pointers) ------------------------------------------ */
@@ -275,9 +276,10 @@ int mpm_dev_stress(const uint32_t *flags, const mpm_particle_parameters *params,
 int mpm_dev_svd(const float *F, float *U, float *s, float *V, uint64_t n, void *stream);
 
 /* One substep stage at a time on an engine, for the per-kernel parity tests:
- * stage 0 = sort, 1 = P2G, 2 = grid update, 3 = G2P (first substep of a call:
- * P2G evaluates the stress from F and C).  Host-side grid/particle readback
- * goes through mpm_read_grid / mpm_read_particles. */
+ * stage 0 = sort, 1 = P2G from the stored state (stress from F and C),
+ * 2 = grid update, 3 = G2P writing the state (x, v, F, C), as the last
+ * substep of a call does.  Host-side grid/particle readback goes through
+ * mpm_read_grid / mpm_read_particles. */
 int mpm_run_stage(mpm_engine *engine, int32_t stage, const mpm_frame_input *input);
 
 /* Raw P2G result (mass, momentum before normalisation) of the last P2G stage

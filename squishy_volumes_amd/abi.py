@@ -147,10 +147,9 @@ class MpmGridView(C.Structure):
 
 class MpmTuning(C.Structure):
     _fields_ = [
-        ("p2g_waves_per_block", C.c_int32),
-        ("g2p_threads", C.c_int32),
+        ("resort_interval", C.c_int32),
         ("persistent_blocks", C.c_int32),
-        ("sort_items_per_thread", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
 

@@ -21,8 +21,9 @@ using namespace svamd;
 
 namespace {
 
-enum Stage { ST_KEYS = 0, ST_SORT, ST_BLOCKS, ST_P2G, ST_GRID, ST_G2P, ST_COPY, ST_COUNT };
-const char *kStageNames[ST_COUNT] = {"keys", "radix_sort", "block_list", "p2g", "grid_update", "g2p", "copy_dead"};
+enum Stage { ST_KEYS = 0, ST_SORT, ST_BLOCKS, ST_P2G, ST_GRID, ST_G2P2G, ST_G2P, ST_COPY, ST_COUNT };
+const char *kStageNames[ST_COUNT] = {"keys",        "radix_sort", "block_list", "p2g",
+                                     "grid_update", "g2p2g",      "g2p",        "copy_dead"};
 
 thread_local std::string g_last_error;
 
@@ -62,9 +63,17 @@ struct mpm_engine {
     void *blk_temp = nullptr;
     float4 *grid = nullptr, *vgrid = nullptr;
     uint32_t *blk_flag = nullptr, *blk_epoch = nullptr;
-    uint32_t *cand = nullptr;     // 8 candidate touched blocks per occupied block
+    uint32_t *cand = nullptr;     // NCAND candidate touched blocks per occupied block
+    uint32_t *ovf[2] = {nullptr, nullptr};  // overflow lists of claimed blocks, by epoch parity
+    float4 *part = nullptr;       // partial region arena per block-list entry
+    uint32_t part_capacity = 0;
+    uint64_t *entry = nullptr;    // per block: (sort generation << 32) | list index
+    uint32_t *blk_atomic = nullptr;
+    uint32_t sort_gen = 0;
     uint32_t *partial = nullptr;  // per-wave active-node counts of the grid update
     bool keys_valid = false;      // e->keys holds the keys of the current particle order
+    bool pending_permute = false; // a sort ran since the last transfer
+    int resort_interval = 4;      // substeps between sorts
     DeviceScalars *ds = nullptr;
     DeviceScalars *ds_host = nullptr;  // pinned
 
@@ -72,7 +81,6 @@ struct mpm_engine {
     uint32_t epoch = 0;
     uint32_t grid_epoch = 0;   // epoch stamped by the last grid update
     bool grid_valid = false;   // a grid update has run
-    bool next_from_c = true;
     bool stepped = false;
     bool goals_allocated = false;
 
@@ -256,7 +264,23 @@ int upload_goals(mpm_engine *e, const mpm_frame_input *in, bool &goals) {
     return MPM_OK;
 }
 
-// Sort, block list: everything before P2G.
+GridBuffers grid_buffers(const mpm_engine *e, uint32_t epoch) {
+    GridBuffers B;
+    B.grid = e->grid;
+    B.vgrid = e->vgrid;
+    B.part = e->part;
+    B.part_capacity = e->part_capacity;
+    B.entry = e->entry;
+    B.gen = e->sort_gen;
+    B.blk_flag = e->blk_flag;
+    B.blk_atomic = e->blk_atomic;
+    B.blk_epoch = e->blk_epoch;
+    B.cand = e->cand;
+    B.ovf = e->ovf[epoch & 1u];
+    return B;
+}
+
+// Sort, block list: everything before a transfer that permutes.
 int run_sort(mpm_engine *e) {
     EventRecord r{};
     ParticleArrays &P = e->P[e->cur];
@@ -271,47 +295,60 @@ int run_sort(mpm_engine *e) {
                        e->stream));
     e->end(r);
     e->begin(ST_BLOCKS, r);
+    e->sort_gen += 1;
     HIP_TRY(launch_block_list(e->keys_sorted, e->n, e->geom.tomb_key, e->blk_start, e->blk_key, e->blk_capacity,
-                              e->ds, e->blk_temp, e->stream));
+                              e->geom, e->entry, e->sort_gen, e->ds, e->blk_temp, e->stream));
     e->end(r);
+    e->keys_valid = false;
+    e->pending_permute = true;
     return MPM_OK;
 }
 
-int run_p2g(mpm_engine *e, const SubstepParams &S) {
+// One transfer launch.  After a sort the transfer reads slot i from
+// order[i] and writes the other particle set; otherwise it works in place.
+// want_keys: write the block keys of the new positions for the next sort.
+int run_transfer(mpm_engine *e, TransferKind kind, const SubstepParams &S0, const SubstepParams &S1, bool want_keys) {
     EventRecord r{};
-    e->begin(ST_P2G, r);
-    HIP_TRY(launch_p2g(e->launch, e->next_from_c, e->P[e->cur], e->X, e->order, e->blk_start, e->blk_key, e->geom, S,
-                       e->grid, e->blk_flag, e->cand, e->ds, e->stream));
+    TransferCall t{};
+    t.kind = kind;
+    t.permute = e->pending_permute;
+    t.Pin = e->P[e->cur];
+    t.Pout = e->P[1 - e->cur];
+    t.X = e->X;
+    t.order = e->order;
+    t.blk_start = e->blk_start;
+    t.blk_key = e->blk_key;
+    t.G = e->geom;
+    t.S0 = S0;
+    t.S1 = S1;
+    t.B = grid_buffers(e, S1.epoch);
+    t.keys_out = want_keys ? e->keys : nullptr;
+    t.ds = e->ds;
+    const int stage = kind == TransferKind::P2G ? ST_P2G : (kind == TransferKind::G2P2G ? ST_G2P2G : ST_G2P);
+    e->begin(stage, r);
+    HIP_TRY(launch_transfer(e->launch, t, e->stream));
     e->end(r);
-    e->next_from_c = false;
+    if (t.permute || want_keys) {
+        // Slots past the live particles: copied when permuting, tombstone keys.
+        e->begin(ST_COPY, r);
+        HIP_TRY(launch_copy_dead(t.Pin, t.permute ? t.Pout : t.Pin, e->order, e->n, t.keys_out, e->geom.tomb_key,
+                                 e->ds, e->stream));
+        e->end(r);
+    }
+    if (t.permute) e->cur = 1 - e->cur;
+    e->pending_permute = false;
+    e->keys_valid = want_keys;
     return MPM_OK;
 }
 
+// Grid update of the grid the last transfer scattered (epoch S.epoch).
 int run_grid(mpm_engine *e, const SubstepParams &S) {
     EventRecord r{};
     e->begin(ST_GRID, r);
-    HIP_TRY(launch_grid_update(e->launch, e->cand, e->geom, S, e->grid, e->vgrid, e->blk_flag, e->blk_epoch,
-                               e->partial, e->ds, e->stream));
+    HIP_TRY(launch_grid_update(e->launch, grid_buffers(e, S.epoch), e->geom, S, false, e->partial, e->ds, e->stream));
     e->end(r);
     e->grid_epoch = S.epoch;
     e->grid_valid = true;
-    return MPM_OK;
-}
-
-int run_g2p(mpm_engine *e, const SubstepParams &S, bool last) {
-    EventRecord r{};
-    ParticleArrays &Pin = e->P[e->cur];
-    ParticleArrays &Pout = e->P[1 - e->cur];
-    e->begin(ST_G2P, r);
-    HIP_TRY(launch_g2p(e->launch, last, Pin, Pout, e->X, e->order, e->blk_start, e->blk_key, e->geom, S, e->vgrid,
-                       e->keys, e->ds, e->stream));
-    e->end(r);
-    e->begin(ST_COPY, r);
-    HIP_TRY(launch_copy_dead(Pin, Pout, e->order, e->n, e->keys, e->geom.tomb_key, e->ds, e->stream));
-    e->end(r);
-    e->keys_valid = true;
-    e->cur = 1 - e->cur;
-    if (last) e->next_from_c = true;
     e->epoch += 1;
     return MPM_OK;
 }
@@ -391,11 +428,9 @@ int mpm_create(const mpm_consts *consts, const mpm_particles_view *pv, const mpm
     if (hipGetDeviceProperties(&prop, e->device) != hipSuccess) return fail(MPM_ERR_HIP);
     e->launch.persistent_blocks = 0;
     e->launch.cu_count = prop.multiProcessorCount;
-    e->launch.g2p_threads = 256;
     if (tuning) {
         if (tuning->persistent_blocks > 0) e->launch.persistent_blocks = tuning->persistent_blocks;
-        if (tuning->g2p_threads >= 64 && tuning->g2p_threads <= 1024 && tuning->g2p_threads % 64 == 0)
-            e->launch.g2p_threads = tuning->g2p_threads;
+        if (tuning->resort_interval > 0) e->resort_interval = tuning->resort_interval;
     }
     // Kernels are specialised on the constitutive models present: snow and
     // sand need the full SVD, every other model a polar decomposition at most.
@@ -405,8 +440,6 @@ int mpm_create(const mpm_consts *consts, const mpm_particles_view *pv, const mpm
             e->launch.model_set = SET_ALL;
             break;
         }
-    // The G2P kernel is compiled for at most 256 threads per work-group.
-    e->launch.g2p_threads = std::min(e->launch.g2p_threads, 256);
 
     const size_t n = e->n;
     const uint64_t max_blocks = std::max<uint64_t>(std::min<uint64_t>(n, e->geom.num_blocks_total), 1);
@@ -423,15 +456,23 @@ int mpm_create(const mpm_consts *consts, const mpm_particles_view *pv, const mpm
         e->alloc(e->blk_key, max_blocks) || e->alloc(bt, block_list_temp_bytes(n)) || e->alloc(e->grid, nodes) ||
         e->alloc(e->vgrid, nodes) || e->alloc(e->blk_flag, e->geom.num_blocks_total) ||
         e->alloc(e->blk_epoch, e->geom.num_blocks_total) || e->alloc(e->ds, 1) ||
-        e->alloc(e->cand, 8 * max_blocks) || e->alloc(e->partial, grid_update_waves(e->launch)))
+        e->alloc(e->cand, NCAND * max_blocks) || e->alloc(e->ovf[0], e->geom.num_blocks_total) ||
+        e->alloc(e->ovf[1], e->geom.num_blocks_total) || e->alloc(e->partial, grid_update_waves(e->launch)) ||
+        e->alloc(e->entry, e->geom.num_blocks_total) || e->alloc(e->blk_atomic, e->geom.num_blocks_total))
         return fail(MPM_ERR_OUT_OF_MEMORY);
     e->sort_temp = st;
     e->blk_temp = bt;
+    // Partial region arenas (8 KiB per block-list entry) for up to 2^18
+    // entries; entries past them flush with atomics.
+    e->part_capacity = (uint32_t)std::min<uint64_t>(max_blocks, 1u << 18);
+    if (e->alloc(e->part, (size_t)e->part_capacity * 512)) return fail(MPM_ERR_OUT_OF_MEMORY);
     if (hipHostMalloc((void **)&e->ds_host, sizeof(DeviceScalars), hipHostMallocDefault) != hipSuccess)
         return fail(MPM_ERR_OUT_OF_MEMORY);
     if (hipMemsetAsync(e->grid, 0, nodes * 16, e->stream) != hipSuccess ||
         hipMemsetAsync(e->vgrid, 0, nodes * 16, e->stream) != hipSuccess ||
         hipMemsetAsync(e->blk_flag, 0, (size_t)e->geom.num_blocks_total * 4, e->stream) != hipSuccess ||
+        hipMemsetAsync(e->blk_atomic, 0, (size_t)e->geom.num_blocks_total * 4, e->stream) != hipSuccess ||
+        hipMemsetAsync(e->entry, 0, (size_t)e->geom.num_blocks_total * 8, e->stream) != hipSuccess ||
         hipMemsetAsync(e->blk_epoch, 0xff, (size_t)e->geom.num_blocks_total * 4, e->stream) != hipSuccess ||
         hipMemsetAsync(e->ds, 0, sizeof(DeviceScalars), e->stream) != hipSuccess)
         return fail(MPM_ERR_HIP);
@@ -521,22 +562,32 @@ int mpm_step_to(mpm_engine *e, double target_time, const mpm_frame_input *input,
     }
     HIP_TRY(hipMemsetAsync(&e->ds->status, 0, 4, e->stream));
 
+    // Transfer t gathers substep t-1 (t >= 1) and scatters substep t (t < k):
+    // a scatter from the stored state, k-1 fused gather-scatters, a last
+    // gather.  The particles are sorted before the first transfer and then
+    // every resort_interval substeps; in between they keep their slots.
+    const size_t k = params.size();
+    const size_t K = (size_t)std::max(1, e->resort_interval);
+    auto sorts_before = [&](size_t t) { return t == 0 || (t < k && t % K == 0); };
     uint64_t done = 0;
     int result = MPM_OK;
-    for (size_t s = 0; s < params.size(); ++s) {
-        if (cancel && *cancel) {
+    for (size_t t = 0; k > 0 && t <= k; ++t) {
+        if (t > 0 && cancel && *cancel) {
             result = MPM_ERR_CANCELED;
             break;
         }
-        SubstepParams S = params[s];
-        S.epoch = e->epoch;
-        const bool last = s + 1 == params.size();
-        if ((rc = run_sort(e)) != MPM_OK) return rc;
-        if ((rc = run_p2g(e, S)) != MPM_OK) return rc;
-        if ((rc = run_grid(e, S)) != MPM_OK) return rc;
-        if ((rc = run_g2p(e, S, last)) != MPM_OK) return rc;
-        e->time = times[s] + (double)e->consts.time_step;
-        ++done;
+        if (sorts_before(t) && (rc = run_sort(e)) != MPM_OK) return rc;
+        const bool want_keys = t == k || sorts_before(t + 1);
+        SubstepParams S0 = params[t > 0 ? t - 1 : 0];
+        SubstepParams S1 = params[t < k ? t : k - 1];
+        S1.epoch = e->epoch;
+        const TransferKind kind = t == 0 ? TransferKind::P2G : (t == k ? TransferKind::G2P : TransferKind::G2P2G);
+        if ((rc = run_transfer(e, kind, S0, S1, want_keys)) != MPM_OK) return rc;
+        if (t < k && (rc = run_grid(e, S1)) != MPM_OK) return rc;
+        if (t > 0) {
+            e->time = times[t - 1] + (double)e->consts.time_step;
+            ++done;
+        }
     }
     if (done > 0) e->stepped = true;
     if ((rc = fetch_scalars(e)) != MPM_OK) return rc;
@@ -612,13 +663,21 @@ static int read_grid_impl(mpm_engine *e, mpm_grid_view *out, bool momentum) {
     const uint64_t nblk = e->geom.num_blocks_total;
     std::vector<float4> g(nblk * 64);
     std::vector<uint32_t> ep(nblk);
+    if (momentum) {
+        // Sum the pending scatter (the partial arenas of the last transfer
+        // and its global-memory scatters) into the velocity grid, stamped
+        // with the pending epoch; the grid update recomputes it.
+        SubstepParams S{};
+        S.epoch = e->epoch;
+        HIP_TRY(launch_grid_update(e->launch, grid_buffers(e, S.epoch), e->geom, S, true, e->partial, e->ds, e->stream));
+    }
     HIP_TRY(hipStreamSynchronize(e->stream));
-    HIP_TRY(hipMemcpy(g.data(), momentum ? e->grid : e->vgrid, nblk * 64 * 16, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(g.data(), e->vgrid, nblk * 64 * 16, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(ep.data(), e->blk_epoch, nblk * 4, hipMemcpyDeviceToHost));
     uint64_t count = 0;
     const int *nb = e->geom.nb;
     for (uint64_t B = 0; B < nblk; ++B) {
-        if (!momentum && (!e->grid_valid || ep[B] != e->grid_epoch)) continue;
+        if (momentum ? ep[B] != e->epoch : (!e->grid_valid || ep[B] != e->grid_epoch)) continue;
         const int bx = (int)(B % nb[0]), by = (int)((B / nb[0]) % nb[1]), bz = (int)(B / ((uint64_t)nb[0] * nb[1]));
         for (int s = 0; s < 64; ++s) {
             const float4 q = g[B * 64 + s];
@@ -655,12 +714,13 @@ int mpm_run_stage(mpm_engine *e, int32_t stage, const mpm_frame_input *input) {
     bool goals = false;
     if ((rc = upload_goals(e, input, goals)) != MPM_OK) return rc;
     SubstepParams S = make_params(e, g, fb, goals, input && input->has_b);
+    S.epoch = e->epoch;
     switch (stage) {
         case 0: rc = run_sort(e); break;
-        case 1: rc = run_p2g(e, S); break;
+        case 1: rc = run_transfer(e, TransferKind::P2G, S, S, false); break;
         case 2: rc = run_grid(e, S); break;
         case 3:
-            rc = run_g2p(e, S, true);
+            rc = run_transfer(e, TransferKind::G2P, S, S, true);
             if (rc == MPM_OK) {
                 e->time += (double)e->consts.time_step;
                 e->stepped = true;

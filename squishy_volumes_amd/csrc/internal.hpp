@@ -78,8 +78,13 @@ struct DeviceScalars {
     uint32_t live_count;    // particles with a valid key (sorted to the front)
     uint32_t status;        // MPM_STATUS_* bits | kernel id << 16
     uint32_t active_nodes;  // nodes with m > 0 after the last grid update (summed on request)
-    uint32_t pad[4];
+    uint32_t ovf_count[2];  // overflow-list lengths, by parity of the grid's epoch
+    uint32_t pad[2];
 };
+
+// Candidate blocks listed per block-list entry for the grid update: the 27
+// blocks around the entry's block that its region reaches.
+constexpr int NCAND = 27;
 
 // Block key of a position: Morton code of the 4^3 block holding the base
 // cell floor(x/h - 1/2) (position_to_shift_quadratic,
@@ -143,32 +148,68 @@ hipError_t exclusive_scan(const uint32_t *in, uint32_t *out, uint32_t *total, ui
 hipError_t launch_keys(const ParticleArrays &P, const GridGeom &G, uint64_t n, uint32_t *keys, DeviceScalars *ds,
                        hipStream_t stream);
 size_t block_list_temp_bytes(uint64_t n);
+// The occupied-block list of the sorted keys; also stamps entry[block] =
+// (gen << 32) | list index for every listed block.
 hipError_t launch_block_list(const uint32_t *keys, uint64_t n, uint32_t tomb_key, uint32_t *blk_start,
-                             uint32_t *blk_key, uint32_t capacity, DeviceScalars *ds, void *temp, hipStream_t stream);
+                             uint32_t *blk_key, uint32_t capacity, const GridGeom &G, uint64_t *entry, uint32_t gen,
+                             DeviceScalars *ds, void *temp, hipStream_t stream);
 hipError_t launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t stream);
 hipError_t launch_reset_scalars(DeviceScalars *ds, hipStream_t stream);
 
 // ---- transfer.hip
 struct TransferLaunch {
     int persistent_blocks;  // work-groups of the block-loop kernels; 0: all resident ones
-    int g2p_threads;
     uint32_t model_set;     // SET_POLAR or SET_ALL
     int cu_count;
 };
-hipError_t launch_p2g(const TransferLaunch &L, bool from_c, const ParticleArrays &P, const AuxArrays &X,
-                      const uint32_t *order, const uint32_t *blk_start, const uint32_t *blk_key,
-                      const GridGeom &G, const SubstepParams &S, float4 *grid, uint32_t *blk_flag, uint32_t *cand,
-                      DeviceScalars *ds, hipStream_t stream);
+
+// The three modes of the transfer kernel: the first scatter of a call (from
+// the stored v, C, F), gather + advance + next scatter, and the last gather.
+enum class TransferKind { P2G, G2P2G, G2P };
+
+// Grid-side buffers shared by the transfer kernel and the grid update.
+// The scatter of each block-list entry leaves its 8^3 region arena in part
+// (plain stores); the grid update sums, for every node of a touched block,
+// the arenas of the (up to 8) entries whose regions cover it.  Only the rare
+// global-memory scatters (and entries beyond part_capacity) add into grid
+// with atomics, marking blk_atomic.
+struct GridBuffers {
+    float4 *grid;               // (m, p) from global-memory scatters
+    float4 *vgrid;              // (m, v) after the grid update
+    float4 *part;               // [entry][arena slot]
+    uint32_t part_capacity;     // entries that have a partial arena
+    uint64_t *entry;            // per block: (sort generation << 32) | list index
+    uint32_t gen;               // current sort generation
+    uint32_t *blk_flag;         // claimed for the grid update
+    uint32_t *blk_atomic;       // grid holds atomic contributions for the block
+    uint32_t *blk_epoch;        // epoch of the block's last grid update
+    uint32_t *cand;             // NCAND claimed blocks per list entry
+    uint32_t *ovf;              // overflow list of claimed blocks (parity of the epoch)
+};
+
+struct TransferCall {
+    TransferKind kind;
+    bool permute;               // slot i takes Pin[order[i]] and is written to Pout; else in place in Pin
+    ParticleArrays Pin, Pout;
+    AuxArrays X;
+    const uint32_t *order;
+    const uint32_t *blk_start;
+    const uint32_t *blk_key;
+    GridGeom G;
+    SubstepParams S0;           // substep of the gather
+    SubstepParams S1;           // substep of the scatter
+    GridBuffers B;              // B.ovf: the list for S1's grid (parity S1.epoch & 1)
+    uint32_t *keys_out;         // null: keys not needed
+    DeviceScalars *ds;
+};
+hipError_t launch_transfer(const TransferLaunch &L, const TransferCall &t, hipStream_t stream);
 uint32_t grid_update_waves(const TransferLaunch &L);
-hipError_t launch_grid_update(const TransferLaunch &L, const uint32_t *cand, const GridGeom &G,
-                              const SubstepParams &S, float4 *grid, float4 *vgrid, uint32_t *blk_flag,
-                              uint32_t *blk_epoch, uint32_t *partial, DeviceScalars *ds, hipStream_t stream);
+// momentum: only sum (m, p) into vgrid and stamp the epoch (a read-back of
+// the scattered momentum), leaving claims and atomic contributions in place.
+hipError_t launch_grid_update(const TransferLaunch &L, const GridBuffers &B, const GridGeom &G, const SubstepParams &S,
+                              bool momentum, uint32_t *partial, DeviceScalars *ds, hipStream_t stream);
 hipError_t launch_sum_partials(const TransferLaunch &L, const uint32_t *partial, DeviceScalars *ds,
                                hipStream_t stream);
-hipError_t launch_g2p(const TransferLaunch &L, bool last, const ParticleArrays &Pin, const ParticleArrays &Pout,
-                      const AuxArrays &X, const uint32_t *order, const uint32_t *blk_start, const uint32_t *blk_key,
-                      const GridGeom &G, const SubstepParams &S, const float4 *vgrid, uint32_t *keys_out,
-                      DeviceScalars *ds, hipStream_t stream);
 hipError_t launch_copy_dead(const ParticleArrays &Pin, const ParticleArrays &Pout, const uint32_t *order,
                             uint64_t n, uint32_t *keys_out, uint32_t tomb, DeviceScalars *ds, hipStream_t stream);
 hipError_t launch_unpermute(const ParticleArrays &P, uint64_t n, float *x, float *v, float *F, float *C,

@@ -1,27 +1,35 @@
-// transfer.hip - P2G, grid update and G2P of the MLS-MPM substep (gfx950).
+// transfer.hip - particle/grid transfers of the MLS-MPM substep (gfx950).
 //
-// Dataflow of one substep (after the block sort):
-//   P2G   per occupied 4^3 block, 4 waves: the block's particles are loaded
-//         once (coalesced through the sort order), reduced to a 16-float
-//         payload, and grouped by cell in LDS.  Lane l of every wave owns cell
-//         l and accumulates its particles' stencil one 3x3 plane at a time in
-//         registers; each wave adds its planes into its own 6^3-node float4
-//         arena with plain LDS read-add-write (within one wave no two lanes
-//         touch the same node, and the arena layout puts them on distinct
-//         banks).  The summed arenas are flushed to the dense node grid with
-//         global float atomics.  Mirrors scatter_momentum
-//         (rust/crates/cpu/src/phase/scatter_momentum.rs:22-93).
-//   grid  v = p / m per node of every touched block (meld_grid without
-//         collider twins, rust/crates/cpu/src/phase/meld_grid.rs:61-66) plus the
-//         optional grid gravity and boundary conditions; clears the
-//         accumulation grid.
-//   G2P   per occupied block: the 6^3 node velocities are staged in LDS, each
-//         thread gathers v and C of one particle (collect_velocity,
-//         rust/crates/cpu/src/phase/collect_velocity.rs:19-75), advances x and
-//         F (advance_particles, rust/crates/cpu/src/phase/advance_particles.rs:17-93),
-//         culls (rust/crates/cpu/src/phase/cull_particles.rs:17-41), and writes
-//         the fused affine momentum A = m C - dt 4/h^2 V0 (tau + J sigma_visc)
-//         for the next P2G, permuting the particle arrays into block order.
+// One persistent kernel, k_transfer<MODE, PERMUTE, GOALS, SET>, moves data
+// between particles and grid.  It walks the occupied 4^3 blocks of the sorted
+// block list (XCD-aware contiguous ranges) with one 4-wave work-group per
+// block at a time.  Its modes:
+//
+//   XM_P2G    first substep of a call: the stress from the stored F and C, the
+//             external force and the scatter (scatter_momentum,
+//             rust/crates/cpu/src/phase/scatter_momentum.rs:22-93).
+//   XM_G2P2G  every further substep: the gather of substep n (collect_velocity,
+//             rust/crates/cpu/src/phase/collect_velocity.rs:19-75), the
+//             advance and cull of substep n (advance_particles.rs:17-93,
+//             cull_particles.rs:17-41), then the external force and scatter of
+//             substep n+1 from the registers.  The fused affine momentum
+//             A = m C - dt 4/h^2 V0 (tau + J sigma_visc) and the particle
+//             velocity never leave the chip.
+//   XM_G2P    last substep of a call: the gather, advance and cull; writes the
+//             reference's state x, v, F, C and the elastic energies.
+//
+// Block frame.  Between sorts a particle keeps its slot in its block's range
+// while it moves, so the kernel accepts base cells one cell outside the
+// block: cells -1..4 per axis, stencil nodes -1..6 (an 8^3 node region).  The
+// gather reads the region's node velocities from LDS; the scatter buckets the
+// particles by cell (216 cells) and accumulates with lane = cell: for one
+// stencil offset the lanes of a wave then touch distinct nodes, so the
+// region's arena takes conflict-free LDS float atomics.  Waves 0-2 take the 64
+// cells inside the block, one stencil plane each; wave 3 takes the sparse
+// shell of cells around it.  A particle outside the region (it moved more
+// than a cell since the last sort) takes a global-memory gather and scatter.
+// The arena is flushed with global float atomics; the blocks it touched are
+// listed for the grid update (first claimant only).
 //
 // The kernels are specialised on the set of constitutive models present: the
 // common set (Neo-Hookean, fluid, fixed-corotated, viscosity) needs only a
@@ -35,18 +43,27 @@ namespace svamd {
 
 namespace {
 
-constexpr int P2G_THREADS = 256;
-constexpr int P2G_WAVES = P2G_THREADS / 64;
-constexpr int P2G_CAP = 256;  // particles staged per pass
-constexpr int P2G_PER_THREAD = P2G_CAP / P2G_THREADS;
-constexpr int ARENA = 512;    // float4 slots of the 6^3 arena (8 rows x 64 banks)
-constexpr int G2P_NODES = 216;
+constexpr int XF_THREADS = 256;
+constexpr int XF_CAP = 512;            // particles per pass of a block
+constexpr int XF_PER_THREAD = XF_CAP / XF_THREADS;
+constexpr int NSLOT = 512;             // 8^3 node region
+constexpr int NCELL = 216;             // 6^3 cells
+constexpr int XF_RANKS = 8;            // core payloads per cell and pass
+constexpr int XF_OVF = 128;            // overflow payloads per pass
+// Node velocities of the region in LDS: node (x, y, z) at x + VS_Y y + VS_Z z.
+// The strides make the 2x2x3 base cells that 16 consecutive particles of a
+// jittered 8-per-cell lattice share fall on distinct 16-byte bank groups.
+constexpr int VS_Y = 10;
+constexpr int VS_Z = 84;
+constexpr int VS_SLOTS = 7 + 7 * VS_Y + 7 * VS_Z + 1;
 constexpr uint32_t NO_BLOCK = 0xffffffffu;
 
-// LDS slot of arena node (x, y, z), 0 <= x, y, z < 6.  The low six bits are
-// (x&3) | (y&3)<<2 | (z&3)<<4: for one stencil offset the 64 lanes (cells)
-// of a wave address 64 distinct slots, and any 16 consecutive lanes (one
-// ds_read_b128 phase) address 16 distinct 16-byte bank groups.
+enum : int { XM_P2G = 0, XM_G2P2G = 1, XM_G2P = 2 };
+
+// Arena slot of region node (x, y, z), 0 <= x, y, z < 8 (region node = block
+// node + 1).  The low six bits are (x&3) | (y&3)<<2 | (z&3)<<4: for one
+// stencil offset the 64 cells of a 4^3 cube address 64 distinct slots, and 16
+// consecutive lanes 16 distinct 16-byte bank groups.
 __device__ __forceinline__ int arena_slot(int x, int y, int z) {
     return (((x >> 2) | ((y >> 2) << 1) | ((z >> 2) << 2)) << 6) | (x & 3) | ((y & 3) << 2) | ((z & 3) << 4);
 }
@@ -61,9 +78,13 @@ __device__ __forceinline__ void decode_block(uint32_t key, uint32_t b[3]) {
     b[2] = compact1by2(key >> 2);
 }
 
-// Dense node index of local node (x, y, z).
+// Dense node index of local node (x, y, z) (all >= 0).
 __device__ __forceinline__ uint32_t node_index(const GridGeom &G, uint32_t x, uint32_t y, uint32_t z) {
     return block_linear(G, x >> 2, y >> 2, z >> 2) * 64u + (x & 3u) + ((y & 3u) << 2) + ((z & 3u) << 4);
+}
+
+__device__ __forceinline__ bool node_in_grid(const GridGeom &G, int x, int y, int z) {
+    return x >= 0 && y >= 0 && z >= 0 && x < 4 * G.nb[0] && y < 4 * G.nb[1] && z < 4 * G.nb[2];
 }
 
 // The block-list range of this work-group.  Work-group w runs on XCD w % 8;
@@ -80,8 +101,33 @@ __device__ __forceinline__ BlockRange xcd_block_range(uint32_t nblocks) {
     return {b0 + r, b1, per};
 }
 
-__device__ __noinline__ bool svd3_call(const float F[9], float U[9], float s[3], float V[9]) {
-    return svd3(F, U, s, V);
+// The SVD is out of line (it is rare and large); its operands travel by value
+// so that no array of the caller has to live in private memory.
+struct Mat9 {
+    float v[9];
+};
+struct Svd3 {
+    float U[9], s[3], V[9];
+    bool fp64;
+};
+__device__ __noinline__ Svd3 svd3_value(Mat9 F) {
+    Svd3 r;
+    r.fp64 = svd3(F.v, r.U, r.s, r.V);
+    return r;
+}
+__device__ __forceinline__ bool svd3_call(const float F[9], float U[9], float s[3], float V[9]) {
+    Mat9 m;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) m.v[k] = F[k];
+    const Svd3 r = svd3_value(m);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        U[k] = r.U[k];
+        V[k] = r.V[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) s[k] = r.s[k];
+    return r.fp64;
 }
 
 // Kirchhoff stress tau = P F^T of a particle.  R must hold the polar factor
@@ -116,25 +162,19 @@ __device__ __forceinline__ void fused_affine(float m, float vol, float scaling, 
     }
 }
 
-// Stress inputs of a particle at P2G time (first substep of a call: the
-// state holds C, not A).
+// Stress inputs of a particle whose state holds F and C (first substep of a
+// call): polar factor / SVD, close-to-inverted check, A.
 template <uint32_t SET>
-__device__ __forceinline__ void affine_from_state(const ParticleArrays &P, const AuxArrays &X, uint32_t p, float m,
-                                                  float scaling, float hardening, DeviceScalars *ds, float A[9]) {
-    const uint32_t flags = P.flags(p);
-    float F[9], C[9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        F[k] = P.F(k, p);
-        C[k] = P.A(k, p);
-    }
+__device__ __forceinline__ void affine_from_state(uint32_t flags, const float F[9], const float C[9], float m, float vol,
+                                                  float p0, float p1, uint32_t id, const AuxArrays &X, float scaling,
+                                                  float hardening, DeviceScalars *ds, float A[9]) {
     float J = det3(F);
     float R[9], U[9], s[3] = {1.f, 1.f, 1.f};
     float hard = 1.0f;
     if ((SET & M_SNOW) && (flags & F_SNOW)) {
         float V[9];
         if (svd3_call(F, U, s, V)) atomicOr(&ds->status, MPM_STATUS_SVD_FP64);
-        hard = expf(hardening * (1.0f - X.jp[P.id(p)]));
+        hard = expf(hardening * (1.0f - X.jp[id]));
     } else if ((SET & M_FCR) && (flags & F_FCR)) {
         if (!polar3(F, R)) {
             float V[9];
@@ -145,242 +185,665 @@ __device__ __forceinline__ void affine_from_state(const ParticleArrays &P, const
     if (!(J >= 1e-7f) && !(flags & F_FLUID))
         atomicOr(&ds->status, MPM_STATUS_PARTICLE_CLOSE_TO_INVERTED | (KID_P2G << 16));
     float tau[9];
-    kirchhoff<SET>(flags, P.p0(p), P.p1(p), hard, F, J, R, U, s, tau);
+    kirchhoff<SET>(flags, p0, p1, hard, F, J, R, U, s, tau);
     const bool visc = flags & F_VISC;
     float vd = 0.f, vb = 0.f;
     if ((SET & M_VISC) && visc) {
-        vd = X.visc_dyn[P.id(p)];
-        vb = X.visc_bulk[P.id(p)];
+        vd = X.visc_dyn[id];
+        vb = X.visc_bulk[id];
     }
-    fused_affine<SET>(m, P.vol(p), scaling, C, tau, J, visc, vd, vb, A);
+    fused_affine<SET>(m, vol, scaling, C, tau, J, visc, vd, vb, A);
 }
 
-// ------------------------------------------------------------------- P2G
-template <bool FROM_C, bool GOALS, uint32_t SET>
-__global__ __launch_bounds__(P2G_THREADS) void k_p2g(ParticleArrays P, AuxArrays X, const uint32_t *__restrict__ order,
-                                                     const uint32_t *__restrict__ blk_start,
-                                                     const uint32_t *__restrict__ blk_key, GridGeom G, SubstepParams S,
-                                                     float4 *__restrict__ grid, uint32_t *__restrict__ blk_flag,
-                                                     uint32_t *__restrict__ cand, DeviceScalars *ds) {
-    __shared__ float4 arena[P2G_WAVES][ARENA];  // one arena per wave: no barriers between their adds
-    __shared__ float4 pay[P2G_CAP][4];
-    __shared__ uint32_t cell_count[64];
-    __shared__ uint32_t cell_start[64];
-    __shared__ uint32_t cell_fill[64];
+// Separable gather of v = sum w v_i and C = 4/h sum w v_i (i - f)^T over the
+// 3x3x3 stencil (collect_velocity.rs:19-75), with w_ijk = wx_i wy_j wz_k:
+// rows over k first, then j, then the planes i.  node(i, j, k) returns the
+// node velocity as a float4 (m, v).
+template <class Node>
+__device__ __forceinline__ void gather_stencil(const float w[3][3], const float f[3], float apic, Node node, float v[3],
+                                               float C[9]) {
+    float T[3][3];  // T[a] = sum w v (node offset along axis a)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        v[r] = 0.f;
+        T[0][r] = T[1][r] = T[2][r] = 0.f;
+    }
+#pragma unroll 1
+    for (int i = 0; i < 3; ++i) {
+        float S[3] = {0.f, 0.f, 0.f}, Sy[3] = {0.f, 0.f, 0.f}, Sz[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            float R[3] = {0.f, 0.f, 0.f}, Rz[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const float4 g = node(i, j, k);
+                const float wz = w[2][k];
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    const float gr = r == 0 ? g.y : (r == 1 ? g.z : g.w);
+                    R[r] += wz * gr;
+                    if (k > 0) Rz[r] += ((float)k * wz) * gr;
+                }
+            }
+            const float wy = w[1][j];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                S[r] += wy * R[r];
+                if (j > 0) Sy[r] += ((float)j * wy) * R[r];
+                Sz[r] += wy * Rz[r];
+            }
+        }
+        const float wx = i == 0 ? w[0][0] : (i == 1 ? w[0][1] : w[0][2]);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            v[r] += wx * S[r];
+            T[0][r] += ((float)i * wx) * S[r];
+            T[1][r] += wx * Sy[r];
+            T[2][r] += wx * Sz[r];
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) C[3 * a + r] = apic * (T[a][r] - f[a] * v[r]);
+}
+
+// external_force (rust/crates/cpu/src/phase/external_force.rs:36-47): goal
+// velocity when the particle has a goal in frames a and b, else gravity
+// (unless gravity is applied on the grid).
+template <bool GOALS>
+__device__ __forceinline__ void external_velocity(const SubstepParams &S, const AuxArrays &X, uint32_t id,
+                                                  const float xp[3], float v[3]) {
+    if constexpr (GOALS) {
+        const uint32_t fb = S.has_b ? X.goal_flags_b[id] : X.goal_flags_a[id];
+        if ((X.goal_flags_a[id] & F_GOAL) && (fb & F_GOAL)) {
+            const float fa = 1.0f - S.factor_b;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const float gb = S.has_b ? X.goal_b[a][id] : X.goal_a[a][id];
+                const float gp = fa * X.goal_a[a][id] + S.factor_b * gb;
+                v[a] = (gp - xp[a]) / S.dt;
+            }
+            return;
+        }
+    }
+    if (!(S.options & MPM_OPT_GRID_GRAVITY)) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) v[a] += S.dt * S.gravity[a];
+    }
+}
+
+struct XferArgs {
+    ParticleArrays Pin;          // read; sorted order when PERMUTE, else the current order
+    ParticleArrays Pout;         // written; equals Pin when not PERMUTE
+    AuxArrays X;
+    const uint32_t *order;       // PERMUTE: slot i of the sorted order holds Pin[order[i]]
+    const uint32_t *blk_start;
+    const uint32_t *blk_key;
+    GridGeom G;
+    SubstepParams S0;            // the substep whose gather runs (G2P modes)
+    SubstepParams S1;            // the substep whose scatter runs (P2G modes)
+    GridBuffers B;               // B.ovf: the overflow list for S1's grid
+    uint32_t *keys_out;          // block keys of the new positions, or null
+    DeviceScalars *ds;
+};
+
+// Scatter of one particle straight into the global accumulation grid, for a
+// particle outside its block's region; claims the blocks it touches for the
+// grid update (overflow list of the epoch's parity).
+__device__ __forceinline__ void scatter_global(const GridBuffers &B, const GridGeom &G, uint32_t epoch,
+                                               DeviceScalars *ds, const int cell[3], const float pl[16]) {
+    if (!node_in_grid(G, cell[0], cell[1], cell[2]) || !node_in_grid(G, cell[0] + 2, cell[1] + 2, cell[2] + 2)) {
+        atomicOr(&ds->status, MPM_STATUS_OUT_OF_GRID | (KID_P2G << 16));
+        return;
+    }
+    float wx[3], wy[3], wz[3];
+    bspline_weights(pl[0], wx);
+    bspline_weights(pl[1], wy);
+    bspline_weights(pl[2], wz);
+#pragma unroll 1
+    for (int i = 0; i < 3; ++i)
+#pragma unroll 1
+        for (int j = 0; j < 3; ++j)
+#pragma unroll 1
+            for (int k = 0; k < 3; ++k) {
+                const float wt = (i == 0 ? wx[0] : (i == 1 ? wx[1] : wx[2])) *
+                                 (j == 0 ? wy[0] : (j == 1 ? wy[1] : wy[2])) *
+                                 (k == 0 ? wz[0] : (k == 1 ? wz[1] : wz[2]));
+                const uint32_t x = (uint32_t)(cell[0] + i), y = (uint32_t)(cell[1] + j), z = (uint32_t)(cell[2] + k);
+                float *g = reinterpret_cast<float *>(B.grid + node_index(G, x, y, z));
+                unsafeAtomicAdd(g + 0, wt * pl[3]);
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+                    unsafeAtomicAdd(g + 1 + r, wt * (pl[4 + r] + (float)i * pl[7 + r] + (float)j * pl[10 + r] +
+                                                     (float)k * pl[13 + r]));
+            }
+#pragma unroll 1
+    for (int bz = cell[2] >> 2; bz <= (cell[2] + 2) >> 2; ++bz)
+#pragma unroll 1
+        for (int by = cell[1] >> 2; by <= (cell[1] + 2) >> 2; ++by)
+#pragma unroll 1
+            for (int bx = cell[0] >> 2; bx <= (cell[0] + 2) >> 2; ++bx) {
+                const uint32_t T = block_linear(G, (uint32_t)bx, (uint32_t)by, (uint32_t)bz);
+                B.blk_atomic[T] = 1u;
+                if (atomicExch(&B.blk_flag[T], 1u) == 0u) {
+                    const uint32_t slot = atomicAdd(&ds->ovf_count[epoch & 1u], 1u);
+                    if (slot < G.num_blocks_total) B.ovf[slot] = T;
+                }
+            }
+}
+
+// Payload of one particle for the scatter, 16 floats in four float4:
+//   (f, m | q | h A columns); its contribution at stencil node (i, j, k) is
+//   w_ijk (m v + A d) = w_ijk (q + i c0 + j c1 + k c2).
+// Contributions of stencil plane i of the cnt particles of one cell (one
+// lane, region cell (cx, cy, cz)): fetch(r, a0..a3) returns the r-th
+// payload.  The 9 nodes of the plane are accumulated in registers, then added
+// into the wave's own arena with a plain read-add-write, one stencil offset
+// per round: for one offset the lanes (distinct cells) touch distinct nodes,
+// whose arena slots differ in their low six bits (no bank conflict).
+template <class Fetch>
+__device__ __forceinline__ void accumulate_plane(int i, uint32_t cnt, int cx, int cy, int cz, Fetch fetch,
+                                                 float4 *arena) {
+    float acc[9][4];
+#pragma unroll
+    for (int n = 0; n < 9; ++n)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[n][c] = 0.f;
+    const float fi = (float)i;
+    for (uint32_t r = 0; r < cnt; ++r) {
+        float4 a0, a1, a2, a3;
+        fetch(r, a0, a1, a2, a3);
+        const float m = a0.w;
+        const float c0[3] = {a1.w, a2.x, a2.y};
+        const float c1[3] = {a2.z, a2.w, a3.x};
+        const float c2[3] = {a3.y, a3.z, a3.w};
+        float wy[3], wz[3];
+        bspline_weights(a0.y, wy);
+        bspline_weights(a0.z, wz);
+        const float d = i == 0 ? 1.5f - a0.x : (i == 1 ? a0.x - 1.0f : a0.x - 0.5f);
+        const float wi = i == 1 ? 0.75f - d * d : 0.5f * d * d;
+        const float ti[3] = {a1.x + fi * c0[0], a1.y + fi * c0[1], a1.z + fi * c0[2]};
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const float wij = wi * wy[j];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const float wt = wij * wz[k];
+                const int n = 3 * j + k;
+                acc[n][0] += wt * m;
+#pragma unroll
+                for (int r3 = 0; r3 < 3; ++r3) acc[n][1 + r3] += wt * (ti[r3] + (float)j * c1[r3] + (float)k * c2[r3]);
+            }
+        }
+    }
+    if (cnt == 0) return;
+#pragma unroll
+    for (int n = 0; n < 9; ++n) {
+        float4 &g = arena[arena_slot(cx + i, cy + n / 3, cz + n % 3)];
+        float4 t = g;
+        t.x += acc[n][0];
+        t.y += acc[n][1];
+        t.z += acc[n][2];
+        t.w += acc[n][3];
+        g = t;
+        // The next round's read must not move above this write (two rounds
+        // of two lanes can hit one node; LDS ops of a wave run in order).
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+template <int MODE, bool PERMUTE, bool GOALS, uint32_t SET>
+__global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
+    constexpr bool HAS_G2P = MODE != XM_P2G;
+    constexpr bool HAS_P2G = MODE != XM_G2P;
+    constexpr bool LAST = MODE == XM_G2P;
+    // The region's node velocities (gather) and the four arenas (scatter:
+    // one per core plane wave, one for the overflow wave) share LDS; a pass
+    // restages the velocities after the previous pass's flush.
+    constexpr int SHARED4 = (HAS_P2G ? 4 * NSLOT : 0) > (HAS_G2P ? VS_SLOTS : 0) ? (HAS_P2G ? 4 * NSLOT : 0)
+                                                                                : (HAS_G2P ? VS_SLOTS : 1);
+    __shared__ float4 shared4[SHARED4];
+    float4 *vstage = shared4;                                    // node (x, y, z) at x + VS_Y y + VS_Z z
+    float4 *arenas = shared4;                                    // [wave][arena_slot]
+    __shared__ float4 payc[HAS_P2G ? 4 * XF_RANKS * 64 : 1];     // core payloads [quarter][rank][cell]
+    __shared__ float4 payx[HAS_P2G ? 4 * XF_OVF : 1];            // overflow payloads [quarter][entry]
+    __shared__ uint8_t xcell[HAS_P2G ? XF_OVF : 1];              // region cell of each overflow entry
+    __shared__ uint8_t xrank[HAS_P2G ? XF_OVF : 1];              // rank of the entry in its cell
+    __shared__ uint8_t xperm[HAS_P2G ? XF_OVF : 1];              // entries in cell order
+    __shared__ uint32_t xcount[HAS_P2G ? NCELL : 1];
+    __shared__ uint32_t xstart[HAS_P2G ? NCELL : 1];
+    __shared__ uint32_t core_count[64];
+    __shared__ uint32_t nx;
+    __shared__ uint32_t sub_used[NCAND];
 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
     const uint32_t wave = tid >> 6;
+    const GridGeom &G = a.G;
+    DeviceScalars *ds = a.ds;
     const uint32_t nblocks = ds->num_blocks;
     const uint32_t live = ds->live_count;
-    const float h = S.h;
-    const float scaling = S.dt * 4.0f / (h * h);
+    const float h = G.h;
+    const float dt = a.S0.dt;
+    const float scaling = a.S1.dt * 4.0f / (h * h);
+    const float apic = 4.0f / h;
 
     const BlockRange range = xcd_block_range(nblocks);
     for (uint32_t bi = range.begin; bi < range.end; bi += range.step) {
-        const uint32_t start = blk_start[bi];
-        const uint32_t end = bi + 1 < nblocks ? blk_start[bi + 1] : live;
+        const uint32_t start = a.blk_start[bi];
+        const uint32_t end = bi + 1 < nblocks ? a.blk_start[bi + 1] : live;
         uint32_t b[3];
-        decode_block(blk_key[bi], b);
-        int cell0[3];
+        decode_block(a.blk_key[bi], b);
+        int cell0[3];  // global cell of the block's first cell
 #pragma unroll
-        for (int a = 0; a < 3; ++a) cell0[a] = G.origin[a] + 4 * (int)b[a];
-        // Claim the 8 blocks the arena covers (first claimant lists the block
-        // for the grid update); issued first so the latency overlaps the staging.
-        if (tid < 8) {
-            const uint32_t T = block_linear(G, b[0] + (tid & 1u), b[1] + ((tid >> 1) & 1u), b[2] + (tid >> 2));
-            const uint32_t old = atomicExch(&blk_flag[T], 1u);
-            cand[8u * bi + tid] = old == 0u ? T : NO_BLOCK;
+        for (int q = 0; q < 3; ++q) cell0[q] = G.origin[q] + 4 * (int)b[q];
+        if constexpr (HAS_P2G) {
+            if (tid < 64u) core_count[tid] = 0u;
+            if (tid == 0) nx = 0u;
+            if (tid < (uint32_t)NCAND) sub_used[tid] = 0u;
         }
-        for (int k = lane; k < ARENA; k += 64) arena[wave][k] = make_float4(0.f, 0.f, 0.f, 0.f);
 
-        for (uint32_t cs = start; cs < end; cs += P2G_CAP) {
-            const uint32_t ce = min(end, cs + (uint32_t)P2G_CAP);
-            if (tid < 64) {
-                cell_count[tid] = 0;
-                cell_fill[tid] = 0;
+        for (uint32_t cs = start; cs < end; cs += XF_CAP) {
+            const uint32_t ce = min(end, cs + (uint32_t)XF_CAP);
+            if constexpr (HAS_G2P) {
+                for (uint32_t s = tid; s < (uint32_t)NSLOT; s += XF_THREADS) {
+                    const int rx = (int)(s & 7u), ry = (int)((s >> 3) & 7u), rz = (int)(s >> 6);
+                    const int x = 4 * (int)b[0] + rx - 1, y = 4 * (int)b[1] + ry - 1, z = 4 * (int)b[2] + rz - 1;
+                    vstage[rx + VS_Y * ry + VS_Z * rz] =
+                        node_in_grid(G, x, y, z) ? a.B.vgrid[node_index(G, (uint32_t)x, (uint32_t)y, (uint32_t)z)]
+                                                 : make_float4(0.f, 0.f, 0.f, 0.f);
+                }
             }
             __syncthreads();
-            // Stage: each thread reduces its particles to payloads
-            //   (fx, m | q | h A columns), contribution at stencil node (i,j,k):
-            //   w_ijk (m v + A d) = w_ijk (q + i c0 + j c1 + k c2).
-            float pl[P2G_PER_THREAD][16];
-            uint32_t mycell[P2G_PER_THREAD];
 #pragma unroll
-            for (int k = 0; k < P2G_PER_THREAD; ++k) {
-                const uint32_t i = cs + tid + (uint32_t)(k * P2G_THREADS);
-                mycell[k] = 64u;
+            for (int kk = 0; kk < XF_PER_THREAD; ++kk) {
+                const uint32_t i = cs + tid + (uint32_t)(kk * XF_THREADS);
                 if (i >= ce) continue;
-                const uint32_t p = order[i];
-                float xp[3], fx[3];
-                int c[3];
-                bool inside = true;
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    xp[a] = P.x(a, p);
-                    const float xn = xp[a] / h;
-                    const float fl = floorf(xn - 0.5f);
-                    fx[a] = xn - fl;
-                    c[a] = (int)fl - cell0[a];
-                    inside = inside && c[a] >= 0 && c[a] < 4;
-                }
-                if (!inside) {
-                    atomicOr(&ds->status, MPM_STATUS_OUT_OF_GRID | (KID_P2G << 16));
+                const uint32_t src = PERMUTE ? a.order[i] : i;
+                const ParticleArrays &Pin = a.Pin;
+                const ParticleArrays &Pout = a.Pout;
+                uint32_t flags = Pin.flags(src);
+                if (flags & F_TOMB) {
+                    // Culled since the last sort: keeps its slot, takes no part.
+                    if (PERMUTE) {
+                        for (int f = 0; f < NFIELDS; ++f) Pout.at(i)[f * TILE] = Pin.at(src)[f * TILE];
+                    }
+                    if (a.keys_out) a.keys_out[i] = G.tomb_key;
                     continue;
                 }
-                const float m = P.m(p);
-                float v[3] = {P.v(0, p), P.v(1, p), P.v(2, p)};
-                // external_force (rust/crates/cpu/src/phase/external_force.rs:36-47).
-                bool goal = false;
-                if constexpr (GOALS) {
-                    const uint32_t o = P.id(p);
-                    const uint32_t fb = S.has_b ? X.goal_flags_b[o] : X.goal_flags_a[o];
-                    if ((X.goal_flags_a[o] & F_GOAL) && (fb & F_GOAL)) {
-                        goal = true;
-                        const float fa = 1.0f - S.factor_b;
+                float xp[3], F[9];
 #pragma unroll
-                        for (int a = 0; a < 3; ++a) {
-                            const float gb = S.has_b ? X.goal_b[a][o] : X.goal_a[a][o];
-                            const float gp = fa * X.goal_a[a][o] + S.factor_b * gb;
-                            v[a] = (gp - xp[a]) / S.dt;
-                        }
-                    }
-                }
-                if (!goal && !(S.options & MPM_OPT_GRID_GRAVITY)) {
+                for (int q = 0; q < 3; ++q) xp[q] = Pin.x(q, src);
 #pragma unroll
-                    for (int a = 0; a < 3; ++a) v[a] += S.dt * S.gravity[a];
-                }
+                for (int k = 0; k < 9; ++k) F[k] = Pin.F(k, src);
+                const float m = Pin.m(src), vol = Pin.vol(src), p0 = Pin.p0(src), p1 = Pin.p1(src);
+                const uint32_t id = Pin.id(src);
+                float v[3], C[9];
+                float J = 1.f;
+                float R[9], U[9], sv[3] = {1.f, 1.f, 1.f};
+                float hard = 1.0f;
+                bool inside = true;
                 float A[9];
-                if constexpr (FROM_C) {
-                    affine_from_state<SET>(P, X, p, m, scaling, S.snow_hardening, ds, A);
-                } else {
-#pragma unroll
-                    for (int q = 0; q < 9; ++q) A[q] = P.A(q, p);
-                }
-                float *o = pl[k];
-                o[0] = fx[0];
-                o[1] = fx[1];
-                o[2] = fx[2];
-                o[3] = m;
-#pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    o[4 + r] = m * v[r] - h * (A[r] * fx[0] + A[3 + r] * fx[1] + A[6 + r] * fx[2]);
-                    o[7 + r] = h * A[r];
-                    o[10 + r] = h * A[3 + r];
-                    o[13 + r] = h * A[6 + r];
-                }
-                const uint32_t cell = (uint32_t)(c[0] | (c[1] << 2) | (c[2] << 4));
-                atomicAdd(&cell_count[cell], 1u);
-                mycell[k] = cell;
-            }
-            __syncthreads();
-            if (tid < 64) {
-                const uint32_t cnt = cell_count[tid];
-                uint32_t incl = cnt;
-#pragma unroll
-                for (int o = 1; o < 64; o <<= 1) {
-                    const uint32_t y = __shfl_up(incl, o, 64);
-                    if (lane >= (uint32_t)o) incl += y;
-                }
-                cell_start[tid] = incl - cnt;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < P2G_PER_THREAD; ++k) {
-                if (mycell[k] < 64u) {
-                    const uint32_t slot = cell_start[mycell[k]] + atomicAdd(&cell_fill[mycell[k]], 1u);
-                    const float *o = pl[k];
-                    pay[slot][0] = make_float4(o[0], o[1], o[2], o[3]);
-                    pay[slot][1] = make_float4(o[4], o[5], o[6], o[7]);
-                    pay[slot][2] = make_float4(o[8], o[9], o[10], o[11]);
-                    pay[slot][3] = make_float4(o[12], o[13], o[14], o[15]);
-                }
-            }
-            __syncthreads();
 
-            // Accumulate: lane = cell, wave w takes every P2G_WAVES-th particle
-            // of the cell; one stencil plane (fixed i) at a time, 36 registers.
-            const uint32_t cnt = cell_count[lane];
-            const uint32_t st = cell_start[lane];
-            const int cx = (int)(lane & 3u), cy = (int)((lane >> 2) & 3u), cz = (int)(lane >> 4);
-            if (cnt > wave) {
-#pragma unroll 1
-                for (int i = 0; i < 3; ++i) {
-                    float acc[9][4];
+                if constexpr (HAS_G2P) {
+                    // collect_velocity: v = sum w v_i, C = 4/h^2 sum w v_i (x_i - x_p)^T.
+                    float f[3], w[3][3];
+                    int c[3];
+                    bool in_region = true;
 #pragma unroll
-                    for (int n = 0; n < 9; ++n)
+                    for (int q = 0; q < 3; ++q) {
+                        const float xn = xp[q] / h;
+                        const float fl = floorf(xn - 0.5f);
+                        f[q] = xn - fl;
+                        c[q] = (int)fl - cell0[q];
+                        in_region = in_region && c[q] >= -1 && c[q] <= 4;
+                        bspline_weights(f[q], w[q]);
+                    }
+                    if (in_region) {
+                        const float4 *base = vstage + (c[0] + 1) + VS_Y * (c[1] + 1) + VS_Z * (c[2] + 1);
+                        gather_stencil(
+                            w, f, apic, [&](int i3, int j3, int k3) { return base[i3 + VS_Y * j3 + VS_Z * k3]; }, v, C);
+                    } else {
+                        const int g0[3] = {c[0] + 4 * (int)b[0], c[1] + 4 * (int)b[1], c[2] + 4 * (int)b[2]};
+                        const bool ok = node_in_grid(G, g0[0], g0[1], g0[2]) &&
+                                        node_in_grid(G, g0[0] + 2, g0[1] + 2, g0[2] + 2);
+                        if (!ok) atomicOr(&ds->status, MPM_STATUS_OUT_OF_GRID | (KID_G2P << 16));
+                        gather_stencil(
+                            w, f, apic,
+                            [&](int i3, int j3, int k3) {
+                                return ok ? a.B.vgrid[node_index(G, (uint32_t)(g0[0] + i3), (uint32_t)(g0[1] + j3),
+                                                               (uint32_t)(g0[2] + k3))]
+                                          : make_float4(0.f, 0.f, 0.f, 0.f);
+                            },
+                            v, C);
+                    }
+
+                    // advance_particles: x += dt v, F += (C F) dt, plasticity, energy.
+                    float CF[9];
+                    matmul3(C, F, CF);
 #pragma unroll
-                        for (int c = 0; c < 4; ++c) acc[n][c] = 0.f;
-                    for (uint32_t r = wave; r < cnt; r += P2G_WAVES) {
-                        const float4 a0 = pay[st + r][0], a1 = pay[st + r][1], a2 = pay[st + r][2], a3 = pay[st + r][3];
-                        const float m = a0.w;
-                        const float c0[3] = {a1.w, a2.x, a2.y};
-                        const float c1[3] = {a2.z, a2.w, a3.x};
-                        const float c2[3] = {a3.y, a3.z, a3.w};
-                        float wx[3], wy[3], wz[3];
-                        bspline_weights(a0.x, wx);
-                        bspline_weights(a0.y, wy);
-                        bspline_weights(a0.z, wz);
-                        const float wi = i == 0 ? wx[0] : (i == 1 ? wx[1] : wx[2]);
-                        const float fi = (float)i;
-                        const float ti[3] = {a1.x + fi * c0[0], a1.y + fi * c0[1], a1.z + fi * c0[2]};
+                    for (int k = 0; k < 9; ++k) F[k] += CF[k] * dt;
 #pragma unroll
-                        for (int j = 0; j < 3; ++j) {
-                            const float wij = wi * wy[j];
+                    for (int q = 0; q < 3; ++q) xp[q] += v[q] * dt;
+                    float energy = 0.f;
+                    J = det3(F);
+                    if ((SET & M_FLUID) && (flags & F_FLUID)) {
+                        // advance_particles.rs:76-86: singular values <- (s1 s2 s3)^(1/3),
+                        // i.e. F <- J^(1/3) R with R the polar factor of F.
+                        if (!(J > 0.f) || !polar3(F, R)) {
+                            float V[9];
+                            if (svd3_call(F, U, sv, V)) atomicOr(&ds->status, MPM_STATUS_SVD_FP64);
+                            matmul3_bt(U, V, R);
+                            J = sv[0] * sv[1] * sv[2];
+                        }
+                        const float g = cbrtf(J);
 #pragma unroll
-                            for (int k = 0; k < 3; ++k) {
-                                const float wt = wij * wz[k];
-                                const int n = 3 * j + k;
-                                acc[n][0] += wt * m;
+                        for (int k = 0; k < 9; ++k) F[k] = g * R[k];
+                        J = det3(F);
+                        if (LAST) energy = energy_inviscid(p0, (int)p1, J);
+                    } else if ((SET & M_SNOW) && (flags & F_SNOW)) {
+                        float V[9];
+                        if (svd3_call(F, U, sv, V)) atomicOr(&ds->status, MPM_STATUS_SVD_FP64);
+                        float jp = a.X.jp[id];
 #pragma unroll
-                                for (int r3 = 0; r3 < 3; ++r3)
-                                    acc[n][1 + r3] += wt * (ti[r3] + (float)j * c1[r3] + (float)k * c2[r3]);
+                        for (int q = 0; q < 3; ++q) {
+                            const float cl = fminf(fmaxf(sv[q], a.S0.snow_lo), a.S0.snow_hi);
+                            jp *= sv[q] / cl;
+                            sv[q] = cl;
+                        }
+                        a.X.jp[id] = jp;
+                        recompose(U, sv, V, F);
+                        J = sv[0] * sv[1] * sv[2];
+                        hard = expf(a.S0.snow_hardening * (1.0f - jp));
+                        if (LAST) energy = energy_fixed_corotated(p0 * hard, p1 * hard, sv);
+                    } else {
+                        if ((SET & M_SAND) && (flags & F_SAND)) {
+                            // advance_particles.rs:50-71: Drucker-Prager return map in log strain.
+                            float V[9];
+                            if (svd3_call(F, U, sv, V)) atomicOr(&ds->status, MPM_STATUS_SVD_FP64);
+                            const float alpha = a.X.sand_alpha[id];
+                            const float e[3] = {logf(sv[0]), logf(sv[1]), logf(sv[2])};
+                            const float etr = e[0] + e[1] + e[2];
+                            const float eh[3] = {e[0] - etr / 3.f, e[1] - etr / 3.f, e[2] - etr / 3.f};
+                            const float ehn = sqrtf(eh[0] * eh[0] + eh[1] * eh[1] + eh[2] * eh[2]);
+                            if (etr < 0.f && ehn > 0.f) {
+                                const float dg = ehn + (3.f * p1 + 2.f * p0) / 2.f / p0 * etr * alpha;
+                                if (dg > 0.f) {
+                                    float d[3];
+#pragma unroll
+                                    for (int q = 0; q < 3; ++q) d[q] = expf(e[q] - dg / ehn * eh[q]);
+                                    recompose(U, d, V, F);
+                                }
+                            } else {
+                                const float one[3] = {1.f, 1.f, 1.f};
+                                recompose(U, one, V, F);
+                            }
+                            J = det3(F);
+                        }
+                        if ((SET & M_FCR) && (flags & F_FCR)) {
+                            if (!polar3(F, R)) {
+                                float V[9];
+                                svd3_call(F, U, sv, V);
+                                matmul3_bt(U, V, R);
+                            }
+                            if (LAST) energy = energy_fixed_corotated_polar(p0, p1, F, R, J);
+                        } else {
+                            if (!(J > 0.f)) {
+                                // try_elastic_energy_neo_hookean fails: EnergyError, particle FAILED.
+                                flags |= F_FAILED;
+                                atomicOr(&ds->status, MPM_STATUS_ENERGY_ERROR | (KID_G2P << 16));
+                            } else if (LAST) {
+                                energy = energy_neo_hookean(p0, p1, F, J);
                             }
                         }
                     }
-                    // Add the plane into this wave's arena, one stencil offset per
-                    // read-add-write round: for a fixed offset the 64 lanes hit 64
-                    // distinct nodes, while two different offsets of two lanes can
-                    // hit the same node, so a round's write must land before the
-                    // next round's read.
+
+                    // cull_particles: strict inequalities.
 #pragma unroll
-                    for (int n = 0; n < 9; ++n) {
-                        const int slot = arena_slot(cx + i, cy + n / 3, cz + n % 3);
-                        float4 g = arena[wave][slot];
-                        g.x += acc[n][0];
-                        g.y += acc[n][1];
-                        g.z += acc[n][2];
-                        g.w += acc[n][3];
-                        arena[wave][slot] = g;
-                        // Keeps the compiler from moving the next round's read
-                        // above this write (LDS ops of one wave execute in order).
-                        __builtin_amdgcn_wave_barrier();
+                    for (int q = 0; q < 3; ++q)
+                        inside = inside && xp[q] > a.S0.domain_min[q] && xp[q] < a.S0.domain_max[q];
+                    if (!inside) flags |= F_TOMB;
+
+                    if (a.keys_out) a.keys_out[i] = inside ? particle_block_key(xp, G, ds) : G.tomb_key;
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) Pout.x(q, i) = xp[q];
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) Pout.F(k, i) = F[k];
+                    if (LAST) {
+#pragma unroll
+                        for (int q = 0; q < 3; ++q) Pout.v(q, i) = v[q];
+#pragma unroll
+                        for (int k = 0; k < 9; ++k) Pout.A(k, i) = C[k];
+                        a.X.energy[id] = energy;
+                    }
+                    if (PERMUTE || flags != Pin.flags(src)) Pout.flags(i) = flags;
+                    if (PERMUTE) {
+                        Pout.m(i) = m;
+                        Pout.vol(i) = vol;
+                        Pout.p0(i) = p0;
+                        Pout.p1(i) = p1;
+                        Pout.id(i) = id;
+                    }
+                    if (LAST || !inside) continue;
+                    // Stress of the new F for the next substep's scatter.
+                    float tau[9];
+                    kirchhoff<SET>(flags, p0, p1, hard, F, J, R, U, sv, tau);
+                    const bool visc = flags & F_VISC;
+                    float vd = 0.f, vb = 0.f;
+                    if ((SET & M_VISC) && visc) {
+                        vd = a.X.visc_dyn[id];
+                        vb = a.X.visc_bulk[id];
+                    }
+                    fused_affine<SET>(m, vol, scaling, C, tau, J, visc, vd, vb, A);
+                } else {
+                    // First substep of a call: the state holds v and C.
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) v[q] = Pin.v(q, src);
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) C[k] = Pin.A(k, src);
+                    affine_from_state<SET>(flags, F, C, m, vol, p0, p1, id, a.X, scaling, a.S1.snow_hardening, ds, A);
+                    if (PERMUTE) {
+                        for (int f = 0; f < NFIELDS; ++f) Pout.at(i)[f * TILE] = Pin.at(src)[f * TILE];
+                    }
+                }
+
+                if constexpr (HAS_P2G) {
+                    // external_force of the substep that starts, then the payload
+                    //   (f, m | q | h A columns): contribution at stencil node (i,j,k)
+                    //   w_ijk (m v + A d) = w_ijk (q + i c0 + j c1 + k c2).
+                    external_velocity<GOALS>(a.S1, a.X, id, xp, v);
+                    float pl[16];
+                    int c[3];
+                    bool in_region = true;
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        const float xn = xp[q] / h;
+                        const float fl = floorf(xn - 0.5f);
+                        pl[q] = xn - fl;
+                        c[q] = (int)fl - cell0[q];
+                        in_region = in_region && c[q] >= -1 && c[q] <= 4;
+                    }
+                    pl[3] = m;
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) {
+                        pl[4 + r] = m * v[r] - h * (A[r] * pl[0] + A[3 + r] * pl[1] + A[6 + r] * pl[2]);
+                        pl[7 + r] = h * A[r];
+                        pl[10 + r] = h * A[3 + r];
+                        pl[13 + r] = h * A[6 + r];
+                    }
+                    // Core cell: rank-major slot; shell cell or a rank past the
+                    // core slots: overflow entry; overflow full or outside the
+                    // region: straight to the global grid.
+                    float4 *dst = nullptr;
+                    uint32_t stride = 0;
+                    if (in_region) {
+                        const bool core = c[0] >= 0 && c[0] <= 3 && c[1] >= 0 && c[1] <= 3 && c[2] >= 0 && c[2] <= 3;
+                        if (core) {
+                            const uint32_t l = (uint32_t)(c[0] | (c[1] << 2) | (c[2] << 4));
+                            const uint32_t r = atomicAdd(&core_count[l], 1u);
+                            if (r < (uint32_t)XF_RANKS) {
+                                dst = payc + r * 64u + l;
+                                stride = XF_RANKS * 64;
+                            }
+                        }
+                        if (!dst) {
+                            const uint32_t e = atomicAdd(&nx, 1u);
+                            if (e < (uint32_t)XF_OVF) {
+                                dst = payx + e;
+                                stride = XF_OVF;
+                                xcell[e] = (uint8_t)((c[0] + 1) + 6 * (c[1] + 1) + 36 * (c[2] + 1));
+                            }
+                        }
+                    }
+                    if (dst) {
+                        dst[0] = make_float4(pl[0], pl[1], pl[2], pl[3]);
+                        dst[stride] = make_float4(pl[4], pl[5], pl[6], pl[7]);
+                        dst[2 * stride] = make_float4(pl[8], pl[9], pl[10], pl[11]);
+                        dst[3 * stride] = make_float4(pl[12], pl[13], pl[14], pl[15]);
+                    } else {
+                        const int gc[3] = {c[0] + 4 * (int)b[0], c[1] + 4 * (int)b[1], c[2] + 4 * (int)b[2]};
+                        scatter_global(a.B, G, a.S1.epoch, ds, gc, pl);
                     }
                 }
             }
-            __syncthreads();
+
+            if constexpr (HAS_P2G) {
+                __syncthreads();
+                for (uint32_t s = tid; s < 4u * NSLOT; s += XF_THREADS) arenas[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+                __syncthreads();
+                if (wave < 3u) {
+                    // The block's own 64 cells, one stencil plane per wave.
+                    const uint32_t cnt = min(core_count[lane], (uint32_t)XF_RANKS);
+                    const float4 *pr = payc + lane;
+                    accumulate_plane(
+                        (int)wave, cnt, (int)(lane & 3u) + 1, (int)((lane >> 2) & 3u) + 1, (int)(lane >> 4) + 1,
+                        [&](uint32_t r, float4 &a0, float4 &a1, float4 &a2, float4 &a3) {
+                            const float4 *q = pr + r * 64u;
+                            a0 = q[0];
+                            a1 = q[XF_RANKS * 64];
+                            a2 = q[2 * XF_RANKS * 64];
+                            a3 = q[3 * XF_RANKS * 64];
+                        },
+                        arenas + wave * NSLOT);
+                } else {
+                    // Overflow entries (shell cells, ranks past the core slots):
+                    // bucketed by cell, then lane = cell over the 216 cells.
+                    const uint32_t ne = min(nx, (uint32_t)XF_OVF);
+                    for (uint32_t q = lane; q < (uint32_t)NCELL; q += 64u) xcount[q] = 0u;
+                    __builtin_amdgcn_wave_barrier();
+                    for (uint32_t e = lane; e < ne; e += 64u) xrank[e] = (uint8_t)atomicAdd(&xcount[xcell[e]], 1u);
+                    __builtin_amdgcn_wave_barrier();
+                    uint32_t cnt4[4], sum = 0;
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        const uint32_t q = 4u * lane + (uint32_t)t;
+                        cnt4[t] = q < (uint32_t)NCELL ? xcount[q] : 0u;
+                        sum += cnt4[t];
+                    }
+                    uint32_t incl = sum;
+#pragma unroll
+                    for (int o = 1; o < 64; o <<= 1) {
+                        const uint32_t y = __shfl_up(incl, o, 64);
+                        if (lane >= (uint32_t)o) incl += y;
+                    }
+                    uint32_t run = incl - sum;
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        const uint32_t q = 4u * lane + (uint32_t)t;
+                        if (q < (uint32_t)NCELL) xstart[q] = run;
+                        run += cnt4[t];
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    for (uint32_t e = lane; e < ne; e += 64u) xperm[xstart[xcell[e]] + xrank[e]] = (uint8_t)e;
+                    __builtin_amdgcn_wave_barrier();
+#pragma unroll 1
+                    for (uint32_t q0 = 0; ne > 0 && q0 < (uint32_t)NCELL; q0 += 64u) {
+                        const uint32_t q = q0 + lane;
+                        const uint32_t cnt = q < (uint32_t)NCELL ? xcount[q] : 0u;
+                        if (__ballot(cnt > 0u) == 0ull) continue;
+                        const uint32_t st = cnt ? xstart[q] : 0u;
+                        const int cx = (int)(q % 6u), cy = (int)((q / 6u) % 6u), cz = (int)(q / 36u);
+#pragma unroll 1
+                        for (int i = 0; i < 3; ++i)
+                            accumulate_plane(
+                                i, cnt, cx, cy, cz,
+                                [&](uint32_t r, float4 &a0, float4 &a1, float4 &a2, float4 &a3) {
+                                    const float4 *px = payx + xperm[st + r];
+                                    a0 = px[0];
+                                    a1 = px[XF_OVF];
+                                    a2 = px[2 * XF_OVF];
+                                    a3 = px[3 * XF_OVF];
+                                },
+                                arenas + 3 * NSLOT);
+                    }
+                }
+                __syncthreads();
+                // Flush the four arenas: the entry's partial arena (plain
+                // stores; later passes of the block add to it), or global
+                // atomics past the partial capacity.  Note which of the 27
+                // blocks around the block received mass.
+                const bool has_part = bi < a.B.part_capacity;
+                float4 *part = a.B.part + (uint64_t)bi * NSLOT;
+                for (uint32_t s = tid; s < (uint32_t)NSLOT; s += XF_THREADS) {
+                    float4 g4 = arenas[s];
+#pragma unroll
+                    for (int w = 1; w < 4; ++w) {
+                        const float4 o = arenas[w * NSLOT + s];
+                        g4.x += o.x;
+                        g4.y += o.y;
+                        g4.z += o.z;
+                        g4.w += o.w;
+                    }
+                    if (has_part) {
+                        if (cs != start) {
+                            const float4 o = part[s];
+                            g4.x += o.x;
+                            g4.y += o.y;
+                            g4.z += o.z;
+                            g4.w += o.w;
+                        }
+                        part[s] = g4;
+                    }
+                    if (!(g4.x != 0.f)) continue;
+                    const uint32_t lo = s & 63u, hi = s >> 6;
+                    const int rx = (int)((lo & 3u) | ((hi & 1u) << 2)),
+                              ry = (int)(((lo >> 2) & 3u) | (((hi >> 1) & 1u) << 2)),
+                              rz = (int)(((lo >> 4) & 3u) | (((hi >> 2) & 1u) << 2));
+                    const int x = 4 * (int)b[0] + rx - 1, y = 4 * (int)b[1] + ry - 1, z = 4 * (int)b[2] + rz - 1;
+                    if (!node_in_grid(G, x, y, z)) {
+                        atomicOr(&ds->status, MPM_STATUS_OUT_OF_GRID | (KID_P2G << 16));
+                        continue;
+                    }
+                    sub_used[((rx + 3) >> 2) + 3 * ((ry + 3) >> 2) + 9 * ((rz + 3) >> 2)] = 1u;
+                    if (!has_part) {
+                        float *g =
+                            reinterpret_cast<float *>(a.B.grid + node_index(G, (uint32_t)x, (uint32_t)y, (uint32_t)z));
+                        unsafeAtomicAdd(g + 0, g4.x);
+                        unsafeAtomicAdd(g + 1, g4.y);
+                        unsafeAtomicAdd(g + 2, g4.z);
+                        unsafeAtomicAdd(g + 3, g4.w);
+                        a.B.blk_atomic[block_linear(G, (uint32_t)x >> 2, (uint32_t)y >> 2, (uint32_t)z >> 2)] = 1u;
+                    }
+                }
+                if (tid < 64u) core_count[tid] = 0u;
+                if (tid == 0) nx = 0u;
+                __syncthreads();
+            } else {
+                __syncthreads();  // the next pass restages the velocities
+            }
         }
 
-        // Flush the sum of the wave arenas (6^3 nodes) to the grid.
-        for (int nidx = tid; nidx < 216; nidx += P2G_THREADS) {
-            const int ax = nidx % 6, ay = (nidx / 6) % 6, az = nidx / 36;
-            const int slot = arena_slot(ax, ay, az);
-            float4 g4 = arena[0][slot];
-#pragma unroll
-            for (int w = 1; w < P2G_WAVES; ++w) {
-                const float4 o = arena[w][slot];
-                g4.x += o.x;
-                g4.y += o.y;
-                g4.z += o.z;
-                g4.w += o.w;
-            }
-            if (g4.x != 0.f) {
-                const uint32_t node = node_index(G, 4u * b[0] + ax, 4u * b[1] + ay, 4u * b[2] + az);
-                float *g = reinterpret_cast<float *>(grid + node);
-                unsafeAtomicAdd(g + 0, g4.x);
-                unsafeAtomicAdd(g + 1, g4.y);
-                unsafeAtomicAdd(g + 2, g4.z);
-                unsafeAtomicAdd(g + 3, g4.w);
+        if constexpr (HAS_P2G) {
+            if (tid < (uint32_t)NCAND) {
+                const int dx = (int)(tid % 3u) - 1, dy = (int)((tid / 3u) % 3u) - 1, dz = (int)(tid / 9u) - 1;
+                const int bx = (int)b[0] + dx, by = (int)b[1] + dy, bz = (int)b[2] + dz;
+                uint32_t out = NO_BLOCK;
+                if (sub_used[tid] && bx >= 0 && by >= 0 && bz >= 0 && bx < G.nb[0] && by < G.nb[1] && bz < G.nb[2]) {
+                    const uint32_t T = block_linear(G, (uint32_t)bx, (uint32_t)by, (uint32_t)bz);
+                    if (atomicExch(&a.B.blk_flag[T], 1u) == 0u) out = T;
+                }
+                a.B.cand[NCAND * bi + tid] = out;
             }
         }
         __syncthreads();
@@ -388,62 +851,106 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g(ParticleArrays P, AuxArrays
 }
 
 // ------------------------------------------------------------ grid update
-// One wave per touched block: the candidate slots written by P2G are read 64
-// at a time and the claimed blocks processed in turn.
-__global__ __launch_bounds__(256) void k_grid_update(const uint32_t *__restrict__ cand, GridGeom G, SubstepParams S,
-                                                     float4 *__restrict__ grid, float4 *__restrict__ vgrid,
-                                                     uint32_t *__restrict__ blk_flag, uint32_t *__restrict__ blk_epoch,
+// One wave per touched block T (lane = node): the candidate slots written by
+// the transfer kernel (NCAND per block-list entry, then the overflow list)
+// are read 64 at a time and the claimed blocks processed in turn.  A node of
+// T lies in the regions of T and, along each axis, of one neighbour (T-1 for
+// the first three nodes, T+1 for the last): its (m, p) is the sum of those up
+// to 8 partial arenas, plus the global-memory scatters when T has any.
+template <bool MOMENTUM>
+__global__ __launch_bounds__(256) void k_grid_update(GridBuffers B, GridGeom G, SubstepParams S,
                                                      uint32_t *__restrict__ partial, DeviceScalars *ds) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gw = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
     const uint32_t nw = gridDim.x * (blockDim.x / 64);
-    const uint32_t ncand = ds->num_blocks * 8u;
+    const uint32_t nlist = ds->num_blocks * (uint32_t)NCAND;
+    const uint32_t novf = min(ds->ovf_count[S.epoch & 1u], G.num_blocks_total);
+    if (!MOMENTUM && gw == 0 && lane == 0) ds->ovf_count[(S.epoch + 1u) & 1u] = 0u;  // for the next transfer
+    const uint32_t ncand = nlist + novf;
+    const uint32_t x = lane & 3u, y = (lane >> 2) & 3u, z = lane >> 4;
+    const int ax = x <= 2u ? -1 : 1, ay = y <= 2u ? -1 : 1, az = z <= 2u ? -1 : 1;
+    const int ndx = (int)(lane % 3u) - 1, ndy = (int)((lane / 3u) % 3u) - 1, ndz = (int)(lane / 9u) - 1;
     uint32_t active = 0;
     for (uint32_t chunk = gw; chunk * 64u < ncand; chunk += nw) {
         const uint32_t c = chunk * 64u + lane;
-        const uint32_t mine = c < ncand ? cand[c] : NO_BLOCK;
+        const uint32_t mine = c < nlist ? B.cand[c] : (c < ncand ? B.ovf[c - nlist] : NO_BLOCK);
         uint64_t todo = __ballot(mine != NO_BLOCK);
         while (todo) {
             const int l = __ffsll((unsigned long long)todo) - 1;
             todo &= todo - 1ull;
-            const uint32_t B = __shfl(mine, l, 64);
-            const uint32_t node = B * 64u + lane;
-            const float4 g = grid[node];
+            const uint32_t T = __shfl(mine, l, 64);
+            const int bx = (int)(T % (uint32_t)G.nb[0]), by = (int)((T / (uint32_t)G.nb[0]) % (uint32_t)G.nb[1]),
+                      bz = (int)(T / ((uint32_t)G.nb[0] * (uint32_t)G.nb[1]));
+            // Lanes 0..26: the block-list entry of neighbour T + d, if it has a partial arena.
+            uint32_t ent = NO_BLOCK;
+            if (lane < (uint32_t)NCAND) {
+                const int qx = bx + ndx, qy = by + ndy, qz = bz + ndz;
+                if (qx >= 0 && qy >= 0 && qz >= 0 && qx < G.nb[0] && qy < G.nb[1] && qz < G.nb[2]) {
+                    const uint64_t v = B.entry[block_linear(G, (uint32_t)qx, (uint32_t)qy, (uint32_t)qz)];
+                    if ((uint32_t)(v >> 32) == B.gen && (uint32_t)v < B.part_capacity) ent = (uint32_t)v;
+                }
+            }
+            float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int dx = (k & 1) ? ax : 0, dy = (k & 2) ? ay : 0, dz = (k & 4) ? az : 0;
+                const uint32_t e = __shfl(ent, (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1), 64);
+                if (e != NO_BLOCK) {
+                    const float4 o = B.part[(uint64_t)e * NSLOT +
+                                            arena_slot((int)x + 1 - 4 * dx, (int)y + 1 - 4 * dy, (int)z + 1 - 4 * dz)];
+                    g.x += o.x;
+                    g.y += o.y;
+                    g.z += o.z;
+                    g.w += o.w;
+                }
+            }
+            const uint32_t node = T * 64u + lane;
+            const bool atomic_part = B.blk_atomic[T] != 0u;
+            if (atomic_part) {
+                const float4 o = B.grid[node];
+                g.x += o.x;
+                g.y += o.y;
+                g.z += o.z;
+                g.w += o.w;
+            }
+            if (MOMENTUM) {
+                B.vgrid[node] = g;
+                if (lane == 0) B.blk_epoch[T] = S.epoch;
+                continue;
+            }
             const float m = g.x;
             float v[3] = {0.f, 0.f, 0.f};
             if (m > 0.f) {
+                // meld_grid without collider twins (rust/crates/cpu/src/phase/meld_grid.rs:61-66).
                 v[0] = g.y / m;
                 v[1] = g.z / m;
                 v[2] = g.w / m;
                 // Extensions without a reference counterpart.
                 if (S.options & MPM_OPT_GRID_GRAVITY) {
 #pragma unroll
-                    for (int a = 0; a < 3; ++a) v[a] += S.gravity[a] * S.dt;
+                    for (int q = 0; q < 3; ++q) v[q] += S.gravity[q] * S.dt;
                 }
                 if (S.options & (MPM_OPT_STICKY_PLANE | MPM_OPT_BOX_WALLS)) {
-                    const uint32_t bx = B % (uint32_t)G.nb[0];
-                    const uint32_t by = (B / (uint32_t)G.nb[0]) % (uint32_t)G.nb[1];
-                    const uint32_t bz = B / ((uint32_t)G.nb[0] * (uint32_t)G.nb[1]);
-                    const int lx = (int)(4u * bx + (lane & 3u)), ly = (int)(4u * by + ((lane >> 2) & 3u)),
-                              lz = (int)(4u * bz + (lane >> 4));
+                    const int lx = 4 * bx + (int)x, ly = 4 * by + (int)y, lz = 4 * bz + (int)z;
                     const float xi[3] = {(float)(lx + G.origin[0]) * S.h, (float)(ly + G.origin[1]) * S.h,
                                          (float)(lz + G.origin[2]) * S.h};
                     if ((S.options & MPM_OPT_STICKY_PLANE) && xi[2] < S.plane_z) v[0] = v[1] = v[2] = 0.f;
                     if (S.options & MPM_OPT_BOX_WALLS) {
 #pragma unroll
-                        for (int a = 0; a < 3; ++a) {
-                            if (xi[a] < S.wall_min[a] && v[a] < 0.f) v[a] = 0.f;
-                            if (xi[a] > S.wall_max[a] && v[a] > 0.f) v[a] = 0.f;
+                        for (int q = 0; q < 3; ++q) {
+                            if (xi[q] < S.wall_min[q] && v[q] < 0.f) v[q] = 0.f;
+                            if (xi[q] > S.wall_max[q] && v[q] > 0.f) v[q] = 0.f;
                         }
                     }
                 }
                 ++active;
             }
-            vgrid[node] = make_float4(m, v[0], v[1], v[2]);
-            grid[node] = make_float4(0.f, 0.f, 0.f, 0.f);
+            B.vgrid[node] = make_float4(m, v[0], v[1], v[2]);
+            if (atomic_part) B.grid[node] = make_float4(0.f, 0.f, 0.f, 0.f);
             if (lane == 0) {
-                blk_flag[B] = 0u;
-                blk_epoch[B] = S.epoch;
+                B.blk_flag[T] = 0u;
+                B.blk_atomic[T] = 0u;
+                B.blk_epoch[T] = S.epoch;
             }
         }
     }
@@ -465,242 +972,20 @@ __global__ __launch_bounds__(256) void k_sum_partials(const uint32_t *__restrict
     if (threadIdx.x == 0) ds->active_nodes = ws[0] + ws[1] + ws[2] + ws[3];
 }
 
-// ------------------------------------------------------------------- G2P
-template <bool LAST, uint32_t SET>
-__global__ __launch_bounds__(256) void k_g2p(ParticleArrays Pin, ParticleArrays Pout, AuxArrays X,
-                                             const uint32_t *__restrict__ order, const uint32_t *__restrict__ blk_start,
-                                             const uint32_t *__restrict__ blk_key, GridGeom G, SubstepParams S,
-                                             const float4 *__restrict__ vgrid, uint32_t *__restrict__ keys_out,
-                                             DeviceScalars *ds) {
-    __shared__ float4 vloc[G2P_NODES];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t NT = blockDim.x;
-    const uint32_t nblocks = ds->num_blocks;
-    const uint32_t live = ds->live_count;
-    const float h = S.h;
-    const float dt = S.dt;
-    const float scaling = dt * 4.0f / (h * h);
-    const float apic = 4.0f / h;
-
-    const BlockRange range = xcd_block_range(nblocks);
-    for (uint32_t bi = range.begin; bi < range.end; bi += range.step) {
-        const uint32_t start = blk_start[bi];
-        const uint32_t end = bi + 1 < nblocks ? blk_start[bi + 1] : live;
-        uint32_t b[3];
-        decode_block(blk_key[bi], b);
-        for (uint32_t nidx = tid; nidx < (uint32_t)G2P_NODES; nidx += NT) {
-            const uint32_t ax = nidx % 6u, ay = (nidx / 6u) % 6u, az = nidx / 36u;
-            vloc[nidx] = vgrid[node_index(G, 4u * b[0] + ax, 4u * b[1] + ay, 4u * b[2] + az)];
-        }
-        __syncthreads();
-        int cell0[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) cell0[a] = G.origin[a] + 4 * (int)b[a];
-
-        for (uint32_t i = start + tid; i < end; i += NT) {
-            const uint32_t p = order[i];
-            float xp[3], fx[3], w[3][3];
-            int c[3];
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                xp[a] = Pin.x(a, p);
-                const float xn = xp[a] / h;
-                const float fl = floorf(xn - 0.5f);
-                fx[a] = xn - fl;
-                c[a] = min(max((int)fl - cell0[a], 0), 3);
-                bspline_weights(fx[a], w[a]);
-            }
-            // collect_velocity: v = sum w v_i, C = 4/h^2 sum w v_i (x_i - x_p)^T
-            float v[3] = {0.f, 0.f, 0.f};
-            float C[9];
-#pragma unroll
-            for (int k = 0; k < 9; ++k) C[k] = 0.f;
-            // Planes one at a time: keeps the LDS reads of a single plane in flight.
-#pragma unroll 1
-            for (int i3 = 0; i3 < 3; ++i3) {
-                const float dx = (float)i3 - fx[0];
-#pragma unroll
-                for (int j3 = 0; j3 < 3; ++j3) {
-                    const float dy = (float)j3 - fx[1];
-                    const float wij = w[0][i3] * w[1][j3];
-#pragma unroll
-                    for (int k3 = 0; k3 < 3; ++k3) {
-                        const float dz = (float)k3 - fx[2];
-                        const float wt = wij * w[2][k3];
-                        const float4 gv = vloc[(c[0] + i3) + 6 * ((c[1] + j3) + 6 * (c[2] + k3))];
-                        const float wv[3] = {wt * gv.y, wt * gv.z, wt * gv.w};
-#pragma unroll
-                        for (int r = 0; r < 3; ++r) {
-                            v[r] += wv[r];
-                            C[r] += wv[r] * dx;
-                            C[3 + r] += wv[r] * dy;
-                            C[6 + r] += wv[r] * dz;
-                        }
-                    }
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < 9; ++k) C[k] *= apic;
-
-            // advance_particles: x += dt v, F += (C F) dt, plasticity, energy.
-            uint32_t flags = Pin.flags(p);
-            const uint32_t id = Pin.id(p);
-            const float m = Pin.m(p), vol = Pin.vol(p), p0 = Pin.p0(p), p1 = Pin.p1(p);
-            float F[9], CF[9];
-#pragma unroll
-            for (int k = 0; k < 9; ++k) F[k] = Pin.F(k, p);
-            matmul3(C, F, CF);
-#pragma unroll
-            for (int k = 0; k < 9; ++k) F[k] += CF[k] * dt;
-            float xnew[3];
-#pragma unroll
-            for (int a = 0; a < 3; ++a) xnew[a] = xp[a] + v[a] * dt;
-
-            float R[9], U[9], s[3] = {1.f, 1.f, 1.f};
-            float hard = 1.0f;
-            float energy = 0.f;
-            float J = det3(F);
-            if ((SET & M_FLUID) && (flags & F_FLUID)) {
-                // advance_particles.rs:76-86: singular values <- (s1 s2 s3)^(1/3),
-                // i.e. F <- J^(1/3) R with R the polar factor of F.
-                if (!(J > 0.f) || !polar3(F, R)) {
-                    float V[9];
-                    if (svd3_call(F, U, s, V)) atomicOr(&ds->status, MPM_STATUS_SVD_FP64);
-                    matmul3_bt(U, V, R);
-                    J = s[0] * s[1] * s[2];
-                }
-                const float g = cbrtf(J);
-#pragma unroll
-                for (int k = 0; k < 9; ++k) F[k] = g * R[k];
-                J = det3(F);
-                if (LAST) energy = energy_inviscid(p0, (int)p1, J);
-            } else if ((SET & M_SNOW) && (flags & F_SNOW)) {
-                float V[9];
-                if (svd3_call(F, U, s, V)) atomicOr(&ds->status, MPM_STATUS_SVD_FP64);
-                float jp = X.jp[id];
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    const float cl = fminf(fmaxf(s[a], S.snow_lo), S.snow_hi);
-                    jp *= s[a] / cl;
-                    s[a] = cl;
-                }
-                X.jp[id] = jp;
-                recompose(U, s, V, F);
-                J = s[0] * s[1] * s[2];
-                hard = expf(S.snow_hardening * (1.0f - jp));
-                if (LAST) energy = energy_fixed_corotated(p0 * hard, p1 * hard, s);
-            } else {
-                if ((SET & M_SAND) && (flags & F_SAND)) {
-                    // advance_particles.rs:50-71: Drucker-Prager return map in log strain.
-                    float V[9];
-                    if (svd3_call(F, U, s, V)) atomicOr(&ds->status, MPM_STATUS_SVD_FP64);
-                    const float alpha = X.sand_alpha[id];
-                    const float e[3] = {logf(s[0]), logf(s[1]), logf(s[2])};
-                    const float etr = e[0] + e[1] + e[2];
-                    const float eh[3] = {e[0] - etr / 3.f, e[1] - etr / 3.f, e[2] - etr / 3.f};
-                    const float ehn = sqrtf(eh[0] * eh[0] + eh[1] * eh[1] + eh[2] * eh[2]);
-                    if (etr < 0.f && ehn > 0.f) {
-                        const float dg = ehn + (3.f * p1 + 2.f * p0) / 2.f / p0 * etr * alpha;
-                        if (dg > 0.f) {
-                            float d[3];
-#pragma unroll
-                            for (int a = 0; a < 3; ++a) d[a] = expf(e[a] - dg / ehn * eh[a]);
-                            recompose(U, d, V, F);
-                        }
-                    } else {
-                        const float one[3] = {1.f, 1.f, 1.f};
-                        recompose(U, one, V, F);
-                    }
-                    J = det3(F);
-                }
-                if ((SET & M_FCR) && (flags & F_FCR)) {
-                    if (!polar3(F, R)) {
-                        float V[9];
-                        svd3_call(F, U, s, V);
-                        matmul3_bt(U, V, R);
-                    }
-                    if (LAST) energy = energy_fixed_corotated_polar(p0, p1, F, R, J);
-                } else {
-                    if (!(J > 0.f)) {
-                        // try_elastic_energy_neo_hookean fails: EnergyError, particle FAILED.
-                        flags |= F_FAILED;
-                        atomicOr(&ds->status, MPM_STATUS_ENERGY_ERROR | (KID_G2P << 16));
-                    } else if (LAST) {
-                        energy = energy_neo_hookean(p0, p1, F, J);
-                    }
-                }
-            }
-
-            // cull_particles: strict inequalities.
-            bool inside = true;
-#pragma unroll
-            for (int a = 0; a < 3; ++a) inside = inside && xnew[a] > S.domain_min[a] && xnew[a] < S.domain_max[a];
-            if (!inside) flags |= F_TOMB;
-
-            float Aout[9];
-            if (LAST || !inside) {
-#pragma unroll
-                for (int k = 0; k < 9; ++k) Aout[k] = C[k];
-            } else {
-                float tau[9];
-                kirchhoff<SET>(flags, p0, p1, hard, F, J, R, U, s, tau);
-                const bool visc = flags & F_VISC;
-                float vd = 0.f, vb = 0.f;
-                if ((SET & M_VISC) && visc) {
-                    vd = X.visc_dyn[id];
-                    vb = X.visc_bulk[id];
-                }
-                fused_affine<SET>(m, vol, scaling, C, tau, J, visc, vd, vb, Aout);
-            }
-            if (LAST) X.energy[id] = energy;
-            if (keys_out) keys_out[i] = inside ? particle_block_key(xnew, G, ds) : G.tomb_key;
-
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                Pout.x(a, i) = xnew[a];
-                Pout.v(a, i) = v[a];
-            }
-#pragma unroll
-            for (int k = 0; k < 9; ++k) {
-                Pout.F(k, i) = F[k];
-                Pout.A(k, i) = Aout[k];
-            }
-            Pout.m(i) = m;
-            Pout.vol(i) = vol;
-            Pout.p0(i) = p0;
-            Pout.p1(i) = p1;
-            Pout.flags(i) = flags;
-            Pout.id(i) = id;
-        }
-        __syncthreads();
-    }
-}
-
-// Particles that are not in any block (tombstoned, sorted to the end) are
-// copied to their sorted slot unchanged.
+// Particles that are not in any block (tombstoned, sorted to the end): when
+// the particle set is being permuted (Pout differs from Pin) they are copied
+// to their sorted slot unchanged; their keys are the tombstone key.
 __global__ __launch_bounds__(256) void k_copy_dead(ParticleArrays Pin, ParticleArrays Pout,
                                                    const uint32_t *__restrict__ order, uint64_t n,
                                                    uint32_t *__restrict__ keys_out, uint32_t tomb, DeviceScalars *ds) {
     const uint64_t live = ds->live_count;
+    const bool copy = Pin.base != Pout.base;
     for (uint64_t i = live + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t p = order[i];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            Pout.x(a, i) = Pin.x(a, p);
-            Pout.v(a, i) = Pin.v(a, p);
+        if (copy) {
+            const uint32_t p = order[i];
+            for (int f = 0; f < NFIELDS; ++f) Pout.at(i)[f * TILE] = Pin.at(p)[f * TILE];
         }
-#pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            Pout.F(k, i) = Pin.F(k, p);
-            Pout.A(k, i) = Pin.A(k, p);
-        }
-        Pout.m(i) = Pin.m(p);
-        Pout.vol(i) = Pin.vol(p);
-        Pout.p0(i) = Pin.p0(p);
-        Pout.p1(i) = Pin.p1(p);
-        Pout.flags(i) = Pin.flags(p);
-        Pout.id(i) = Pin.id(p);
         if (keys_out) keys_out[i] = tomb;
     }
 }
@@ -713,9 +998,9 @@ __global__ __launch_bounds__(256) void k_unpermute(ParticleArrays P, uint64_t n,
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t o = P.id(i);
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            x[3ull * o + a] = P.x(a, i);
-            v[3ull * o + a] = P.v(a, i);
+        for (int q = 0; q < 3; ++q) {
+            x[3ull * o + q] = P.x(q, i);
+            v[3ull * o + q] = P.v(q, i);
         }
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
@@ -781,7 +1066,7 @@ __global__ __launch_bounds__(256) void k_svd(const float *__restrict__ Fin, floa
         Vo[9 * i + k] = V[k];
     }
 #pragma unroll
-    for (int a = 0; a < 3; ++a) so[3 * i + a] = s[a];
+    for (int q = 0; q < 3; ++q) so[3 * i + q] = s[q];
 }
 
 // Work-groups of a persistent launch: all resident at once (the occupancy of
@@ -794,41 +1079,54 @@ int resident_groups(const void *kernel, int threads, const TransferLaunch &L) {
     return std::max(8, (per_cu * L.cu_count) & ~7);
 }
 
-template <uint32_t SET>
-hipError_t p2g_dispatch(const TransferLaunch &L, bool from_c, bool goals, const ParticleArrays &P, const AuxArrays &X,
-                        const uint32_t *order, const uint32_t *blk_start, const uint32_t *blk_key, const GridGeom &G,
-                        const SubstepParams &S, float4 *grid, uint32_t *blk_flag, uint32_t *cand, DeviceScalars *ds,
-                        hipStream_t stream) {
-    const dim3 blk(P2G_THREADS);
-#define SV_P2G(FC, GO)                                                                                                  \
-    do {                                                                                                                \
-        static const int groups = resident_groups(reinterpret_cast<const void *>(&k_p2g<FC, GO, SET>), P2G_THREADS, L); \
-        hipLaunchKernelGGL((k_p2g<FC, GO, SET>), dim3(groups), blk, 0, stream, P, X, order, blk_start, blk_key, G, S,    \
-                           grid, blk_flag, cand, ds);                                                                   \
-    } while (0)
-    if (from_c) {
-        if (goals) SV_P2G(true, true);
-        else SV_P2G(true, false);
-    } else {
-        if (goals) SV_P2G(false, true);
-        else SV_P2G(false, false);
-    }
-#undef SV_P2G
+template <int MODE, bool PERMUTE, bool GOALS, uint32_t SET>
+hipError_t launch_one(const TransferLaunch &L, const XferArgs &a, hipStream_t stream) {
+    static const int groups =
+        resident_groups(reinterpret_cast<const void *>(&k_transfer<MODE, PERMUTE, GOALS, SET>), XF_THREADS, L);
+    hipLaunchKernelGGL((k_transfer<MODE, PERMUTE, GOALS, SET>), dim3(groups), dim3(XF_THREADS), 0, stream, a);
     return hipGetLastError();
+}
+
+template <int MODE, uint32_t SET>
+hipError_t dispatch_flags(const TransferLaunch &L, bool permute, bool goals, const XferArgs &a, hipStream_t stream) {
+    if constexpr (MODE == XM_G2P) {
+        return permute ? launch_one<MODE, true, false, SET>(L, a, stream) : launch_one<MODE, false, false, SET>(L, a, stream);
+    } else {
+        if (permute)
+            return goals ? launch_one<MODE, true, true, SET>(L, a, stream) : launch_one<MODE, true, false, SET>(L, a, stream);
+        return goals ? launch_one<MODE, false, true, SET>(L, a, stream) : launch_one<MODE, false, false, SET>(L, a, stream);
+    }
+}
+
+template <int MODE>
+hipError_t dispatch_set(const TransferLaunch &L, bool permute, bool goals, const XferArgs &a, hipStream_t stream) {
+    if (L.model_set == SET_POLAR) return dispatch_flags<MODE, SET_POLAR>(L, permute, goals, a, stream);
+    return dispatch_flags<MODE, SET_ALL>(L, permute, goals, a, stream);
 }
 
 }  // namespace
 
-hipError_t launch_p2g(const TransferLaunch &L, bool from_c, const ParticleArrays &P, const AuxArrays &X,
-                      const uint32_t *order, const uint32_t *blk_start, const uint32_t *blk_key, const GridGeom &G,
-                      const SubstepParams &S, float4 *grid, uint32_t *blk_flag, uint32_t *cand, DeviceScalars *ds,
-                      hipStream_t stream) {
-    const bool goals = S.has_goals != 0;
-    if (L.model_set == SET_POLAR)
-        return p2g_dispatch<SET_POLAR>(L, from_c, goals, P, X, order, blk_start, blk_key, G, S, grid, blk_flag, cand, ds,
-                                       stream);
-    return p2g_dispatch<SET_ALL>(L, from_c, goals, P, X, order, blk_start, blk_key, G, S, grid, blk_flag, cand, ds,
-                                 stream);
+hipError_t launch_transfer(const TransferLaunch &L, const TransferCall &t, hipStream_t stream) {
+    XferArgs a;
+    a.Pin = t.Pin;
+    a.Pout = t.permute ? t.Pout : t.Pin;
+    a.X = t.X;
+    a.order = t.order;
+    a.blk_start = t.blk_start;
+    a.blk_key = t.blk_key;
+    a.G = t.G;
+    a.S0 = t.S0;
+    a.S1 = t.S1;
+    a.B = t.B;
+    a.keys_out = t.keys_out;
+    a.ds = t.ds;
+    const bool goals = t.S1.has_goals != 0;
+    switch (t.kind) {
+        case TransferKind::P2G: return dispatch_set<XM_P2G>(L, t.permute, goals, a, stream);
+        case TransferKind::G2P2G: return dispatch_set<XM_G2P2G>(L, t.permute, goals, a, stream);
+        case TransferKind::G2P: return dispatch_set<XM_G2P>(L, t.permute, false, a, stream);
+    }
+    return hipErrorInvalidValue;
 }
 
 // The grid update runs on a fixed grid so that its per-wave partial counts
@@ -836,39 +1134,18 @@ hipError_t launch_p2g(const TransferLaunch &L, bool from_c, const ParticleArrays
 uint32_t grid_update_groups(const TransferLaunch &L) { return (uint32_t)std::max(8, 4 * L.cu_count); }
 uint32_t grid_update_waves(const TransferLaunch &L) { return grid_update_groups(L) * 4u; }
 
-hipError_t launch_grid_update(const TransferLaunch &L, const uint32_t *cand, const GridGeom &G, const SubstepParams &S,
-                              float4 *grid, float4 *vgrid, uint32_t *blk_flag, uint32_t *blk_epoch, uint32_t *partial,
-                              DeviceScalars *ds, hipStream_t stream) {
-    hipLaunchKernelGGL(k_grid_update, dim3(grid_update_groups(L)), dim3(256), 0, stream, cand, G, S, grid, vgrid, blk_flag,
-                       blk_epoch, partial, ds);
+hipError_t launch_grid_update(const TransferLaunch &L, const GridBuffers &B, const GridGeom &G, const SubstepParams &S,
+                              bool momentum, uint32_t *partial, DeviceScalars *ds, hipStream_t stream) {
+    if (momentum)
+        hipLaunchKernelGGL(k_grid_update<true>, dim3(grid_update_groups(L)), dim3(256), 0, stream, B, G, S, partial, ds);
+    else
+        hipLaunchKernelGGL(k_grid_update<false>, dim3(grid_update_groups(L)), dim3(256), 0, stream, B, G, S, partial, ds);
     return hipGetLastError();
 }
 
 hipError_t launch_sum_partials(const TransferLaunch &L, const uint32_t *partial, DeviceScalars *ds,
                                hipStream_t stream) {
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, stream, partial, grid_update_waves(L), ds);
-    return hipGetLastError();
-}
-
-hipError_t launch_g2p(const TransferLaunch &L, bool last, const ParticleArrays &Pin, const ParticleArrays &Pout,
-                      const AuxArrays &X, const uint32_t *order, const uint32_t *blk_start, const uint32_t *blk_key,
-                      const GridGeom &G, const SubstepParams &S, const float4 *vgrid, uint32_t *keys_out,
-                      DeviceScalars *ds, hipStream_t stream) {
-    const dim3 blk(L.g2p_threads);
-#define SV_G2P(LA, SE)                                                                                              \
-    do {                                                                                                            \
-        const int groups = resident_groups(reinterpret_cast<const void *>(&k_g2p<LA, SE>), L.g2p_threads, L);       \
-        hipLaunchKernelGGL((k_g2p<LA, SE>), dim3(groups), blk, 0, stream, Pin, Pout, X, order, blk_start, blk_key, G, \
-                           S, vgrid, keys_out, ds);                                                                 \
-    } while (0)
-    if (L.model_set == SET_POLAR) {
-        if (last) SV_G2P(true, SET_POLAR);
-        else SV_G2P(false, SET_POLAR);
-    } else {
-        if (last) SV_G2P(true, SET_ALL);
-        else SV_G2P(false, SET_ALL);
-    }
-#undef SV_G2P
     return hipGetLastError();
 }
 

@@ -31,8 +31,17 @@ import re
 import sys
 from pathlib import Path
 
-# Kernel name prefix -> substep stage name used by bench.py.
-STAGES = {"k_p2g": "p2g", "k_g2p": "g2p", "k_grid_update": "grid_update"}
+# Kernel name prefix -> substep stage name used by bench.py; k_transfer's
+# first template argument is its mode.
+STAGES = {"k_grid_update": "grid_update"}
+TRANSFER_MODES = {"0": "p2g", "1": "g2p2g", "2": "g2p"}
+
+
+def stage_of(name: str) -> str | None:
+    base = name.split("<")[0]
+    if base == "k_transfer" and "<" in name:
+        return TRANSFER_MODES.get(name.split("<")[1].split(",")[0].strip())
+    return STAGES.get(base)
 
 
 def short_name(name: str) -> str:
@@ -122,14 +131,14 @@ def cmd_traffic(args) -> None:
         rd = vf * f_fetch * 1024 / 1e6
         wr = vw * f_write * 1024 / 1e6
         print(f"| `{k}` | {max(nf, nw)} | {vf:.0f} | {vw:.0f} | {rd:.2f} | {wr:.2f} |")
-        base = k.split("<")[0]
-        if base in STAGES and k in fetch and k in write:
+        stage = stage_of(k)
+        if stage and k in fetch and k in write:
             n = min(nf, nw)
-            if base not in best or n > best[base][0]:
-                best[base] = (n, rd * 1e6, wr * 1e6, k)
+            if stage not in best or n > best[stage][0]:
+                best[stage] = (n, rd * 1e6, wr * 1e6, k)
     if args.write_json:
-        for base, (n, rd, wr, k) in best.items():
-            out = Path(f"{args.write_json}{STAGES[base]}_{args.workload}.json")
+        for stage, (n, rd, wr, k) in best.items():
+            out = Path(f"{args.write_json}{stage}_{args.workload}.json")
             out.write_text(
                 json.dumps(
                     {
