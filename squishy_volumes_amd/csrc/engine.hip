@@ -65,11 +65,6 @@ struct mpm_engine {
     uint32_t *blk_flag = nullptr, *blk_epoch = nullptr;
     uint32_t *cand = nullptr;     // NCAND candidate touched blocks per occupied block
     uint32_t *ovf[2] = {nullptr, nullptr};  // overflow lists of claimed blocks, by epoch parity
-    float4 *part = nullptr;       // partial region arena per block-list entry
-    uint32_t part_capacity = 0;
-    uint64_t *entry = nullptr;    // per block: (sort generation << 32) | list index
-    uint32_t *blk_atomic = nullptr;
-    uint32_t sort_gen = 0;
     uint32_t *partial = nullptr;  // per-wave active-node counts of the grid update
     bool keys_valid = false;      // e->keys holds the keys of the current particle order
     bool pending_permute = false; // a sort ran since the last transfer
@@ -268,12 +263,7 @@ GridBuffers grid_buffers(const mpm_engine *e, uint32_t epoch) {
     GridBuffers B;
     B.grid = e->grid;
     B.vgrid = e->vgrid;
-    B.part = e->part;
-    B.part_capacity = e->part_capacity;
-    B.entry = e->entry;
-    B.gen = e->sort_gen;
     B.blk_flag = e->blk_flag;
-    B.blk_atomic = e->blk_atomic;
     B.blk_epoch = e->blk_epoch;
     B.cand = e->cand;
     B.ovf = e->ovf[epoch & 1u];
@@ -295,9 +285,8 @@ int run_sort(mpm_engine *e) {
                        e->stream));
     e->end(r);
     e->begin(ST_BLOCKS, r);
-    e->sort_gen += 1;
     HIP_TRY(launch_block_list(e->keys_sorted, e->n, e->geom.tomb_key, e->blk_start, e->blk_key, e->blk_capacity,
-                              e->geom, e->entry, e->sort_gen, e->ds, e->blk_temp, e->stream));
+                              e->ds, e->blk_temp, e->stream));
     e->end(r);
     e->keys_valid = false;
     e->pending_permute = true;
@@ -345,7 +334,7 @@ int run_transfer(mpm_engine *e, TransferKind kind, const SubstepParams &S0, cons
 int run_grid(mpm_engine *e, const SubstepParams &S) {
     EventRecord r{};
     e->begin(ST_GRID, r);
-    HIP_TRY(launch_grid_update(e->launch, grid_buffers(e, S.epoch), e->geom, S, false, e->partial, e->ds, e->stream));
+    HIP_TRY(launch_grid_update(e->launch, grid_buffers(e, S.epoch), e->geom, S, e->partial, e->ds, e->stream));
     e->end(r);
     e->grid_epoch = S.epoch;
     e->grid_valid = true;
@@ -457,22 +446,17 @@ int mpm_create(const mpm_consts *consts, const mpm_particles_view *pv, const mpm
         e->alloc(e->vgrid, nodes) || e->alloc(e->blk_flag, e->geom.num_blocks_total) ||
         e->alloc(e->blk_epoch, e->geom.num_blocks_total) || e->alloc(e->ds, 1) ||
         e->alloc(e->cand, NCAND * max_blocks) || e->alloc(e->ovf[0], e->geom.num_blocks_total) ||
-        e->alloc(e->ovf[1], e->geom.num_blocks_total) || e->alloc(e->partial, grid_update_waves(e->launch)) ||
-        e->alloc(e->entry, e->geom.num_blocks_total) || e->alloc(e->blk_atomic, e->geom.num_blocks_total))
+        e->alloc(e->ovf[1], e->geom.num_blocks_total) || e->alloc(e->partial, grid_update_waves(e->launch)))
         return fail(MPM_ERR_OUT_OF_MEMORY);
     e->sort_temp = st;
     e->blk_temp = bt;
-    // Partial region arenas (8 KiB per block-list entry) for up to 2^18
-    // entries; entries past them flush with atomics.
-    e->part_capacity = (uint32_t)std::min<uint64_t>(max_blocks, 1u << 18);
-    if (e->alloc(e->part, (size_t)e->part_capacity * 512)) return fail(MPM_ERR_OUT_OF_MEMORY);
+
     if (hipHostMalloc((void **)&e->ds_host, sizeof(DeviceScalars), hipHostMallocDefault) != hipSuccess)
         return fail(MPM_ERR_OUT_OF_MEMORY);
     if (hipMemsetAsync(e->grid, 0, nodes * 16, e->stream) != hipSuccess ||
         hipMemsetAsync(e->vgrid, 0, nodes * 16, e->stream) != hipSuccess ||
         hipMemsetAsync(e->blk_flag, 0, (size_t)e->geom.num_blocks_total * 4, e->stream) != hipSuccess ||
-        hipMemsetAsync(e->blk_atomic, 0, (size_t)e->geom.num_blocks_total * 4, e->stream) != hipSuccess ||
-        hipMemsetAsync(e->entry, 0, (size_t)e->geom.num_blocks_total * 8, e->stream) != hipSuccess ||
+
         hipMemsetAsync(e->blk_epoch, 0xff, (size_t)e->geom.num_blocks_total * 4, e->stream) != hipSuccess ||
         hipMemsetAsync(e->ds, 0, sizeof(DeviceScalars), e->stream) != hipSuccess)
         return fail(MPM_ERR_HIP);
@@ -663,21 +647,13 @@ static int read_grid_impl(mpm_engine *e, mpm_grid_view *out, bool momentum) {
     const uint64_t nblk = e->geom.num_blocks_total;
     std::vector<float4> g(nblk * 64);
     std::vector<uint32_t> ep(nblk);
-    if (momentum) {
-        // Sum the pending scatter (the partial arenas of the last transfer
-        // and its global-memory scatters) into the velocity grid, stamped
-        // with the pending epoch; the grid update recomputes it.
-        SubstepParams S{};
-        S.epoch = e->epoch;
-        HIP_TRY(launch_grid_update(e->launch, grid_buffers(e, S.epoch), e->geom, S, true, e->partial, e->ds, e->stream));
-    }
     HIP_TRY(hipStreamSynchronize(e->stream));
-    HIP_TRY(hipMemcpy(g.data(), e->vgrid, nblk * 64 * 16, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(g.data(), momentum ? e->grid : e->vgrid, nblk * 64 * 16, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(ep.data(), e->blk_epoch, nblk * 4, hipMemcpyDeviceToHost));
     uint64_t count = 0;
     const int *nb = e->geom.nb;
     for (uint64_t B = 0; B < nblk; ++B) {
-        if (momentum ? ep[B] != e->epoch : (!e->grid_valid || ep[B] != e->grid_epoch)) continue;
+        if (!momentum && (!e->grid_valid || ep[B] != e->grid_epoch)) continue;
         const int bx = (int)(B % nb[0]), by = (int)((B / nb[0]) % nb[1]), bz = (int)(B / ((uint64_t)nb[0] * nb[1]));
         for (int s = 0; s < 64; ++s) {
             const float4 q = g[B * 64 + s];

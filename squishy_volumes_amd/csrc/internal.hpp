@@ -148,11 +148,8 @@ hipError_t exclusive_scan(const uint32_t *in, uint32_t *out, uint32_t *total, ui
 hipError_t launch_keys(const ParticleArrays &P, const GridGeom &G, uint64_t n, uint32_t *keys, DeviceScalars *ds,
                        hipStream_t stream);
 size_t block_list_temp_bytes(uint64_t n);
-// The occupied-block list of the sorted keys; also stamps entry[block] =
-// (gen << 32) | list index for every listed block.
 hipError_t launch_block_list(const uint32_t *keys, uint64_t n, uint32_t tomb_key, uint32_t *blk_start,
-                             uint32_t *blk_key, uint32_t capacity, const GridGeom &G, uint64_t *entry, uint32_t gen,
-                             DeviceScalars *ds, void *temp, hipStream_t stream);
+                             uint32_t *blk_key, uint32_t capacity, DeviceScalars *ds, void *temp, hipStream_t stream);
 hipError_t launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t stream);
 hipError_t launch_reset_scalars(DeviceScalars *ds, hipStream_t stream);
 
@@ -168,20 +165,10 @@ struct TransferLaunch {
 enum class TransferKind { P2G, G2P2G, G2P };
 
 // Grid-side buffers shared by the transfer kernel and the grid update.
-// The scatter of each block-list entry leaves its 8^3 region arena in part
-// (plain stores); the grid update sums, for every node of a touched block,
-// the arenas of the (up to 8) entries whose regions cover it.  Only the rare
-// global-memory scatters (and entries beyond part_capacity) add into grid
-// with atomics, marking blk_atomic.
 struct GridBuffers {
-    float4 *grid;               // (m, p) from global-memory scatters
+    float4 *grid;               // (m, p): the scatter adds into it with float atomics
     float4 *vgrid;              // (m, v) after the grid update
-    float4 *part;               // [entry][arena slot]
-    uint32_t part_capacity;     // entries that have a partial arena
-    uint64_t *entry;            // per block: (sort generation << 32) | list index
-    uint32_t gen;               // current sort generation
     uint32_t *blk_flag;         // claimed for the grid update
-    uint32_t *blk_atomic;       // grid holds atomic contributions for the block
     uint32_t *blk_epoch;        // epoch of the block's last grid update
     uint32_t *cand;             // NCAND claimed blocks per list entry
     uint32_t *ovf;              // overflow list of claimed blocks (parity of the epoch)
@@ -204,10 +191,8 @@ struct TransferCall {
 };
 hipError_t launch_transfer(const TransferLaunch &L, const TransferCall &t, hipStream_t stream);
 uint32_t grid_update_waves(const TransferLaunch &L);
-// momentum: only sum (m, p) into vgrid and stamp the epoch (a read-back of
-// the scattered momentum), leaving claims and atomic contributions in place.
 hipError_t launch_grid_update(const TransferLaunch &L, const GridBuffers &B, const GridGeom &G, const SubstepParams &S,
-                              bool momentum, uint32_t *partial, DeviceScalars *ds, hipStream_t stream);
+                              uint32_t *partial, DeviceScalars *ds, hipStream_t stream);
 hipError_t launch_sum_partials(const TransferLaunch &L, const uint32_t *partial, DeviceScalars *ds,
                                hipStream_t stream);
 hipError_t launch_copy_dead(const ParticleArrays &Pin, const ParticleArrays &Pout, const uint32_t *order,

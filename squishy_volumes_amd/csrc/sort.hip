@@ -263,7 +263,6 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_heads_write(const uint32_t *__
                                                                uint32_t tomb, const uint32_t *__restrict__ tile_offsets,
                                                                uint32_t *__restrict__ blk_start,
                                                                uint32_t *__restrict__ blk_key, uint32_t capacity,
-                                                               GridGeom G, uint64_t *__restrict__ entry, uint32_t gen,
                                                                DeviceScalars *ds) {
     __shared__ uint32_t ws[SCAN_THREADS / 64];
     const uint32_t lane = lane_id();
@@ -288,9 +287,6 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_heads_write(const uint32_t *__
             if (slot < capacity) {
                 blk_start[slot] = (uint32_t)i;
                 blk_key[slot] = key;
-                const uint32_t b = compact1by2(key) +
-                                   (uint32_t)G.nb[0] * (compact1by2(key >> 1) + (uint32_t)G.nb[1] * compact1by2(key >> 2));
-                entry[b] = ((uint64_t)gen << 32) | slot;
             } else {
                 atomicOr(&ds->status, MPM_STATUS_INDIRECT_LIMIT_EXCEEDED | (KID_BLOCKS << 16));
             }
@@ -396,15 +392,14 @@ hipError_t launch_keys(const ParticleArrays &P, const GridGeom &G, uint64_t n, u
 size_t block_list_temp_bytes(uint64_t n) { return (size_t)std::max<uint32_t>(tiles_of(n, SCAN_TILE), 1u) * 4; }
 
 hipError_t launch_block_list(const uint32_t *keys, uint64_t n, uint32_t tomb_key, uint32_t *blk_start,
-                             uint32_t *blk_key, uint32_t capacity, const GridGeom &G, uint64_t *entry, uint32_t gen,
-                             DeviceScalars *ds, void *temp, hipStream_t stream) {
+                             uint32_t *blk_key, uint32_t capacity, DeviceScalars *ds, void *temp, hipStream_t stream) {
     uint32_t nt = tiles_of(n, SCAN_TILE);
     uint32_t *tile = static_cast<uint32_t *>(temp);
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_heads_count, dim3(nt), dim3(SCAN_THREADS), 0, stream, keys, n, tomb_key, tile, ds);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(TOP_THREADS), 0, stream, tile, nt, &ds->num_blocks);
     hipLaunchKernelGGL(k_heads_write, dim3(nt), dim3(SCAN_THREADS), 0, stream, keys, n, tomb_key, tile, blk_start,
-                       blk_key, capacity, G, entry, gen, ds);
+                       blk_key, capacity, ds);
     return hipGetLastError();
 }
 

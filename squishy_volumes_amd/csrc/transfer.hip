@@ -44,12 +44,13 @@ namespace svamd {
 namespace {
 
 constexpr int XF_THREADS = 256;
-constexpr int XF_CAP = 512;            // particles per pass of a block
+constexpr int XF_CAP = 768;            // particles per pass of a block
 constexpr int XF_PER_THREAD = XF_CAP / XF_THREADS;
 constexpr int NSLOT = 512;             // 8^3 node region
 constexpr int NCELL = 216;             // 6^3 cells
-constexpr int XF_RANKS = 8;            // core payloads per cell and pass
-constexpr int XF_OVF = 128;            // overflow payloads per pass
+constexpr int XF_RANKS = 12;           // core payloads per cell and pass
+constexpr int XF_OVF = 96;             // overflow payloads per pass
+constexpr int PSLOT = 256;             // nodes one stencil plane of the core cells reaches
 // Node velocities of the region in LDS: node (x, y, z) at x + VS_Y y + VS_Z z.
 // The strides make the 2x2x3 base cells that 16 consecutive particles of a
 // jittered 8-per-cell lattice share fall on distinct 16-byte bank groups.
@@ -66,6 +67,13 @@ enum : int { XM_P2G = 0, XM_G2P2G = 1, XM_G2P = 2 };
 // consecutive lanes 16 distinct 16-byte bank groups.
 __device__ __forceinline__ int arena_slot(int x, int y, int z) {
     return (((x >> 2) | ((y >> 2) << 1) | ((z >> 2) << 2)) << 6) | (x & 3) | ((y & 3) << 2) | ((z & 3) << 4);
+}
+
+// Slot of region node (x, y, z) in the arena of core stencil plane i: the 64
+// core cells reach x in [1 + i, 4 + i] only, so u = x - 1 - i in [0, 3] and
+// the plane needs 4 x 8 x 8 = 256 slots, with the same bank property.
+__device__ __forceinline__ int plane_slot(int u, int y, int z) {
+    return (((y >> 2) | ((z >> 2) << 1)) << 6) | u | ((y & 3) << 2) | ((z & 3) << 4);
 }
 
 __device__ __forceinline__ uint32_t block_linear(const GridGeom &G, uint32_t bx, uint32_t by, uint32_t bz) {
@@ -325,7 +333,6 @@ __device__ __forceinline__ void scatter_global(const GridBuffers &B, const GridG
 #pragma unroll 1
             for (int bx = cell[0] >> 2; bx <= (cell[0] + 2) >> 2; ++bx) {
                 const uint32_t T = block_linear(G, (uint32_t)bx, (uint32_t)by, (uint32_t)bz);
-                B.blk_atomic[T] = 1u;
                 if (atomicExch(&B.blk_flag[T], 1u) == 0u) {
                     const uint32_t slot = atomicAdd(&ds->ovf_count[epoch & 1u], 1u);
                     if (slot < G.num_blocks_total) B.ovf[slot] = T;
@@ -342,7 +349,8 @@ __device__ __forceinline__ void scatter_global(const GridBuffers &B, const GridG
 // into the wave's own arena with a plain read-add-write, one stencil offset
 // per round: for one offset the lanes (distinct cells) touch distinct nodes,
 // whose arena slots differ in their low six bits (no bank conflict).
-template <class Fetch>
+// PLANE: arena is core plane i's (plane_slot), else the full region arena.
+template <bool PLANE, class Fetch>
 __device__ __forceinline__ void accumulate_plane(int i, uint32_t cnt, int cx, int cy, int cz, Fetch fetch,
                                                  float4 *arena) {
     float acc[9][4];
@@ -380,7 +388,7 @@ __device__ __forceinline__ void accumulate_plane(int i, uint32_t cnt, int cx, in
     if (cnt == 0) return;
 #pragma unroll
     for (int n = 0; n < 9; ++n) {
-        float4 &g = arena[arena_slot(cx + i, cy + n / 3, cz + n % 3)];
+        float4 &g = arena[PLANE ? plane_slot(cx - 1, cy + n / 3, cz + n % 3) : arena_slot(cx + i, cy + n / 3, cz + n % 3)];
         float4 t = g;
         t.x += acc[n][0];
         t.y += acc[n][1];
@@ -398,19 +406,23 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
     constexpr bool HAS_G2P = MODE != XM_P2G;
     constexpr bool HAS_P2G = MODE != XM_G2P;
     constexpr bool LAST = MODE == XM_G2P;
-    // The region's node velocities (gather) and the four arenas (scatter:
-    // one per core plane wave, one for the overflow wave) share LDS; a pass
-    // restages the velocities after the previous pass's flush.
-    constexpr int SHARED4 = (HAS_P2G ? 4 * NSLOT : 0) > (HAS_G2P ? VS_SLOTS : 0) ? (HAS_P2G ? 4 * NSLOT : 0)
-                                                                                : (HAS_G2P ? VS_SLOTS : 1);
+    // The region's node velocities (gather) and the arenas (scatter: one
+    // 256-slot arena per core plane wave, one 512-slot arena for the overflow
+    // wave) share LDS; a pass restages the velocities after the previous
+    // pass's flush.
+    constexpr int ARENA4 = 3 * PSLOT + NSLOT;
+    constexpr int SHARED4 = (HAS_P2G ? ARENA4 : 0) > (HAS_G2P ? VS_SLOTS : 0) ? (HAS_P2G ? ARENA4 : 0)
+                                                                              : (HAS_G2P ? VS_SLOTS : 1);
     __shared__ float4 shared4[SHARED4];
     float4 *vstage = shared4;                                    // node (x, y, z) at x + VS_Y y + VS_Z z
-    float4 *arenas = shared4;                                    // [wave][arena_slot]
+    float4 *planes = shared4;                                    // [wave][plane_slot], waves 0-2
+    float4 *shell = shared4 + 3 * PSLOT;                         // [arena_slot], wave 3
     __shared__ float4 payc[HAS_P2G ? 4 * XF_RANKS * 64 : 1];     // core payloads [quarter][rank][cell]
     __shared__ float4 payx[HAS_P2G ? 4 * XF_OVF : 1];            // overflow payloads [quarter][entry]
     __shared__ uint8_t xcell[HAS_P2G ? XF_OVF : 1];              // region cell of each overflow entry
     __shared__ uint8_t xrank[HAS_P2G ? XF_OVF : 1];              // rank of the entry in its cell
     __shared__ uint8_t xperm[HAS_P2G ? XF_OVF : 1];              // entries in cell order
+    __shared__ uint8_t xocc[HAS_P2G ? NCELL : 1];                // occupied overflow cells
     __shared__ uint32_t xcount[HAS_P2G ? NCELL : 1];
     __shared__ uint32_t xstart[HAS_P2G ? NCELL : 1];
     __shared__ uint32_t core_count[64];
@@ -456,7 +468,7 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
                 }
             }
             __syncthreads();
-#pragma unroll
+#pragma unroll 1
             for (int kk = 0; kk < XF_PER_THREAD; ++kk) {
                 const uint32_t i = cs + tid + (uint32_t)(kk * XF_THREADS);
                 if (i >= ce) continue;
@@ -710,13 +722,13 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
 
             if constexpr (HAS_P2G) {
                 __syncthreads();
-                for (uint32_t s = tid; s < 4u * NSLOT; s += XF_THREADS) arenas[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+                for (uint32_t s = tid; s < (uint32_t)ARENA4; s += XF_THREADS) shared4[s] = make_float4(0.f, 0.f, 0.f, 0.f);
                 __syncthreads();
                 if (wave < 3u) {
                     // The block's own 64 cells, one stencil plane per wave.
                     const uint32_t cnt = min(core_count[lane], (uint32_t)XF_RANKS);
                     const float4 *pr = payc + lane;
-                    accumulate_plane(
+                    accumulate_plane<true>(
                         (int)wave, cnt, (int)(lane & 3u) + 1, (int)((lane >> 2) & 3u) + 1, (int)(lane >> 4) + 1,
                         [&](uint32_t r, float4 &a0, float4 &a1, float4 &a2, float4 &a3) {
                             const float4 *q = pr + r * 64u;
@@ -725,106 +737,100 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
                             a2 = q[2 * XF_RANKS * 64];
                             a3 = q[3 * XF_RANKS * 64];
                         },
-                        arenas + wave * NSLOT);
+                        planes + wave * PSLOT);
                 } else {
                     // Overflow entries (shell cells, ranks past the core slots):
-                    // bucketed by cell, then lane = cell over the 216 cells.
+                    // bucketed by cell; lane = occupied cell, 64 at a time.
                     const uint32_t ne = min(nx, (uint32_t)XF_OVF);
-                    for (uint32_t q = lane; q < (uint32_t)NCELL; q += 64u) xcount[q] = 0u;
-                    __builtin_amdgcn_wave_barrier();
-                    for (uint32_t e = lane; e < ne; e += 64u) xrank[e] = (uint8_t)atomicAdd(&xcount[xcell[e]], 1u);
-                    __builtin_amdgcn_wave_barrier();
-                    uint32_t cnt4[4], sum = 0;
+                    if (ne > 0) {
+                        for (uint32_t q = lane; q < (uint32_t)NCELL; q += 64u) xcount[q] = 0u;
+                        __builtin_amdgcn_wave_barrier();
+                        for (uint32_t e = lane; e < ne; e += 64u)
+                            xrank[e] = (uint8_t)atomicAdd(&xcount[xcell[e]], 1u);
+                        __builtin_amdgcn_wave_barrier();
+                        // Exclusive scans of the counts and of the occupied cells, 4 cells per lane.
+                        uint32_t cnt4[4], sum = 0, occ = 0;
 #pragma unroll
-                    for (int t = 0; t < 4; ++t) {
-                        const uint32_t q = 4u * lane + (uint32_t)t;
-                        cnt4[t] = q < (uint32_t)NCELL ? xcount[q] : 0u;
-                        sum += cnt4[t];
-                    }
-                    uint32_t incl = sum;
+                        for (int t = 0; t < 4; ++t) {
+                            const uint32_t q = 4u * lane + (uint32_t)t;
+                            cnt4[t] = q < (uint32_t)NCELL ? xcount[q] : 0u;
+                            sum += cnt4[t];
+                            occ += cnt4[t] ? 1u : 0u;
+                        }
+                        uint32_t incl = sum, iocc = occ;
 #pragma unroll
-                    for (int o = 1; o < 64; o <<= 1) {
-                        const uint32_t y = __shfl_up(incl, o, 64);
-                        if (lane >= (uint32_t)o) incl += y;
-                    }
-                    uint32_t run = incl - sum;
+                        for (int o = 1; o < 64; o <<= 1) {
+                            const uint32_t y = __shfl_up(incl, o, 64), yo = __shfl_up(iocc, o, 64);
+                            if (lane >= (uint32_t)o) {
+                                incl += y;
+                                iocc += yo;
+                            }
+                        }
+                        const uint32_t nocc = __shfl(iocc, 63, 64);
+                        uint32_t run = incl - sum, orun = iocc - occ;
 #pragma unroll
-                    for (int t = 0; t < 4; ++t) {
-                        const uint32_t q = 4u * lane + (uint32_t)t;
-                        if (q < (uint32_t)NCELL) xstart[q] = run;
-                        run += cnt4[t];
-                    }
-                    __builtin_amdgcn_wave_barrier();
-                    for (uint32_t e = lane; e < ne; e += 64u) xperm[xstart[xcell[e]] + xrank[e]] = (uint8_t)e;
-                    __builtin_amdgcn_wave_barrier();
+                        for (int t = 0; t < 4; ++t) {
+                            const uint32_t q = 4u * lane + (uint32_t)t;
+                            if (q < (uint32_t)NCELL) xstart[q] = run;
+                            if (cnt4[t]) xocc[orun++] = (uint8_t)q;
+                            run += cnt4[t];
+                        }
+                        __builtin_amdgcn_wave_barrier();
+                        for (uint32_t e = lane; e < ne; e += 64u) xperm[xstart[xcell[e]] + xrank[e]] = (uint8_t)e;
+                        __builtin_amdgcn_wave_barrier();
 #pragma unroll 1
-                    for (uint32_t q0 = 0; ne > 0 && q0 < (uint32_t)NCELL; q0 += 64u) {
-                        const uint32_t q = q0 + lane;
-                        const uint32_t cnt = q < (uint32_t)NCELL ? xcount[q] : 0u;
-                        if (__ballot(cnt > 0u) == 0ull) continue;
-                        const uint32_t st = cnt ? xstart[q] : 0u;
-                        const int cx = (int)(q % 6u), cy = (int)((q / 6u) % 6u), cz = (int)(q / 36u);
+                        for (uint32_t o0 = 0; o0 < nocc; o0 += 64u) {
+                            const uint32_t o = o0 + lane;
+                            const uint32_t q = o < nocc ? xocc[o] : 0u;
+                            const uint32_t cnt = o < nocc ? xcount[q] : 0u;
+                            const uint32_t st = cnt ? xstart[q] : 0u;
+                            const int cx = (int)(q % 6u), cy = (int)((q / 6u) % 6u), cz = (int)(q / 36u);
 #pragma unroll 1
-                        for (int i = 0; i < 3; ++i)
-                            accumulate_plane(
-                                i, cnt, cx, cy, cz,
-                                [&](uint32_t r, float4 &a0, float4 &a1, float4 &a2, float4 &a3) {
-                                    const float4 *px = payx + xperm[st + r];
-                                    a0 = px[0];
-                                    a1 = px[XF_OVF];
-                                    a2 = px[2 * XF_OVF];
-                                    a3 = px[3 * XF_OVF];
-                                },
-                                arenas + 3 * NSLOT);
+                            for (int i = 0; i < 3; ++i)
+                                accumulate_plane<false>(
+                                    i, cnt, cx, cy, cz,
+                                    [&](uint32_t r, float4 &a0, float4 &a1, float4 &a2, float4 &a3) {
+                                        const float4 *px = payx + xperm[st + r];
+                                        a0 = px[0];
+                                        a1 = px[XF_OVF];
+                                        a2 = px[2 * XF_OVF];
+                                        a3 = px[3 * XF_OVF];
+                                    },
+                                    shell);
+                        }
                     }
                 }
                 __syncthreads();
-                // Flush the four arenas: the entry's partial arena (plain
-                // stores; later passes of the block add to it), or global
-                // atomics past the partial capacity.  Note which of the 27
-                // blocks around the block received mass.
-                const bool has_part = bi < a.B.part_capacity;
-                float4 *part = a.B.part + (uint64_t)bi * NSLOT;
+                // Flush the arenas to the grid with global float atomics;
+                // note which of the 27 blocks around the block received mass.
                 for (uint32_t s = tid; s < (uint32_t)NSLOT; s += XF_THREADS) {
-                    float4 g4 = arenas[s];
+                    const uint32_t lo = s & 63u, hi = s >> 6;
+                    const int rx = (int)((lo & 3u) | ((hi & 1u) << 2)),
+                              ry = (int)(((lo >> 2) & 3u) | (((hi >> 1) & 1u) << 2)),
+                              rz = (int)(((lo >> 4) & 3u) | (((hi >> 2) & 1u) << 2));
+                    float4 g4 = shell[s];
 #pragma unroll
-                    for (int w = 1; w < 4; ++w) {
-                        const float4 o = arenas[w * NSLOT + s];
+                    for (int w = 0; w < 3; ++w) {
+                        const int u = rx - 1 - w;
+                        if (u < 0 || u > 3) continue;
+                        const float4 o = planes[w * PSLOT + plane_slot(u, ry, rz)];
                         g4.x += o.x;
                         g4.y += o.y;
                         g4.z += o.z;
                         g4.w += o.w;
                     }
-                    if (has_part) {
-                        if (cs != start) {
-                            const float4 o = part[s];
-                            g4.x += o.x;
-                            g4.y += o.y;
-                            g4.z += o.z;
-                            g4.w += o.w;
-                        }
-                        part[s] = g4;
-                    }
                     if (!(g4.x != 0.f)) continue;
-                    const uint32_t lo = s & 63u, hi = s >> 6;
-                    const int rx = (int)((lo & 3u) | ((hi & 1u) << 2)),
-                              ry = (int)(((lo >> 2) & 3u) | (((hi >> 1) & 1u) << 2)),
-                              rz = (int)(((lo >> 4) & 3u) | (((hi >> 2) & 1u) << 2));
                     const int x = 4 * (int)b[0] + rx - 1, y = 4 * (int)b[1] + ry - 1, z = 4 * (int)b[2] + rz - 1;
                     if (!node_in_grid(G, x, y, z)) {
                         atomicOr(&ds->status, MPM_STATUS_OUT_OF_GRID | (KID_P2G << 16));
                         continue;
                     }
                     sub_used[((rx + 3) >> 2) + 3 * ((ry + 3) >> 2) + 9 * ((rz + 3) >> 2)] = 1u;
-                    if (!has_part) {
-                        float *g =
-                            reinterpret_cast<float *>(a.B.grid + node_index(G, (uint32_t)x, (uint32_t)y, (uint32_t)z));
-                        unsafeAtomicAdd(g + 0, g4.x);
-                        unsafeAtomicAdd(g + 1, g4.y);
-                        unsafeAtomicAdd(g + 2, g4.z);
-                        unsafeAtomicAdd(g + 3, g4.w);
-                        a.B.blk_atomic[block_linear(G, (uint32_t)x >> 2, (uint32_t)y >> 2, (uint32_t)z >> 2)] = 1u;
-                    }
+                    float *g = reinterpret_cast<float *>(a.B.grid + node_index(G, (uint32_t)x, (uint32_t)y, (uint32_t)z));
+                    unsafeAtomicAdd(g + 0, g4.x);
+                    unsafeAtomicAdd(g + 1, g4.y);
+                    unsafeAtomicAdd(g + 2, g4.z);
+                    unsafeAtomicAdd(g + 3, g4.w);
                 }
                 if (tid < 64u) core_count[tid] = 0u;
                 if (tid == 0) nx = 0u;
@@ -853,11 +859,8 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
 // ------------------------------------------------------------ grid update
 // One wave per touched block T (lane = node): the candidate slots written by
 // the transfer kernel (NCAND per block-list entry, then the overflow list)
-// are read 64 at a time and the claimed blocks processed in turn.  A node of
-// T lies in the regions of T and, along each axis, of one neighbour (T-1 for
-// the first three nodes, T+1 for the last): its (m, p) is the sum of those up
-// to 8 partial arenas, plus the global-memory scatters when T has any.
-template <bool MOMENTUM>
+// are read 64 at a time and the claimed blocks processed in turn: v = p / m,
+// and the accumulation grid is cleared for the next scatter.
 __global__ __launch_bounds__(256) void k_grid_update(GridBuffers B, GridGeom G, SubstepParams S,
                                                      uint32_t *__restrict__ partial, DeviceScalars *ds) {
     const uint32_t lane = threadIdx.x & 63u;
@@ -865,11 +868,9 @@ __global__ __launch_bounds__(256) void k_grid_update(GridBuffers B, GridGeom G, 
     const uint32_t nw = gridDim.x * (blockDim.x / 64);
     const uint32_t nlist = ds->num_blocks * (uint32_t)NCAND;
     const uint32_t novf = min(ds->ovf_count[S.epoch & 1u], G.num_blocks_total);
-    if (!MOMENTUM && gw == 0 && lane == 0) ds->ovf_count[(S.epoch + 1u) & 1u] = 0u;  // for the next transfer
+    if (gw == 0 && lane == 0) ds->ovf_count[(S.epoch + 1u) & 1u] = 0u;  // for the next transfer
     const uint32_t ncand = nlist + novf;
     const uint32_t x = lane & 3u, y = (lane >> 2) & 3u, z = lane >> 4;
-    const int ax = x <= 2u ? -1 : 1, ay = y <= 2u ? -1 : 1, az = z <= 2u ? -1 : 1;
-    const int ndx = (int)(lane % 3u) - 1, ndy = (int)((lane / 3u) % 3u) - 1, ndz = (int)(lane / 9u) - 1;
     uint32_t active = 0;
     for (uint32_t chunk = gw; chunk * 64u < ncand; chunk += nw) {
         const uint32_t c = chunk * 64u + lane;
@@ -881,43 +882,8 @@ __global__ __launch_bounds__(256) void k_grid_update(GridBuffers B, GridGeom G, 
             const uint32_t T = __shfl(mine, l, 64);
             const int bx = (int)(T % (uint32_t)G.nb[0]), by = (int)((T / (uint32_t)G.nb[0]) % (uint32_t)G.nb[1]),
                       bz = (int)(T / ((uint32_t)G.nb[0] * (uint32_t)G.nb[1]));
-            // Lanes 0..26: the block-list entry of neighbour T + d, if it has a partial arena.
-            uint32_t ent = NO_BLOCK;
-            if (lane < (uint32_t)NCAND) {
-                const int qx = bx + ndx, qy = by + ndy, qz = bz + ndz;
-                if (qx >= 0 && qy >= 0 && qz >= 0 && qx < G.nb[0] && qy < G.nb[1] && qz < G.nb[2]) {
-                    const uint64_t v = B.entry[block_linear(G, (uint32_t)qx, (uint32_t)qy, (uint32_t)qz)];
-                    if ((uint32_t)(v >> 32) == B.gen && (uint32_t)v < B.part_capacity) ent = (uint32_t)v;
-                }
-            }
-            float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const int dx = (k & 1) ? ax : 0, dy = (k & 2) ? ay : 0, dz = (k & 4) ? az : 0;
-                const uint32_t e = __shfl(ent, (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1), 64);
-                if (e != NO_BLOCK) {
-                    const float4 o = B.part[(uint64_t)e * NSLOT +
-                                            arena_slot((int)x + 1 - 4 * dx, (int)y + 1 - 4 * dy, (int)z + 1 - 4 * dz)];
-                    g.x += o.x;
-                    g.y += o.y;
-                    g.z += o.z;
-                    g.w += o.w;
-                }
-            }
             const uint32_t node = T * 64u + lane;
-            const bool atomic_part = B.blk_atomic[T] != 0u;
-            if (atomic_part) {
-                const float4 o = B.grid[node];
-                g.x += o.x;
-                g.y += o.y;
-                g.z += o.z;
-                g.w += o.w;
-            }
-            if (MOMENTUM) {
-                B.vgrid[node] = g;
-                if (lane == 0) B.blk_epoch[T] = S.epoch;
-                continue;
-            }
+            const float4 g = B.grid[node];
             const float m = g.x;
             float v[3] = {0.f, 0.f, 0.f};
             if (m > 0.f) {
@@ -946,10 +912,9 @@ __global__ __launch_bounds__(256) void k_grid_update(GridBuffers B, GridGeom G, 
                 ++active;
             }
             B.vgrid[node] = make_float4(m, v[0], v[1], v[2]);
-            if (atomic_part) B.grid[node] = make_float4(0.f, 0.f, 0.f, 0.f);
+            B.grid[node] = make_float4(0.f, 0.f, 0.f, 0.f);
             if (lane == 0) {
                 B.blk_flag[T] = 0u;
-                B.blk_atomic[T] = 0u;
                 B.blk_epoch[T] = S.epoch;
             }
         }
@@ -1135,11 +1100,8 @@ uint32_t grid_update_groups(const TransferLaunch &L) { return (uint32_t)std::max
 uint32_t grid_update_waves(const TransferLaunch &L) { return grid_update_groups(L) * 4u; }
 
 hipError_t launch_grid_update(const TransferLaunch &L, const GridBuffers &B, const GridGeom &G, const SubstepParams &S,
-                              bool momentum, uint32_t *partial, DeviceScalars *ds, hipStream_t stream) {
-    if (momentum)
-        hipLaunchKernelGGL(k_grid_update<true>, dim3(grid_update_groups(L)), dim3(256), 0, stream, B, G, S, partial, ds);
-    else
-        hipLaunchKernelGGL(k_grid_update<false>, dim3(grid_update_groups(L)), dim3(256), 0, stream, B, G, S, partial, ds);
+                              uint32_t *partial, DeviceScalars *ds, hipStream_t stream) {
+    hipLaunchKernelGGL(k_grid_update, dim3(grid_update_groups(L)), dim3(256), 0, stream, B, G, S, partial, ds);
     return hipGetLastError();
 }
 
