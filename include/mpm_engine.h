@@ -189,7 +189,7 @@ typedef struct mpm_engine mpm_engine;
 
 /* Engine tuning (the autotuner's knobs).  Zero fields take defaults. */
 typedef struct mpm_tuning {
-    int32_t resort_interval;             /* substeps between particle sorts (default 4); between
+    int32_t resort_interval;             /* substeps between particle sorts (default 8); between
                                             sorts a particle keeps its slot and may drift one cell
                                             out of its block; farther drift takes a slower path */
     int32_t persistent_blocks;           /* work-groups of the transfer kernel (default: all resident) */

@@ -68,7 +68,7 @@ struct mpm_engine {
     uint32_t *partial = nullptr;  // per-wave active-node counts of the grid update
     bool keys_valid = false;      // e->keys holds the keys of the current particle order
     bool pending_permute = false; // a sort ran since the last transfer
-    int resort_interval = 4;      // substeps between sorts
+    int resort_interval = 8;      // substeps between sorts
     DeviceScalars *ds = nullptr;
     DeviceScalars *ds_host = nullptr;  // pinned
 

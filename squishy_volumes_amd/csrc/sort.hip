@@ -35,6 +35,9 @@ __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 // Lanes of the calling wave (among the active ones) whose digit equals d.
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, int bits) {
     uint64_t peers = __ballot(1);
+    // Nearly sorted keys: usually every active lane holds the same digit.
+    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+    if (__ballot(d == d0) == peers) return peers;
     for (int b = 0; b < bits; ++b) {
         uint32_t bit = (d >> b) & 1u;
         uint64_t m = __ballot(bit);
@@ -139,11 +142,18 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint32_t *__r
     __syncthreads();
     uint64_t base = (uint64_t)blockIdx.x * SORT_TILE;
     const uint32_t mask = (1u << bits) - 1u;
-#pragma unroll 4
+    // All of the tile's loads are issued before the first is used.
+    uint32_t kreg[SORT_ITEMS];
+#pragma unroll
+    for (int k = 0; k < SORT_ITEMS; ++k) {
+        const uint64_t i = base + (uint64_t)k * SORT_THREADS + threadIdx.x;
+        kreg[k] = i < n ? keys[i] : 0u;
+    }
+#pragma unroll
     for (int k = 0; k < SORT_ITEMS; ++k) {
         uint64_t i = base + (uint64_t)k * SORT_THREADS + threadIdx.x;
         if (i < n) {
-            uint32_t d = (keys[i] >> shift) & mask;
+            uint32_t d = (kreg[k] >> shift) & mask;
             uint64_t peers = match_digit(d, bits);
             uint32_t leader = (uint32_t)__ffsll((unsigned long long)peers) - 1u;
             if (lane_id() == leader) atomicAdd(&h[d], (uint32_t)__popcll(peers));
@@ -166,6 +176,15 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint32_t *
     const uint32_t mask = (1u << bits) - 1u;
     digit_base[threadIdx.x] = offsets[(uint64_t)threadIdx.x * T + blockIdx.x];
     uint64_t base = (uint64_t)blockIdx.x * SORT_TILE;
+    // All of the tile's loads are issued before the first round.
+    uint32_t kreg[SORT_ITEMS], vreg[SORT_ITEMS];
+#pragma unroll
+    for (int k = 0; k < SORT_ITEMS; ++k) {
+        const uint64_t i = base + (uint64_t)k * SORT_THREADS + threadIdx.x;
+        kreg[k] = i < n ? keys_in[i] : 0u;
+        vreg[k] = i < n ? (vals_in ? vals_in[i] : (uint32_t)i) : 0u;
+    }
+#pragma unroll
     for (int k = 0; k < SORT_ITEMS; ++k) {
 #pragma unroll
         for (int w = 0; w < WAVES; ++w) wave_cnt[w][threadIdx.x] = 0;
@@ -174,8 +193,8 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint32_t *
         bool valid = i < n;
         uint32_t key = 0, val = 0, d = 0, rank = 0;
         if (valid) {
-            key = keys_in[i];
-            val = vals_in ? vals_in[i] : (uint32_t)i;
+            key = kreg[k];
+            val = vreg[k];
             d = (key >> shift) & mask;
             uint64_t peers = match_digit(d, bits);
             rank = (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
@@ -247,7 +266,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_heads_count(const uint32_t *__
     __shared__ uint32_t ws[SCAN_THREADS / 64 + 1];
     const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + threadIdx.x;
     uint32_t cnt = 0;
-#pragma unroll 4
+#pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) {
         const uint64_t i = base + (uint64_t)k * SCAN_THREADS;
         uint32_t key = tomb;
@@ -269,10 +288,19 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_heads_write(const uint32_t *__
     const uint32_t wave = threadIdx.x / 64;
     const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + threadIdx.x;
     uint32_t running = tile_offsets[blockIdx.x];
+    // The head tests (and their loads) of all rounds come first.
+    uint32_t kreg[SCAN_ITEMS];
+    bool hreg[SCAN_ITEMS];
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        kreg[k] = tomb;
+        hreg[k] = is_head(keys, base + (uint64_t)k * SCAN_THREADS, n, tomb, kreg[k]);
+    }
+#pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) {
         const uint64_t i = base + (uint64_t)k * SCAN_THREADS;
-        uint32_t key = tomb;
-        const bool head = is_head(keys, i, n, tomb, key);
+        const uint32_t key = kreg[k];
+        const bool head = hreg[k];
         const uint64_t m = __ballot(head);
         if (lane == 0) ws[wave] = (uint32_t)__popcll(m);
         __syncthreads();

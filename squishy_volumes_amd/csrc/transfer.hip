@@ -359,9 +359,12 @@ __device__ __forceinline__ void accumulate_plane(int i, uint32_t cnt, int cx, in
 #pragma unroll
         for (int c = 0; c < 4; ++c) acc[n][c] = 0.f;
     const float fi = (float)i;
+    // The next payload's reads are issued before the current one is used.
+    float4 n0, n1, n2, n3;
+    if (cnt > 0) fetch(0u, n0, n1, n2, n3);
     for (uint32_t r = 0; r < cnt; ++r) {
-        float4 a0, a1, a2, a3;
-        fetch(r, a0, a1, a2, a3);
+        const float4 a0 = n0, a1 = n1, a2 = n2, a3 = n3;
+        if (r + 1u < cnt) fetch(r + 1u, n0, n1, n2, n3);
         const float m = a0.w;
         const float c0[3] = {a1.w, a2.x, a2.y};
         const float c1[3] = {a2.z, a2.w, a3.x};

@@ -278,3 +278,53 @@ def test_spheres_and_slab_invariants_at_full_size():
     Ff = p.matrix_F()[fl][:20000].astype(np.float64)
     J = np.linalg.det(Ff)
     assert np.abs(Ff @ np.transpose(Ff, (0, 2, 1)) - (J ** (2 / 3))[:, None, None] * np.eye(3)).max() < 1e-4
+
+
+def spin(parts: Particles, omega):
+    """Adds the rigid rotation omega x (x - centre) to the velocities and its
+    gradient to the velocity gradients (column-major)."""
+    w = np.asarray(omega, np.float64)
+    c = parts.positions.astype(np.float64).mean(0)
+    parts.velocities[:] += np.cross(w, parts.positions - c).astype(np.float32)
+    W = np.array([[0.0, -w[2], w[1]], [w[2], 0.0, -w[0]], [-w[1], w[0], 0.0]])
+    parts.velocity_gradients[:] += W.T.astype(np.float32)
+
+
+@pytest.mark.parametrize("interval", [1, 16])
+def test_fast_particles_drift_out_of_their_blocks(interval):
+    """A cube moving about 0.4 cells per substep: without a sort for 16
+    substeps its particles leave the one-cell margin of their blocks, which
+    takes the global-memory gather and scatter.  Both schedules must hold the
+    parity tolerance."""
+    sc = scenes.cube_scene(n_cells=4, corner_cells=(20, 20, 20), velocity=(60.0, -40.0, 30.0),
+                           domain=((-0.5, -0.5, -0.5), (1.5, 1.5, 1.5)), dt=1e-4)
+    spin(sc.io_state.particles, (40.0, 10.0, -20.0))
+    T = scenes.advance_time(0.0, sc.time_step, 12)
+    g = engine(sc, tuning={"resort_interval": interval})
+    st, err = g.produce_next_state(sc.frame_input, GpuRunParameters(target_time=T))
+    assert err is None and g.last_status.substeps == 12
+    o32, o64 = oracles(sc)
+    o64.step_to(T, sc.frame_input)
+    compare_states(st.particles, o64.particles_f64(), sc.consts.grid_node_size, f"fast cube, resort interval {interval}")
+
+
+def test_dense_blocks_take_several_passes_and_overflow_ranks():
+    """Two jittered lattices (16 particles per cell) in an 8^3-cell cube, spun
+    about its centre: blocks hold up to 1024 particles (more than one pass of
+    the transfer kernel) and cells more than the 12 ranked payload slots (the
+    overflow list)."""
+    a = scenes.cube_scene(n_cells=8, corner_cells=(20, 20, 20), velocity=(0.5, -0.3, 0.2), seed=1,
+                          domain=((-0.5, -0.5, -0.5), (1.5, 1.5, 1.5)))
+    b = scenes.cube_scene(n_cells=8, corner_cells=(20, 20, 20), velocity=(0.5, -0.3, 0.2), seed=2,
+                          domain=((-0.5, -0.5, -0.5), (1.5, 1.5, 1.5)))
+    parts = scenes.concat([a.io_state.particles, b.io_state.particles])
+    spin(parts, (3.0, -2.0, 4.0))
+    sc = scenes.Scene(name="dense_cube", consts=a.consts, time_step=a.time_step, options=a.options,
+                      io_state=IoState(time=0.0, particles=parts), frame_input=a.frame_input, substeps=8)
+    T = scenes.advance_time(0.0, sc.time_step, 8)
+    g = engine(sc, tuning={"resort_interval": 3})
+    st, err = g.produce_next_state(sc.frame_input, GpuRunParameters(target_time=T))
+    assert err is None
+    o32, o64 = oracles(sc)
+    o64.step_to(T, sc.frame_input)
+    compare_states(st.particles, o64.particles_f64(), sc.consts.grid_node_size, "dense cube")
