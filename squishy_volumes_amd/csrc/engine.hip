@@ -296,7 +296,7 @@ int run_sort(mpm_engine *e) {
 // One transfer launch.  After a sort the transfer reads slot i from
 // order[i] and writes the other particle set; otherwise it works in place.
 // want_keys: write the block keys of the new positions for the next sort.
-int run_transfer(mpm_engine *e, TransferKind kind, const SubstepParams &S0, const SubstepParams &S1, bool want_keys) {
+int run_transfer(mpm_engine *e, TransferKind kind, const SubstepParams &S1, bool want_keys) {
     EventRecord r{};
     TransferCall t{};
     t.kind = kind;
@@ -308,7 +308,6 @@ int run_transfer(mpm_engine *e, TransferKind kind, const SubstepParams &S0, cons
     t.blk_start = e->blk_start;
     t.blk_key = e->blk_key;
     t.G = e->geom;
-    t.S0 = S0;
     t.S1 = S1;
     t.B = grid_buffers(e, S1.epoch);
     t.keys_out = want_keys ? e->keys : nullptr;
@@ -562,11 +561,10 @@ int mpm_step_to(mpm_engine *e, double target_time, const mpm_frame_input *input,
         }
         if (sorts_before(t) && (rc = run_sort(e)) != MPM_OK) return rc;
         const bool want_keys = t == k || sorts_before(t + 1);
-        SubstepParams S0 = params[t > 0 ? t - 1 : 0];
         SubstepParams S1 = params[t < k ? t : k - 1];
         S1.epoch = e->epoch;
         const TransferKind kind = t == 0 ? TransferKind::P2G : (t == k ? TransferKind::G2P : TransferKind::G2P2G);
-        if ((rc = run_transfer(e, kind, S0, S1, want_keys)) != MPM_OK) return rc;
+        if ((rc = run_transfer(e, kind, S1, want_keys)) != MPM_OK) return rc;
         if (t < k && (rc = run_grid(e, S1)) != MPM_OK) return rc;
         if (t > 0) {
             e->time = times[t - 1] + (double)e->consts.time_step;
@@ -693,10 +691,10 @@ int mpm_run_stage(mpm_engine *e, int32_t stage, const mpm_frame_input *input) {
     S.epoch = e->epoch;
     switch (stage) {
         case 0: rc = run_sort(e); break;
-        case 1: rc = run_transfer(e, TransferKind::P2G, S, S, false); break;
+        case 1: rc = run_transfer(e, TransferKind::P2G, S, false); break;
         case 2: rc = run_grid(e, S); break;
         case 3:
-            rc = run_transfer(e, TransferKind::G2P, S, S, true);
+            rc = run_transfer(e, TransferKind::G2P, S, true);
             if (rc == MPM_OK) {
                 e->time += (double)e->consts.time_step;
                 e->stepped = true;

@@ -183,8 +183,8 @@ struct TransferCall {
     const uint32_t *blk_start;
     const uint32_t *blk_key;
     GridGeom G;
-    SubstepParams S0;           // substep of the gather
-    SubstepParams S1;           // substep of the scatter
+    SubstepParams S1;           // substep of the scatter (the gather's dt, domain and
+                                // plasticity constants are the same every substep)
     GridBuffers B;              // B.ovf: the list for S1's grid (parity S1.epoch & 1)
     uint32_t *keys_out;         // null: keys not needed
     DeviceScalars *ds;

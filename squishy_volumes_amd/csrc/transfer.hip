@@ -289,7 +289,7 @@ struct XferArgs {
     const uint32_t *blk_start;
     const uint32_t *blk_key;
     GridGeom G;
-    SubstepParams S0;            // the substep whose gather runs (G2P modes)
+
     SubstepParams S1;            // the substep whose scatter runs (P2G modes)
     GridBuffers B;               // B.ovf: the overflow list for S1's grid
     uint32_t *keys_out;          // block keys of the new positions, or null
@@ -359,12 +359,9 @@ __device__ __forceinline__ void accumulate_plane(int i, uint32_t cnt, int cx, in
 #pragma unroll
         for (int c = 0; c < 4; ++c) acc[n][c] = 0.f;
     const float fi = (float)i;
-    // The next payload's reads are issued before the current one is used.
-    float4 n0, n1, n2, n3;
-    if (cnt > 0) fetch(0u, n0, n1, n2, n3);
     for (uint32_t r = 0; r < cnt; ++r) {
-        const float4 a0 = n0, a1 = n1, a2 = n2, a3 = n3;
-        if (r + 1u < cnt) fetch(r + 1u, n0, n1, n2, n3);
+        float4 a0, a1, a2, a3;
+        fetch(r, a0, a1, a2, a3);
         const float m = a0.w;
         const float c0[3] = {a1.w, a2.x, a2.y};
         const float c1[3] = {a2.z, a2.w, a3.x};
@@ -374,18 +371,27 @@ __device__ __forceinline__ void accumulate_plane(int i, uint32_t cnt, int cx, in
         bspline_weights(a0.z, wz);
         const float d = i == 0 ? 1.5f - a0.x : (i == 1 ? a0.x - 1.0f : a0.x - 0.5f);
         const float wi = i == 1 ? 0.75f - d * d : 0.5f * d * d;
-        const float ti[3] = {a1.x + fi * c0[0], a1.y + fi * c0[1], a1.z + fi * c0[2]};
+        // Node values built by running sums (no multiplications by the
+        // offsets 0, which IEEE arithmetic cannot fold away).
+        float tj[3] = {a1.x + fi * c0[0], a1.y + fi * c0[1], a1.z + fi * c0[2]};
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             const float wij = wi * wy[j];
+            float tk[3] = {tj[0], tj[1], tj[2]};
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 const float wt = wij * wz[k];
                 const int n = 3 * j + k;
                 acc[n][0] += wt * m;
 #pragma unroll
-                for (int r3 = 0; r3 < 3; ++r3) acc[n][1 + r3] += wt * (ti[r3] + (float)j * c1[r3] + (float)k * c2[r3]);
+                for (int r3 = 0; r3 < 3; ++r3) acc[n][1 + r3] += wt * tk[r3];
+                if (k < 2)
+#pragma unroll
+                    for (int r3 = 0; r3 < 3; ++r3) tk[r3] += c2[r3];
             }
+            if (j < 2)
+#pragma unroll
+                for (int r3 = 0; r3 < 3; ++r3) tj[r3] += c1[r3];
         }
     }
     if (cnt == 0) return;
@@ -440,7 +446,7 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
     const uint32_t nblocks = ds->num_blocks;
     const uint32_t live = ds->live_count;
     const float h = G.h;
-    const float dt = a.S0.dt;
+    const float dt = a.S1.dt;
     const float scaling = a.S1.dt * 4.0f / (h * h);
     const float apic = 4.0f / h;
 
@@ -563,14 +569,14 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
                         float jp = a.X.jp[id];
 #pragma unroll
                         for (int q = 0; q < 3; ++q) {
-                            const float cl = fminf(fmaxf(sv[q], a.S0.snow_lo), a.S0.snow_hi);
+                            const float cl = fminf(fmaxf(sv[q], a.S1.snow_lo), a.S1.snow_hi);
                             jp *= sv[q] / cl;
                             sv[q] = cl;
                         }
                         a.X.jp[id] = jp;
                         recompose(U, sv, V, F);
                         J = sv[0] * sv[1] * sv[2];
-                        hard = expf(a.S0.snow_hardening * (1.0f - jp));
+                        hard = expf(a.S1.snow_hardening * (1.0f - jp));
                         if (LAST) energy = energy_fixed_corotated(p0 * hard, p1 * hard, sv);
                     } else {
                         if ((SET & M_SAND) && (flags & F_SAND)) {
@@ -617,7 +623,7 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
                     // cull_particles: strict inequalities.
 #pragma unroll
                     for (int q = 0; q < 3; ++q)
-                        inside = inside && xp[q] > a.S0.domain_min[q] && xp[q] < a.S0.domain_max[q];
+                        inside = inside && xp[q] > a.S1.domain_min[q] && xp[q] < a.S1.domain_max[q];
                     if (!inside) flags |= F_TOMB;
 
                     if (a.keys_out) a.keys_out[i] = inside ? particle_block_key(xp, G, ds) : G.tomb_key;
@@ -1083,7 +1089,7 @@ hipError_t launch_transfer(const TransferLaunch &L, const TransferCall &t, hipSt
     a.blk_start = t.blk_start;
     a.blk_key = t.blk_key;
     a.G = t.G;
-    a.S0 = t.S0;
+
     a.S1 = t.S1;
     a.B = t.B;
     a.keys_out = t.keys_out;
