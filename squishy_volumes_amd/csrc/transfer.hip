@@ -340,6 +340,32 @@ __device__ __forceinline__ void scatter_global(const GridBuffers &B, const GridG
             }
 }
 
+// The fields of one particle that the transfer kernel reads in every mode.
+struct ParticleIn {
+    float x[3], F[9], m, vol, p0, p1;
+    uint32_t id, flags, src;
+};
+
+// Issues all loads of slot i's particle (its source slot is order[i] when
+// permuting) at once; nothing here waits on a loaded value except the source
+// slot.
+template <bool PERMUTE>
+__device__ __forceinline__ void load_particle(const ParticleArrays &P, const uint32_t *order, uint32_t i,
+                                              ParticleIn &r) {
+    r.src = PERMUTE ? order[i] : i;
+    const float *p = P.at(r.src);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) r.x[q] = p[(FX + q) * TILE];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) r.F[k] = p[(FF + k) * TILE];
+    r.m = p[FM * TILE];
+    r.vol = p[FVOL * TILE];
+    r.p0 = p[FP0 * TILE];
+    r.p1 = p[FP1 * TILE];
+    r.id = __float_as_uint(p[FID * TILE]);
+    r.flags = __float_as_uint(p[FFLAGS * TILE]);
+}
+
 // Payload of one particle for the scatter, 16 floats in four float4:
 //   (f, m | q | h A columns); its contribution at stencil node (i, j, k) is
 //   w_ijk (m v + A d) = w_ijk (q + i c0 + j c1 + k c2).
@@ -467,24 +493,53 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
 
         for (uint32_t cs = start; cs < end; cs += XF_CAP) {
             const uint32_t ce = min(end, cs + (uint32_t)XF_CAP);
+            // The fields of a thread's next particle are loaded while the
+            // current one computes (3 particles per thread, 256 apart).  The
+            // copy into cur at the top of an iteration is the one wait for
+            // them: afterwards no load that cur depends on is outstanding, so
+            // the conditional prefetch leaves the compiler no wait to guess.
+            // The first particle's loads are issued with the two staged nodes
+            // of each thread, all unconditionally (clamped indices), so that
+            // the staging waits for its own loads only.
+            ParticleIn cur, nxt;
+            const uint32_t i_first = min(cs + tid, ce - 1u);
             if constexpr (HAS_G2P) {
-                for (uint32_t s = tid; s < (uint32_t)NSLOT; s += XF_THREADS) {
+                static_assert(NSLOT == 2 * XF_THREADS, "two staged nodes per thread");
+                float4 sv[2];
+                int so[2];
+                bool sin[2];
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    const uint32_t s = tid + (uint32_t)(t * XF_THREADS);
                     const int rx = (int)(s & 7u), ry = (int)((s >> 3) & 7u), rz = (int)(s >> 6);
                     const int x = 4 * (int)b[0] + rx - 1, y = 4 * (int)b[1] + ry - 1, z = 4 * (int)b[2] + rz - 1;
-                    vstage[rx + VS_Y * ry + VS_Z * rz] =
-                        node_in_grid(G, x, y, z) ? a.B.vgrid[node_index(G, (uint32_t)x, (uint32_t)y, (uint32_t)z)]
-                                                 : make_float4(0.f, 0.f, 0.f, 0.f);
+                    sin[t] = node_in_grid(G, x, y, z);
+                    sv[t] = a.B.vgrid[sin[t] ? node_index(G, (uint32_t)x, (uint32_t)y, (uint32_t)z) : 0u];
+                    so[t] = rx + VS_Y * ry + VS_Z * rz;
                 }
+                // Keeps the node loads ahead of the particle loads (the
+                // in-order load counter then lets the staging wait for them
+                // alone).
+                __builtin_amdgcn_sched_barrier(0);
+                load_particle<PERMUTE>(a.Pin, a.order, i_first, nxt);
+#pragma unroll
+                for (int t = 0; t < 2; ++t) vstage[so[t]] = sin[t] ? sv[t] : make_float4(0.f, 0.f, 0.f, 0.f);
+            } else {
+                load_particle<PERMUTE>(a.Pin, a.order, i_first, nxt);
             }
             __syncthreads();
 #pragma unroll 1
             for (int kk = 0; kk < XF_PER_THREAD; ++kk) {
                 const uint32_t i = cs + tid + (uint32_t)(kk * XF_THREADS);
-                if (i >= ce) continue;
-                const uint32_t src = PERMUTE ? a.order[i] : i;
+                if (i >= ce) break;
+                cur = nxt;
+                if (kk + 1 < XF_PER_THREAD && i + XF_THREADS < ce)
+                    load_particle<PERMUTE>(a.Pin, a.order, i + XF_THREADS, nxt);
+                const uint32_t src = cur.src;
                 const ParticleArrays &Pin = a.Pin;
                 const ParticleArrays &Pout = a.Pout;
-                uint32_t flags = Pin.flags(src);
+                const uint32_t flags0 = cur.flags;
+                uint32_t flags = flags0;
                 if (flags & F_TOMB) {
                     // Culled since the last sort: keeps its slot, takes no part.
                     if (PERMUTE) {
@@ -495,11 +550,11 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
                 }
                 float xp[3], F[9];
 #pragma unroll
-                for (int q = 0; q < 3; ++q) xp[q] = Pin.x(q, src);
+                for (int q = 0; q < 3; ++q) xp[q] = cur.x[q];
 #pragma unroll
-                for (int k = 0; k < 9; ++k) F[k] = Pin.F(k, src);
-                const float m = Pin.m(src), vol = Pin.vol(src), p0 = Pin.p0(src), p1 = Pin.p1(src);
-                const uint32_t id = Pin.id(src);
+                for (int k = 0; k < 9; ++k) F[k] = cur.F[k];
+                const float m = cur.m, vol = cur.vol, p0 = cur.p0, p1 = cur.p1;
+                const uint32_t id = cur.id;
                 float v[3], C[9];
                 float J = 1.f;
                 float R[9], U[9], sv[3] = {1.f, 1.f, 1.f};
@@ -638,7 +693,7 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
                         for (int k = 0; k < 9; ++k) Pout.A(k, i) = C[k];
                         a.X.energy[id] = energy;
                     }
-                    if (PERMUTE || flags != Pin.flags(src)) Pout.flags(i) = flags;
+                    if (PERMUTE || flags != flags0) Pout.flags(i) = flags;
                     if (PERMUTE) {
                         Pout.m(i) = m;
                         Pout.vol(i) = vol;
