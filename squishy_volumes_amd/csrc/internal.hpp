@@ -168,9 +168,10 @@ enum class TransferKind { P2G, G2P2G, G2P };
 struct GridBuffers {
     float4 *grid;               // (m, p): the scatter adds into it with float atomics
     float4 *vgrid;              // (m, v) after the grid update
-    uint32_t *blk_flag;         // claimed for the grid update
+    uint32_t *blk_flag;         // set for blocks that received mass; cleared by the grid update
     uint32_t *blk_epoch;        // epoch of the block's last grid update
-    uint32_t *cand;             // NCAND claimed blocks per list entry
+    uint32_t *cand;             // NCAND candidate blocks per list entry (a block may be listed
+                                // by several entries)
     uint32_t *ovf;              // overflow list of claimed blocks (parity of the epoch)
 };
 

@@ -340,6 +340,24 @@ __device__ __forceinline__ void scatter_global(const GridBuffers &B, const GridG
             }
 }
 
+// The blocks that the stencil of a particle in region cell c (relative to the
+// block's first cell, each component in [-1, 4]) reaches, as a 27-bit mask:
+// bit (ox + 1) + 3 (oy + 1) + 9 (oz + 1) for block offset (ox, oy, oz).  Its
+// nodes c .. c + 2 lie in block offsets floor(c / 4) .. floor((c + 2) / 4).
+__device__ __forceinline__ uint32_t touched_blocks(const int c[3]) {
+    uint32_t m[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const int lo = (c[q] >> 2) + 1, hi = ((c[q] + 2) >> 2) + 1;
+        m[q] = ((2u << hi) - 1u) & ~((1u << lo) - 1u);
+    }
+    const uint32_t X = m[0] * 0x1249249u;  // bit i at i + 3t, t < 9
+    const uint32_t y3 = ((m[1] & 1u) ? 07u : 0u) | ((m[1] & 2u) ? 070u : 0u) | ((m[1] & 4u) ? 0700u : 0u);
+    const uint32_t Y = y3 * 0x40201u;      // repeated at 0, 9, 18
+    const uint32_t Z = ((m[2] & 1u) ? 0x1ffu : 0u) | ((m[2] & 2u) ? 0x1ffu << 9 : 0u) | ((m[2] & 4u) ? 0x1ffu << 18 : 0u);
+    return X & Y & Z;
+}
+
 // The fields of one particle that the transfer kernel reads in every mode.
 struct ParticleIn {
     float x[3], F[9], m, vol, p0, p1;
@@ -462,7 +480,7 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
     __shared__ uint32_t xstart[HAS_P2G ? NCELL : 1];
     __shared__ uint32_t core_count[64];
     __shared__ uint32_t nx;
-    __shared__ uint32_t sub_used[NCAND];
+    __shared__ uint32_t blk_touch;  // blocks reached by the block's particles (touched_blocks)
 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
@@ -476,58 +494,101 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
     const float scaling = a.S1.dt * 4.0f / (h * h);
     const float apic = 4.0f / h;
 
+    // The work-group visits its block-list entries j = 0, 1, ... (list index
+    // range.begin + j * range.step), each in passes of at most XF_CAP
+    // particles.  The loads that start a pass (the region's node velocities,
+    // two per thread, and each thread's first particle) are issued as soon
+    // as the previous pass's gather is done, so that they travel while it
+    // accumulates and flushes.  Lane l of every wave holds entry
+    // meta_base + l's particle range and key, refilled 64 entries at a time.
     const BlockRange range = xcd_block_range(nblocks);
-    for (uint32_t bi = range.begin; bi < range.end; bi += range.step) {
-        const uint32_t start = a.blk_start[bi];
-        const uint32_t end = bi + 1 < nblocks ? a.blk_start[bi + 1] : live;
+    uint32_t meta_base = 0, m_start = 0, m_end = 0, m_key = 0;
+    auto load_meta = [&](uint32_t base) {
+        meta_base = base;
+        const uint32_t bj = range.begin + (base + lane) * range.step;
+        if (bj < range.end) {
+            m_start = a.blk_start[bj];
+            m_end = bj + 1 < nblocks ? a.blk_start[bj + 1] : live;
+            m_key = a.blk_key[bj];
+        }
+    };
+    struct Item {
+        uint32_t j, bi, start, end, cs;
         uint32_t b[3];
-        decode_block(a.blk_key[bi], b);
+    };
+    auto get_block = [&](uint32_t j, Item &w) -> bool {
+        w.bi = range.begin + j * range.step;
+        if (w.bi >= range.end) return false;
+        if (j - meta_base >= 64u) load_meta(j);
+        const int l = (int)(j - meta_base);
+        w.j = j;
+        w.start = (uint32_t)__builtin_amdgcn_readlane((int)m_start, l);
+        w.end = (uint32_t)__builtin_amdgcn_readlane((int)m_end, l);
+        w.cs = w.start;
+        decode_block((uint32_t)__builtin_amdgcn_readlane((int)m_key, l), w.b);
+        return true;
+    };
+    static_assert(NSLOT == 2 * XF_THREADS, "two staged nodes per thread");
+    int so[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const uint32_t s = tid + (uint32_t)(t * XF_THREADS);
+        so[t] = (int)(s & 7u) + VS_Y * (int)((s >> 3) & 7u) + VS_Z * (int)(s >> 6);
+    }
+    // The fields of a thread's next particle are loaded while the current
+    // one computes (3 particles per thread, 256 apart).  The copy into cur at
+    // the top of an iteration is the one wait for them: afterwards no load
+    // that cur depends on is outstanding, so the conditional prefetch leaves
+    // the compiler no wait to guess.  All loads that start a pass are
+    // unconditional (clamped indices).
+    ParticleIn cur, nxt;
+    float4 sv[2];
+    bool sin[2];
+    auto issue = [&](const Item &w) {
+        const uint32_t ce = min(w.end, w.cs + (uint32_t)XF_CAP);
+        if constexpr (HAS_G2P) {
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const uint32_t s = tid + (uint32_t)(t * XF_THREADS);
+                const int x = 4 * (int)w.b[0] + (int)(s & 7u) - 1, y = 4 * (int)w.b[1] + (int)((s >> 3) & 7u) - 1,
+                          z = 4 * (int)w.b[2] + (int)(s >> 6) - 1;
+                sin[t] = node_in_grid(G, x, y, z);
+                sv[t] = a.B.vgrid[sin[t] ? node_index(G, (uint32_t)x, (uint32_t)y, (uint32_t)z) : 0u];
+            }
+            // Keeps the node loads ahead of the particle loads (the in-order
+            // load counter then lets the staging wait for them alone).
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        load_particle<PERMUTE>(a.Pin, a.order, min(w.cs + tid, ce - 1u), nxt);
+    };
+
+    load_meta(0);
+    Item w;
+    bool have = get_block(0, w);
+    if (have) issue(w);
+    if constexpr (HAS_P2G) {
+        if (tid < 64u) core_count[tid] = 0u;
+        if (tid == 0) nx = 0u;
+    }
+    while (have) {
+        const uint32_t bi = w.bi, start = w.start, end = w.end, cs = w.cs;
+        const uint32_t ce = min(end, cs + (uint32_t)XF_CAP);
+        const uint32_t b[3] = {w.b[0], w.b[1], w.b[2]};
         int cell0[3];  // global cell of the block's first cell
 #pragma unroll
         for (int q = 0; q < 3; ++q) cell0[q] = G.origin[q] + 4 * (int)b[q];
-        if constexpr (HAS_P2G) {
-            if (tid < 64u) core_count[tid] = 0u;
-            if (tid == 0) nx = 0u;
-            if (tid < (uint32_t)NCAND) sub_used[tid] = 0u;
-        }
-
-        for (uint32_t cs = start; cs < end; cs += XF_CAP) {
-            const uint32_t ce = min(end, cs + (uint32_t)XF_CAP);
-            // The fields of a thread's next particle are loaded while the
-            // current one computes (3 particles per thread, 256 apart).  The
-            // copy into cur at the top of an iteration is the one wait for
-            // them: afterwards no load that cur depends on is outstanding, so
-            // the conditional prefetch leaves the compiler no wait to guess.
-            // The first particle's loads are issued with the two staged nodes
-            // of each thread, all unconditionally (clamped indices), so that
-            // the staging waits for its own loads only.
-            ParticleIn cur, nxt;
-            const uint32_t i_first = min(cs + tid, ce - 1u);
+        Item wn;
+        bool have_next;
+        {
             if constexpr (HAS_G2P) {
-                static_assert(NSLOT == 2 * XF_THREADS, "two staged nodes per thread");
-                float4 sv[2];
-                int so[2];
-                bool sin[2];
-#pragma unroll
-                for (int t = 0; t < 2; ++t) {
-                    const uint32_t s = tid + (uint32_t)(t * XF_THREADS);
-                    const int rx = (int)(s & 7u), ry = (int)((s >> 3) & 7u), rz = (int)(s >> 6);
-                    const int x = 4 * (int)b[0] + rx - 1, y = 4 * (int)b[1] + ry - 1, z = 4 * (int)b[2] + rz - 1;
-                    sin[t] = node_in_grid(G, x, y, z);
-                    sv[t] = a.B.vgrid[sin[t] ? node_index(G, (uint32_t)x, (uint32_t)y, (uint32_t)z) : 0u];
-                    so[t] = rx + VS_Y * ry + VS_Z * rz;
-                }
-                // Keeps the node loads ahead of the particle loads (the
-                // in-order load counter then lets the staging wait for them
-                // alone).
-                __builtin_amdgcn_sched_barrier(0);
-                load_particle<PERMUTE>(a.Pin, a.order, i_first, nxt);
 #pragma unroll
                 for (int t = 0; t < 2; ++t) vstage[so[t]] = sin[t] ? sv[t] : make_float4(0.f, 0.f, 0.f, 0.f);
-            } else {
-                load_particle<PERMUTE>(a.Pin, a.order, i_first, nxt);
+            }
+            if constexpr (HAS_P2G) {
+                if (cs == start && tid == 0) blk_touch = 0u;
             }
             __syncthreads();
+            uint32_t touch = 0;  // touched_blocks of this thread's particles
 #pragma unroll 1
             for (int kk = 0; kk < XF_PER_THREAD; ++kk) {
                 const uint32_t i = cs + tid + (uint32_t)(kk * XF_THREADS);
@@ -740,6 +801,7 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
                         c[q] = (int)fl - cell0[q];
                         in_region = in_region && c[q] >= -1 && c[q] <= 4;
                     }
+                    if (in_region) touch |= touched_blocks(c);
                     pl[3] = m;
 #pragma unroll
                     for (int r = 0; r < 3; ++r) {
@@ -783,6 +845,22 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
                     }
                 }
             }
+
+            if constexpr (HAS_P2G) {
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) touch |= (uint32_t)__shfl_xor((int)touch, o, 64);
+                if (lane == 0) atomicOr(&blk_touch, touch);
+            }
+
+            // The next pass: the block's next XF_CAP particles, or the next block.
+            if (ce < end) {
+                wn = w;
+                wn.cs = ce;
+                have_next = true;
+            } else {
+                have_next = get_block(w.j + 1, wn);
+            }
+            if (have_next) issue(wn);
 
             if constexpr (HAS_P2G) {
                 __syncthreads();
@@ -864,9 +942,23 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
                         }
                     }
                 }
+                // Last pass of the block: list the blocks around it that its
+                // particles reach as candidates for the grid update and flag
+                // them (plain stores: several entries may list one block, and
+                // the grid update keeps the one whose exchange finds the flag).
+                if (ce >= end && tid < (uint32_t)NCAND) {
+                    const int dx = (int)(tid % 3u) - 1, dy = (int)((tid / 3u) % 3u) - 1, dz = (int)(tid / 9u) - 1;
+                    const int bx = (int)b[0] + dx, by = (int)b[1] + dy, bz = (int)b[2] + dz;
+                    uint32_t T = NO_BLOCK;
+                    if (((blk_touch >> tid) & 1u) && bx >= 0 && by >= 0 && bz >= 0 && bx < G.nb[0] && by < G.nb[1] &&
+                        bz < G.nb[2]) {
+                        T = block_linear(G, (uint32_t)bx, (uint32_t)by, (uint32_t)bz);
+                        a.B.blk_flag[T] = 1u;
+                    }
+                    a.B.cand[NCAND * bi + tid] = T;
+                }
                 __syncthreads();
-                // Flush the arenas to the grid with global float atomics;
-                // note which of the 27 blocks around the block received mass.
+                // Flush the arenas to the grid with global float atomics.
                 for (uint32_t s = tid; s < (uint32_t)NSLOT; s += XF_THREADS) {
                     const uint32_t lo = s & 63u, hi = s >> 6;
                     const int rx = (int)((lo & 3u) | ((hi & 1u) << 2)),
@@ -889,7 +981,6 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
                         atomicOr(&ds->status, MPM_STATUS_OUT_OF_GRID | (KID_P2G << 16));
                         continue;
                     }
-                    sub_used[((rx + 3) >> 2) + 3 * ((ry + 3) >> 2) + 9 * ((rz + 3) >> 2)] = 1u;
                     float *g = reinterpret_cast<float *>(a.B.grid + node_index(G, (uint32_t)x, (uint32_t)y, (uint32_t)z));
                     unsafeAtomicAdd(g + 0, g4.x);
                     unsafeAtomicAdd(g + 1, g4.y);
@@ -904,26 +995,15 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
             }
         }
 
-        if constexpr (HAS_P2G) {
-            if (tid < (uint32_t)NCAND) {
-                const int dx = (int)(tid % 3u) - 1, dy = (int)((tid / 3u) % 3u) - 1, dz = (int)(tid / 9u) - 1;
-                const int bx = (int)b[0] + dx, by = (int)b[1] + dy, bz = (int)b[2] + dz;
-                uint32_t out = NO_BLOCK;
-                if (sub_used[tid] && bx >= 0 && by >= 0 && bz >= 0 && bx < G.nb[0] && by < G.nb[1] && bz < G.nb[2]) {
-                    const uint32_t T = block_linear(G, (uint32_t)bx, (uint32_t)by, (uint32_t)bz);
-                    if (atomicExch(&a.B.blk_flag[T], 1u) == 0u) out = T;
-                }
-                a.B.cand[NCAND * bi + tid] = out;
-            }
-        }
-        __syncthreads();
+        w = wn;
+        have = have_next;
     }
 }
 
 // ------------------------------------------------------------ grid update
 // One wave per touched block T (lane = node): the candidate slots written by
 // the transfer kernel (NCAND per block-list entry, then the overflow list)
-// are read 64 at a time and the claimed blocks processed in turn: v = p / m,
+// are read 64 at a time, and each flagged block is processed once: v = p / m,
 // and the accumulation grid is cleared for the next scatter.
 __global__ __launch_bounds__(256) void k_grid_update(GridBuffers B, GridGeom G, SubstepParams S,
                                                      uint32_t *__restrict__ partial, DeviceScalars *ds) {
@@ -938,7 +1018,10 @@ __global__ __launch_bounds__(256) void k_grid_update(GridBuffers B, GridGeom G, 
     uint32_t active = 0;
     for (uint32_t chunk = gw; chunk * 64u < ncand; chunk += nw) {
         const uint32_t c = chunk * 64u + lane;
-        const uint32_t mine = c < nlist ? B.cand[c] : (c < ncand ? B.ovf[c - nlist] : NO_BLOCK);
+        uint32_t mine = c < nlist ? B.cand[c] : (c < ncand ? B.ovf[c - nlist] : NO_BLOCK);
+        // A block may be listed by several entries: the one whose exchange
+        // finds the flag set processes it (and clears the flag).
+        if (mine != NO_BLOCK && atomicExch(&B.blk_flag[mine], 0u) == 0u) mine = NO_BLOCK;
         uint64_t todo = __ballot(mine != NO_BLOCK);
         while (todo) {
             const int l = __ffsll((unsigned long long)todo) - 1;
@@ -977,10 +1060,7 @@ __global__ __launch_bounds__(256) void k_grid_update(GridBuffers B, GridGeom G, 
             }
             B.vgrid[node] = make_float4(m, v[0], v[1], v[2]);
             B.grid[node] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (lane == 0) {
-                B.blk_flag[T] = 0u;
-                B.blk_epoch[T] = S.epoch;
-            }
+            if (lane == 0) B.blk_epoch[T] = S.epoch;
         }
     }
     // Per-wave count of active nodes (summed once per step_to call).
