@@ -177,6 +177,7 @@ int compute_geometry(const mpm_consts &c, GridGeom &g, int &key_bits) {
     key_bits = (int)(3 * k);
     g.tomb_key = (key_bits == 32) ? 0xffffffffu : ((1u << key_bits) - 1u);
     g.h = c.grid_node_size;
+    g.inv_h = 1.0f / g.h;
     uint64_t total = (uint64_t)g.nb[0] * g.nb[1] * g.nb[2];
     // Two float4 node grids: accumulation and velocity.
     if (total * 64ull * 16ull * 2ull > (96ull << 30)) return MPM_ERR_GRID_TOO_LARGE;

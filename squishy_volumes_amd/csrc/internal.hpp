@@ -67,7 +67,8 @@ struct GridGeom {
     int origin[3];
     int nb[3];
     float h;
-    uint32_t axis_bits;  // Morton bits per axis; 2^axis_bits > nb[a]
+    float inv_h;         // 1/h rounded to nearest: x / h is div_by(x, h, inv_h)
+    uint32_t axis_bits;  // Morton bits per axis; 2^axis_bits >= nb[a]
     uint32_t tomb_key;   // all-ones in 3*axis_bits bits: never a valid block
     uint32_t num_blocks_total;
 };
@@ -95,7 +96,7 @@ __device__ __forceinline__ uint32_t particle_block_key(const float x[3], const G
     bool ok = true;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const float xn = x[a] / G.h - 0.5f;
+        const float xn = div_by(x[a], G.h, G.inv_h) - 0.5f;
         ok = ok && isfinite(xn);
         const float fl = floorf(xn);
         ok = ok && fl > -1.0e9f && fl < 1.0e9f;

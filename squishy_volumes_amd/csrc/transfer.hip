@@ -544,7 +544,12 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
     ParticleIn cur, nxt;
     float4 sv[2];
     bool sin[2];
-    auto issue = [&](const Item &w) {
+    // Thread t takes slot t ^ flip of each group of XF_THREADS particles, with
+    // flip alternating between 0 and 255 from pass to pass: the short last
+    // group of a pass falls on waves 0-1 and on waves 2-3 in turn, so that the
+    // SIMDs (wave w of every work-group runs on SIMD w) share the load.
+    uint32_t npass = 0;
+    auto issue = [&](const Item &w, uint32_t flip) {
         const uint32_t ce = min(w.end, w.cs + (uint32_t)XF_CAP);
         if constexpr (HAS_G2P) {
 #pragma unroll
@@ -559,13 +564,13 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
             // load counter then lets the staging wait for them alone).
             __builtin_amdgcn_sched_barrier(0);
         }
-        load_particle<PERMUTE>(a.Pin, a.order, min(w.cs + tid, ce - 1u), nxt);
+        load_particle<PERMUTE>(a.Pin, a.order, min(w.cs + (tid ^ flip), ce - 1u), nxt);
     };
 
     load_meta(0);
     Item w;
     bool have = get_block(0, w);
-    if (have) issue(w);
+    if (have) issue(w, 0u);
     if constexpr (HAS_P2G) {
         if (tid < 64u) core_count[tid] = 0u;
         if (tid == 0) nx = 0u;
@@ -573,6 +578,7 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
     while (have) {
         const uint32_t bi = w.bi, start = w.start, end = w.end, cs = w.cs;
         const uint32_t ce = min(end, cs + (uint32_t)XF_CAP);
+        const uint32_t flip = npass & 1u ? 255u : 0u;
         const uint32_t b[3] = {w.b[0], w.b[1], w.b[2]};
         int cell0[3];  // global cell of the block's first cell
 #pragma unroll
@@ -591,7 +597,7 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
             uint32_t touch = 0;  // touched_blocks of this thread's particles
 #pragma unroll 1
             for (int kk = 0; kk < XF_PER_THREAD; ++kk) {
-                const uint32_t i = cs + tid + (uint32_t)(kk * XF_THREADS);
+                const uint32_t i = cs + (tid ^ flip) + (uint32_t)(kk * XF_THREADS);
                 if (i >= ce) break;
                 cur = nxt;
                 if (kk + 1 < XF_PER_THREAD && i + XF_THREADS < ce)
@@ -630,7 +636,7 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
                     bool in_region = true;
 #pragma unroll
                     for (int q = 0; q < 3; ++q) {
-                        const float xn = xp[q] / h;
+                        const float xn = div_by(xp[q], h, G.inv_h);
                         const float fl = floorf(xn - 0.5f);
                         f[q] = xn - fl;
                         c[q] = (int)fl - cell0[q];
@@ -795,7 +801,7 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
                     bool in_region = true;
 #pragma unroll
                     for (int q = 0; q < 3; ++q) {
-                        const float xn = xp[q] / h;
+                        const float xn = div_by(xp[q], h, G.inv_h);
                         const float fl = floorf(xn - 0.5f);
                         pl[q] = xn - fl;
                         c[q] = (int)fl - cell0[q];
@@ -860,7 +866,7 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
             } else {
                 have_next = get_block(w.j + 1, wn);
             }
-            if (have_next) issue(wn);
+            if (have_next) issue(wn, (npass + 1u) & 1u ? 255u : 0u);
 
             if constexpr (HAS_P2G) {
                 __syncthreads();
@@ -869,11 +875,11 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
                 if (wave < 3u) {
                     // The block's own 64 cells, one stencil plane per wave.
                     const uint32_t cnt = min(core_count[lane], (uint32_t)XF_RANKS);
-                    const float4 *pr = payc + lane;
+
                     accumulate_plane<true>(
                         (int)wave, cnt, (int)(lane & 3u) + 1, (int)((lane >> 2) & 3u) + 1, (int)(lane >> 4) + 1,
                         [&](uint32_t r, float4 &a0, float4 &a1, float4 &a2, float4 &a3) {
-                            const float4 *q = pr + r * 64u;
+                            const float4 *q = payc + r * 64u + lane;
                             a0 = q[0];
                             a1 = q[XF_RANKS * 64];
                             a2 = q[2 * XF_RANKS * 64];
@@ -997,6 +1003,7 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
 
         w = wn;
         have = have_next;
+        ++npass;
     }
 }
 
