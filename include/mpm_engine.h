@@ -231,7 +231,9 @@ double mpm_time(const mpm_engine *engine);
 /* Per-kernel timing of the substeps run since the last call with reset=1:
  * names[i] / total_ms[i] / launches[i] for i < *count (at most `capacity`).
  * Timing uses HIP events on the engine's stream and is enabled by
- * mpm_set_profiling. */
+ * mpm_set_profiling: enabled = 0 off, 1 every stage, otherwise a bit mask
+ * with bit i for the i-th name (each timed launch adds two event records to
+ * the stream, which cost a little time of their own). */
 int mpm_set_profiling(mpm_engine *engine, int32_t enabled);
 int mpm_kernel_times(mpm_engine *engine, int32_t reset, int32_t capacity, const char **names,
                      double *total_ms, uint64_t *launches, int32_t *count);

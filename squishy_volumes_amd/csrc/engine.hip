@@ -85,6 +85,7 @@ struct mpm_engine {
     std::vector<uint8_t> dead_at_create;
 
     bool profiling = false;
+    uint32_t profile_mask = 0;  // stages timed while profiling (bit = Stage)
     std::vector<EventRecord> records;
     std::vector<hipEvent_t> event_pool;
     double stage_ms[ST_COUNT] = {0};
@@ -120,20 +121,23 @@ struct mpm_engine {
             event_pool.pop_back();
             return e;
         }
+        // Timing only, read after the stream has synchronized: no system-scope
+        // fence (its cache writeback would slow the kernels it measures).
         hipEvent_t e = nullptr;
-        (void)hipEventCreate(&e);
+        (void)hipEventCreateWithFlags(&e, hipEventDisableSystemFence);
         return e;
     }
 
     void begin(int stage, EventRecord &r) {
-        if (!profiling) return;
+        r.a = r.b = nullptr;
+        if (!profiling || !((profile_mask >> stage) & 1u)) return;
         r.stage = stage;
         r.a = take_event();
         r.b = take_event();
         (void)hipEventRecord(r.a, stream);
     }
     void end(EventRecord &r) {
-        if (!profiling) return;
+        if (!r.a) return;
         (void)hipEventRecord(r.b, stream);
         records.push_back(r);
     }
@@ -710,6 +714,7 @@ int mpm_run_stage(mpm_engine *e, int32_t stage, const mpm_frame_input *input) {
 int mpm_set_profiling(mpm_engine *e, int32_t enabled) {
     if (!e) return MPM_ERR_INVALID_ARGUMENT;
     e->profiling = enabled != 0;
+    e->profile_mask = enabled == 1 ? ((1u << ST_COUNT) - 1u) : (uint32_t)enabled;
     return MPM_OK;
 }
 

@@ -154,8 +154,21 @@ class GpuState:
         fi, keep = frame_input.to_c()
         _check(self._lib, self._lib.mpm_run_stage(self._h, stage, C.byref(fi)))
 
-    def set_profiling(self, enabled: bool):
-        _check(self._lib, self._lib.mpm_set_profiling(self._h, 1 if enabled else 0))
+    STAGES = ("keys", "radix_sort", "block_list", "p2g", "grid_update", "g2p2g", "g2p", "copy_dead")
+
+    def set_profiling(self, enabled: bool, stages=None):
+        """HIP-event timing of every stage, or of the named `stages` only."""
+        mask = 0
+        if enabled:
+            if stages is None:
+                mask = 1
+            else:
+                mask = sum(1 << self.STAGES.index(s) for s in set(stages))
+                if mask == 1:  # the keys stage alone: 1 would mean every stage
+                    mask |= 1 << 31
+        if mask >= 1 << 31:
+            mask -= 1 << 32  # int32 argument
+        _check(self._lib, self._lib.mpm_set_profiling(self._h, mask))
 
     def kernel_times(self, reset: bool = True) -> dict:
         cap = 16

@@ -328,3 +328,27 @@ def test_dense_blocks_take_several_passes_and_overflow_ranks():
     o32, o64 = oracles(sc)
     o64.step_to(T, sc.frame_input)
     compare_states(st.particles, o64.particles_f64(), sc.consts.grid_node_size, "dense cube")
+
+
+def test_profiling_times_the_selected_stages_only():
+    """mpm_set_profiling: 1 times every stage, a mask only the named ones, 0 none."""
+    sc = perturbed_cube()
+    g = engine(sc, {"resort_interval": 2})
+    step = lambda k: g.step_to(scenes.advance_time(g.time, sc.time_step, k), sc.frame_input)
+    g.set_profiling(True)
+    step(4)
+    every = g.kernel_times(reset=True)
+    assert {"radix_sort", "block_list", "p2g", "grid_update", "g2p2g", "g2p"} <= {k for k, v in every.items() if v[1]}
+    assert every["g2p2g"][1] == 3 and every["grid_update"][1] == 4 and every["p2g"][1] == 1
+    g.set_profiling(True, stages=("g2p2g", "grid_update"))
+    step(4)
+    some = {k: v for k, v in g.kernel_times(reset=True).items() if v[1]}
+    assert set(some) == {"g2p2g", "grid_update"} and some["g2p2g"][1] == 3 and some["grid_update"][1] == 4
+    assert all(v[0] > 0 for v in some.values())
+    g.set_profiling(True, stages=("keys",))
+    step(2)
+    assert {k for k, v in g.kernel_times(reset=True).items() if v[1]} <= {"keys"}
+    g.set_profiling(False)
+    step(2)
+    assert not any(v[1] for v in g.kernel_times(reset=True).values())
+    g.close()
