@@ -964,34 +964,33 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
                     a.B.cand[NCAND * bi + tid] = T;
                 }
                 __syncthreads();
-                // Flush the arenas to the grid with global float atomics.
-                for (uint32_t s = tid; s < (uint32_t)NSLOT; s += XF_THREADS) {
+                // Flush the arenas to the grid with global float atomics, one
+                // component per lane: the four lanes of a quad add the four
+                // components of one node, so that each atomic request to
+                // memory (the float atomics are carried out beyond the L2)
+                // carries whole nodes.  Zero components are skipped.
+                for (uint32_t q = tid; q < 4u * (uint32_t)NSLOT; q += XF_THREADS) {
+                    const uint32_t s = q >> 2, comp = q & 3u;
                     const uint32_t lo = s & 63u, hi = s >> 6;
                     const int rx = (int)((lo & 3u) | ((hi & 1u) << 2)),
                               ry = (int)(((lo >> 2) & 3u) | (((hi >> 1) & 1u) << 2)),
                               rz = (int)(((lo >> 4) & 3u) | (((hi >> 2) & 1u) << 2));
-                    float4 g4 = shell[s];
+                    float g = reinterpret_cast<const float *>(shell + s)[comp];
 #pragma unroll
                     for (int w = 0; w < 3; ++w) {
                         const int u = rx - 1 - w;
                         if (u < 0 || u > 3) continue;
-                        const float4 o = planes[w * PSLOT + plane_slot(u, ry, rz)];
-                        g4.x += o.x;
-                        g4.y += o.y;
-                        g4.z += o.z;
-                        g4.w += o.w;
+                        g += reinterpret_cast<const float *>(planes + w * PSLOT + plane_slot(u, ry, rz))[comp];
                     }
-                    if (!(g4.x != 0.f)) continue;
+                    if (!(g != 0.f)) continue;
                     const int x = 4 * (int)b[0] + rx - 1, y = 4 * (int)b[1] + ry - 1, z = 4 * (int)b[2] + rz - 1;
                     if (!node_in_grid(G, x, y, z)) {
                         atomicOr(&ds->status, MPM_STATUS_OUT_OF_GRID | (KID_P2G << 16));
                         continue;
                     }
-                    float *g = reinterpret_cast<float *>(a.B.grid + node_index(G, (uint32_t)x, (uint32_t)y, (uint32_t)z));
-                    unsafeAtomicAdd(g + 0, g4.x);
-                    unsafeAtomicAdd(g + 1, g4.y);
-                    unsafeAtomicAdd(g + 2, g4.z);
-                    unsafeAtomicAdd(g + 3, g4.w);
+                    unsafeAtomicAdd(reinterpret_cast<float *>(a.B.grid + node_index(G, (uint32_t)x, (uint32_t)y,
+                                                                                   (uint32_t)z)) + comp,
+                                    g);
                 }
                 if (tid < 64u) core_count[tid] = 0u;
                 if (tid == 0) nx = 0u;
