@@ -969,28 +969,36 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
                 // components of one node, so that each atomic request to
                 // memory (the float atomics are carried out beyond the L2)
                 // carries whole nodes.  Zero components are skipped.
-                for (uint32_t q = tid; q < 4u * (uint32_t)NSLOT; q += XF_THREADS) {
-                    const uint32_t s = q >> 2, comp = q & 3u;
-                    const uint32_t lo = s & 63u, hi = s >> 6;
-                    const int rx = (int)((lo & 3u) | ((hi & 1u) << 2)),
-                              ry = (int)(((lo >> 2) & 3u) | (((hi >> 1) & 1u) << 2)),
-                              rz = (int)(((lo >> 4) & 3u) | (((hi >> 2) & 1u) << 2));
-                    float g = reinterpret_cast<const float *>(shell + s)[comp];
+                // Thread t takes component t & 3 of region slot 64 k + (t >> 2)
+                // in each of the 8 region sub-blocks k.
+                static_assert(4 * 64 == XF_THREADS, "one quad per slot of a sub-block");
+                {
+                    const uint32_t lo = tid >> 2, comp = tid & 3u;
+                    const int lx = (int)(lo & 3u), ly = (int)((lo >> 2) & 3u), lz = (int)(lo >> 4);
+                    const float *shellf = reinterpret_cast<const float *>(shell) + comp;
+                    const float *planef = reinterpret_cast<const float *>(planes) + comp;
+                    float *gridf = reinterpret_cast<float *>(a.B.grid) + comp;
+#pragma unroll 1
+                    for (int k = 0; k < 8; ++k) {
+                        const int rx = lx | ((k & 1) << 2), ry = ly | ((k & 2) << 1), rz = lz | (k & 4);
+                        // All four reads are issued together (a plane that does
+                        // not reach the slot is read at a valid slot and dropped).
+                        float g = shellf[4 * (64 * k + (int)lo)];
 #pragma unroll
-                    for (int w = 0; w < 3; ++w) {
-                        const int u = rx - 1 - w;
-                        if (u < 0 || u > 3) continue;
-                        g += reinterpret_cast<const float *>(planes + w * PSLOT + plane_slot(u, ry, rz))[comp];
+                        for (int w = 0; w < 3; ++w) {
+                            const int u = rx - 1 - w;
+                            const bool reach = u >= 0 && u <= 3;
+                            const float o = planef[4 * (w * PSLOT + plane_slot(reach ? u : 0, ry, rz))];
+                            g += reach ? o : 0.f;
+                        }
+                        if (!(g != 0.f)) continue;
+                        const int x = 4 * (int)b[0] + rx - 1, y = 4 * (int)b[1] + ry - 1, z = 4 * (int)b[2] + rz - 1;
+                        if (!node_in_grid(G, x, y, z)) {
+                            atomicOr(&ds->status, MPM_STATUS_OUT_OF_GRID | (KID_P2G << 16));
+                            continue;
+                        }
+                        unsafeAtomicAdd(gridf + 4u * node_index(G, (uint32_t)x, (uint32_t)y, (uint32_t)z), g);
                     }
-                    if (!(g != 0.f)) continue;
-                    const int x = 4 * (int)b[0] + rx - 1, y = 4 * (int)b[1] + ry - 1, z = 4 * (int)b[2] + rz - 1;
-                    if (!node_in_grid(G, x, y, z)) {
-                        atomicOr(&ds->status, MPM_STATUS_OUT_OF_GRID | (KID_P2G << 16));
-                        continue;
-                    }
-                    unsafeAtomicAdd(reinterpret_cast<float *>(a.B.grid + node_index(G, (uint32_t)x, (uint32_t)y,
-                                                                                   (uint32_t)z)) + comp,
-                                    g);
                 }
                 if (tid < 64u) core_count[tid] = 0u;
                 if (tid == 0) nx = 0u;
