@@ -88,7 +88,9 @@ typedef enum mpm_result {
 enum {
     MPM_OPT_GRID_GRAVITY = 1u << 0,  /* add dt*g on the grid instead of on particles */
     MPM_OPT_STICKY_PLANE = 1u << 1,  /* nodes with z < plane_z get v = 0 */
-    MPM_OPT_BOX_WALLS = 1u << 2      /* separating slip on the faces of [wall_min, wall_max] */
+    MPM_OPT_BOX_WALLS = 1u << 2,     /* separating slip on the faces of [wall_min, wall_max] */
+    MPM_OPT_SPHERE = 1u << 3         /* separating slip on a sphere: nodes inside lose their
+                                        velocity component into it */
 };
 
 /* Per-simulation constants: InputConsts (rust/crates/file_input/src/header.rs:9-69)
@@ -111,6 +113,8 @@ typedef struct mpm_consts {
     float snow_stretch;
     float snow_hardening;
     int32_t device;              /* HIP device ordinal; GpuState's adapter name */
+    float sphere_center[3];      /* MPM_OPT_SPHERE */
+    float sphere_radius;
 } mpm_consts;
 
 /* Per-particle parameters, the flattened form of ParticleParameters
@@ -276,6 +280,53 @@ int mpm_dev_stress(const uint32_t *flags, const mpm_particle_parameters *params,
 
 /* 3x3 signed SVD F = U diag(s) V^T with U, V rotations (n matrices, column-major). */
 int mpm_dev_svd(const float *F, float *U, float *s, float *V, uint64_t n, void *stream);
+
+/* ---- Learned constitutive closure (no counterpart in the reference, which
+ * has no learned stress model; it stands in for the analytic snow stress).
+ * A 6-64-64-3 perceptron with ReLU hidden layers, evaluated with BF16
+ * matrix-core products and FP32 accumulation.  Its FP32 parameters are one
+ * device array of mpm_mlp_param_count() floats, padded:
+ *   W1 [64][16] (columns 6..15 zero), b1 [64], W2 [64][64], b2 [64],
+ *   W3 [16][64] (rows 3..15 zero), b3 [16];  layer k: z = W h + b.
+ * Samples are row-major: x [n][6], y and targets [n][3].  Weights, inputs,
+ * hidden activations and back-propagated errors are rounded to BF16 (nearest
+ * even) where they enter a product; biases, sums and outputs are FP32. */
+int32_t mpm_mlp_param_count(void);
+int mpm_mlp_forward(const float *params, const float *x, float *y, uint64_t n, void *stream);
+/* Adds sum((y - t)^2) over the n samples to *loss and scale times its
+ * gradient to grad (scale = 1/N gives the gradient of the mean over N). */
+int mpm_mlp_grad(const float *params, const float *x, const float *targets, uint64_t n, float scale,
+                 float *grad, float *loss, void *stream);
+/* One Adam step (bias-corrected, step >= 1) on params; clears grad. */
+int mpm_mlp_adam(float *params, float *grad, float *m, float *v, float lr, float beta1, float beta2,
+                 float eps, int32_t step, void *stream);
+
+/* The learned closure in the substep: with it set, the stress of every snow
+ * particle in each substep's scatter comes from the network, evaluated on
+ * the particle's sorted singular values s (SVD of F), E, nu and Jp.  Network
+ * inputs: (s_q - 1) / 0.025, log10(E / 1e5), (nu - 0.3) / 0.1, (Jp - 1) / 0.1;
+ * outputs: principal Kirchhoff stresses in units of 0.025 E exp(xi (1 - Jp))
+ * (xi = snow_hardening), composed as tau = U diag(out) U^T.  A particle whose
+ * inputs leave [hull_lo, hull_hi] (the training set's range) takes the
+ * analytic stress; every judge_stride-th particle (by id, rotating each
+ * substep) takes both and the difference goes into the statistics.  A
+ * substep then runs as a gather writing the state, the sort when due, the
+ * closure, a scatter from the state and the grid update.  No counterpart in
+ * the reference.  params is a host pointer (copied); NULL turns it off. */
+typedef struct mpm_closure {
+    const float *params;             /* mpm_mlp_param_count() floats */
+    float hull_lo[6], hull_hi[6];
+    int32_t judge_stride;            /* 0: no judge */
+} mpm_closure;
+typedef struct mpm_closure_stats {
+    uint64_t evaluated;              /* network stresses used */
+    uint64_t fallbacks;              /* analytic stresses outside the hull */
+    uint64_t judged;                 /* particles checked against the analytic stress */
+    double err2, ref2;               /* sum |tau - tau_analytic|^2, sum |tau_analytic|^2 (judged) */
+} mpm_closure_stats;
+int mpm_set_closure(mpm_engine *engine, const mpm_closure *closure);
+/* Statistics since the last reset. */
+int mpm_read_closure_stats(mpm_engine *engine, int32_t reset, mpm_closure_stats *out);
 
 /* One substep stage at a time on an engine, for the per-kernel parity tests:
  * stage 0 = sort, 1 = P2G from the stored state (stress from F and C),

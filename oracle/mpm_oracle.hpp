@@ -580,7 +580,9 @@ struct State {
         }
     }
 
-    // Extension (no reference counterpart): grid gravity, sticky plane, box walls.
+    // Extension (no reference counterpart): grid gravity, sticky plane, box
+    // walls, sphere (separating slip: a node inside loses its velocity
+    // component along the inward normal).
     void apply_grid_extensions(const int id[3], V3<R> &v, R hh, R t, bool grid_gravity) const {
         if (grid_gravity) v += gravity * t;
         V3<R> xi(R(id[0]) * hh, R(id[1]) * hh, R(id[2]) * hh);
@@ -589,6 +591,16 @@ struct State {
             for (int a = 0; a < 3; ++a) {
                 if (xi[a] < R(consts.wall_min[a]) && v[a] < 0) v[a] = 0;
                 if (xi[a] > R(consts.wall_max[a]) && v[a] > 0) v[a] = 0;
+            }
+        }
+        if (consts.options & MPM_OPT_SPHERE) {
+            const V3<R> d(xi.x - R(consts.sphere_center[0]), xi.y - R(consts.sphere_center[1]),
+                          xi.z - R(consts.sphere_center[2]));
+            const R r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+            const R rad = R(consts.sphere_radius);
+            if (r2 < rad * rad && r2 > 0) {
+                const R vn = (v.x * d.x + v.y * d.y + v.z * d.z) / r2;
+                if (vn < 0) v = v - d * vn;
             }
         }
     }

@@ -36,6 +36,7 @@ STATUS_SVD_FP64 = 64
 OPT_GRID_GRAVITY = 1 << 0
 OPT_STICKY_PLANE = 1 << 1
 OPT_BOX_WALLS = 1 << 2
+OPT_SPHERE = 1 << 3
 
 OK = 0
 ERR_ENERGY = 1
@@ -82,6 +83,8 @@ class MpmConsts(C.Structure):
         ("snow_stretch", C.c_float),
         ("snow_hardening", C.c_float),
         ("device", C.c_int32),
+        ("sphere_center", C.c_float * 3),
+        ("sphere_radius", C.c_float),
     ]
 
 
@@ -153,6 +156,25 @@ class MpmTuning(C.Structure):
     ]
 
 
+class MpmClosure(C.Structure):
+    _fields_ = [
+        ("params", C.c_void_p),
+        ("hull_lo", C.c_float * 6),
+        ("hull_hi", C.c_float * 6),
+        ("judge_stride", C.c_int32),
+    ]
+
+
+class MpmClosureStats(C.Structure):
+    _fields_ = [
+        ("evaluated", C.c_uint64),
+        ("fallbacks", C.c_uint64),
+        ("judged", C.c_uint64),
+        ("err2", C.c_double),
+        ("ref2", C.c_double),
+    ]
+
+
 # Every exported symbol, with its ctypes signature.
 SIGNATURES = {
     "mpm_create": (C.c_int, [C.POINTER(MpmConsts), C.POINTER(MpmParticlesView), C.POINTER(MpmTuning), C.POINTER(C.c_void_p)]),
@@ -182,6 +204,18 @@ SIGNATURES = {
         [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p],
     ),
     "mpm_dev_svd": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "mpm_mlp_param_count": (C.c_int32, []),
+    "mpm_mlp_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "mpm_mlp_grad": (
+        C.c_int,
+        [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p],
+    ),
+    "mpm_mlp_adam": (
+        C.c_int,
+        [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int32, C.c_void_p],
+    ),
+    "mpm_set_closure": (C.c_int, [C.c_void_p, C.POINTER(MpmClosure)]),
+    "mpm_read_closure_stats": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(MpmClosureStats)]),
     "mpm_run_stage": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(MpmFrameInput)]),
     "mpm_read_grid_momentum": (C.c_int, [C.c_void_p, C.POINTER(MpmGridView)]),
 }

@@ -21,9 +21,9 @@ using namespace svamd;
 
 namespace {
 
-enum Stage { ST_KEYS = 0, ST_SORT, ST_BLOCKS, ST_P2G, ST_GRID, ST_G2P2G, ST_G2P, ST_COPY, ST_COUNT };
-const char *kStageNames[ST_COUNT] = {"keys",        "radix_sort", "block_list", "p2g",
-                                     "grid_update", "g2p2g",      "g2p",        "copy_dead"};
+enum Stage { ST_KEYS = 0, ST_SORT, ST_BLOCKS, ST_P2G, ST_GRID, ST_G2P2G, ST_G2P, ST_COPY, ST_CLOSURE, ST_COUNT };
+const char *kStageNames[ST_COUNT] = {"keys",        "radix_sort", "block_list", "p2g",    "grid_update",
+                                     "g2p2g",       "g2p",        "copy_dead",  "closure"};
 
 thread_local std::string g_last_error;
 
@@ -83,6 +83,15 @@ struct mpm_engine {
     std::vector<float> initial_positions;
     std::vector<uint32_t> collider_bits;
     std::vector<uint8_t> dead_at_create;
+
+    // Learned closure (mpm_set_closure): the snow stress of each substep's
+    // scatter comes from the network.
+    bool closure_on = false;
+    float *closure_params = nullptr;  // MLP_NPARAM
+    float *closure_tau = nullptr;     // 9 rows of n
+    ClosureStats *closure_stats = nullptr;
+    float hull_lo[MLP_IN] = {0}, hull_hi[MLP_IN] = {0};
+    uint32_t judge_stride = 0, judge_phase = 0;
 
     bool profiling = false;
     uint32_t profile_mask = 0;  // stages timed while profiling (bit = Stage)
@@ -208,7 +217,9 @@ SubstepParams make_params(const mpm_engine *e, const float gravity[3], float fac
         S.wall_max[a] = c.wall_max[a];
         S.domain_min[a] = c.domain_min[a];
         S.domain_max[a] = c.domain_max[a];
+        S.sphere_center[a] = c.sphere_center[a];
     }
+    S.sphere_radius = c.sphere_radius;
     S.factor_b = factor_b;
     S.has_goals = goals ? 1 : 0;
     S.has_b = has_b ? 1 : 0;
@@ -317,6 +328,10 @@ int run_transfer(mpm_engine *e, TransferKind kind, const SubstepParams &S1, bool
     t.B = grid_buffers(e, S1.epoch);
     t.keys_out = want_keys ? e->keys : nullptr;
     t.ds = e->ds;
+    if (kind == TransferKind::P2G && e->closure_on) {
+        t.tau_ext = e->closure_tau;
+        t.tau_stride = e->n;
+    }
     const int stage = kind == TransferKind::P2G ? ST_P2G : (kind == TransferKind::G2P2G ? ST_G2P2G : ST_G2P);
     e->begin(stage, r);
     HIP_TRY(launch_transfer(e->launch, t, e->stream));
@@ -343,6 +358,33 @@ int run_grid(mpm_engine *e, const SubstepParams &S) {
     e->grid_epoch = S.epoch;
     e->grid_valid = true;
     e->epoch += 1;
+    return MPM_OK;
+}
+
+// The learned stress of the snow particles for the scatter that follows
+// (which permutes when a sort is pending, so slot i reads order[i] then).
+int run_closure(mpm_engine *e) {
+    EventRecord r{};
+    ClosureLaunch c{};
+    c.P = e->P[e->cur];
+    c.order = e->pending_permute ? e->order : nullptr;
+    c.X = e->X;
+    c.params = e->closure_params;
+    c.tau = e->closure_tau;
+    c.stride = e->n;
+    c.n = e->n;
+    for (int q = 0; q < MLP_IN; ++q) {
+        c.hull_lo[q] = e->hull_lo[q];
+        c.hull_hi[q] = e->hull_hi[q];
+    }
+    c.hardening = e->consts.snow_hardening;
+    c.judge_stride = e->judge_stride;
+    c.judge_phase = e->judge_phase;
+    c.stats = e->closure_stats;
+    e->begin(ST_CLOSURE, r);
+    HIP_TRY(launch_closure(c, e->stream));
+    e->end(r);
+    if (e->judge_stride) e->judge_phase = (e->judge_phase + 1) % e->judge_stride;
     return MPM_OK;
 }
 
@@ -559,7 +601,30 @@ int mpm_step_to(mpm_engine *e, double target_time, const mpm_frame_input *input,
     auto sorts_before = [&](size_t t) { return t == 0 || (t < k && t % K == 0); };
     uint64_t done = 0;
     int result = MPM_OK;
-    for (size_t t = 0; k > 0 && t <= k; ++t) {
+    // With the learned closure, substep t is a gather (of substep t - 1, it
+    // writes the state), the sort when due, the closure, a scatter from the
+    // state and the grid update.
+    for (size_t t = 0; e->closure_on && k > 0 && t <= k; ++t) {
+        if (t > 0 && cancel && *cancel) {
+            result = MPM_ERR_CANCELED;
+            break;
+        }
+        if (t > 0) {
+            SubstepParams S0 = params[t - 1];
+            S0.epoch = e->epoch;
+            if ((rc = run_transfer(e, TransferKind::G2P, S0, t == k || sorts_before(t))) != MPM_OK) return rc;
+            e->time = times[t - 1] + (double)e->consts.time_step;
+            ++done;
+        }
+        if (t == k) break;
+        if (sorts_before(t) && (rc = run_sort(e)) != MPM_OK) return rc;
+        SubstepParams S1 = params[t];
+        S1.epoch = e->epoch;
+        if ((rc = run_closure(e)) != MPM_OK) return rc;
+        if ((rc = run_transfer(e, TransferKind::P2G, S1, false)) != MPM_OK) return rc;
+        if ((rc = run_grid(e, S1)) != MPM_OK) return rc;
+    }
+    for (size_t t = 0; !e->closure_on && k > 0 && t <= k; ++t) {
         if (t > 0 && cancel && *cancel) {
             result = MPM_ERR_CANCELED;
             break;
@@ -774,6 +839,74 @@ int mpm_dev_stress(const uint32_t *flags, const mpm_particle_parameters *params,
 int mpm_dev_svd(const float *F, float *U, float *s, float *V, uint64_t n, void *stream) {
     if (n > 0 && (!F || !U || !s || !V)) return MPM_ERR_INVALID_ARGUMENT;
     HIP_TRY(launch_svd(F, U, s, V, n, static_cast<hipStream_t>(stream)));
+    return MPM_OK;
+}
+
+int32_t mpm_mlp_param_count(void) { return MLP_NPARAM; }
+
+int mpm_set_closure(mpm_engine *e, const mpm_closure *c) {
+    if (!e) return MPM_ERR_INVALID_ARGUMENT;
+    if (!c) {
+        e->closure_on = false;
+        return MPM_OK;
+    }
+    if (!c->params || c->judge_stride < 0) return MPM_ERR_INVALID_ARGUMENT;
+    for (int q = 0; q < MLP_IN; ++q)
+        if (!(c->hull_lo[q] <= c->hull_hi[q])) return MPM_ERR_INVALID_ARGUMENT;
+    HIP_TRY(hipSetDevice(e->device));
+    if (!e->closure_params) {
+        if (e->alloc(e->closure_params, MLP_NPARAM) != hipSuccess ||
+            e->alloc(e->closure_tau, 9 * std::max<uint64_t>(e->n, 1)) != hipSuccess ||
+            e->alloc(e->closure_stats, 1) != hipSuccess)
+            return MPM_ERR_OUT_OF_MEMORY;
+        HIP_TRY(hipMemsetAsync(e->closure_stats, 0, sizeof(ClosureStats), e->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(e->closure_params, c->params, MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    for (int q = 0; q < MLP_IN; ++q) {
+        e->hull_lo[q] = c->hull_lo[q];
+        e->hull_hi[q] = c->hull_hi[q];
+    }
+    e->judge_stride = (uint32_t)c->judge_stride;
+    e->judge_phase = 0;
+    e->closure_on = true;
+    return MPM_OK;
+}
+
+int mpm_read_closure_stats(mpm_engine *e, int32_t reset, mpm_closure_stats *out) {
+    if (!e || !out) return MPM_ERR_INVALID_ARGUMENT;
+    std::memset(out, 0, sizeof(*out));
+    if (!e->closure_stats) return MPM_OK;
+    HIP_TRY(hipSetDevice(e->device));
+    ClosureStats h{};
+    HIP_TRY(hipMemcpyAsync(&h, e->closure_stats, sizeof(h), hipMemcpyDeviceToHost, e->stream));
+    if (reset) HIP_TRY(hipMemsetAsync(e->closure_stats, 0, sizeof(ClosureStats), e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    out->evaluated = h.evaluated;
+    out->fallbacks = h.fallbacks;
+    out->judged = h.judged;
+    out->err2 = h.err2;
+    out->ref2 = h.ref2;
+    return MPM_OK;
+}
+
+int mpm_mlp_forward(const float *params, const float *x, float *y, uint64_t n, void *stream) {
+    if (!params || (n > 0 && (!x || !y))) return MPM_ERR_INVALID_ARGUMENT;
+    HIP_TRY(launch_mlp_forward(params, x, y, n, static_cast<hipStream_t>(stream)));
+    return MPM_OK;
+}
+
+int mpm_mlp_grad(const float *params, const float *x, const float *targets, uint64_t n, float scale, float *grad,
+                 float *loss, void *stream) {
+    if (!params || !grad || !loss || (n > 0 && (!x || !targets))) return MPM_ERR_INVALID_ARGUMENT;
+    HIP_TRY(launch_mlp_grad(params, x, targets, n, scale, grad, loss, static_cast<hipStream_t>(stream)));
+    return MPM_OK;
+}
+
+int mpm_mlp_adam(float *params, float *grad, float *m, float *v, float lr, float beta1, float beta2, float eps,
+                 int32_t step, void *stream) {
+    if (!params || !grad || !m || !v || step < 1) return MPM_ERR_INVALID_ARGUMENT;
+    HIP_TRY(launch_mlp_adam(params, grad, m, v, lr, beta1, beta2, eps, step, static_cast<hipStream_t>(stream)));
     return MPM_OK;
 }
 

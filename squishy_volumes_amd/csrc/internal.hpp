@@ -125,6 +125,7 @@ struct SubstepParams {
     float domain_min[3];
     float domain_max[3];
     float snow_lo, snow_hi, snow_hardening;
+    float sphere_center[3], sphere_radius;
     uint32_t epoch;    // substep counter, stamps grid blocks
 };
 
@@ -190,6 +191,8 @@ struct TransferCall {
     GridBuffers B;              // B.ovf: the list for S1's grid (parity S1.epoch & 1)
     uint32_t *keys_out;         // null: keys not needed
     DeviceScalars *ds;
+    const float *tau_ext = nullptr;  // P2G: snow stress from the learned closure (rows of tau_stride)
+    uint64_t tau_stride = 0;
 };
 hipError_t launch_transfer(const TransferLaunch &L, const TransferCall &t, hipStream_t stream);
 uint32_t grid_update_waves(const TransferLaunch &L);
@@ -205,5 +208,46 @@ hipError_t launch_stress(const uint32_t *flags, const mpm_particle_parameters *p
                          const float *jp, float snow_hardening, float *tau, float *energy, uint64_t n,
                          hipStream_t stream);
 hipError_t launch_svd(const float *F, float *U, float *s, float *V, uint64_t n, hipStream_t stream);
+
+// ---- closure.hip: the learned stress model, a 6-64-64-3 perceptron.  Its
+// FP32 parameters are one padded array: W1 [64][16] (input columns 6..15
+// zero), b1 [64], W2 [64][64], b2 [64], W3 [16][64] (output rows 3..15
+// zero), b3 [16]; W[out][in], so layer k computes z = W h + b.
+constexpr int MLP_IN = 6, MLP_H = 64, MLP_OUT = 3, MLP_KIN = 16;
+constexpr int MLP_OFF_W1 = 0;
+constexpr int MLP_OFF_B1 = MLP_OFF_W1 + MLP_H * MLP_KIN;
+constexpr int MLP_OFF_W2 = MLP_OFF_B1 + MLP_H;
+constexpr int MLP_OFF_B2 = MLP_OFF_W2 + MLP_H * MLP_H;
+constexpr int MLP_OFF_W3 = MLP_OFF_B2 + MLP_H;
+constexpr int MLP_OFF_B3 = MLP_OFF_W3 + 16 * MLP_H;
+constexpr int MLP_NPARAM = MLP_OFF_B3 + 16;
+hipError_t launch_mlp_forward(const float *P, const float *x, float *y, uint64_t n, hipStream_t stream);
+hipError_t launch_mlp_grad(const float *P, const float *x, const float *t, uint64_t n, float scale, float *grad,
+                           float *loss, hipStream_t stream);
+hipError_t launch_mlp_adam(float *P, float *G, float *M, float *V, float lr, float b1, float b2, float eps, int step,
+                           hipStream_t stream);
+
+// Network inputs of a snow particle: (s_q - 1) / SIGMA for its singular
+// values s_0 >= s_1 >= s_2, log10(E / 1e5), (nu - 0.3) / 0.1 and
+// (Jp - 1) / JP; outputs: principal Kirchhoff stresses in units of
+// E exp(xi (1 - Jp)) STRESS (the hardened Young's modulus).
+constexpr float CLOSURE_SIGMA_SCALE = 0.025f, CLOSURE_JP_SCALE = 0.1f, CLOSURE_STRESS_SCALE = 0.025f;
+struct ClosureStats {
+    unsigned long long evaluated, fallbacks, judged, pad;
+    double err2, ref2;  // sum |tau - tau_analytic|^2 and sum |tau_analytic|^2 over the kept particles
+};
+struct ClosureLaunch {
+    ParticleArrays P;
+    const uint32_t *order;  // slot i reads P at order[i], or null
+    AuxArrays X;
+    const float *params;
+    float *tau;             // row k of slot i at tau[k * stride + i]
+    uint64_t stride, n;
+    float hull_lo[MLP_IN], hull_hi[MLP_IN];
+    float hardening;
+    uint32_t judge_stride, judge_phase;
+    ClosureStats *stats;
+};
+hipError_t launch_closure(const ClosureLaunch &c, hipStream_t stream);
 
 }  // namespace svamd

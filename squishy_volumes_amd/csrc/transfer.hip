@@ -171,15 +171,20 @@ __device__ __forceinline__ void fused_affine(float m, float vol, float scaling, 
 }
 
 // Stress inputs of a particle whose state holds F and C (first substep of a
-// call): polar factor / SVD, close-to-inverted check, A.
+// call): polar factor / SVD, close-to-inverted check, A.  With tau_ext, a
+// snow particle's Kirchhoff stress is read from it (row k at tau_ext[k *
+// stride + slot]) instead of being evaluated.
 template <uint32_t SET>
 __device__ __forceinline__ void affine_from_state(uint32_t flags, const float F[9], const float C[9], float m, float vol,
                                                   float p0, float p1, uint32_t id, const AuxArrays &X, float scaling,
-                                                  float hardening, DeviceScalars *ds, float A[9]) {
+                                                  float hardening, DeviceScalars *ds, const float *tau_ext,
+                                                  uint64_t stride, uint64_t slot, float A[9]) {
     float J = det3(F);
     float R[9], U[9], s[3] = {1.f, 1.f, 1.f};
     float hard = 1.0f;
-    if ((SET & M_SNOW) && (flags & F_SNOW)) {
+    const bool ext = (SET & M_SNOW) && (flags & F_SNOW) && tau_ext;
+    if (ext) {
+    } else if ((SET & M_SNOW) && (flags & F_SNOW)) {
         float V[9];
         if (svd3_call(F, U, s, V)) atomicOr(&ds->status, MPM_STATUS_SVD_FP64);
         hard = expf(hardening * (1.0f - X.jp[id]));
@@ -193,7 +198,12 @@ __device__ __forceinline__ void affine_from_state(uint32_t flags, const float F[
     if (!(J >= 1e-7f) && !(flags & F_FLUID))
         atomicOr(&ds->status, MPM_STATUS_PARTICLE_CLOSE_TO_INVERTED | (KID_P2G << 16));
     float tau[9];
-    kirchhoff<SET>(flags, p0, p1, hard, F, J, R, U, s, tau);
+    if (ext) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) tau[k] = tau_ext[(uint64_t)k * stride + slot];
+    } else {
+        kirchhoff<SET>(flags, p0, p1, hard, F, J, R, U, s, tau);
+    }
     const bool visc = flags & F_VISC;
     float vd = 0.f, vb = 0.f;
     if ((SET & M_VISC) && visc) {
@@ -294,6 +304,8 @@ struct XferArgs {
     GridBuffers B;               // B.ovf: the overflow list for S1's grid
     uint32_t *keys_out;          // block keys of the new positions, or null
     DeviceScalars *ds;
+    const float *tau_ext;        // P2G mode: Kirchhoff stress of snow particles by slot (9 rows
+    uint64_t tau_stride;         // of tau_stride floats), from the learned closure; or null
 };
 
 // Scatter of one particle straight into the global accumulation grid, for a
@@ -785,7 +797,8 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
                     for (int q = 0; q < 3; ++q) v[q] = Pin.v(q, src);
 #pragma unroll
                     for (int k = 0; k < 9; ++k) C[k] = Pin.A(k, src);
-                    affine_from_state<SET>(flags, F, C, m, vol, p0, p1, id, a.X, scaling, a.S1.snow_hardening, ds, A);
+                    affine_from_state<SET>(flags, F, C, m, vol, p0, p1, id, a.X, scaling, a.S1.snow_hardening, ds,
+                                           a.tau_ext, a.tau_stride, i, A);
                     if (PERMUTE) {
                         for (int f = 0; f < NFIELDS; ++f) Pout.at(i)[f * TILE] = Pin.at(src)[f * TILE];
                     }
@@ -1057,7 +1070,7 @@ __global__ __launch_bounds__(256) void k_grid_update(GridBuffers B, GridGeom G, 
 #pragma unroll
                     for (int q = 0; q < 3; ++q) v[q] += S.gravity[q] * S.dt;
                 }
-                if (S.options & (MPM_OPT_STICKY_PLANE | MPM_OPT_BOX_WALLS)) {
+                if (S.options & (MPM_OPT_STICKY_PLANE | MPM_OPT_BOX_WALLS | MPM_OPT_SPHERE)) {
                     const int lx = 4 * bx + (int)x, ly = 4 * by + (int)y, lz = 4 * bz + (int)z;
                     const float xi[3] = {(float)(lx + G.origin[0]) * S.h, (float)(ly + G.origin[1]) * S.h,
                                          (float)(lz + G.origin[2]) * S.h};
@@ -1067,6 +1080,18 @@ __global__ __launch_bounds__(256) void k_grid_update(GridBuffers B, GridGeom G, 
                         for (int q = 0; q < 3; ++q) {
                             if (xi[q] < S.wall_min[q] && v[q] < 0.f) v[q] = 0.f;
                             if (xi[q] > S.wall_max[q] && v[q] > 0.f) v[q] = 0.f;
+                        }
+                    }
+                    if (S.options & MPM_OPT_SPHERE) {
+                        const float d[3] = {xi[0] - S.sphere_center[0], xi[1] - S.sphere_center[1],
+                                            xi[2] - S.sphere_center[2]};
+                        const float r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                        if (r2 < S.sphere_radius * S.sphere_radius && r2 > 0.f) {
+                            const float vn = (v[0] * d[0] + v[1] * d[1] + v[2] * d[2]) / r2;
+                            if (vn < 0.f) {
+#pragma unroll
+                                for (int q = 0; q < 3; ++q) v[q] -= vn * d[q];
+                            }
                         }
                     }
                 }
@@ -1243,6 +1268,8 @@ hipError_t launch_transfer(const TransferLaunch &L, const TransferCall &t, hipSt
     a.B = t.B;
     a.keys_out = t.keys_out;
     a.ds = t.ds;
+    a.tau_ext = t.tau_ext;
+    a.tau_stride = t.tau_stride;
     const bool goals = t.S1.has_goals != 0;
     switch (t.kind) {
         case TransferKind::P2G: return dispatch_set<XM_P2G>(L, t.permute, goals, a, stream);

@@ -182,3 +182,27 @@ class GpuState:
     @property
     def stream(self) -> int:
         return int(self._lib.mpm_stream(self._h) or 0)
+
+    def set_closure(self, model, judge_stride: int = 0):
+        """Learned snow stress (a closure.ClosureModel) in every substep, or the
+        analytic stress again with model=None (mpm_set_closure)."""
+        if model is None:
+            _check(self._lib, self._lib.mpm_set_closure(self._h, None))
+            return
+        params = np.ascontiguousarray(model.params, dtype=np.float32)
+        if params.size != int(self._lib.mpm_mlp_param_count()):
+            raise ValueError("closure parameters have the wrong length")
+        c = abi.MpmClosure()
+        c.params = params.ctypes.data
+        c.hull_lo[:] = [float(v) for v in model.hull_lo]
+        c.hull_hi[:] = [float(v) for v in model.hull_hi]
+        c.judge_stride = int(judge_stride)
+        _check(self._lib, self._lib.mpm_set_closure(self._h, C.byref(c)))
+
+    def closure_stats(self, reset: bool = True) -> dict:
+        """Network / fallback / judged counts and the judged relative stress error."""
+        st = abi.MpmClosureStats()
+        _check(self._lib, self._lib.mpm_read_closure_stats(self._h, 1 if reset else 0, C.byref(st)))
+        rel = float(np.sqrt(st.err2 / st.ref2)) if st.ref2 > 0 else float("nan")
+        return {"evaluated": int(st.evaluated), "fallbacks": int(st.fallbacks), "judged": int(st.judged),
+                "relative_stress_error": rel}

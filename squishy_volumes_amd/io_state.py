@@ -144,6 +144,7 @@ class EngineOptions:
     grid_gravity: bool = False
     sticky_plane_z: Optional[float] = None
     box_walls: Optional[tuple] = None  # ((xmin, ymin, zmin), (xmax, ymax, zmax))
+    sphere: Optional[tuple] = None  # ((cx, cy, cz), radius): separating slip on the sphere
     snow_compression: float = 2.5e-2
     snow_stretch: float = 4.5e-3
     snow_hardening: float = 10.0
@@ -167,6 +168,10 @@ def make_consts(consts: InputConsts, time_step: float, options: EngineOptions | 
         opt |= abi.OPT_BOX_WALLS
         c.wall_min[:] = options.box_walls[0]
         c.wall_max[:] = options.box_walls[1]
+    if options.sphere is not None:
+        opt |= abi.OPT_SPHERE
+        c.sphere_center[:] = options.sphere[0]
+        c.sphere_radius = options.sphere[1]
     c.options = opt
     c.snow_compression = options.snow_compression
     c.snow_stretch = options.snow_stretch

@@ -172,3 +172,34 @@ def spheres_and_slab(*, h: float = 1.0 / 256, target_particles: int = 4_200_000,
         substeps=substeps,
         info={"particles": allp.count, "sphere_particles": n_spheres, "slab_layers": layers, "E": Es.tolist(), "h": h},
     )
+
+
+def snow_block_on_sphere(*, h: float = 1.0 / 128, target_particles: int = 1_000_000, seed: int = 0,
+                         dt: float = 1e-4, substeps: int = 1000, fps: int = 1, E: float = 1.4e5, nu: float = 0.2,
+                         rho: float = 400.0) -> Scene:
+    """A snow block (fixed-corotated elasticity with the singular values of F
+    clamped to [1 - 2.5e-2, 1 + 4.5e-3] and hardening 10) falling at 3 m/s
+    onto a sphere collider of radius 0.15, in a unit box with separating-slip
+    walls, 8 particles per cell."""
+    rng = np.random.default_rng(seed)
+    side = int(round((target_particles / 8) ** (1.0 / 3.0)))
+    lo_xy = int(round(0.5 / h)) - side // 2
+    z0 = int(round(0.42 / h))
+    cells = box_cells(np.array([lo_xy, lo_xy, z0]), np.array([lo_xy + side, lo_xy + side, z0 + side]))
+    pts = jittered_cells(cells, h, rng)
+    p = particles_from(pts, abi.IS_SOLID | abi.SNOW, rho, h, (0.0, 0.0, -3.0), E=E, nu=nu, with_jp=True)
+    p.plastic_jp[:] = 1.0
+    wall = 4 * h
+    consts = InputConsts(grid_node_size=h, frames_per_second=fps, domain_min=(0.0, 0.0, 0.0), domain_max=(1.0, 1.0, 1.0))
+    opts = EngineOptions(box_walls=((wall, wall, wall), (1.0 - wall, 1.0 - wall, 1.0 - wall)),
+                         sphere=((0.5, 0.5, 0.22), 0.15))
+    return Scene(
+        name="snow_block_on_sphere",
+        consts=consts,
+        time_step=dt,
+        options=opts,
+        io_state=IoState(time=0.0, particles=p),
+        frame_input=FrameInput(frame=0, gravity_a=(0.0, 0.0, -9.81)),
+        substeps=substeps,
+        info={"particles": p.count, "block_cells": side, "E": E, "nu": nu, "h": h},
+    )
