@@ -96,6 +96,110 @@ def traffic_for(kernel: str, workload: str):
     return None
 
 
+def learned_form(args) -> dict:
+    """The learned closure on the 1M-particle snow block impacting a sphere:
+    pairs from the plain (analytic) run, training on the GPU, then the same
+    window of substeps timed with the analytic and with the learned stress."""
+    import numpy as np
+    import torch
+
+    from squishy_volumes_amd import closure, scenes
+    from squishy_volumes_amd.gpu_state import GpuState
+
+    sc = scenes.snow_block_on_sphere()
+    n = sc.io_state.particles.count
+    hard = sc.options.snow_hardening
+    adv = lambda g, k: g.step_to(scenes.advance_time(g.time, sc.time_step, k), sc.frame_input)
+    tuning = {"resort_interval": args.resort_interval} if args.resort_interval else {}
+
+    # Pairs from the plain run: every `every` substeps over `gen` substeps,
+    # about args.pairs in total.
+    gen, every = args.closure_substeps, 50
+    plain = GpuState.from_io_state(0, sc.consts, sc.time_step, sc.io_state, sc.options, tuning)
+    rng = np.random.default_rng(0)
+    chunks = []
+    per = max(1, args.pairs // (gen // every))
+    for _ in range(gen // every):
+        adv(plain, every)
+        p = closure.pairs_from_particles(plain.to_io_state().particles, hard)
+        chunks.append(p.subset(rng.choice(len(p), size=min(per, len(p)), replace=False)))
+    data = closure.Pairs.concat(chunks)
+    log(f"closure: {len(data)} pairs from {gen} plain substeps")
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    net = closure.Mlp(seed=0)
+    x = torch.from_numpy(data.x).cuda()
+    t = torch.from_numpy(data.t).cuda()
+    tlog = net.fit(x, t, epochs=args.epochs, batch=8192, lr=3e-3, lr_final=3e-5)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    model = net.model(data.x)
+    log(f"closure: trained {args.epochs} epochs in {train_s:.1f} s, mse {tlog.epochs[0]:.3e} -> {tlog.epochs[-1]:.3e}")
+
+    def timed(learned: bool):
+        g = GpuState.from_io_state(0, sc.consts, sc.time_step, sc.io_state, sc.options, tuning)
+        if learned:
+            g.set_closure(model, judge_stride=64)
+        adv(g, args.closure_start)           # up to the impact
+        if learned:
+            g.closure_stats(reset=True)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        adv(g, args.closure_window)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t1
+        stats = g.closure_stats(reset=True) if learned else None
+        parts = g.to_io_state().particles
+        # Per-stage breakdown over 50 further substeps (untimed).
+        g.kernel_times(reset=True)
+        g.set_profiling(True)
+        adv(g, 50)
+        g.set_profiling(False)
+        stages = {k: round(v[0] / 50, 4) for k, v in g.kernel_times(reset=True).items() if v[1]}
+        g.close()
+        return el, stats, parts, stages
+
+    el_plain, _, parts_plain, stages_plain = timed(False)
+    el_learned, stats, parts_learned, stages_learned = timed(True)
+
+    # Offline: the network against the analytic stress in FP64 on the final
+    # learned state (the judge above compares with the FP32 analytic stress).
+    fin = closure.pairs_from_particles(parts_learned, hard)
+    inside = np.all((fin.x >= model.hull_lo) & (fin.x <= model.hull_hi), axis=1)
+    y = net.forward(torch.from_numpy(fin.x[inside]).cuda()).cpu().numpy().astype(np.float64)
+    unit = closure.stress_unit(fin.E[inside], fin.jp[inside], hard)
+    tau64 = closure.analytic_principal_stress(fin.s[inside], fin.E[inside], fin.nu[inside], fin.jp[inside], hard)
+    err64 = float(np.linalg.norm(y * unit[:, None] - tau64) / np.linalg.norm(tau64))
+    ex = float(np.linalg.norm(parts_learned.positions - parts_plain.positions) / np.linalg.norm(parts_plain.positions))
+    w = args.closure_window
+    return {
+        "metric": "particle-substeps/s (learned closure vs analytic stress)",
+        "value": n * w / el_learned,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": w,
+        "ms_per_step": el_learned / w * 1e3,
+        "higher_is_better": True,
+        "dtype": "f32 state, bf16 MLP with f32 accumulation",
+        "data": "synthetic (seeded snow block)",
+        "config": {"workload": "snow_block_on_sphere_1M", "particles": n, "h": sc.consts.grid_node_size,
+                   "dt": sc.time_step, "window": [args.closure_start, args.closure_start + w]},
+        "plain": {"value": n * w / el_plain, "ms_per_step": el_plain / w * 1e3,
+                  "stage_ms_per_substep": stages_plain},
+        "learned_stage_ms_per_substep": stages_learned,
+        "learned_vs_plain": el_plain / el_learned,
+        "stress_error_vs_analytic_f32": stats["relative_stress_error"],
+        "stress_error_vs_analytic_f64": err64,
+        "network_evaluations": stats["evaluated"],
+        "hull_fallbacks": stats["fallbacks"],
+        "judged": stats["judged"],
+        "position_error_vs_plain": ex,
+        "training": {"pairs": len(data), "epochs": args.epochs, "seconds": train_s,
+                     "mse_first_epoch": tlog.epochs[0], "mse_last_epoch": tlog.epochs[-1]},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +217,13 @@ def main():
         help="first launch the plain copy kernel (k_copy_bytes) three times over a buffer of this many MiB, "
         "outside the timed region, so that hardware counters can be scaled by a copy of known size",
     )
+    ap.add_argument("--form", choices=("plain", "learned"), default="plain",
+                    help="plain: the headline 4.2M-particle substep; learned: the learned closure on the snow scene")
+    ap.add_argument("--pairs", type=int, default=2_000_000)
+    ap.add_argument("--epochs", type=int, default=600)
+    ap.add_argument("--closure-substeps", type=int, default=1000)
+    ap.add_argument("--closure-start", type=int, default=300)
+    ap.add_argument("--closure-window", type=int, default=200)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -130,6 +241,9 @@ def main():
 
     if args.copy_mib:
         copy_calibration(args.copy_mib)
+    if args.form == "learned":
+        print(json.dumps(learned_form(args)))
+        return
 
     scene = scenes.spheres_and_slab()
     n = scene.io_state.particles.count

@@ -88,6 +88,67 @@ __device__ __forceinline__ void forward16(const float *__restrict__ P, bf16x4 xo
     for (int i = 0; i < 4; ++i) y[i] = acc[i] + P[MLP_OFF_B3 + r0 + i];
 }
 
+// The network's operands staged in LDS in the order the lanes read them:
+// op[o][l] is lane l's BF16 operand o (o = t for W1 tile t, 4 + 4 t + kk for
+// W2 tile (t, kk), 20 + kk for W3 tile kk), so that every operand is one
+// conflict-free 8-byte read; the biases follow as floats.
+struct StagedNet {
+    bf16x4 op[24][64];
+    float b1[MLP_H], b2[MLP_H], b3[16];
+};
+
+__device__ __forceinline__ void stage_net(const float *__restrict__ P, StagedNet &S) {
+    for (int idx = (int)threadIdx.x; idx < 24 * 64; idx += (int)blockDim.x) {
+        const int o = idx >> 6, ln = idx & 63, r0 = 4 * (ln >> 4), c = ln & 15;
+        const float *src;
+        if (o < 4)
+            src = P + MLP_OFF_W1 + (16 * o + c) * MLP_KIN + r0;
+        else if (o < 20)
+            src = P + MLP_OFF_W2 + (16 * ((o - 4) >> 2) + c) * MLP_H + 16 * ((o - 4) & 3) + r0;
+        else
+            src = P + MLP_OFF_W3 + c * MLP_H + 16 * (o - 20) + r0;
+        S.op[o][ln] = op4(src, 1);
+    }
+    for (int idx = (int)threadIdx.x; idx < 2 * MLP_H + 16; idx += (int)blockDim.x) {
+        if (idx < MLP_H)
+            S.b1[idx] = P[MLP_OFF_B1 + idx];
+        else if (idx < 2 * MLP_H)
+            S.b2[idx - MLP_H] = P[MLP_OFF_B2 + idx - MLP_H];
+        else
+            S.b3[idx - 2 * MLP_H] = P[MLP_OFF_B3 + idx - 2 * MLP_H];
+    }
+}
+
+// forward16 with the operands from LDS.
+__device__ __forceinline__ void forward16_staged(const StagedNet &S, bf16x4 xop, f32x4 &y) {
+    const int l = (int)(threadIdx.x & 63u), r0 = 4 * (l >> 4);
+    bf16x4 h1[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const f32x4 acc = mfma(S.op[t][l], xop, zero4());
+        f32x4 h;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) h[i] = fmaxf(acc[i] + S.b1[16 * t + r0 + i], 0.f);
+        h1[t] = pack(h);
+    }
+    bf16x4 h2[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        f32x4 acc = zero4();
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) acc = mfma(S.op[4 + 4 * t + kk][l], h1[kk], acc);
+        f32x4 h;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) h[i] = fmaxf(acc[i] + S.b2[16 * t + r0 + i], 0.f);
+        h2[t] = pack(h);
+    }
+    f32x4 acc = zero4();
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) acc = mfma(S.op[20 + kk][l], h2[kk], acc);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) y[i] = acc[i] + S.b3[r0 + i];
+}
+
 // The B operand of 16 samples' features, from row-major x[n][MLP_IN].
 __device__ __forceinline__ bf16x4 feature_op(const float *__restrict__ x, uint64_t s0, uint64_t n) {
     const int l = (int)(threadIdx.x & 63u), r0 = 4 * (l >> 4), c = l & 15;
@@ -263,7 +324,8 @@ __global__ void k_mlp_adam(float *__restrict__ P, float *__restrict__ G, float *
 // Kirchhoff stress of every live snow particle from the network, written
 // by slot (row k at tau[k * stride + slot]) for the scatter that follows.
 // Per lane: the SVD of F, the network inputs and the training-hull test; per
-// wave: the network on its 64 particles, 16 at a time.  A particle whose
+// wave: the network on its 64 particles, 16 at a time, with the work-group's
+// copy of the operands in LDS.  A particle whose
 // inputs leave the hull takes the analytic stress; the particles with
 // id % judge_stride == judge_phase take both, and the difference is summed
 // into the statistics.
@@ -281,8 +343,11 @@ struct ClosureArgs {
 };
 
 __global__ __launch_bounds__(256) void k_closure(ClosureArgs a) {
+    __shared__ StagedNet net;
     __shared__ float feat[4][64 * 8];
     __shared__ float outs[4][64 * 4];
+    stage_net(a.params, net);
+    __syncthreads();
     const int l = (int)(threadIdx.x & 63u), w = (int)(threadIdx.x >> 6), r0 = 4 * (l >> 4), c = l & 15;
     uint32_t evaluated = 0, fallbacks = 0, judged = 0;
     double err2 = 0.0, ref2 = 0.0;
@@ -333,9 +398,8 @@ __global__ __launch_bounds__(256) void k_closure(ClosureArgs a) {
                 xop.y = real ? to_bf16(fp[1]) : (short)0;
                 xop.z = real ? to_bf16(fp[2]) : (short)0;
                 xop.w = real ? to_bf16(fp[3]) : (short)0;
-                Tiles h1, h2;
                 f32x4 y;
-                forward16(a.params, xop, h1, h2, y);
+                forward16_staged(net, xop, y);
                 if (l < 16) {
 #pragma unroll
                     for (int q = 0; q < 4; ++q) outs[w][(16 * g + l) * 4 + q] = y[q];
@@ -377,12 +441,22 @@ __global__ __launch_bounds__(256) void k_closure(ClosureArgs a) {
         err2 += __shfl_xor(err2, o, 64);
         ref2 += __shfl_xor(ref2, o, 64);
     }
-    if (l == 0) {
-        atomicAdd(&a.stats->evaluated, (unsigned long long)evaluated);
-        atomicAdd(&a.stats->fallbacks, (unsigned long long)fallbacks);
-        atomicAdd(&a.stats->judged, (unsigned long long)judged);
-        atomicAdd(&a.stats->err2, err2);
-        atomicAdd(&a.stats->ref2, ref2);
+    // Work-group w adds its sums to slot w with a plain read-add-write: one
+    // launch touches a slot once, and launches on the stream are ordered
+    // (atomics on five common addresses would serialize in memory).
+    __shared__ ClosureStats part[4];
+    if (l == 0) part[w] = ClosureStats{evaluated, fallbacks, judged, 0ull, err2, ref2};
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ClosureStats &S = a.stats[blockIdx.x];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            S.evaluated += part[q].evaluated;
+            S.fallbacks += part[q].fallbacks;
+            S.judged += part[q].judged;
+            S.err2 += part[q].err2;
+            S.ref2 += part[q].ref2;
+        }
     }
 }
 
@@ -406,7 +480,7 @@ hipError_t launch_closure(const ClosureLaunch &c, hipStream_t stream) {
     a.judge_stride = c.judge_stride;
     a.judge_phase = c.judge_phase;
     a.stats = c.stats;
-    const uint32_t blocks = (uint32_t)std::min<uint64_t>((c.n + 255) / 256, 8192);
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((c.n + 255) / 256, CLOSURE_STATS_SLOTS);
     hipLaunchKernelGGL(k_closure, dim3(blocks), dim3(256), 0, stream, a);
     return hipGetLastError();
 }

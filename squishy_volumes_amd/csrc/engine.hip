@@ -857,9 +857,9 @@ int mpm_set_closure(mpm_engine *e, const mpm_closure *c) {
     if (!e->closure_params) {
         if (e->alloc(e->closure_params, MLP_NPARAM) != hipSuccess ||
             e->alloc(e->closure_tau, 9 * std::max<uint64_t>(e->n, 1)) != hipSuccess ||
-            e->alloc(e->closure_stats, 1) != hipSuccess)
+            e->alloc(e->closure_stats, CLOSURE_STATS_SLOTS) != hipSuccess)
             return MPM_ERR_OUT_OF_MEMORY;
-        HIP_TRY(hipMemsetAsync(e->closure_stats, 0, sizeof(ClosureStats), e->stream));
+        HIP_TRY(hipMemsetAsync(e->closure_stats, 0, CLOSURE_STATS_SLOTS * sizeof(ClosureStats), e->stream));
     }
     HIP_TRY(hipMemcpyAsync(e->closure_params, c->params, MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
@@ -878,15 +878,18 @@ int mpm_read_closure_stats(mpm_engine *e, int32_t reset, mpm_closure_stats *out)
     std::memset(out, 0, sizeof(*out));
     if (!e->closure_stats) return MPM_OK;
     HIP_TRY(hipSetDevice(e->device));
-    ClosureStats h{};
-    HIP_TRY(hipMemcpyAsync(&h, e->closure_stats, sizeof(h), hipMemcpyDeviceToHost, e->stream));
-    if (reset) HIP_TRY(hipMemsetAsync(e->closure_stats, 0, sizeof(ClosureStats), e->stream));
+    std::vector<ClosureStats> h(CLOSURE_STATS_SLOTS);
+    HIP_TRY(hipMemcpyAsync(h.data(), e->closure_stats, h.size() * sizeof(ClosureStats), hipMemcpyDeviceToHost,
+                           e->stream));
+    if (reset) HIP_TRY(hipMemsetAsync(e->closure_stats, 0, h.size() * sizeof(ClosureStats), e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
-    out->evaluated = h.evaluated;
-    out->fallbacks = h.fallbacks;
-    out->judged = h.judged;
-    out->err2 = h.err2;
-    out->ref2 = h.ref2;
+    for (const ClosureStats &s : h) {
+        out->evaluated += s.evaluated;
+        out->fallbacks += s.fallbacks;
+        out->judged += s.judged;
+        out->err2 += s.err2;
+        out->ref2 += s.ref2;
+    }
     return MPM_OK;
 }
 

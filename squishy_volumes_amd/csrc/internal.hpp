@@ -236,6 +236,9 @@ struct ClosureStats {
     unsigned long long evaluated, fallbacks, judged, pad;
     double err2, ref2;  // sum |tau - tau_analytic|^2 and sum |tau_analytic|^2 over the kept particles
 };
+// The closure kernel runs at most this many work-groups, each summing into
+// its own statistics slot.
+constexpr uint32_t CLOSURE_STATS_SLOTS = 4096;
 struct ClosureLaunch {
     ParticleArrays P;
     const uint32_t *order;  // slot i reads P at order[i], or null
@@ -246,7 +249,7 @@ struct ClosureLaunch {
     float hull_lo[MLP_IN], hull_hi[MLP_IN];
     float hardening;
     uint32_t judge_stride, judge_phase;
-    ClosureStats *stats;
+    ClosureStats *stats;    // CLOSURE_STATS_SLOTS slots
 };
 hipError_t launch_closure(const ClosureLaunch &c, hipStream_t stream);
 

@@ -81,8 +81,16 @@ def test_trained_closure_tracks_the_analytic_run():
     net = closure.Mlp(seed=3)
     x = torch.from_numpy(data.x).cuda()
     t = torch.from_numpy(data.t).cuda()
+    # What training gains: the stress error of the network on the pairs before
+    # and after (in stress units, relative to the analytic stress).
+    unit = closure.stress_unit(data.E, data.jp, hard)[:, None]
+    tau = data.t.astype(np.float64) * unit
+    untrained = rel_norm_err(net.forward(x).cpu().numpy() * unit, tau)
     log = net.fit(x, t, epochs=300, batch=1024, lr=3e-3, lr_final=1e-4)
-    print(f"{len(data)} pairs, loss {log.epochs[0]:.3e} -> {log.epochs[-1]:.3e}")
+    trained = rel_norm_err(net.forward(x).cpu().numpy() * unit, tau)
+    print(f"{len(data)} pairs, loss {log.epochs[0]:.3e} -> {log.epochs[-1]:.3e}; "
+          f"stress error untrained {untrained:.3f}, trained {trained:.2e}")
+    assert trained <= 0.01 * untrained
     model = net.model(data.x)
 
     learned = engine(sc)
