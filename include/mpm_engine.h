@@ -115,6 +115,16 @@ typedef struct mpm_consts {
     int32_t device;              /* HIP device ordinal; GpuState's adapter name */
     float sphere_center[3];      /* MPM_OPT_SPHERE */
     float sphere_radius;
+    /* Batched scenes (no reference counterpart): with scene_repeat[a] > 1 the
+     * domain [domain_min, domain_max] is one scene's, repeated scene_repeat[a]
+     * times along axis a with period scene_period[a] (> the domain's width).
+     * The grid covers all copies; the cull, walls, plane and sphere act on
+     * each particle's and node's position within its own copy, copies
+     * meeting in the middle of the gap between them (scene_period[a] must be
+     * at least the domain's width + 4 grid_node_size, else
+     * MPM_ERR_INVALID_ARGUMENT).  0 or 1 means one copy. */
+    float scene_period[3];
+    uint32_t scene_repeat[3];
 } mpm_consts;
 
 /* Per-particle parameters, the flattened form of ParticleParameters

@@ -85,6 +85,8 @@ class MpmConsts(C.Structure):
         ("device", C.c_int32),
         ("sphere_center", C.c_float * 3),
         ("sphere_radius", C.c_float),
+        ("scene_period", C.c_float * 3),
+        ("scene_repeat", C.c_uint32 * 3),
     ]
 
 

@@ -172,8 +172,10 @@ int compute_geometry(const mpm_consts &c, GridGeom &g, int &key_bits) {
     const double h = c.grid_node_size;
     int nbmax = 0;
     for (int a = 0; a < 3; ++a) {
+        double extent_max = c.domain_max[a];
+        if (c.scene_repeat[a] > 1) extent_max += (double)c.scene_period[a] * (c.scene_repeat[a] - 1);
         double lo = std::floor((double)c.domain_min[a] / h - 0.5);
-        double hi = std::floor((double)c.domain_max[a] / h - 0.5);
+        double hi = std::floor(extent_max / h - 0.5);
         if (!(hi - lo < 4.0 * 1023)) return MPM_ERR_GRID_TOO_LARGE;
         long long origin = 4 * (long long)std::floor((lo - 2.0) / 4.0);
         long long cmax = (long long)hi + 2 - origin;
@@ -218,6 +220,12 @@ SubstepParams make_params(const mpm_engine *e, const float gravity[3], float fac
         S.domain_min[a] = c.domain_min[a];
         S.domain_max[a] = c.domain_max[a];
         S.sphere_center[a] = c.sphere_center[a];
+        const bool rep = c.scene_repeat[a] > 1;
+        S.scene_period[a] = rep ? c.scene_period[a] : 0.f;
+        S.scene_inv_period[a] = rep ? 1.0f / c.scene_period[a] : 0.f;
+        S.scene_last[a] = rep ? (float)(c.scene_repeat[a] - 1) : 0.f;
+        // copies meet in the middle of the gap between them
+        S.scene_origin[a] = rep ? c.domain_min[a] - 0.5f * (c.scene_period[a] - (c.domain_max[a] - c.domain_min[a])) : 0.f;
     }
     S.sphere_radius = c.sphere_radius;
     S.factor_b = factor_b;
@@ -431,8 +439,13 @@ int mpm_create(const mpm_consts *consts, const mpm_particles_view *pv, const mpm
     if (!(consts->grid_node_size > 0.f) || !(consts->time_step >= 0.f) || consts->frames_per_second == 0)
         return MPM_ERR_INVALID_ARGUMENT;
     if (!finite3(consts->domain_min) || !finite3(consts->domain_max)) return MPM_ERR_INVALID_ARGUMENT;
-    for (int a = 0; a < 3; ++a)
+    for (int a = 0; a < 3; ++a) {
         if (!(consts->domain_min[a] < consts->domain_max[a])) return MPM_ERR_INVALID_ARGUMENT;
+        // copies of a batch are at least 4 nodes apart: no stencil reaches across
+        if (consts->scene_repeat[a] > 1 &&
+            !(consts->scene_period[a] >= consts->domain_max[a] - consts->domain_min[a] + 4.f * consts->grid_node_size))
+            return MPM_ERR_INVALID_ARGUMENT;
+    }
     if (pv->count > 0 && (!pv->flags || !pv->parameters || !pv->positions || !pv->position_gradients ||
                           !pv->velocities || !pv->velocity_gradients))
         return MPM_ERR_INVALID_ARGUMENT;

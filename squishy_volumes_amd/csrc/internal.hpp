@@ -126,8 +126,19 @@ struct SubstepParams {
     float domain_max[3];
     float snow_lo, snow_hi, snow_hardening;
     float sphere_center[3], sphere_radius;
+    // Batched scenes: copy k of the domain spans [scene_origin + k period,
+    // scene_origin + (k + 1) period) along each axis, k = 0 .. scene_last.
+    float scene_period[3], scene_inv_period[3], scene_last[3], scene_origin[3];
     uint32_t epoch;    // substep counter, stamps grid blocks
 };
+
+// Coordinate a of a position within its own scene of a batch (the domain
+// repeated with period scene_period[a]); unchanged for one scene.
+__device__ __forceinline__ float scene_local(float x, int a, const SubstepParams &S) {
+    if (!(S.scene_last[a] > 0.f)) return x;
+    const float k = fminf(fmaxf(floorf((x - S.scene_origin[a]) * S.scene_inv_period[a]), 0.f), S.scene_last[a]);
+    return x - k * S.scene_period[a];
+}
 
 enum KernelId : uint32_t {
     KID_KEYS = 1,

@@ -757,7 +757,8 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
                     // cull_particles: strict inequalities.
 #pragma unroll
                     for (int q = 0; q < 3; ++q)
-                        inside = inside && xp[q] > a.S1.domain_min[q] && xp[q] < a.S1.domain_max[q];
+                        inside = inside && scene_local(xp[q], q, a.S1) > a.S1.domain_min[q] &&
+                                 scene_local(xp[q], q, a.S1) < a.S1.domain_max[q];
                     if (!inside) flags |= F_TOMB;
 
                     if (a.keys_out) a.keys_out[i] = inside ? particle_block_key(xp, G, ds) : G.tomb_key;
@@ -1072,8 +1073,9 @@ __global__ __launch_bounds__(256) void k_grid_update(GridBuffers B, GridGeom G, 
                 }
                 if (S.options & (MPM_OPT_STICKY_PLANE | MPM_OPT_BOX_WALLS | MPM_OPT_SPHERE)) {
                     const int lx = 4 * bx + (int)x, ly = 4 * by + (int)y, lz = 4 * bz + (int)z;
-                    const float xi[3] = {(float)(lx + G.origin[0]) * S.h, (float)(ly + G.origin[1]) * S.h,
-                                         (float)(lz + G.origin[2]) * S.h};
+                    const float xi[3] = {scene_local((float)(lx + G.origin[0]) * S.h, 0, S),
+                                         scene_local((float)(ly + G.origin[1]) * S.h, 1, S),
+                                         scene_local((float)(lz + G.origin[2]) * S.h, 2, S)};
                     if ((S.options & MPM_OPT_STICKY_PLANE) && xi[2] < S.plane_z) v[0] = v[1] = v[2] = 0.f;
                     if (S.options & MPM_OPT_BOX_WALLS) {
 #pragma unroll

@@ -145,6 +145,8 @@ class EngineOptions:
     sticky_plane_z: Optional[float] = None
     box_walls: Optional[tuple] = None  # ((xmin, ymin, zmin), (xmax, ymax, zmax))
     sphere: Optional[tuple] = None  # ((cx, cy, cz), radius): separating slip on the sphere
+    scene_period: Optional[tuple] = None  # batched scenes: period of the repeated domain per axis
+    scene_repeat: Optional[tuple] = None  # and the number of copies per axis
     snow_compression: float = 2.5e-2
     snow_stretch: float = 4.5e-3
     snow_hardening: float = 10.0
@@ -172,6 +174,9 @@ def make_consts(consts: InputConsts, time_step: float, options: EngineOptions | 
         opt |= abi.OPT_SPHERE
         c.sphere_center[:] = options.sphere[0]
         c.sphere_radius = options.sphere[1]
+    if options.scene_repeat is not None:
+        c.scene_period[:] = options.scene_period
+        c.scene_repeat[:] = options.scene_repeat
     c.options = opt
     c.snow_compression = options.snow_compression
     c.snow_stretch = options.snow_stretch
