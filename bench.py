@@ -200,6 +200,90 @@ def learned_form(args) -> dict:
     }
 
 
+def batched_form(args) -> dict:
+    """64 uneven random-object scenes (5k..500k particles each, log-uniform,
+    scaled to 8M in all; h = 1/128, dt = 5e-5) advanced together by one
+    engine with autotuned launch knobs, against the plain form: each scene on
+    its own engine, one after the other, with the default knobs."""
+    import torch
+
+    from squishy_volumes_amd import autotune, scenes
+    from squishy_volumes_amd.batch import Batch
+    from squishy_volumes_amd.gpu_state import GpuState
+
+    scs = scenes.uneven_scene_set(args.scenes, seed=0, total=args.batch_particles)
+    b = Batch(scs)
+    sc = b.scene
+    n = b.count
+    steps = args.batch_substeps
+    log(f"batched: {len(scs)} scenes, {n} particles, repeat {b.repeat}")
+    if args.no_autotune:
+        tuning = {"resort_interval": args.resort_interval or 8, "persistent_blocks": args.persistent_blocks}
+        tune = None
+    else:
+        tune = autotune.autotune(sc, log=log)
+        tuning = tune.best
+    adv = lambda g, k: g.step_to(scenes.advance_time(g.time, sc.time_step, k), b.frame_input)
+    g = GpuState.from_io_state(0, sc.consts, sc.time_step, sc.io_state, sc.options, tuning)
+    if args.warmup:
+        adv(g, args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    status = adv(g, steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    assert status.result == 0 and status.substeps == steps, (status.result, status.substeps)
+    nodes = int(status.active_nodes)
+    g.kernel_times(reset=True)
+    g.set_profiling(True)
+    adv(g, 50)
+    g.set_profiling(False)
+    breakdown = g.kernel_times(reset=True)
+    g.close()
+    log(f"batched: {steps} substeps in {el:.3f} s")
+
+    # The plain form: every scene alone, the same substeps, default knobs.
+    el_plain = 0.0
+    for s1 in scs:
+        g1 = GpuState.from_io_state(0, s1.consts, s1.time_step, s1.io_state, s1.options)
+        if args.warmup:
+            g1.step_to(scenes.advance_time(g1.time, s1.time_step, args.warmup), s1.frame_input)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        st1 = g1.step_to(scenes.advance_time(g1.time, s1.time_step, steps), s1.frame_input)
+        torch.cuda.synchronize()
+        el_plain += time.perf_counter() - t1
+        assert st1.result == 0
+        g1.close()
+    log(f"plain per-scene form: {steps} substeps of every scene in {el_plain:.3f} s")
+    substep_bytes = 224 * n + 64 * nodes
+    counts = [s1.io_state.particles.count for s1 in scs]
+    return {
+        "metric": "particle-substeps/s (batched scenes vs one engine per scene)",
+        "value": n * steps / el,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / steps * 1e3,
+        "higher_is_better": True,
+        "dtype": "f32",
+        "data": "synthetic (seeded random cubes and spheres)",
+        "config": {"workload": f"uneven_scenes_{len(scs)}", "scenes": len(scs), "particles": n,
+                   "particles_per_scene": {"min": min(counts), "max": max(counts)},
+                   "h": sc.consts.grid_node_size, "dt": sc.time_step, "scene_repeat": list(b.repeat),
+                   "active_nodes": nodes, "active_blocks": int(status.active_blocks)},
+        "tuning": tuning,
+        "autotune": None if tune is None else {"ms_per_substep": tune.ms_per_substep, "trials": tune.trials},
+        "plain": {"value": n * steps / el_plain, "ms_per_step": el_plain / steps * 1e3},
+        "batched_vs_plain": el_plain / el,
+        "substep_roofline": {"bytes_per_substep": substep_bytes,
+                             "achieved": substep_bytes * steps / el / 1e9,
+                             "frac": substep_bytes * steps / el / 1e9 / HBM_PEAK_GBS},
+        "stage_ms_per_substep": {k: v[0] / 50 for k, v in breakdown.items() if v[1]},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -217,8 +301,13 @@ def main():
         help="first launch the plain copy kernel (k_copy_bytes) three times over a buffer of this many MiB, "
         "outside the timed region, so that hardware counters can be scaled by a copy of known size",
     )
-    ap.add_argument("--form", choices=("plain", "learned"), default="plain",
-                    help="plain: the headline 4.2M-particle substep; learned: the learned closure on the snow scene")
+    ap.add_argument("--form", choices=("plain", "learned", "batched"), default="plain",
+                    help="plain: the headline 4.2M-particle substep; learned: the learned closure on the snow "
+                    "scene; batched: 64 uneven scenes in one engine against one engine per scene")
+    ap.add_argument("--scenes", type=int, default=64)
+    ap.add_argument("--batch-substeps", type=int, default=200)
+    ap.add_argument("--batch-particles", type=int, default=8_000_000)
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--pairs", type=int, default=2_000_000)
     ap.add_argument("--epochs", type=int, default=600)
     ap.add_argument("--closure-substeps", type=int, default=1000)
@@ -243,6 +332,9 @@ def main():
         copy_calibration(args.copy_mib)
     if args.form == "learned":
         print(json.dumps(learned_form(args)))
+        return
+    if args.form == "batched":
+        print(json.dumps(batched_form(args)))
         return
 
     scene = scenes.spheres_and_slab()

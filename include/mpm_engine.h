@@ -186,6 +186,7 @@ typedef struct mpm_status {
     uint64_t active_blocks;              /* occupied 4^3 blocks in the last substep */
     uint64_t active_nodes;               /* grid nodes with mass > 0 in the last substep */
     uint64_t svd_fp64_fallbacks;         /* 1 when an SVD was recomputed in FP64 during the call */
+    uint64_t graph_launches;             /* sort cycles run as one HIP graph launch by the call */
 } mpm_status;
 
 /* Grid nodes, as GridNodes (rust/crates/file_frame/src/grid_nodes.rs:9-15):
@@ -207,7 +208,11 @@ typedef struct mpm_tuning {
                                             sorts a particle keeps its slot and may drift one cell
                                             out of its block; farther drift takes a slower path */
     int32_t persistent_blocks;           /* work-groups of the transfer kernel (default: all resident) */
-    int32_t reserved[2];
+    int32_t graph;                       /* -1: launch every kernel of the substep loop; otherwise
+                                            (default) each whole sort cycle inside a call is one
+                                            launch of a captured HIP graph (not while profiling,
+                                            not with the learned closure) */
+    int32_t reserved;
 } mpm_tuning;
 
 /* ---- engine API (host pointers) ------------------------------------------ */

@@ -136,6 +136,7 @@ class MpmStatus(C.Structure):
         ("active_blocks", C.c_uint64),
         ("active_nodes", C.c_uint64),
         ("svd_fp64_fallbacks", C.c_uint64),
+        ("graph_launches", C.c_uint64),
     ]
 
 
@@ -154,7 +155,8 @@ class MpmTuning(C.Structure):
     _fields_ = [
         ("resort_interval", C.c_int32),
         ("persistent_blocks", C.c_int32),
-        ("reserved", C.c_int32 * 2),
+        ("graph", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 

@@ -270,6 +270,9 @@ SV_DEV void recompose(const float U[9], const float d[3], const float V[9], floa
 // Material selector carried in the particle flags.
 // Model sets the substep kernels are specialised on.
 enum : uint32_t { M_NH = 1, M_FLUID = 2, M_FCR = 4, M_SNOW = 8, M_SAND = 16, M_VISC = 32 };
+// Not a model: the variant for batched scenes (the cull in each particle's
+// own copy of the domain).
+constexpr uint32_t M_BATCH = 64;
 constexpr uint32_t SET_POLAR = M_NH | M_FLUID | M_FCR | M_VISC;  // no SVD in the hot path
 constexpr uint32_t SET_ALL = M_NH | M_FLUID | M_FCR | M_SNOW | M_SAND | M_VISC;
 

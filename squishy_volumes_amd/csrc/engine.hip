@@ -72,6 +72,17 @@ struct mpm_engine {
     DeviceScalars *ds = nullptr;
     DeviceScalars *ds_host = nullptr;  // pinned
 
+    // Substep parameters of a call, one entry per substep, uploaded before
+    // its first launch; transfers read entry ds->param_index, which each
+    // grid update advances.  Kernel arguments are therefore the same in every
+    // sort cycle, and a cycle captured as a HIP graph replays unchanged.
+    SubstepParams *param_table = nullptr;  // device
+    SubstepParams *param_host = nullptr;   // pinned staging
+    uint32_t param_capacity = 0;
+    bool use_graph = true;
+    hipGraphExec_t cycle_graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [cur][goals]
+    int graph_interval = 0;                // resort interval the graphs were captured with
+
     double time = 0;
     uint32_t epoch = 0;
     uint32_t grid_epoch = 0;   // epoch stamped by the last grid update
@@ -105,7 +116,10 @@ struct mpm_engine {
     ~mpm_engine() {
         if (device >= 0) (void)hipSetDevice(device);
         if (stream) (void)hipStreamSynchronize(stream);
+        drop_graphs();
         for (void *p : allocations) (void)hipFree(p);
+        if (param_table) (void)hipFree(param_table);
+        if (param_host) (void)hipHostFree(param_host);
         if (ds_host) (void)hipHostFree(ds_host);
         for (auto &r : records) {
             (void)hipEventDestroy(r.a);
@@ -113,6 +127,15 @@ struct mpm_engine {
         }
         for (auto e : event_pool) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void drop_graphs() {
+        for (auto &row : cycle_graph)
+            for (auto &g : row)
+                if (g) {
+                    (void)hipGraphExecDestroy(g);
+                    g = nullptr;
+                }
     }
 
     template <class T>
@@ -268,6 +291,7 @@ int upload_goals(mpm_engine *e, const mpm_frame_input *in, bool &goals) {
         HIP_TRY(e->alloc(e->X.goal_flags_a, n));
         HIP_TRY(e->alloc(e->X.goal_flags_b, n));
         e->goals_allocated = true;
+        e->drop_graphs();  // captured with the old (absent) goal arrays
     }
     std::vector<float> tmp(n);
     for (int which = 0; which < 2; ++which) {
@@ -283,15 +307,44 @@ int upload_goals(mpm_engine *e, const mpm_frame_input *in, bool &goals) {
     return MPM_OK;
 }
 
-GridBuffers grid_buffers(const mpm_engine *e, uint32_t epoch) {
+GridBuffers grid_buffers(const mpm_engine *e) {
     GridBuffers B;
     B.grid = e->grid;
     B.vgrid = e->vgrid;
     B.blk_flag = e->blk_flag;
     B.blk_epoch = e->blk_epoch;
     B.cand = e->cand;
-    B.ovf = e->ovf[epoch & 1u];
+    B.ovf[0] = e->ovf[0];
+    B.ovf[1] = e->ovf[1];
     return B;
+}
+
+// The parameters of the call's substeps, params[t] for substep t (the
+// engine's epoch + t), to the device table, and the table index to 0.
+int upload_params(mpm_engine *e, std::vector<SubstepParams> &params) {
+    if (params.empty()) return MPM_OK;
+    if (params.size() > e->param_capacity) {
+        e->drop_graphs();  // they hold the old table's address
+        if (e->param_table) (void)hipFree(e->param_table);
+        if (e->param_host) (void)hipHostFree(e->param_host);
+        e->param_table = nullptr;
+        e->param_host = nullptr;
+        e->param_capacity = 0;
+        const uint32_t cap = (uint32_t)std::max<size_t>(params.size(), 1024);
+        HIP_TRY(hipMalloc((void **)&e->param_table, cap * sizeof(SubstepParams)));
+        HIP_TRY(hipHostMalloc((void **)&e->param_host, cap * sizeof(SubstepParams), hipHostMallocDefault));
+        e->param_capacity = cap;
+    }
+    for (size_t t = 0; t < params.size(); ++t) {
+        params[t].index = (uint32_t)t;
+        params[t].epoch = e->epoch + (uint32_t)t;
+    }
+    // The previous call has drained the stream: the staging buffer is free.
+    std::memcpy(e->param_host, params.data(), params.size() * sizeof(SubstepParams));
+    HIP_TRY(hipMemcpyAsync(e->param_table, e->param_host, params.size() * sizeof(SubstepParams),
+                           hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemsetAsync(&e->ds->param_index, 0, 2 * sizeof(uint32_t), e->stream));
+    return MPM_OK;
 }
 
 // Sort, block list: everything before a transfer that permutes.
@@ -320,7 +373,7 @@ int run_sort(mpm_engine *e) {
 // One transfer launch.  After a sort the transfer reads slot i from
 // order[i] and writes the other particle set; otherwise it works in place.
 // want_keys: write the block keys of the new positions for the next sort.
-int run_transfer(mpm_engine *e, TransferKind kind, const SubstepParams &S1, bool want_keys) {
+int run_transfer(mpm_engine *e, TransferKind kind, bool goals, bool want_keys) {
     EventRecord r{};
     TransferCall t{};
     t.kind = kind;
@@ -332,8 +385,9 @@ int run_transfer(mpm_engine *e, TransferKind kind, const SubstepParams &S1, bool
     t.blk_start = e->blk_start;
     t.blk_key = e->blk_key;
     t.G = e->geom;
-    t.S1 = S1;
-    t.B = grid_buffers(e, S1.epoch);
+    t.params = e->param_table;
+    t.goals = goals;
+    t.B = grid_buffers(e);
     t.keys_out = want_keys ? e->keys : nullptr;
     t.ds = e->ds;
     if (kind == TransferKind::P2G && e->closure_on) {
@@ -357,13 +411,13 @@ int run_transfer(mpm_engine *e, TransferKind kind, const SubstepParams &S1, bool
     return MPM_OK;
 }
 
-// Grid update of the grid the last transfer scattered (epoch S.epoch).
-int run_grid(mpm_engine *e, const SubstepParams &S) {
+// Grid update of the grid the last transfer scattered (epoch e->epoch).
+int run_grid(mpm_engine *e) {
     EventRecord r{};
     e->begin(ST_GRID, r);
-    HIP_TRY(launch_grid_update(e->launch, grid_buffers(e, S.epoch), e->geom, S, e->partial, e->ds, e->stream));
+    HIP_TRY(launch_grid_update(e->launch, grid_buffers(e), e->geom, e->param_table, e->partial, e->ds, e->stream));
     e->end(r);
-    e->grid_epoch = S.epoch;
+    e->grid_epoch = e->epoch;
     e->grid_valid = true;
     e->epoch += 1;
     return MPM_OK;
@@ -393,6 +447,54 @@ int run_closure(mpm_engine *e) {
     HIP_TRY(launch_closure(c, e->stream));
     e->end(r);
     if (e->judge_stride) e->judge_phase = (e->judge_phase + 1) % e->judge_stride;
+    return MPM_OK;
+}
+
+// One sort cycle of the plain loop (resort_interval iterations, launched by
+// `iterations(u)` for u = 0 .. K - 1) as a HIP graph: captured from the same
+// launches on the first use for this particle buffer and goal setting, then
+// replayed.  The launches read their substep from the parameter table, so
+// the replay needs no argument update; the host-side state advances as the
+// captured launches advanced it.
+template <class Iter>
+int run_cycle_graph(mpm_engine *e, bool goals, Iter iterations) {
+    const int K = std::max(1, e->resort_interval);
+    if (e->graph_interval != K) {
+        e->drop_graphs();
+        e->graph_interval = K;
+    }
+    // The state after the cycle: one permuting transfer, K grid updates, the
+    // next cycle's keys written.
+    const int cur_after = 1 - e->cur;
+    const uint32_t epoch_after = e->epoch + (uint32_t)K;
+    hipGraphExec_t &exec = e->cycle_graph[e->cur][goals ? 1 : 0];
+    if (!exec) {
+        HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
+        int rc = MPM_OK;
+        for (int u = 0; u < K && rc == MPM_OK; ++u) rc = iterations((size_t)u);
+        hipGraph_t graph = nullptr;
+        const hipError_t ce = hipStreamEndCapture(e->stream, &graph);
+        if (rc != MPM_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        HIP_TRY(ce);
+        const hipError_t ie = hipGraphInstantiateWithFlags(&exec, graph, 0);
+        (void)hipGraphDestroy(graph);
+        HIP_TRY(ie);
+        if (e->cur != cur_after || e->epoch != epoch_after || !e->keys_valid || e->pending_permute) {
+            g_last_error = "captured sort cycle left an unexpected engine state";
+            return MPM_ERR_HIP;
+        }
+    } else {
+        e->cur = cur_after;
+        e->keys_valid = true;
+        e->pending_permute = false;
+        e->grid_epoch = epoch_after - 1u;
+        e->epoch = epoch_after;
+        e->grid_valid = true;
+    }
+    HIP_TRY(hipGraphLaunch(exec, e->stream));
     return MPM_OK;
 }
 
@@ -479,9 +581,11 @@ int mpm_create(const mpm_consts *consts, const mpm_particles_view *pv, const mpm
     if (tuning) {
         if (tuning->persistent_blocks > 0) e->launch.persistent_blocks = tuning->persistent_blocks;
         if (tuning->resort_interval > 0) e->resort_interval = tuning->resort_interval;
+        if (tuning->graph < 0) e->use_graph = false;
     }
     // Kernels are specialised on the constitutive models present: snow and
     // sand need the full SVD, every other model a polar decomposition at most.
+    e->launch.batched = consts->scene_repeat[0] > 1 || consts->scene_repeat[1] > 1 || consts->scene_repeat[2] > 1;
     e->launch.model_set = SET_POLAR;
     for (uint64_t i = 0; i < pv->count; ++i)
         if (pv->flags[i] & (MPM_SNOW | MPM_USE_SAND_ALPHA)) {
@@ -604,12 +708,15 @@ int mpm_step_to(mpm_engine *e, double target_time, const mpm_frame_input *input,
         params.push_back(make_params(e, g, fb, goals, input && input->has_b));
     }
     HIP_TRY(hipMemsetAsync(&e->ds->status, 0, 4, e->stream));
+    // Entry k (the last gather's) repeats entry k - 1.
+    if (!params.empty()) params.push_back(params.back());
+    if ((rc = upload_params(e, params)) != MPM_OK) return rc;
 
     // Transfer t gathers substep t-1 (t >= 1) and scatters substep t (t < k):
     // a scatter from the stored state, k-1 fused gather-scatters, a last
     // gather.  The particles are sorted before the first transfer and then
     // every resort_interval substeps; in between they keep their slots.
-    const size_t k = params.size();
+    const size_t k = times.size();
     const size_t K = (size_t)std::max(1, e->resort_interval);
     auto sorts_before = [&](size_t t) { return t == 0 || (t < k && t % K == 0); };
     uint64_t done = 0;
@@ -623,36 +730,50 @@ int mpm_step_to(mpm_engine *e, double target_time, const mpm_frame_input *input,
             break;
         }
         if (t > 0) {
-            SubstepParams S0 = params[t - 1];
-            S0.epoch = e->epoch;
-            if ((rc = run_transfer(e, TransferKind::G2P, S0, t == k || sorts_before(t))) != MPM_OK) return rc;
+            if ((rc = run_transfer(e, TransferKind::G2P, false, t == k || sorts_before(t))) != MPM_OK) return rc;
             e->time = times[t - 1] + (double)e->consts.time_step;
             ++done;
         }
         if (t == k) break;
         if (sorts_before(t) && (rc = run_sort(e)) != MPM_OK) return rc;
-        SubstepParams S1 = params[t];
-        S1.epoch = e->epoch;
         if ((rc = run_closure(e)) != MPM_OK) return rc;
-        if ((rc = run_transfer(e, TransferKind::P2G, S1, false)) != MPM_OK) return rc;
-        if ((rc = run_grid(e, S1)) != MPM_OK) return rc;
+        if ((rc = run_transfer(e, TransferKind::P2G, goals, false)) != MPM_OK) return rc;
+        if ((rc = run_grid(e)) != MPM_OK) return rc;
     }
-    for (size_t t = 0; !e->closure_on && k > 0 && t <= k; ++t) {
+    // Plain loop.  Iteration t: the sort when due, transfer t, grid update t.
+    auto iteration = [&](size_t t) -> int {
+        int r;
+        if (sorts_before(t) && (r = run_sort(e)) != MPM_OK) return r;
+        const bool want_keys = t == k || sorts_before(t + 1);
+        const TransferKind kind = t == 0 ? TransferKind::P2G : (t == k ? TransferKind::G2P : TransferKind::G2P2G);
+        if ((r = run_transfer(e, kind, goals, want_keys)) != MPM_OK) return r;
+        if (t < k && (r = run_grid(e)) != MPM_OK) return r;
+        return MPM_OK;
+    };
+    // A sort cycle t .. t + K - 1 that starts after a transfer which wrote
+    // the keys and ends with one that writes the next cycle's keys has the
+    // same launches every time: it runs as one HIP graph launch.
+    const bool graphs = e->use_graph && !e->profiling;
+    uint64_t graph_launches = 0;
+    for (size_t t = 0; !e->closure_on && k > 0 && t <= k;) {
         if (t > 0 && cancel && *cancel) {
             result = MPM_ERR_CANCELED;
             break;
         }
-        if (sorts_before(t) && (rc = run_sort(e)) != MPM_OK) return rc;
-        const bool want_keys = t == k || sorts_before(t + 1);
-        SubstepParams S1 = params[t < k ? t : k - 1];
-        S1.epoch = e->epoch;
-        const TransferKind kind = t == 0 ? TransferKind::P2G : (t == k ? TransferKind::G2P : TransferKind::G2P2G);
-        if ((rc = run_transfer(e, kind, S1, want_keys)) != MPM_OK) return rc;
-        if (t < k && (rc = run_grid(e, S1)) != MPM_OK) return rc;
-        if (t > 0) {
-            e->time = times[t - 1] + (double)e->consts.time_step;
-            ++done;
+        const bool cycle = graphs && t > 0 && t % K == 0 && t + K < k && e->keys_valid && !e->pending_permute;
+        const size_t steps = cycle ? K : 1;
+        if (cycle) {
+            if ((rc = run_cycle_graph(e, goals, [&](size_t u) { return iteration(t + u); })) != MPM_OK) return rc;
+            ++graph_launches;
+        } else if ((rc = iteration(t)) != MPM_OK) {
+            return rc;
         }
+        for (size_t u = t; u < t + steps; ++u)
+            if (u > 0) {
+                e->time = times[u - 1] + (double)e->consts.time_step;
+                ++done;
+            }
+        t += steps;
     }
     if (done > 0) e->stepped = true;
     if ((rc = fetch_scalars(e)) != MPM_OK) return rc;
@@ -671,6 +792,7 @@ int mpm_step_to(mpm_engine *e, double target_time, const mpm_frame_input *input,
         status->active_blocks = e->ds_host->num_blocks;
         status->active_nodes = e->ds_host->active_nodes;
         status->svd_fp64_fallbacks = (e->ds_host->status & MPM_STATUS_SVD_FP64) ? 1 : 0;
+        status->graph_launches = graph_launches;
     }
     return result;
 }
@@ -770,14 +892,14 @@ int mpm_run_stage(mpm_engine *e, int32_t stage, const mpm_frame_input *input) {
     if (rc != MPM_OK) return rc;
     bool goals = false;
     if ((rc = upload_goals(e, input, goals)) != MPM_OK) return rc;
-    SubstepParams S = make_params(e, g, fb, goals, input && input->has_b);
-    S.epoch = e->epoch;
+    std::vector<SubstepParams> one{make_params(e, g, fb, goals, input && input->has_b)};
+    if ((rc = upload_params(e, one)) != MPM_OK) return rc;
     switch (stage) {
         case 0: rc = run_sort(e); break;
-        case 1: rc = run_transfer(e, TransferKind::P2G, S, false); break;
-        case 2: rc = run_grid(e, S); break;
+        case 1: rc = run_transfer(e, TransferKind::P2G, goals, false); break;
+        case 2: rc = run_grid(e); break;
         case 3:
-            rc = run_transfer(e, TransferKind::G2P, S, true);
+            rc = run_transfer(e, TransferKind::G2P, false, true);
             if (rc == MPM_OK) {
                 e->time += (double)e->consts.time_step;
                 e->stepped = true;

@@ -80,7 +80,9 @@ struct DeviceScalars {
     uint32_t status;        // MPM_STATUS_* bits | kernel id << 16
     uint32_t active_nodes;  // nodes with m > 0 after the last grid update (summed on request)
     uint32_t ovf_count[2];  // overflow-list lengths, by parity of the grid's epoch
-    uint32_t pad[2];
+    uint32_t param_index;   // entry of the substep parameter table the next transfer reads
+                            // (the grid update sets it to its own entry + 1)
+    uint32_t grid_param;    // entry the next grid update reads (the transfer sets it)
 };
 
 // Candidate blocks listed per block-list entry for the grid update: the 27
@@ -130,6 +132,7 @@ struct SubstepParams {
     // scene_origin + (k + 1) period) along each axis, k = 0 .. scene_last.
     float scene_period[3], scene_inv_period[3], scene_last[3], scene_origin[3];
     uint32_t epoch;    // substep counter, stamps grid blocks
+    uint32_t index;    // this entry's index in the substep parameter table
 };
 
 // Coordinate a of a position within its own scene of a batch (the domain
@@ -170,6 +173,7 @@ hipError_t launch_reset_scalars(DeviceScalars *ds, hipStream_t stream);
 struct TransferLaunch {
     int persistent_blocks;  // work-groups of the block-loop kernels; 0: all resident ones
     uint32_t model_set;     // SET_POLAR or SET_ALL
+    bool batched;           // batched scenes (mpm_consts.scene_repeat): the M_BATCH kernels
     int cu_count;
 };
 
@@ -185,7 +189,7 @@ struct GridBuffers {
     uint32_t *blk_epoch;        // epoch of the block's last grid update
     uint32_t *cand;             // NCAND candidate blocks per list entry (a block may be listed
                                 // by several entries)
-    uint32_t *ovf;              // overflow list of claimed blocks (parity of the epoch)
+    uint32_t *ovf[2];           // overflow lists of claimed blocks, by parity of the epoch
 };
 
 struct TransferCall {
@@ -197,9 +201,11 @@ struct TransferCall {
     const uint32_t *blk_start;
     const uint32_t *blk_key;
     GridGeom G;
-    SubstepParams S1;           // substep of the scatter (the gather's dt, domain and
-                                // plasticity constants are the same every substep)
-    GridBuffers B;              // B.ovf: the list for S1's grid (parity S1.epoch & 1)
+    const SubstepParams *params; // the engine's table, read at entry ds->param_index: the
+                                 // substep of the scatter (the gather's dt, domain and
+                                 // plasticity constants are the same every substep)
+    bool goals;                 // that substep has goal positions
+    GridBuffers B;
     uint32_t *keys_out;         // null: keys not needed
     DeviceScalars *ds;
     const float *tau_ext = nullptr;  // P2G: snow stress from the learned closure (rows of tau_stride)
@@ -207,8 +213,8 @@ struct TransferCall {
 };
 hipError_t launch_transfer(const TransferLaunch &L, const TransferCall &t, hipStream_t stream);
 uint32_t grid_update_waves(const TransferLaunch &L);
-hipError_t launch_grid_update(const TransferLaunch &L, const GridBuffers &B, const GridGeom &G, const SubstepParams &S,
-                              uint32_t *partial, DeviceScalars *ds, hipStream_t stream);
+hipError_t launch_grid_update(const TransferLaunch &L, const GridBuffers &B, const GridGeom &G,
+                              const SubstepParams *params, uint32_t *partial, DeviceScalars *ds, hipStream_t stream);
 hipError_t launch_sum_partials(const TransferLaunch &L, const uint32_t *partial, DeviceScalars *ds,
                                hipStream_t stream);
 hipError_t launch_copy_dead(const ParticleArrays &Pin, const ParticleArrays &Pout, const uint32_t *order,

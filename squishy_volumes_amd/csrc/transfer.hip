@@ -300,13 +300,27 @@ struct XferArgs {
     const uint32_t *blk_key;
     GridGeom G;
 
-    SubstepParams S1;            // the substep whose scatter runs (P2G modes)
-    GridBuffers B;               // B.ovf: the overflow list for S1's grid
+    const SubstepParams *params; // parameters of substep ds->param_index (the substep whose
+                                 // scatter runs, P2G modes)
+    GridBuffers B;               // B.ovf[epoch & 1]: the overflow list for that substep's grid
     uint32_t *keys_out;          // block keys of the new positions, or null
     DeviceScalars *ds;
     const float *tau_ext;        // P2G mode: Kirchhoff stress of snow particles by slot (9 rows
     uint64_t tau_stride;         // of tau_stride floats), from the learned closure; or null
 };
+
+// The substep parameters do not change while a kernel runs: read through the
+// constant address space, the compiler may load them again where it needs
+// them (as it does kernel arguments) instead of holding them in registers.
+__device__ __forceinline__ SubstepParams load_invariant(const SubstepParams *p) {
+    static_assert(sizeof(SubstepParams) % 4 == 0, "whole words");
+    SubstepParams s;
+    const __attribute__((address_space(4))) uint32_t *w = (const __attribute__((address_space(4))) uint32_t *)p;
+    uint32_t *d = reinterpret_cast<uint32_t *>(&s);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(SubstepParams) / 4); ++i) d[i] = w[i];
+    return s;
+}
 
 // Scatter of one particle straight into the global accumulation grid, for a
 // particle outside its block's region; claims the blocks it touches for the
@@ -347,7 +361,7 @@ __device__ __forceinline__ void scatter_global(const GridBuffers &B, const GridG
                 const uint32_t T = block_linear(G, (uint32_t)bx, (uint32_t)by, (uint32_t)bz);
                 if (atomicExch(&B.blk_flag[T], 1u) == 0u) {
                     const uint32_t slot = atomicAdd(&ds->ovf_count[epoch & 1u], 1u);
-                    if (slot < G.num_blocks_total) B.ovf[slot] = T;
+                    if (slot < G.num_blocks_total) B.ovf[epoch & 1u][slot] = T;
                 }
             }
 }
@@ -502,8 +516,14 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
     const uint32_t nblocks = ds->num_blocks;
     const uint32_t live = ds->live_count;
     const float h = G.h;
-    const float dt = a.S1.dt;
-    const float scaling = a.S1.dt * 4.0f / (h * h);
+    const uint32_t pidx = ds->param_index;
+    const SubstepParams S1 = load_invariant(a.params + pidx);
+    // The grid update that follows reads the same entry.  Each of the two
+    // kernels hands the index on to the other in a slot the other kernel
+    // alone reads, so no work-group races a reader of its own launch.
+    if (blockIdx.x == 0 && tid == 0) ds->grid_param = pidx;
+    const float dt = S1.dt;
+    const float scaling = S1.dt * 4.0f / (h * h);
     const float apic = 4.0f / h;
 
     // The work-group visits its block-list entries j = 0, 1, ... (list index
@@ -703,14 +723,14 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
                         float jp = a.X.jp[id];
 #pragma unroll
                         for (int q = 0; q < 3; ++q) {
-                            const float cl = fminf(fmaxf(sv[q], a.S1.snow_lo), a.S1.snow_hi);
+                            const float cl = fminf(fmaxf(sv[q], S1.snow_lo), S1.snow_hi);
                             jp *= sv[q] / cl;
                             sv[q] = cl;
                         }
                         a.X.jp[id] = jp;
                         recompose(U, sv, V, F);
                         J = sv[0] * sv[1] * sv[2];
-                        hard = expf(a.S1.snow_hardening * (1.0f - jp));
+                        hard = expf(S1.snow_hardening * (1.0f - jp));
                         if (LAST) energy = energy_fixed_corotated(p0 * hard, p1 * hard, sv);
                     } else {
                         if ((SET & M_SAND) && (flags & F_SAND)) {
@@ -756,9 +776,11 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
 
                     // cull_particles: strict inequalities.
 #pragma unroll
-                    for (int q = 0; q < 3; ++q)
-                        inside = inside && scene_local(xp[q], q, a.S1) > a.S1.domain_min[q] &&
-                                 scene_local(xp[q], q, a.S1) < a.S1.domain_max[q];
+                    for (int q = 0; q < 3; ++q) {
+                        // batched scenes: within the particle's own copy of the domain
+                        const float xl = (SET & M_BATCH) ? scene_local(xp[q], q, S1) : xp[q];
+                        inside = inside && xl > S1.domain_min[q] && xl < S1.domain_max[q];
+                    }
                     if (!inside) flags |= F_TOMB;
 
                     if (a.keys_out) a.keys_out[i] = inside ? particle_block_key(xp, G, ds) : G.tomb_key;
@@ -798,7 +820,7 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
                     for (int q = 0; q < 3; ++q) v[q] = Pin.v(q, src);
 #pragma unroll
                     for (int k = 0; k < 9; ++k) C[k] = Pin.A(k, src);
-                    affine_from_state<SET>(flags, F, C, m, vol, p0, p1, id, a.X, scaling, a.S1.snow_hardening, ds,
+                    affine_from_state<SET>(flags, F, C, m, vol, p0, p1, id, a.X, scaling, S1.snow_hardening, ds,
                                            a.tau_ext, a.tau_stride, i, A);
                     if (PERMUTE) {
                         for (int f = 0; f < NFIELDS; ++f) Pout.at(i)[f * TILE] = Pin.at(src)[f * TILE];
@@ -809,7 +831,7 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
                     // external_force of the substep that starts, then the payload
                     //   (f, m | q | h A columns): contribution at stencil node (i,j,k)
                     //   w_ijk (m v + A d) = w_ijk (q + i c0 + j c1 + k c2).
-                    external_velocity<GOALS>(a.S1, a.X, id, xp, v);
+                    external_velocity<GOALS>(S1, a.X, id, xp, v);
                     float pl[16];
                     int c[3];
                     bool in_region = true;
@@ -861,7 +883,7 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
                         dst[3 * stride] = make_float4(pl[12], pl[13], pl[14], pl[15]);
                     } else {
                         const int gc[3] = {c[0] + 4 * (int)b[0], c[1] + 4 * (int)b[1], c[2] + 4 * (int)b[2]};
-                        scatter_global(a.B, G, a.S1.epoch, ds, gc, pl);
+                        scatter_global(a.B, G, S1.epoch, ds, gc, pl);
                     }
                 }
             }
@@ -1033,8 +1055,12 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
 // the transfer kernel (NCAND per block-list entry, then the overflow list)
 // are read 64 at a time, and each flagged block is processed once: v = p / m,
 // and the accumulation grid is cleared for the next scatter.
-__global__ __launch_bounds__(256) void k_grid_update(GridBuffers B, GridGeom G, SubstepParams S,
+__global__ __launch_bounds__(256) void k_grid_update(GridBuffers B, GridGeom G, const SubstepParams *params,
                                                      uint32_t *__restrict__ partial, DeviceScalars *ds) {
+    const SubstepParams S = load_invariant(params + ds->grid_param);
+    const uint32_t *ovf = B.ovf[S.epoch & 1u];
+    // The next transfer reads the next substep's entry.
+    if (blockIdx.x == 0 && threadIdx.x == 0) ds->param_index = S.index + 1u;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gw = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
     const uint32_t nw = gridDim.x * (blockDim.x / 64);
@@ -1046,7 +1072,7 @@ __global__ __launch_bounds__(256) void k_grid_update(GridBuffers B, GridGeom G, 
     uint32_t active = 0;
     for (uint32_t chunk = gw; chunk * 64u < ncand; chunk += nw) {
         const uint32_t c = chunk * 64u + lane;
-        uint32_t mine = c < nlist ? B.cand[c] : (c < ncand ? B.ovf[c - nlist] : NO_BLOCK);
+        uint32_t mine = c < nlist ? B.cand[c] : (c < ncand ? ovf[c - nlist] : NO_BLOCK);
         // A block may be listed by several entries: the one whose exchange
         // finds the flag set processes it (and clears the flag).
         if (mine != NO_BLOCK && atomicExch(&B.blk_flag[mine], 0u) == 0u) mine = NO_BLOCK;
@@ -1219,20 +1245,27 @@ __global__ __launch_bounds__(256) void k_svd(const float *__restrict__ Fin, floa
     for (int q = 0; q < 3; ++q) so[3 * i + q] = s[q];
 }
 
-// Work-groups of a persistent launch: all resident at once (the occupancy of
-// the kernel times the CU count), a multiple of 8 for the XCD mapping.
-int resident_groups(const void *kernel, int threads, const TransferLaunch &L) {
-    if (L.persistent_blocks > 0) return std::max(8, L.persistent_blocks & ~7);
+int groups_per_cu(const void *kernel, int threads) {
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess || per_cu < 1)
         per_cu = 1;
+    return per_cu;
+}
+
+// Work-groups of a persistent launch: the engine's persistent_blocks, or all
+// resident at once (the occupancy of the kernel times the CU count); a
+// multiple of 8 for the XCD mapping.  Each work-group takes a fixed share of
+// the block list, so any count is correct.
+int resident_groups(int per_cu, const TransferLaunch &L) {
+    if (L.persistent_blocks > 0) return std::max(8, L.persistent_blocks & ~7);
     return std::max(8, (per_cu * L.cu_count) & ~7);
 }
 
 template <int MODE, bool PERMUTE, bool GOALS, uint32_t SET>
 hipError_t launch_one(const TransferLaunch &L, const XferArgs &a, hipStream_t stream) {
-    static const int groups =
-        resident_groups(reinterpret_cast<const void *>(&k_transfer<MODE, PERMUTE, GOALS, SET>), XF_THREADS, L);
+    static const int per_cu =
+        groups_per_cu(reinterpret_cast<const void *>(&k_transfer<MODE, PERMUTE, GOALS, SET>), XF_THREADS);
+    const int groups = resident_groups(per_cu, L);
     hipLaunchKernelGGL((k_transfer<MODE, PERMUTE, GOALS, SET>), dim3(groups), dim3(XF_THREADS), 0, stream, a);
     return hipGetLastError();
 }
@@ -1250,6 +1283,10 @@ hipError_t dispatch_flags(const TransferLaunch &L, bool permute, bool goals, con
 
 template <int MODE>
 hipError_t dispatch_set(const TransferLaunch &L, bool permute, bool goals, const XferArgs &a, hipStream_t stream) {
+    if (L.batched) {
+        if (L.model_set == SET_POLAR) return dispatch_flags<MODE, SET_POLAR | M_BATCH>(L, permute, goals, a, stream);
+        return dispatch_flags<MODE, SET_ALL | M_BATCH>(L, permute, goals, a, stream);
+    }
     if (L.model_set == SET_POLAR) return dispatch_flags<MODE, SET_POLAR>(L, permute, goals, a, stream);
     return dispatch_flags<MODE, SET_ALL>(L, permute, goals, a, stream);
 }
@@ -1266,13 +1303,13 @@ hipError_t launch_transfer(const TransferLaunch &L, const TransferCall &t, hipSt
     a.blk_key = t.blk_key;
     a.G = t.G;
 
-    a.S1 = t.S1;
+    a.params = t.params;
     a.B = t.B;
     a.keys_out = t.keys_out;
     a.ds = t.ds;
     a.tau_ext = t.tau_ext;
     a.tau_stride = t.tau_stride;
-    const bool goals = t.S1.has_goals != 0;
+    const bool goals = t.goals;
     switch (t.kind) {
         case TransferKind::P2G: return dispatch_set<XM_P2G>(L, t.permute, goals, a, stream);
         case TransferKind::G2P2G: return dispatch_set<XM_G2P2G>(L, t.permute, goals, a, stream);
@@ -1286,9 +1323,9 @@ hipError_t launch_transfer(const TransferLaunch &L, const TransferCall &t, hipSt
 uint32_t grid_update_groups(const TransferLaunch &L) { return (uint32_t)std::max(8, 4 * L.cu_count); }
 uint32_t grid_update_waves(const TransferLaunch &L) { return grid_update_groups(L) * 4u; }
 
-hipError_t launch_grid_update(const TransferLaunch &L, const GridBuffers &B, const GridGeom &G, const SubstepParams &S,
-                              uint32_t *partial, DeviceScalars *ds, hipStream_t stream) {
-    hipLaunchKernelGGL(k_grid_update, dim3(grid_update_groups(L)), dim3(256), 0, stream, B, G, S, partial, ds);
+hipError_t launch_grid_update(const TransferLaunch &L, const GridBuffers &B, const GridGeom &G,
+                              const SubstepParams *params, uint32_t *partial, DeviceScalars *ds, hipStream_t stream) {
+    hipLaunchKernelGGL(k_grid_update, dim3(grid_update_groups(L)), dim3(256), 0, stream, B, G, params, partial, ds);
     return hipGetLastError();
 }
 

@@ -203,3 +203,73 @@ def snow_block_on_sphere(*, h: float = 1.0 / 128, target_particles: int = 1_000_
         substeps=substeps,
         info={"particles": p.count, "block_cells": side, "E": E, "nu": nu, "h": h},
     )
+
+
+def random_objects_scene(target_particles: int, *, seed: int = 0, h: float = 1.0 / 128, dt: float = 5e-5,
+                         substeps: int = 200, fps: int = 1, max_objects: int = 4) -> Scene:
+    """1 to max_objects fixed-corotated cubes and spheres (rho 1000, E
+    log-uniform in [1e4, 1e6], nu 0.3, random velocities up to 1 m/s) that
+    together hold about target_particles, placed without overlap in a unit
+    box with separating-slip walls, 8 particles per cell."""
+    rng = np.random.default_rng(seed)
+    wall = 4 * h
+    k = int(rng.integers(1, max_objects + 1))
+    shares = rng.dirichlet(np.ones(k)) * target_particles
+    placed = []  # (centre, bounding radius)
+    parts = []
+    for share in shares:
+        cells = max(share / 8.0, 8.0)
+        cube = bool(rng.random() < 0.5)
+        if cube:
+            side = max(2, int(round(cells ** (1.0 / 3.0))))
+            half = 0.5 * side * h
+            bound = half * np.sqrt(3.0)
+        else:
+            radius = (3.0 * cells / (4.0 * np.pi)) ** (1.0 / 3.0) * h
+            bound = radius
+        lo, hi = wall + 2 * h + bound, 1.0 - wall - 2 * h - bound
+        if lo >= hi:
+            continue
+        for _ in range(50):
+            c = rng.uniform(lo, hi, 3)
+            if all(np.linalg.norm(c - c2) > bound + b2 + 2 * h for c2, b2 in placed):
+                break
+        else:
+            continue
+        placed.append((c, bound))
+        if cube:
+            c0 = np.floor(c / h - 0.5 * side).astype(np.int64)
+            pts = jittered_cells(box_cells(c0, c0 + side), h, rng)
+        else:
+            pts = sphere_points(c, radius, h, rng)
+        E = float(np.exp(rng.uniform(np.log(1e4), np.log(1e6))))
+        vel = rng.uniform(-1.0, 1.0, 3)
+        vel *= rng.uniform(0.0, 1.0) / max(np.linalg.norm(vel), 1e-9)
+        parts.append(particles_from(pts, abi.IS_SOLID | abi.FIXED_COROTATED, 1000.0, h, vel, E=E, nu=0.3))
+    allp = concat(parts)
+    consts = InputConsts(grid_node_size=h, frames_per_second=fps, domain_min=(0.0, 0.0, 0.0), domain_max=(1.0, 1.0, 1.0))
+    opts = EngineOptions(box_walls=((wall, wall, wall), (1.0 - wall, 1.0 - wall, 1.0 - wall)))
+    return Scene(
+        name=f"random_objects_{seed}",
+        consts=consts,
+        time_step=dt,
+        options=opts,
+        io_state=IoState(time=0.0, particles=allp),
+        frame_input=FrameInput(frame=0, gravity_a=(0.0, 0.0, -9.81)),
+        substeps=substeps,
+        info={"particles": allp.count, "objects": len(parts), "h": h},
+    )
+
+
+def uneven_scene_set(count: int = 64, lo: int = 5_000, hi: int = 500_000, seed: int = 0,
+                     total: Optional[int] = None, **kw) -> list[Scene]:
+    """`count` random-object scenes whose particle counts are drawn
+    log-uniform in [lo, hi] and, with `total`, scaled (and clipped to
+    [lo, hi]) so that they add up to about `total` (the batched workload: 64
+    scenes, 8M particles in all)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), count))
+    if total is not None:
+        for _ in range(20):
+            sizes = np.clip(sizes * (total / sizes.sum()), lo, hi)
+    return [random_objects_scene(int(s), seed=seed * 1000 + i, **kw) for i, s in enumerate(sizes)]

@@ -69,3 +69,16 @@ def test_engine_rejects_a_period_that_overlaps_the_copies():
     h = C.c_void_p()
     assert lib.mpm_create(C.byref(c), C.byref(view), C.byref(t), C.byref(h)) == abi.ERR_INVALID_ARGUMENT
     assert not h.value
+
+
+def test_uneven_scene_set_sizes_and_placement():
+    scs = scenes.uneven_scene_set(count=12, lo=2000, hi=60000, seed=5, h=1.0 / 64)
+    for sc in scs:
+        p = sc.io_state.particles
+        assert 0.5 * 2000 <= p.count <= 1.5 * 60000
+        wall = sc.options.box_walls
+        assert np.all(p.positions > np.asarray(wall[0])) and np.all(p.positions < np.asarray(wall[1]))
+        E = 2 * p.parameters["mu"] * 1.3
+        assert np.all((E > 0.99e4) & (E < 1.01e6))
+    b = Batch(scs)
+    assert b.repeat == (3, 3, 2)

@@ -66,3 +66,26 @@ def test_culling_acts_in_each_scene_copy():
         assert int((dead != dead1).sum()) <= 2
         live = ~dead & ~dead1
         assert rel_norm_err(p.positions[live], alone.particles.positions[live]) <= 1e-4
+
+
+def test_autotuner_picks_the_fastest_measured_tuning_and_caches_it(tmp_path):
+    from squishy_volumes_amd import autotune
+
+    b = Batch(uneven([500, 4000, 1728]))
+    cache = tmp_path / "tune.json"
+    r = autotune.autotune(b.scene, resort_intervals=(4, 8), persistent_blocks=(0, 64), graph=(0,), warmup=2,
+                          substeps=6, cache=cache)
+    assert len(r.trials) == 4 and not r.cached
+    assert r.ms_per_substep == min(ms for _, ms in r.trials)
+    assert r.best in [t for t, _ in r.trials]
+    again = autotune.autotune(b.scene, cache=cache)
+    assert again.cached and again.best == r.best
+    # any tuning gives the same physics up to round-off
+    T = scenes.advance_time(0.0, b.scene.time_step, 30)
+    runs = []
+    for tuning in ({"resort_interval": 4, "persistent_blocks": 64}, {"resort_interval": 16, "persistent_blocks": 2048}):
+        st, err = engine(b.scene, tuning=tuning).produce_next_state(b.frame_input, GpuRunParameters(target_time=T))
+        assert err is None
+        runs.append(st.particles)
+    assert rel_norm_err(runs[0].positions, runs[1].positions) <= 1e-5
+    assert rel_norm_err(runs[0].velocities, runs[1].velocities) <= 1e-4
