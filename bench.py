@@ -284,6 +284,121 @@ def batched_form(args) -> dict:
     }
 
 
+def combined_form(args) -> dict:
+    """128 batched scenes of elastic, snow and fluid objects (10k..1M
+    particles each, log-uniform, scaled to 16M in all; h = 1/128, dt = 5e-5),
+    300 substeps: the plain batched form against the batched form with the
+    learned closure shared by all scenes, pre-trained on pairs from the first
+    100 plain substeps and fine-tuned on the GPU every 50 substeps on pairs
+    from the current state (the fine-tuning inside the timed region)."""
+    import numpy as np
+    import torch
+
+    from squishy_volumes_amd import abi, closure, scenes
+    from squishy_volumes_amd.batch import Batch
+    from squishy_volumes_amd.gpu_state import GpuState
+
+    scs = scenes.mixed_scene_set(args.combined_scenes, total=args.combined_particles)
+    b = Batch(scs)
+    sc = b.scene
+    n = b.count
+    flags = sc.io_state.particles.flags
+    n_snow = int(((flags & abi.SNOW) != 0).sum())
+    steps, every = args.combined_substeps, 50
+    tuning = {"resort_interval": args.resort_interval or 16, "persistent_blocks": args.persistent_blocks or 2048}
+    adv = lambda g, k: g.step_to(scenes.advance_time(g.time, sc.time_step, k), b.frame_input)
+    make = lambda: GpuState.from_io_state(0, sc.consts, sc.time_step, sc.io_state, sc.options, tuning)
+    log(f"combined: {len(scs)} scenes, {n} particles ({n_snow} snow), repeat {b.repeat}")
+
+    # Pre-training: pairs from the first 100 plain substeps.
+    stride = max(1, (4 * n_snow) // args.pairs)
+    g0 = make()
+    xs, ts = [], []
+    for k in range(4):
+        adv(g0, 25)
+        x, t = g0.closure_pairs(stride, k % stride)
+        xs.append(x)
+        ts.append(t)
+    g0.close()
+    x, t = torch.cat(xs), torch.cat(ts)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    net = closure.Mlp(seed=0)
+    tlog = net.fit(x, t, epochs=args.combined_epochs, batch=8192, lr=3e-3, lr_final=1e-4)
+    torch.cuda.synchronize()
+    pre_s = time.perf_counter() - t0
+    hull = closure.ClosureModel.hull_of(x)
+    log(f"combined: pre-trained on {x.shape[0]} pairs in {pre_s:.1f} s, mse {tlog.epochs[0]:.2e} -> {tlog.epochs[-1]:.2e}")
+
+    # The plain batched form.
+    gp = make()
+    adv(gp, args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = adv(gp, steps)
+    torch.cuda.synchronize()
+    el_plain = time.perf_counter() - t0
+    assert st.result == 0 and st.substeps == steps
+    nodes = int(st.active_nodes)
+    gp.close()
+    log(f"combined: plain batched {steps} substeps in {el_plain:.3f} s")
+
+    # The learned form, fine-tuned online.
+    ft_stride = max(1, n_snow // args.finetune_pairs)
+    gl = make()
+    gl.set_closure(net.device_model(hull), judge_stride=64)
+    adv(gl, args.warmup)
+    gl.closure_stats(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ft_s, ft_pairs, ft_log = 0.0, 0, []
+    for c in range(steps // every):
+        st = adv(gl, every)
+        assert st.result == 0
+        if c == steps // every - 1:
+            break
+        t1 = time.perf_counter()
+        x, t = gl.closure_pairs(ft_stride, c % ft_stride)
+        flog = net.fit(x, t, epochs=args.finetune_epochs, batch=8192, lr=3e-4, lr_final=1e-4, seed=c + 1)
+        hull = closure.ClosureModel.hull_union(hull, closure.ClosureModel.hull_of(x))
+        gl.set_closure(net.device_model(hull), judge_stride=64)
+        ft_s += time.perf_counter() - t1
+        ft_pairs += int(x.shape[0])
+        ft_log.append(flog.epochs[-1])
+    torch.cuda.synchronize()
+    el_learned = time.perf_counter() - t0
+    stats = gl.closure_stats(reset=True)
+    gl.close()
+    log(f"combined: learned batched {steps} substeps in {el_learned:.3f} s ({ft_s:.3f} s fine-tuning), "
+        f"stress error {stats['relative_stress_error']:.2e}")
+    return {
+        "metric": "particle-substeps/s (combined: batched scenes with the learned closure, fine-tuned online)",
+        "value": n * steps / el_learned,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": el_learned / steps * 1e3,
+        "higher_is_better": True,
+        "dtype": "f32 state, bf16 MLP with f32 accumulation",
+        "data": "synthetic (seeded elastic, snow and fluid objects)",
+        "config": {"workload": f"mixed_scenes_{len(scs)}", "scenes": len(scs), "particles": n, "snow_particles": n_snow,
+                   "h": sc.consts.grid_node_size, "dt": sc.time_step, "scene_repeat": list(b.repeat),
+                   "active_nodes": nodes, "tuning": tuning},
+        "plain_batched": {"value": n * steps / el_plain, "ms_per_step": el_plain / steps * 1e3},
+        "learned_vs_plain": el_plain / el_learned,
+        "stress_error_vs_analytic_f32": stats["relative_stress_error"],
+        "network_evaluations": stats["evaluated"],
+        "hull_fallbacks": stats["fallbacks"],
+        "judged": stats["judged"],
+        "pretraining": {"pairs": int(sum(v.shape[0] for v in xs)), "epochs": args.combined_epochs, "seconds": pre_s,
+                        "mse_first_epoch": tlog.epochs[0], "mse_last_epoch": tlog.epochs[-1]},
+        "finetuning": {"every_substeps": every, "rounds": len(ft_log), "pairs": ft_pairs,
+                       "epochs_per_round": args.finetune_epochs, "seconds_in_timed_region": ft_s,
+                       "mse_last_epoch_per_round": ft_log},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -301,9 +416,16 @@ def main():
         help="first launch the plain copy kernel (k_copy_bytes) three times over a buffer of this many MiB, "
         "outside the timed region, so that hardware counters can be scaled by a copy of known size",
     )
-    ap.add_argument("--form", choices=("plain", "learned", "batched"), default="plain",
+    ap.add_argument("--form", choices=("plain", "learned", "batched", "combined"), default="plain",
                     help="plain: the headline 4.2M-particle substep; learned: the learned closure on the snow "
-                    "scene; batched: 64 uneven scenes in one engine against one engine per scene")
+                    "scene; batched: 64 uneven scenes in one engine against one engine per scene; combined: 128 "
+                    "mixed-material scenes with the learned closure fine-tuned online against the plain batched form")
+    ap.add_argument("--combined-scenes", type=int, default=128)
+    ap.add_argument("--combined-particles", type=int, default=16_000_000)
+    ap.add_argument("--combined-substeps", type=int, default=300)
+    ap.add_argument("--combined-epochs", type=int, default=150)
+    ap.add_argument("--finetune-pairs", type=int, default=200_000)
+    ap.add_argument("--finetune-epochs", type=int, default=3)
     ap.add_argument("--scenes", type=int, default=64)
     ap.add_argument("--batch-substeps", type=int, default=200)
     ap.add_argument("--batch-particles", type=int, default=8_000_000)
@@ -335,6 +457,9 @@ def main():
         return
     if args.form == "batched":
         print(json.dumps(batched_form(args)))
+        return
+    if args.form == "combined":
+        print(json.dumps(combined_form(args)))
         return
 
     scene = scenes.spheres_and_slab()

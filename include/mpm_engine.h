@@ -327,7 +327,9 @@ int mpm_mlp_adam(float *params, float *grad, float *m, float *v, float lr, float
  * substep) takes both and the difference goes into the statistics.  A
  * substep then runs as a gather writing the state, the sort when due, the
  * closure, a scatter from the state and the grid update.  No counterpart in
- * the reference.  params is a host pointer (copied); NULL turns it off. */
+ * the reference.  params is a host or device pointer (copied on the engine's
+ * stream; device memory must be complete when the call is made); NULL turns
+ * it off. */
 typedef struct mpm_closure {
     const float *params;             /* mpm_mlp_param_count() floats */
     float hull_lo[6], hull_hi[6];
@@ -342,6 +344,17 @@ typedef struct mpm_closure_stats {
 int mpm_set_closure(mpm_engine *engine, const mpm_closure *closure);
 /* Statistics since the last reset. */
 int mpm_read_closure_stats(mpm_engine *engine, int32_t reset, mpm_closure_stats *out);
+/* Training pairs from the engine's current state, for fine-tuning the
+ * closure online: every live snow particle whose id % stride == phase gives
+ * one row of network inputs x (6 floats) and targets t (3 floats: the
+ * analytic principal Kirchhoff stresses 2 mu h (s - 1) s + lambda h (J - 1) J,
+ * h = exp(xi (1 - Jp)), in the output units above).  x and t are device
+ * arrays of capacity rows; *count receives the number of rows (in an
+ * unspecified order); MPM_ERR_CAPACITY when it exceeds the capacity (the
+ * first capacity rows are written).  Synchronizes the engine's stream.  No
+ * counterpart in the reference. */
+int mpm_closure_pairs(mpm_engine *engine, uint32_t stride, uint32_t phase, float *x, float *t, uint64_t capacity,
+                      uint64_t *count);
 
 /* One substep stage at a time on an engine, for the per-kernel parity tests:
  * stage 0 = sort, 1 = P2G from the stored state (stress from F and C),

@@ -220,6 +220,8 @@ SIGNATURES = {
     ),
     "mpm_set_closure": (C.c_int, [C.c_void_p, C.POINTER(MpmClosure)]),
     "mpm_read_closure_stats": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(MpmClosureStats)]),
+    "mpm_closure_pairs": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint64,
+                                    C.POINTER(C.c_uint64)]),
     "mpm_run_stage": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(MpmFrameInput)]),
     "mpm_read_grid_momentum": (C.c_int, [C.c_void_p, C.POINTER(MpmGridView)]),
 }
@@ -242,6 +244,14 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
             f"{p} is missing: build the HIP engine first (python -m squishy_volumes_amd.build); "
             "the engine has no CPU fallback"
         )
+    # One HIP runtime per process: torch's build bundles its own
+    # libamdhip64.so.7 under the same soname as ROCm's.  Whichever is loaded
+    # first serves both, and torch's GPU initialisation fails on any runtime
+    # but its own, so torch (when installed) is loaded before the engine.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(str(p))
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

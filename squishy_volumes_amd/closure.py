@@ -131,9 +131,17 @@ class ClosureModel:
         width and at least floor (an input that is constant in the training
         set, such as E of a one-material scene, is computed on the device in
         FP32 and may differ from the host's value in the last bits)."""
-        lo, hi = x.min(axis=0).astype(np.float64), x.max(axis=0).astype(np.float64)
+        if hasattr(x, "amin"):  # a device tensor: the range on the device
+            lo, hi = x.amin(0).double().cpu().numpy(), x.amax(0).double().cpu().numpy()
+        else:
+            lo, hi = x.min(axis=0).astype(np.float64), x.max(axis=0).astype(np.float64)
         pad = np.maximum(margin * (hi - lo), floor)
         return (lo - pad).astype(np.float32), (hi + pad).astype(np.float32)
+
+    @staticmethod
+    def hull_union(a, b):
+        """The smallest box holding the hulls a = (lo, hi) and b."""
+        return np.minimum(a[0], b[0]), np.maximum(a[1], b[1])
 
     def save(self, path):
         np.savez(path, params=self.params, hull_lo=self.hull_lo, hull_hi=self.hull_hi)
@@ -210,6 +218,12 @@ class Mlp:
     def model(self, x_train) -> ClosureModel:
         lo, hi = ClosureModel.hull_of(x_train)
         return ClosureModel(self.params.cpu().numpy().copy(), lo, hi)
+
+    def device_model(self, hull) -> ClosureModel:
+        """The current parameters (a device copy, handed to the engine without
+        a trip through host memory) with the training hull `hull` = (lo, hi)."""
+        return ClosureModel(self.params.detach().clone(), np.asarray(hull[0], np.float32),
+                            np.asarray(hull[1], np.float32))
 
     def fit(self, x, t, epochs: int, batch: int = 4096, lr: float = 2e-3, lr_final: float = 1e-4, seed: int = 0) -> TrainLog:
         """Mini-batch Adam on (x, t) (device tensors), learning rate decaying

@@ -460,7 +460,73 @@ __global__ __launch_bounds__(256) void k_closure(ClosureArgs a) {
     }
 }
 
+// Training pairs from the current state: every live snow particle with
+// id % stride == phase contributes its network inputs (as the closure kernel
+// forms them) and, as the target, its analytic principal Kirchhoff stresses
+// 2 mu h (s - 1) s + lambda h (J - 1) J in units of E h CLOSURE_STRESS_SCALE
+// (h the hardening factor): what the closure replaces.  A wave reserves its
+// rows with one atomic; rows past the capacity are counted, not written.
+__global__ __launch_bounds__(256) void k_closure_pairs(ParticleArrays P, AuxArrays X, uint64_t n, float hardening,
+                                                       uint32_t stride, uint32_t phase, float *__restrict__ xo,
+                                                       float *__restrict__ to, uint64_t capacity,
+                                                       unsigned long long *count) {
+    const int l = (int)(threadIdx.x & 63u);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i - (uint64_t)l < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        bool take = false;
+        float x[MLP_IN] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, t[3] = {0.f, 0.f, 0.f};
+        if (i < n) {
+            const uint32_t flags = P.flags(i);
+            const uint32_t id = P.id(i);
+            if ((flags & F_SNOW) && !(flags & F_TOMB) && id % stride == phase) {
+                float F[9], U[9], s[3], V[9];
+#pragma unroll
+                for (int k = 0; k < 9; ++k) F[k] = P.F(k, i);
+                svd3(F, U, s, V);
+                const float mu = P.p0(i), lam = P.p1(i);
+                const float jp = X.jp[id];
+                const float hard = expf(hardening * (1.0f - jp));
+                const float E = mu * (3.f * lam + 2.f * mu) / (lam + mu);
+                const float nu = lam / (2.f * (lam + mu));
+#pragma unroll
+                for (int q = 0; q < 3; ++q) x[q] = (s[q] - 1.0f) * (1.0f / CLOSURE_SIGMA_SCALE);
+                x[3] = log10f(E * 1e-5f);
+                x[4] = (nu - 0.3f) * 10.0f;
+                x[5] = (jp - 1.0f) * (1.0f / CLOSURE_JP_SCALE);
+                const float J = s[0] * s[1] * s[2];
+                const float unit = E * hard * CLOSURE_STRESS_SCALE;
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+                    t[q] = (2.f * mu * hard * (s[q] - 1.f) * s[q] + lam * hard * (J - 1.f) * J) / unit;
+                take = true;
+            }
+        }
+        const uint64_t mask = __ballot(take);
+        if (!mask) continue;
+        unsigned long long base = 0;
+        if (l == __ffsll((unsigned long long)mask) - 1) base = atomicAdd(count, (unsigned long long)__popcll(mask));
+        base = __shfl(base, __ffsll((unsigned long long)mask) - 1, 64);
+        const uint64_t row = base + (uint64_t)__popcll(mask & ((1ull << l) - 1ull));
+        if (take && row < capacity) {
+#pragma unroll
+            for (int q = 0; q < MLP_IN; ++q) xo[row * MLP_IN + q] = x[q];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) to[row * 3 + q] = t[q];
+        }
+    }
+}
+
 }  // namespace
+
+hipError_t launch_closure_pairs(const ParticleArrays &P, const AuxArrays &X, uint64_t n, float hardening,
+                                uint32_t stride, uint32_t phase, float *x, float *t, uint64_t capacity,
+                                unsigned long long *count, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_closure_pairs, dim3(blocks), dim3(256), 0, stream, P, X, n, hardening, stride, phase, x, t,
+                       capacity, count);
+    return hipGetLastError();
+}
 
 hipError_t launch_closure(const ClosureLaunch &c, hipStream_t stream) {
     if (c.n == 0) return hipSuccess;

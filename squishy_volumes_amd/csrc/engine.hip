@@ -103,6 +103,7 @@ struct mpm_engine {
     ClosureStats *closure_stats = nullptr;
     float hull_lo[MLP_IN] = {0}, hull_hi[MLP_IN] = {0};
     uint32_t judge_stride = 0, judge_phase = 0;
+    unsigned long long *pair_count = nullptr;  // mpm_closure_pairs
 
     bool profiling = false;
     uint32_t profile_mask = 0;  // stages timed while profiling (bit = Stage)
@@ -996,7 +997,8 @@ int mpm_set_closure(mpm_engine *e, const mpm_closure *c) {
             return MPM_ERR_OUT_OF_MEMORY;
         HIP_TRY(hipMemsetAsync(e->closure_stats, 0, CLOSURE_STATS_SLOTS * sizeof(ClosureStats), e->stream));
     }
-    HIP_TRY(hipMemcpyAsync(e->closure_params, c->params, MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice, e->stream));
+    // Host or device memory (the parameters of a network trained on the GPU).
+    HIP_TRY(hipMemcpyAsync(e->closure_params, c->params, MLP_NPARAM * sizeof(float), hipMemcpyDefault, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     for (int q = 0; q < MLP_IN; ++q) {
         e->hull_lo[q] = c->hull_lo[q];
@@ -1006,6 +1008,22 @@ int mpm_set_closure(mpm_engine *e, const mpm_closure *c) {
     e->judge_phase = 0;
     e->closure_on = true;
     return MPM_OK;
+}
+
+int mpm_closure_pairs(mpm_engine *e, uint32_t stride, uint32_t phase, float *x, float *t, uint64_t capacity,
+                      uint64_t *count) {
+    if (!e || !count || stride == 0 || phase >= stride || (capacity > 0 && (!x || !t)))
+        return MPM_ERR_INVALID_ARGUMENT;
+    HIP_TRY(hipSetDevice(e->device));
+    if (!e->pair_count && e->alloc(e->pair_count, 1) != hipSuccess) return MPM_ERR_OUT_OF_MEMORY;
+    HIP_TRY(hipMemsetAsync(e->pair_count, 0, sizeof(unsigned long long), e->stream));
+    HIP_TRY(launch_closure_pairs(e->P[e->cur], e->X, e->n, e->consts.snow_hardening, stride, phase, x, t, capacity,
+                                 e->pair_count, e->stream));
+    unsigned long long c = 0;
+    HIP_TRY(hipMemcpyAsync(&c, e->pair_count, sizeof(c), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    *count = (uint64_t)c;
+    return c > capacity ? MPM_ERR_CAPACITY : MPM_OK;
 }
 
 int mpm_read_closure_stats(mpm_engine *e, int32_t reset, mpm_closure_stats *out) {

@@ -189,15 +189,43 @@ class GpuState:
         if model is None:
             _check(self._lib, self._lib.mpm_set_closure(self._h, None))
             return
-        params = np.ascontiguousarray(model.params, dtype=np.float32)
-        if params.size != int(self._lib.mpm_mlp_param_count()):
+        p = model.params
+        if hasattr(p, "data_ptr"):  # a float32 device tensor: the trained parameters in place
+            import torch
+
+            if p.dtype != torch.float32 or not p.is_cuda or not p.is_contiguous():
+                raise ValueError("closure parameters must be a contiguous float32 device tensor")
+            torch.cuda.current_stream().synchronize()  # written on torch's stream
+            params, size, ptr = p, p.numel(), p.data_ptr()
+        else:
+            params = np.ascontiguousarray(p, dtype=np.float32)
+            size, ptr = params.size, params.ctypes.data
+        if size != int(self._lib.mpm_mlp_param_count()):
             raise ValueError("closure parameters have the wrong length")
         c = abi.MpmClosure()
-        c.params = params.ctypes.data
+        c.params = ptr
         c.hull_lo[:] = [float(v) for v in model.hull_lo]
         c.hull_hi[:] = [float(v) for v in model.hull_hi]
         c.judge_stride = int(judge_stride)
         _check(self._lib, self._lib.mpm_set_closure(self._h, C.byref(c)))
+
+    def closure_pairs(self, stride: int, phase: int = 0, capacity: int = 1 << 20):
+        """Training pairs (x [k][6], t [k][3], float32 device tensors) of the
+        live snow particles with id % stride == phase (mpm_closure_pairs)."""
+        import torch
+
+        dev = torch.device("cuda", torch.cuda.current_device())
+        x = torch.empty((capacity, 6), dtype=torch.float32, device=dev)
+        t = torch.empty((capacity, 3), dtype=torch.float32, device=dev)
+        torch.cuda.current_stream().synchronize()  # the buffers exist before the engine's stream writes them
+        count = C.c_uint64()
+        rc = self._lib.mpm_closure_pairs(self._h, int(stride), int(phase), C.c_void_p(x.data_ptr()),
+                                         C.c_void_p(t.data_ptr()), capacity, C.byref(count))
+        if rc == abi.ERR_CAPACITY:
+            return self.closure_pairs(stride, phase, int(count.value))
+        _check(self._lib, rc)
+        k = int(count.value)
+        return x[:k], t[:k]
 
     def closure_stats(self, reset: bool = True) -> dict:
         """Network / fallback / judged counts and the judged relative stress error."""

@@ -206,11 +206,16 @@ def snow_block_on_sphere(*, h: float = 1.0 / 128, target_particles: int = 1_000_
 
 
 def random_objects_scene(target_particles: int, *, seed: int = 0, h: float = 1.0 / 128, dt: float = 5e-5,
-                         substeps: int = 200, fps: int = 1, max_objects: int = 4) -> Scene:
-    """1 to max_objects fixed-corotated cubes and spheres (rho 1000, E
-    log-uniform in [1e4, 1e6], nu 0.3, random velocities up to 1 m/s) that
-    together hold about target_particles, placed without overlap in a unit
-    box with separating-slip walls, 8 particles per cell."""
+                         substeps: int = 200, fps: int = 1, max_objects: int = 4,
+                         materials: tuple = ("elastic",)) -> Scene:
+    """1 to max_objects cubes and spheres that together hold about
+    target_particles, placed without overlap in a unit box with
+    separating-slip walls, 8 particles per cell, random velocities up to
+    1 m/s.  Each object's material is drawn from `materials`:
+      elastic  fixed-corotated, rho 1000, E log-uniform in [1e4, 1e6], nu 0.3;
+      snow     the elastoplastic snow model, rho 400, E log-uniform in
+               [1e5, 5e5], nu 0.2;
+      fluid    weakly compressible, rho 1000, K 1e5, gamma 7."""
     rng = np.random.default_rng(seed)
     wall = 4 * h
     k = int(rng.integers(1, max_objects + 1))
@@ -242,10 +247,19 @@ def random_objects_scene(target_particles: int, *, seed: int = 0, h: float = 1.0
             pts = jittered_cells(box_cells(c0, c0 + side), h, rng)
         else:
             pts = sphere_points(c, radius, h, rng)
-        E = float(np.exp(rng.uniform(np.log(1e4), np.log(1e6))))
+        material = materials[int(rng.integers(len(materials)))]
         vel = rng.uniform(-1.0, 1.0, 3)
         vel *= rng.uniform(0.0, 1.0) / max(np.linalg.norm(vel), 1e-9)
-        parts.append(particles_from(pts, abi.IS_SOLID | abi.FIXED_COROTATED, 1000.0, h, vel, E=E, nu=0.3))
+        if material == "elastic":
+            E = float(np.exp(rng.uniform(np.log(1e4), np.log(1e6))))
+            parts.append(particles_from(pts, abi.IS_SOLID | abi.FIXED_COROTATED, 1000.0, h, vel, E=E, nu=0.3))
+        elif material == "snow":
+            E = float(np.exp(rng.uniform(np.log(1e5), np.log(5e5))))
+            parts.append(particles_from(pts, abi.IS_SOLID | abi.SNOW, 400.0, h, vel, E=E, nu=0.2, with_jp=True))
+        elif material == "fluid":
+            parts.append(particles_from(pts, abi.IS_FLUID, 1000.0, h, vel, bulk_modulus=1e5, exponent=7))
+        else:
+            raise ValueError(f"unknown material {material!r}")
     allp = concat(parts)
     consts = InputConsts(grid_node_size=h, frames_per_second=fps, domain_min=(0.0, 0.0, 0.0), domain_max=(1.0, 1.0, 1.0))
     opts = EngineOptions(box_walls=((wall, wall, wall), (1.0 - wall, 1.0 - wall, 1.0 - wall)))
@@ -273,3 +287,10 @@ def uneven_scene_set(count: int = 64, lo: int = 5_000, hi: int = 500_000, seed: 
         for _ in range(20):
             sizes = np.clip(sizes * (total / sizes.sum()), lo, hi)
     return [random_objects_scene(int(s), seed=seed * 1000 + i, **kw) for i, s in enumerate(sizes)]
+
+
+def mixed_scene_set(count: int = 128, lo: int = 10_000, hi: int = 1_000_000, seed: int = 1,
+                    total: Optional[int] = 16_000_000, **kw) -> list[Scene]:
+    """The combined workload: `count` scenes of elastic, snow and fluid
+    objects, log-uniform sizes in [lo, hi] scaled to about `total`."""
+    return uneven_scene_set(count, lo, hi, seed, total, materials=("elastic", "snow", "fluid"), **kw)

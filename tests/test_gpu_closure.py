@@ -104,3 +104,65 @@ def test_trained_closure_tracks_the_analytic_run():
     assert stats["evaluated"] > 0
     assert stats["relative_stress_error"] <= 3e-2
     assert ex <= 1e-3
+
+
+def test_pairs_from_the_engine_match_the_host_pairs():
+    """mpm_closure_pairs against pairs_from_particles on the same state (the
+    host takes an FP64 SVD; the rows are matched by their inputs): inputs and
+    targets within 1e-4 absolute (their scale is 1), as many rows as live
+    snow particles in the sample."""
+    import torch
+    from scipy.spatial import cKDTree
+
+    sc = small_snow()
+    g = engine(sc)
+    st, _ = g.produce_next_state(sc.frame_input, GpuRunParameters(scenes.advance_time(0.0, sc.time_step, 80)))
+    host = closure.pairs_from_particles(st.particles, sc.options.snow_hardening)
+    x, t = g.closure_pairs(stride=1)
+    x, t = x.cpu().numpy(), t.cpu().numpy()
+    assert x.shape[0] == len(host)
+    d, j = cKDTree(host.x).query(x)
+    d2, _ = cKDTree(x).query(host.x)  # every host row has a device row (rows may repeat: clamped F)
+    assert d2.max() <= 1e-4
+    print(f"{len(host)} pairs; max input distance {d.max():.2e}, max target diff {np.abs(t - host.t[j]).max():.2e}")
+    assert d.max() <= 1e-4 and np.abs(t - host.t[j]).max() <= 1e-4
+    # a sample: the particles with id % 3 == 1
+    xs, _ = g.closure_pairs(stride=3, phase=1)
+    live = ((st.particles.flags & closure.abi.SNOW) != 0) & ((st.particles.flags & closure.abi.TOMBSTONED) == 0)
+    assert xs.shape[0] == int(live[1::3].sum())
+    # a too small buffer reports the count and the library grows it
+    xc, _ = g.closure_pairs(stride=1, capacity=100)
+    assert xc.shape[0] == len(host)
+    assert torch.is_tensor(xc)
+
+
+def test_closure_parameters_from_device_memory():
+    """The engine takes the network straight from device memory (the online
+    fine-tuning path): one substep (the closure sees the uploaded state, so
+    both runs evaluate the same inputs) gives the same statistics as with a
+    host copy of the parameters, and positions within float round-off."""
+    import torch
+
+    sc = small_snow()
+    p = sc.io_state.particles
+    rng = np.random.default_rng(2)
+    F = np.eye(3) + 0.01 * rng.uniform(-1, 1, (p.count, 3, 3))  # nonzero stresses to judge
+    p.position_gradients[:] = np.transpose(F, (0, 2, 1)).astype(np.float32)
+    T = scenes.advance_time(0.0, sc.time_step, 1)
+    net = closure.Mlp(seed=5)
+    data = closure.pairs_from_particles(sc.io_state.particles, sc.options.snow_hardening)
+    net.fit(torch.from_numpy(data.x).cuda(), torch.from_numpy(data.t).cuda(), epochs=20, batch=512, lr=3e-3)
+    lo, hi = closure.ClosureModel.hull_of(data.x)
+    hull = (lo - 1.0, hi + 1.0)
+    runs = []
+    for model in (closure.ClosureModel(net.params.cpu().numpy(), *hull), net.device_model(hull)):
+        e = engine(sc)
+        e.set_closure(model, judge_stride=2)
+        s_, err = e.produce_next_state(sc.frame_input, GpuRunParameters(target_time=T))
+        assert err is None
+        runs.append((s_.particles, e.closure_stats()))
+    print(runs[0][1], runs[1][1])
+    a, b = runs[0][1], runs[1][1]
+    assert a["evaluated"] == b["evaluated"] == sc.io_state.particles.count and a["judged"] == b["judged"]
+    assert abs(a["relative_stress_error"] - b["relative_stress_error"]) <= 1e-12
+    assert rel_norm_err(runs[0][0].positions, runs[1][0].positions) <= 1e-6
