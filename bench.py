@@ -168,9 +168,9 @@ def learned_form(args) -> dict:
     fin = closure.pairs_from_particles(parts_learned, hard)
     inside = np.all((fin.x >= model.hull_lo) & (fin.x <= model.hull_hi), axis=1)
     y = net.forward(torch.from_numpy(fin.x[inside]).cuda()).cpu().numpy().astype(np.float64)
-    unit = closure.stress_unit(fin.E[inside], fin.jp[inside], hard)
+    tau = closure.stress_from_outputs(y, fin.s[inside], fin.E[inside], fin.nu[inside], fin.jp[inside], hard)
     tau64 = closure.analytic_principal_stress(fin.s[inside], fin.E[inside], fin.nu[inside], fin.jp[inside], hard)
-    err64 = float(np.linalg.norm(y * unit[:, None] - tau64) / np.linalg.norm(tau64))
+    err64 = float(np.linalg.norm(tau - tau64) / np.linalg.norm(tau64))
     ex = float(np.linalg.norm(parts_learned.positions - parts_plain.positions) / np.linalg.norm(parts_plain.positions))
     w = args.closure_window
     return {

@@ -320,8 +320,11 @@ int mpm_mlp_adam(float *params, float *grad, float *m, float *v, float lr, float
  * particle in each substep's scatter comes from the network, evaluated on
  * the particle's sorted singular values s (SVD of F), E, nu and Jp.  Network
  * inputs: (s_q - 1) / 0.025, log10(E / 1e5), (nu - 0.3) / 0.1, (Jp - 1) / 0.1;
- * outputs: principal Kirchhoff stresses in units of 0.025 E exp(xi (1 - Jp))
- * (xi = snow_hardening), composed as tau = U diag(out) U^T.  A particle whose
+ * outputs: the principal Kirchhoff stresses minus their linear-elastic part
+ * h (2 mu (s - 1) + lambda tr(s - 1)), in units of 0.025^2 E h m, with
+ * h = exp(xi (1 - Jp)) (xi = snow_hardening) and m = max(|s - 1| / 0.025,
+ * 1e-6) (|.| the Euclidean norm); the closure adds the linear part
+ * (FP32) and composes tau = U diag(linear + out) U^T.  A particle whose
  * inputs leave [hull_lo, hull_hi] (the training set's range) takes the
  * analytic stress; every judge_stride-th particle (by id, rotating each
  * substep) takes both and the difference goes into the statistics.  A
@@ -348,8 +351,8 @@ int mpm_read_closure_stats(mpm_engine *engine, int32_t reset, mpm_closure_stats 
  * closure online: every live snow particle whose id % stride == phase gives
  * one row of network inputs x (6 floats) and targets t (3 floats: the
  * analytic principal Kirchhoff stresses 2 mu h (s - 1) s + lambda h (J - 1) J,
- * h = exp(xi (1 - Jp)), in the output units above).  x and t are device
- * arrays of capacity rows; *count receives the number of rows (in an
+ * h = exp(xi (1 - Jp)), minus their linear part, in the output units
+ * above).  x and t are device arrays of capacity rows; *count receives the number of rows (in an
  * unspecified order); MPM_ERR_CAPACITY when it exceeds the capacity (the
  * first capacity rows are written).  Synchronizes the engine's stream.  No
  * counterpart in the reference. */

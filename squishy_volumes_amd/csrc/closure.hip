@@ -410,8 +410,12 @@ __global__ __launch_bounds__(256) void k_closure(ClosureArgs a) {
         if (snow) {
             float tau[9];
             if (inside) {
-                const float unit = E * hard * CLOSURE_STRESS_SCALE;
-                const float d[3] = {outs[w][l * 4 + 0] * unit, outs[w][l * 4 + 1] * unit, outs[w][l * 4 + 2] * unit};
+                // linear-elastic part (FP32, exact in s) plus the learned remainder
+                const float unit = E * hard * CLOSURE_STRESS_SCALE * closure_strain_scale(s);
+                float lin[3], rem[3], d[3];
+                fcr_principal_parts(mu * hard, lam * hard, s, lin, rem);
+#pragma unroll
+                for (int q = 0; q < 3; ++q) d[q] = lin[q] + outs[w][l * 4 + q] * unit;
                 recompose(U, d, U, tau);
                 ++evaluated;
             } else {
@@ -463,8 +467,10 @@ __global__ __launch_bounds__(256) void k_closure(ClosureArgs a) {
 // Training pairs from the current state: every live snow particle with
 // id % stride == phase contributes its network inputs (as the closure kernel
 // forms them) and, as the target, its analytic principal Kirchhoff stresses
-// 2 mu h (s - 1) s + lambda h (J - 1) J in units of E h CLOSURE_STRESS_SCALE
-// (h the hardening factor): what the closure replaces.  A wave reserves its
+// 2 mu h (s - 1) s + lambda h (J - 1) J minus their linear part
+// h (2 mu (s - 1) + lambda tr(s - 1)), in units of E h CLOSURE_STRESS_SCALE m
+// (h the hardening factor, m the strain scale): what the network adds in the
+// closure.  A wave reserves its
 // rows with one atomic; rows past the capacity are counted, not written.
 __global__ __launch_bounds__(256) void k_closure_pairs(ParticleArrays P, AuxArrays X, uint64_t n, float hardening,
                                                        uint32_t stride, uint32_t phase, float *__restrict__ xo,
@@ -493,11 +499,11 @@ __global__ __launch_bounds__(256) void k_closure_pairs(ParticleArrays P, AuxArra
                 x[3] = log10f(E * 1e-5f);
                 x[4] = (nu - 0.3f) * 10.0f;
                 x[5] = (jp - 1.0f) * (1.0f / CLOSURE_JP_SCALE);
-                const float J = s[0] * s[1] * s[2];
-                const float unit = E * hard * CLOSURE_STRESS_SCALE;
+                const float unit = E * hard * CLOSURE_STRESS_SCALE * closure_strain_scale(s);
+                float lin[3], rem[3];
+                fcr_principal_parts(mu * hard, lam * hard, s, lin, rem);
 #pragma unroll
-                for (int q = 0; q < 3; ++q)
-                    t[q] = (2.f * mu * hard * (s[q] - 1.f) * s[q] + lam * hard * (J - 1.f) * J) / unit;
+                for (int q = 0; q < 3; ++q) t[q] = rem[q] / unit;
                 take = true;
             }
         }

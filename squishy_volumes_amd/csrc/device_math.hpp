@@ -356,11 +356,34 @@ SV_DEV bool polar3(const float F[9], float R[9]) {
 
 // Fixed-corotated (no reference counterpart): tau = 2 mu (F - R) F^T + lambda (J - 1) J I.
 // From the SVD: (F - R) F^T = U diag(s^2 - s) U^T.
+// The principal stresses 2 mu (s - 1) s + lambda (J - 1) J split into the
+// linear part 2 mu e + lambda tr(e) and the second-order remainder
+// 2 mu e^2 + lambda (e0 e1 + e0 e2 + e1 e2 + e0 e1 e2 + (J - 1)^2), e = s - 1
+// (exact in FP32 near 1), J - 1 summed from the same products.  No term
+// cancels, so both parts keep their relative precision however small the
+// strain; s s - s and s0 s1 s2 - 1 carry absolute errors of about 1e-7,
+// which dominate strains below 1e-5.
+SV_DEV void fcr_principal_parts(float mu, float lambda, const float s[3], float lin[3], float rem[3]) {
+    const float e0 = s[0] - 1.0f, e1 = s[1] - 1.0f, e2 = s[2] - 1.0f;
+    const float tr = e0 + e1 + e2;
+    const float p23 = (e0 * e1 + e0 * e2 + e1 * e2) + e0 * e1 * e2;
+    const float jm1 = tr + p23;
+    const float vol2 = lambda * (p23 + jm1 * jm1);
+    const float e[3] = {e0, e1, e2};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        lin[q] = 2.0f * mu * e[q] + lambda * tr;
+        rem[q] = 2.0f * mu * e[q] * e[q] + vol2;
+    }
+}
+
 SV_DEV void tau_fixed_corotated_svd(float mu, float lambda, const float U[9], const float s[3], float tau[9]) {
-    float d[3] = {s[0] * s[0] - s[0], s[1] * s[1] - s[1], s[2] * s[2] - s[2]};
+    // (s - 1) s and (J - 1) J from e = s - 1, without cancellation
+    const float e0 = s[0] - 1.0f, e1 = s[1] - 1.0f, e2 = s[2] - 1.0f;
+    float d[3] = {e0 + e0 * e0, e1 + e1 * e1, e2 + e2 * e2};
     recompose(U, d, U, tau);
-    float J = s[0] * s[1] * s[2];
-    float vol = lambda * (J - 1.0f) * J;
+    const float jm1 = (e0 + e1 + e2) + ((e0 * e1 + e0 * e2 + e1 * e2) + e0 * e1 * e2);
+    float vol = lambda * (jm1 + jm1 * jm1);
 #pragma unroll
     for (int k = 0; k < 9; ++k) tau[k] *= 2.0f * mu;
     tau[0] += vol;

@@ -248,7 +248,17 @@ hipError_t launch_mlp_adam(float *P, float *G, float *M, float *V, float lr, flo
 // values s_0 >= s_1 >= s_2, log10(E / 1e5), (nu - 0.3) / 0.1 and
 // (Jp - 1) / JP; outputs: principal Kirchhoff stresses in units of
 // E exp(xi (1 - Jp)) STRESS (the hardened Young's modulus).
-constexpr float CLOSURE_SIGMA_SCALE = 0.025f, CLOSURE_JP_SCALE = 0.1f, CLOSURE_STRESS_SCALE = 0.025f;
+// The network's output is the principal stress minus its linear-elastic
+// part, in units of E h CLOSURE_STRESS_SCALE m (h the hardening factor,
+// m = max(|s - 1| / 0.025, 1e-6) the strain magnitude): that remainder is
+// second order in the strain, so the output is first order in it, and
+// 0.025^2 makes it order one at the snow model's compression limit.  The
+// network's absolute error then scales with the strain, as the stress does.
+constexpr float CLOSURE_SIGMA_SCALE = 0.025f, CLOSURE_JP_SCALE = 0.1f, CLOSURE_STRESS_SCALE = 0.025f * 0.025f;
+__host__ __device__ __forceinline__ float closure_strain_scale(const float s[3]) {
+    const float e0 = s[0] - 1.f, e1 = s[1] - 1.f, e2 = s[2] - 1.f;
+    return fmaxf(sqrtf(e0 * e0 + e1 * e1 + e2 * e2) * (1.0f / CLOSURE_SIGMA_SCALE), 1e-6f);
+}
 struct ClosureStats {
     unsigned long long evaluated, fallbacks, judged, pad;
     double err2, ref2;  // sum |tau - tau_analytic|^2 and sum |tau_analytic|^2 over the kept particles

@@ -154,15 +154,34 @@ def test_training_learns_the_snow_stress():
 
     s, E, nu, jp = snow_samples(200000, 0)
     tau = closure.analytic_principal_stress(s, E, nu, jp, 10.0)
-    x, t = device_tensors(closure.features(s, E, nu, jp), closure.targets(tau, E, jp, 10.0))
+    x, t = device_tensors(closure.features(s, E, nu, jp), closure.targets(tau, s, E, nu, jp, 10.0))
     net = closure.Mlp(seed=1)
     before = rel_norm_err(net.forward(x).cpu().numpy(), t.cpu().numpy())
     log = net.fit(x, t, epochs=60, batch=4096, lr=3e-3, lr_final=1e-4)
     sv, Ev, nuv, jpv = snow_samples(50000, 1)
     tv = closure.analytic_principal_stress(sv, Ev, nuv, jpv, 10.0)
     yv = net.forward(device_tensors(closure.features(sv, Ev, nuv, jpv))[0]).cpu().numpy()
-    err = rel_norm_err(yv * closure.stress_unit(Ev, jpv, 10.0)[:, None], tv)
+    err = rel_norm_err(closure.stress_from_outputs(yv, sv, Ev, nuv, jpv, 10.0), tv)
     print(f"relative stress error: before {before:.3f}, after {err:.2e}; epoch losses {log.epochs[0]:.3e} .. {log.epochs[-1]:.3e}")
     assert log.epochs[-1] < 0.01 * log.epochs[0]
     assert err <= 2.5e-2
     torch.cuda.synchronize()
+
+
+def test_targets_invert_and_the_remainder_is_first_order_in_the_strain():
+    """The closure's outputs: targets() and stress_from_outputs() are inverse,
+    the linear part alone is the stress to first order, and the target (the
+    second-order remainder over the strain magnitude) shrinks with the strain."""
+    s, E, nu, jp = snow_samples(2000, 3)
+    tau = closure.analytic_principal_stress(s, E, nu, jp, 10.0)
+    t = closure.targets(tau, s, E, nu, jp, 10.0)
+    back = closure.stress_from_outputs(t, s, E, nu, jp, 10.0)
+    assert rel_norm_err(back, tau) <= 1e-6
+    for k in (0.1, 0.01):
+        sk = 1.0 + k * (s - 1.0)
+        tk = closure.targets(closure.analytic_principal_stress(sk, E, nu, jp, 10.0), sk, E, nu, jp, 10.0)
+        ratio = np.linalg.norm(tk) / np.linalg.norm(t)
+        assert 0.5 * k <= ratio <= 2.0 * k, (k, ratio)
+        lin = closure.linear_principal_stress(sk, E, nu, jp, 10.0)
+        tau_k = closure.analytic_principal_stress(sk, E, nu, jp, 10.0)
+        assert rel_norm_err(lin, tau_k) <= 0.1 * k
