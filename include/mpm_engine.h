@@ -370,6 +370,28 @@ int mpm_run_stage(mpm_engine *engine, int32_t stage, const mpm_frame_input *inpu
  * for the active nodes, same layout as mpm_read_grid (velocities hold momentum). */
 int mpm_read_grid_momentum(mpm_engine *engine, mpm_grid_view *out);
 
+/* ---- frame files (host only, no device) --------------------------------- */
+
+/* The reference's frame file of an IoState (IoState::write / IoState::read,
+ * rust/crates/file_frame/src/io_state.rs:31-75): 32 magic bytes, 64 version
+ * bytes ("0.3.4", the squishy_volumes_file_util version whose readers accept
+ * the file, rust/crates/file_util/src/lib.rs:27-44), then time, Particles and
+ * Option<GridNodes> as bincode 1.3's default functions encode them.  The
+ * parameters' enum and options follow the flags (MPM_USE_VISCOSITY,
+ * MPM_IS_FLUID, MPM_USE_SAND_ALPHA), as ParticleFlags::from does in the
+ * reference; plastic_jp is not part of the format (decoded as 1).  grid NULL
+ * writes None. */
+uint64_t mpm_frame_file_size(const mpm_particles_view *particles, const mpm_grid_view *grid);
+/* Encodes into out; *written receives the size (also on MPM_ERR_CAPACITY). */
+int mpm_frame_encode(const mpm_particles_view *particles, const mpm_grid_view *grid, uint8_t *out,
+                     uint64_t capacity, uint64_t *written);
+/* Checks a frame file's magic, version and structure and returns its counts;
+ * *grid_nodes is UINT64_MAX when the file holds no grid. */
+int mpm_frame_decode_counts(const uint8_t *data, uint64_t size, uint64_t *particles, uint64_t *grid_nodes);
+/* Decodes into views of the counts above (grid may be NULL to skip the grid;
+ * MPM_ERR_CAPACITY with grid->count set when its capacity is short). */
+int mpm_frame_decode(const uint8_t *data, uint64_t size, mpm_particles_view *particles, mpm_grid_view *grid);
+
 #ifdef __cplusplus
 }
 #endif

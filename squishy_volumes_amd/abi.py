@@ -223,6 +223,11 @@ SIGNATURES = {
     "mpm_closure_pairs": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint64,
                                     C.POINTER(C.c_uint64)]),
     "mpm_run_stage": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(MpmFrameInput)]),
+    "mpm_frame_file_size": (C.c_uint64, [C.POINTER(MpmParticlesView), C.POINTER(MpmGridView)]),
+    "mpm_frame_encode": (C.c_int, [C.POINTER(MpmParticlesView), C.POINTER(MpmGridView), C.c_void_p, C.c_uint64,
+                                   C.POINTER(C.c_uint64)]),
+    "mpm_frame_decode_counts": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "mpm_frame_decode": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(MpmParticlesView), C.POINTER(MpmGridView)]),
     "mpm_read_grid_momentum": (C.c_int, [C.c_void_p, C.POINTER(MpmGridView)]),
 }
 

@@ -113,6 +113,69 @@ class IoState:
     particles: Particles
     grid_nodes: Optional[GridNodes] = None
 
+    def write(self, path) -> int:
+        """IoState::write (rust/crates/file_frame/src/io_state.rs:31-62): the
+        reference's frame file, written to a temporary file beside `path` and
+        moved into place; returns the bytes written.  Encoded by the engine
+        library's mpm_frame_encode (host code)."""
+        import os
+        from pathlib import Path
+
+        lib = abi.load_library()
+        parts = self.particles.contiguous()
+        view = parts.view()
+        view.time = self.time
+        gview, keep = _grid_view(self.grid_nodes)
+        gptr = C.byref(gview) if gview is not None else None
+        size = int(lib.mpm_frame_file_size(C.byref(view), gptr))
+        if size == 0:
+            raise ValueError("particles or grid nodes are incomplete")
+        buf = np.empty(size, np.uint8)
+        written = C.c_uint64()
+        rc = lib.mpm_frame_encode(C.byref(view), gptr, buf.ctypes.data, size, C.byref(written))
+        if rc != 0:
+            raise RuntimeError(f"mpm_frame_encode: {lib.mpm_error_string(rc).decode()}")
+        path = Path(path)
+        temp = path.parent / "temp.bin"
+        buf.tofile(temp)
+        os.replace(temp, path)
+        return size
+
+    @staticmethod
+    def read(path) -> "IoState":
+        """IoState::read (rust/crates/file_frame/src/io_state.rs:64-74)."""
+        lib = abi.load_library()
+        data = np.fromfile(path, dtype=np.uint8)
+        n, m = C.c_uint64(), C.c_uint64()
+        rc = lib.mpm_frame_decode_counts(data.ctypes.data, data.size, C.byref(n), C.byref(m))
+        if rc != 0:
+            raise ValueError(f"{path} is not a frame file of this format version")
+        parts = Particles.empty(int(n.value))
+        view = parts.view()
+        grid = None
+        gview = None
+        if m.value != 2**64 - 1:
+            k = int(m.value)
+            grid = GridNodes(np.zeros((k, 3), np.int32), np.zeros(k, np.uint32), np.zeros(k, np.float32),
+                             np.zeros((k, 3), np.float32))
+            gview, keep = _grid_view(grid)
+            gview.capacity = k
+        rc = lib.mpm_frame_decode(data.ctypes.data, data.size, C.byref(view), C.byref(gview) if gview is not None else None)
+        if rc != 0:
+            raise ValueError(f"{path}: {lib.mpm_error_string(rc).decode()}")
+        return IoState(time=view.time, particles=parts, grid_nodes=grid)
+
+
+def _grid_view(grid: Optional[GridNodes]):
+    if grid is None:
+        return None, None
+    arrays = [np.ascontiguousarray(grid.node_ids, np.int32), np.ascontiguousarray(grid.collider_bits, np.uint32),
+              np.ascontiguousarray(grid.masses, np.float32), np.ascontiguousarray(grid.velocities, np.float32)]
+    g = abi.MpmGridView()
+    g.capacity = g.count = int(arrays[2].shape[0])
+    g.node_ids, g.collider_bits, g.masses, g.velocities = (abi.ptr(a) for a in arrays)
+    return g, arrays
+
 
 @dataclass
 class InputConsts:

@@ -1,0 +1,138 @@
+"""The reference's frame file (IoState::write / IoState::read,
+rust/crates/file_frame/src/io_state.rs:31-75): the engine library's encoder
+against an independent restatement of the layout with struct.pack, field by
+field, following serde's data model as bincode 1.3's default functions encode
+it (little-endian, fixed-width integers, u64 sequence lengths, u8 Option
+tags, u32 enum variant indices, arrays without lengths); round trips; and the
+checks of the magic and version bytes (rust/crates/file_util/src/lib.rs:63-93).
+Host code only: no GPU."""
+import struct
+
+import numpy as np
+import pytest
+
+from squishy_volumes_amd import abi
+from squishy_volumes_amd.io_state import GridNodes, IoState, Particles
+
+MAGIC = b"Squishy Volumes Frame File Magic"
+VERSION = b"0.3.4".ljust(64, b"\0")
+
+
+def three_particles():
+    p = Particles.empty(3)
+    p.flags[:] = [abi.IS_SOLID, abi.IS_FLUID | abi.USE_VISCOSITY, abi.IS_SOLID | abi.USE_SAND_ALPHA]
+    prm = p.parameters
+    prm["mass"] = [1.0, 2.0, 3.0]
+    prm["initial_volume"] = [0.1, 0.2, 0.3]
+    prm["viscosity_dynamic"][1], prm["viscosity_bulk"][1] = 0.5, 0.25
+    prm["mu"][[0, 2]], prm["lambda"][[0, 2]] = [10.0, 30.0], [11.0, 31.0]
+    prm["sand_alpha"][2] = 0.75
+    prm["bulk_modulus"][1], prm["exponent"][1] = 1e5, 7
+    rng = np.random.default_rng(0)
+    for name in ("elastic_energies", "positions", "position_gradients", "velocities", "velocity_gradients",
+                 "initial_positions"):
+        a = getattr(p, name)
+        a[:] = rng.standard_normal(a.shape).astype(np.float32)
+    p.collider_bits[:] = [0, 5, 2**31]
+    return p
+
+
+def restated(time, p: Particles, grid=None) -> bytes:
+    out = [MAGIC, VERSION, struct.pack("<d", time)]
+    seq = lambda a: struct.pack("<Q", a.shape[0]) + np.ascontiguousarray(a).tobytes()
+    out.append(seq(p.flags.astype("<u4")))
+    out.append(struct.pack("<Q", p.count))
+    for i in range(p.count):
+        f, q = int(p.flags[i]), p.parameters[i]
+        out.append(struct.pack("<ff", q["mass"], q["initial_volume"]))
+        out.append(struct.pack("<Bff", 1, q["viscosity_dynamic"], q["viscosity_bulk"]) if f & abi.USE_VISCOSITY
+                   else struct.pack("<B", 0))
+        if f & abi.IS_FLUID:
+            out.append(struct.pack("<Iif", 1, q["exponent"], q["bulk_modulus"]))
+        else:
+            out.append(struct.pack("<Iff", 0, q["mu"], q["lambda"]))
+            out.append(struct.pack("<Bf", 1, q["sand_alpha"]) if f & abi.USE_SAND_ALPHA else struct.pack("<B", 0))
+    for a in (p.elastic_energies, p.collider_bits, p.positions, p.position_gradients, p.velocities,
+              p.velocity_gradients, p.initial_positions):
+        out.append(seq(a))
+    if grid is None:
+        out.append(b"\0")
+    else:
+        out.append(b"\1")
+        for a in (grid.node_ids, grid.collider_bits, grid.masses, grid.velocities):
+            out.append(seq(a))
+    return b"".join(out)
+
+
+def some_grid(k=4):
+    rng = np.random.default_rng(1)
+    return GridNodes(rng.integers(-50, 50, (k, 3)).astype(np.int32), np.arange(k, dtype=np.uint32),
+                     rng.random(k).astype(np.float32), rng.standard_normal((k, 3)).astype(np.float32))
+
+
+@pytest.mark.parametrize("with_grid", [False, True])
+def test_encoding_matches_the_restated_layout(tmp_path, with_grid):
+    p = three_particles()
+    grid = some_grid() if with_grid else None
+    st = IoState(time=1.25, particles=p, grid_nodes=grid)
+    path = tmp_path / "frame.bin"
+    size = st.write(path)
+    data = path.read_bytes()
+    expect = restated(1.25, p, grid)
+    assert size == len(data) == len(expect)
+    assert data == expect
+    assert not (tmp_path / "temp.bin").exists()
+
+
+def test_round_trip_of_a_large_state(tmp_path):
+    n = 20000
+    rng = np.random.default_rng(3)
+    p = Particles.empty(n)
+    p.flags[:] = rng.choice([abi.IS_SOLID, abi.IS_FLUID, abi.IS_SOLID | abi.USE_VISCOSITY,
+                             abi.IS_SOLID | abi.USE_SAND_ALPHA | abi.TOMBSTONED], n)
+    for name in p.parameters.dtype.names:
+        p.parameters[name] = rng.integers(1, 9, n) if name == "exponent" else rng.random(n)
+    for name in ("elastic_energies", "positions", "position_gradients", "velocities", "velocity_gradients",
+                 "initial_positions"):
+        a = getattr(p, name)
+        a[:] = rng.standard_normal(a.shape).astype(np.float32)
+    p.collider_bits[:] = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+    grid = some_grid(1000)
+    IoState(time=3.5, particles=p, grid_nodes=grid).write(tmp_path / "f.bin")
+    back = IoState.read(tmp_path / "f.bin")
+    assert back.time == 3.5 and back.particles.count == n
+    for name in ("flags", "elastic_energies", "collider_bits", "positions", "position_gradients", "velocities",
+                 "velocity_gradients", "initial_positions"):
+        assert np.array_equal(getattr(back.particles, name), getattr(p, name)), name
+    # the parameters the flags select come back; the others are not stored
+    f = p.flags
+    for name, sel in (("mass", np.ones(n, bool)), ("initial_volume", np.ones(n, bool)),
+                      ("viscosity_dynamic", (f & abi.USE_VISCOSITY) != 0),
+                      ("mu", (f & abi.IS_FLUID) == 0), ("lambda", (f & abi.IS_FLUID) == 0),
+                      ("sand_alpha", (f & abi.USE_SAND_ALPHA) != 0), ("bulk_modulus", (f & abi.IS_FLUID) != 0),
+                      ("exponent", (f & abi.IS_FLUID) != 0)):
+        assert np.array_equal(back.particles.parameters[name][sel], p.parameters[name][sel]), name
+    for name in ("node_ids", "collider_bits", "masses", "velocities"):
+        assert np.array_equal(getattr(back.grid_nodes, name), getattr(grid, name)), name
+    IoState(time=0.0, particles=p).write(tmp_path / "g.bin")
+    assert IoState.read(tmp_path / "g.bin").grid_nodes is None
+
+
+def test_wrong_magic_version_or_truncation_is_refused(tmp_path):
+    path = tmp_path / "f.bin"
+    IoState(time=0.0, particles=three_particles()).write(path)
+    data = bytearray(path.read_bytes())
+    for mutate in (lambda d: d.__setitem__(0, ord("X")), lambda d: d.__setitem__(32 + 2, ord("9")),
+                   lambda d: d.__delitem__(slice(len(d) - 5, None)), lambda d: d.extend(b"\0")):
+        bad = bytearray(data)
+        mutate(bad)
+        (tmp_path / "bad.bin").write_bytes(bytes(bad))
+        with pytest.raises(ValueError):
+            IoState.read(tmp_path / "bad.bin")
+
+
+def test_empty_state(tmp_path):
+    IoState(time=0.5, particles=Particles.empty(0)).write(tmp_path / "e.bin")
+    back = IoState.read(tmp_path / "e.bin")
+    assert back.particles.count == 0 and back.time == 0.5
+    assert (tmp_path / "e.bin").read_bytes() == restated(0.5, Particles.empty(0))
