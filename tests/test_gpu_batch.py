@@ -89,3 +89,52 @@ def test_autotuner_picks_the_fastest_measured_tuning_and_caches_it(tmp_path):
         runs.append(st.particles)
     assert rel_norm_err(runs[0].positions, runs[1].positions) <= 1e-5
     assert rel_norm_err(runs[0].velocities, runs[1].velocities) <= 1e-4
+
+
+def test_batched_scenes_with_one_learned_closure_match_their_single_runs():
+    """The combined form at small scale: one network (trained on pairs the
+    engine generates) shared by a batch of uneven snow scenes gives each
+    scene what it gives the scene alone (x, v, F within 1e-4), and online
+    fine-tuning from the batch's own pairs keeps the judged error small."""
+    import torch
+
+    from squishy_volumes_amd import closure
+
+    scs = uneven([500, 1728, 216])
+    b = Batch(scs)
+    hard = b.scene.options.snow_hardening
+    g = engine(b.scene)
+    g.step_to(scenes.advance_time(0.0, b.scene.time_step, 40), b.frame_input)
+    x, t = g.closure_pairs(stride=1)
+    net = closure.Mlp(seed=4)
+    net.fit(x, t, epochs=150, batch=512, lr=3e-3, lr_final=1e-4)
+    hull = closure.ClosureModel.hull_of(x)
+    hull = (hull[0] - 0.5, hull[1] + 0.5)
+    T = scenes.advance_time(0.0, b.scene.time_step, 60)
+    gb = engine(b.scene)
+    gb.set_closure(net.device_model(hull), judge_stride=2)
+    st, err = gb.produce_next_state(b.frame_input, GpuRunParameters(target_time=T))
+    assert err is None
+    stats = gb.closure_stats()
+    print("batch", stats)
+    assert stats["evaluated"] > 0 and stats["relative_stress_error"] <= 1e-2
+    for i, (sc, p) in enumerate(zip(scs, b.split(st.particles))):
+        e = engine(sc)
+        e.set_closure(net.device_model(hull), judge_stride=2)
+        alone, err = e.produce_next_state(sc.frame_input, GpuRunParameters(target_time=T))
+        assert err is None
+        errs = [rel_norm_err(a, r) for a, r in ((p.positions, alone.particles.positions),
+                                                (p.velocities, alone.particles.velocities),
+                                                (p.position_gradients, alone.particles.position_gradients))]
+        print(f"scene {i} with the closure, batch vs alone: x {errs[0]:.1e} v {errs[1]:.1e} F {errs[2]:.1e}")
+        assert max(errs) <= 1e-4
+    # one round of online fine-tuning on the batch's current pairs
+    x2, t2 = gb.closure_pairs(stride=1)
+    net.fit(x2, t2, epochs=5, batch=512, lr=3e-4, lr_final=1e-4, seed=1)
+    gb.set_closure(net.device_model(closure.ClosureModel.hull_union(hull, closure.ClosureModel.hull_of(x2))),
+                   judge_stride=2)
+    gb.step_to(scenes.advance_time(gb.time, b.scene.time_step, 20), b.frame_input)
+    stats2 = gb.closure_stats()
+    print("after fine-tuning", stats2)
+    assert stats2["evaluated"] > 0 and stats2["relative_stress_error"] <= 1e-2
+    torch.cuda.synchronize()
