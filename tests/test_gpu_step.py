@@ -206,7 +206,8 @@ def test_energy_error_and_cull():
     assert np.array_equal(st2.particles.positions[on], st.particles.positions[on])
 
 
-@pytest.mark.parametrize("model", ["fixed_corotated_plane", "fluid", "snow", "walls_grid_gravity"])
+@pytest.mark.parametrize("model", ["fixed_corotated_plane", "fluid", "snow", "walls_grid_gravity", "sand",
+                                   "viscous_fluid", "viscous_solid"])
 def test_extension_models_match_oracle(model):
     opts = None
     if model == "fixed_corotated_plane":
@@ -221,6 +222,21 @@ def test_extension_models_match_oracle(model):
         elif model == "snow":
             p.flags[:] = abi.IS_SOLID | abi.SNOW
             p.plastic_jp = np.ones(p.count, np.float32)
+        elif model == "sand":
+            # Drucker-Prager in log strain (advance_particles.rs:50-71); friction angle 30 degrees
+            p.flags[:] = abi.IS_SOLID | abi.USE_SAND_ALPHA
+            sin_phi = np.sin(np.radians(30.0))
+            p.parameters["sand_alpha"] = np.sqrt(2.0 / 3.0) * 2 * sin_phi / (3 - sin_phi)
+        elif model == "viscous_fluid":
+            p.flags[:] = abi.IS_FLUID | abi.USE_VISCOSITY
+            p.parameters["bulk_modulus"] = 1e4
+            p.parameters["exponent"] = 7
+            p.parameters["viscosity_dynamic"] = 2.0
+            p.parameters["viscosity_bulk"] = 1.0
+        elif model == "viscous_solid":
+            p.flags[:] = abi.IS_SOLID | abi.USE_VISCOSITY
+            p.parameters["viscosity_dynamic"] = 5.0
+            p.parameters["viscosity_bulk"] = 2.0
         else:
             opts = EngineOptions(grid_gravity=True, box_walls=((0.0, 0.0, 0.0), (0.42, 1.0, 1.0)))
     T = scenes.advance_time(0.0, sc.time_step, 20)
