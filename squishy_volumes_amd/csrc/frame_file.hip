@@ -194,11 +194,22 @@ int mpm_frame_decode_counts(const uint8_t *data, uint64_t size, uint64_t *partic
     if (!r.ok || n > (size / 4)) return MPM_ERR_INVALID_ARGUMENT;
     const uint8_t *flags = r.take(4 * n);
     if (!r.ok || r.get<uint64_t>() != n) return MPM_ERR_INVALID_ARGUMENT;
-    // the parameters' size follows from the flags; skip them and the arrays
-    for (uint64_t i = 0; i < n; ++i) {
+    // The parameters' layout follows from the flags: check each record's
+    // option tags and variant against them, then skip it.
+    for (uint64_t i = 0; i < n && r.ok; ++i) {
         uint32_t f;
         std::memcpy(&f, flags + 4 * i, 4);
-        r.take(parameters_size(f));
+        const uint8_t *q = r.take(parameters_size(f));
+        if (!q) break;
+        const uint8_t visc = q[8];
+        uint32_t variant;
+        std::memcpy(&variant, q + 9 + ((f & MPM_USE_VISCOSITY) ? 8 : 0), 4);
+        bool match = visc == ((f & MPM_USE_VISCOSITY) ? 1 : 0) && variant == ((f & MPM_IS_FLUID) ? 1u : 0u);
+        if (match && !(f & MPM_IS_FLUID)) {
+            const uint8_t sand = q[9 + ((f & MPM_USE_VISCOSITY) ? 8 : 0) + 4 + 8];
+            match = sand == ((f & MPM_USE_SAND_ALPHA) ? 1 : 0);
+        }
+        if (!match) return MPM_ERR_INVALID_ARGUMENT;
     }
     for (uint64_t width : {4ull, 4ull, 12ull, 36ull, 12ull, 36ull, 12ull}) get_seq(r, n, width);
     const uint8_t tag = r.get<uint8_t>();

@@ -131,6 +131,20 @@ def test_wrong_magic_version_or_truncation_is_refused(tmp_path):
             IoState.read(tmp_path / "bad.bin")
 
 
+def test_parameter_records_that_contradict_the_flags_are_refused(tmp_path):
+    path = tmp_path / "f.bin"
+    p = three_particles()
+    IoState(time=0.0, particles=p).write(path)
+    data = bytearray(path.read_bytes())
+    # particle 0 is a plain solid: its viscosity tag sits after mass and volume
+    first = 32 + 64 + 8 + 8 + 4 * p.count + 8
+    assert data[first + 8] == 0
+    data[first + 8] = 1
+    (tmp_path / "bad.bin").write_bytes(bytes(data))
+    with pytest.raises(ValueError):
+        IoState.read(tmp_path / "bad.bin")
+
+
 def test_empty_state(tmp_path):
     IoState(time=0.5, particles=Particles.empty(0)).write(tmp_path / "e.bin")
     back = IoState.read(tmp_path / "e.bin")
