@@ -370,6 +370,29 @@ int mpm_run_stage(mpm_engine *engine, int32_t stage, const mpm_frame_input *inpu
  * for the active nodes, same layout as mpm_read_grid (velocities hold momentum). */
 int mpm_read_grid_momentum(mpm_engine *engine, mpm_grid_view *out);
 
+/* ---- time-step limits ---------------------------------------------------- */
+
+/* The adaptive time-step limits of the reference (limit_time_step.rs:25-223,
+ * rust/crates/cpu/src/phase/) over the live particles of the engine's
+ * current state, as device reductions: the sound-speed limit h / c (c from
+ * the largest diagonal-space stiffness of the Neo-Hookean or inviscid stress
+ * over J and the current density), the isolated-particle limit, the velocity
+ * limit 0.5 h / max|v| and the deformation limit min 0.2 / max(|C_ij|, 1e-8).
+ * A limit without particles is +inf (None in the reference).  The reference
+ * takes these twice per substep; here a caller reads them between calls and
+ * sets the next calls' step with mpm_set_time_step (per-call adaptivity).
+ * Synchronizes the engine's stream. */
+typedef struct mpm_time_step_limits {
+    float by_velocity;
+    float by_deformation;
+    float by_sound;
+    float by_isolated;
+} mpm_time_step_limits;
+int mpm_read_time_step_limits(mpm_engine *engine, mpm_time_step_limits *out);
+/* The time step of the following mpm_step_to calls (>= 0; 0 makes them
+ * return MPM_ERR_ZERO_TIME_STEP). */
+int mpm_set_time_step(mpm_engine *engine, float time_step);
+
 /* ---- frame files (host only, no device) --------------------------------- */
 
 /* The reference's frame file of an IoState (IoState::write / IoState::read,

@@ -169,6 +169,15 @@ class MpmClosure(C.Structure):
     ]
 
 
+class MpmTimeStepLimits(C.Structure):
+    _fields_ = [
+        ("by_velocity", C.c_float),
+        ("by_deformation", C.c_float),
+        ("by_sound", C.c_float),
+        ("by_isolated", C.c_float),
+    ]
+
+
 class MpmClosureStats(C.Structure):
     _fields_ = [
         ("evaluated", C.c_uint64),
@@ -223,6 +232,8 @@ SIGNATURES = {
     "mpm_closure_pairs": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint64,
                                     C.POINTER(C.c_uint64)]),
     "mpm_run_stage": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(MpmFrameInput)]),
+    "mpm_read_time_step_limits": (C.c_int, [C.c_void_p, C.POINTER(MpmTimeStepLimits)]),
+    "mpm_set_time_step": (C.c_int, [C.c_void_p, C.c_float]),
     "mpm_frame_file_size": (C.c_uint64, [C.POINTER(MpmParticlesView), C.POINTER(MpmGridView)]),
     "mpm_frame_encode": (C.c_int, [C.POINTER(MpmParticlesView), C.POINTER(MpmGridView), C.c_void_p, C.c_uint64,
                                    C.POINTER(C.c_uint64)]),

@@ -104,6 +104,7 @@ struct mpm_engine {
     float hull_lo[MLP_IN] = {0}, hull_hi[MLP_IN] = {0};
     uint32_t judge_stride = 0, judge_phase = 0;
     unsigned long long *pair_count = nullptr;  // mpm_closure_pairs
+    float4 *limit_part = nullptr, *limit_out = nullptr;  // mpm_time_step_limits
 
     bool profiling = false;
     uint32_t profile_mask = 0;  // stages timed while profiling (bit = Stage)
@@ -1024,6 +1025,29 @@ int mpm_closure_pairs(mpm_engine *e, uint32_t stride, uint32_t phase, float *x, 
     HIP_TRY(hipStreamSynchronize(e->stream));
     *count = (uint64_t)c;
     return c > capacity ? MPM_ERR_CAPACITY : MPM_OK;
+}
+
+int mpm_read_time_step_limits(mpm_engine *e, mpm_time_step_limits *out) {
+    if (!e || !out) return MPM_ERR_INVALID_ARGUMENT;
+    HIP_TRY(hipSetDevice(e->device));
+    if (!e->limit_part &&
+        (e->alloc(e->limit_part, step_limit_groups(e->n)) != hipSuccess || e->alloc(e->limit_out, 1) != hipSuccess))
+        return MPM_ERR_OUT_OF_MEMORY;
+    HIP_TRY(launch_step_limits(e->P[e->cur], e->n, e->consts.grid_node_size, e->limit_part, e->limit_out, e->stream));
+    float4 r;
+    HIP_TRY(hipMemcpyAsync(&r, e->limit_out, sizeof(r), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    out->by_sound = r.x;
+    out->by_isolated = r.y;
+    out->by_velocity = r.z > 0.f ? 0.5f * e->consts.grid_node_size / r.z : INFINITY;
+    out->by_deformation = r.w;
+    return MPM_OK;
+}
+
+int mpm_set_time_step(mpm_engine *e, float time_step) {
+    if (!e || !(time_step >= 0.f) || !std::isfinite(time_step)) return MPM_ERR_INVALID_ARGUMENT;
+    e->consts.time_step = time_step;
+    return MPM_OK;
 }
 
 int mpm_read_closure_stats(mpm_engine *e, int32_t reset, mpm_closure_stats *out) {

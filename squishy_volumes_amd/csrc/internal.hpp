@@ -279,6 +279,9 @@ struct ClosureLaunch {
     ClosureStats *stats;    // CLOSURE_STATS_SLOTS slots
 };
 hipError_t launch_closure(const ClosureLaunch &c, hipStream_t stream);
+uint32_t step_limit_groups(uint64_t n);
+hipError_t launch_step_limits(const ParticleArrays &P, uint64_t n, float h, float4 *part, float4 *out,
+                              hipStream_t stream);
 hipError_t launch_closure_pairs(const ParticleArrays &P, const AuxArrays &X, uint64_t n, float hardening,
                                 uint32_t stride, uint32_t phase, float *x, float *t, uint64_t capacity,
                                 unsigned long long *count, hipStream_t stream);

@@ -1,0 +1,186 @@
+// time_step.hip - the adaptive time-step limits of the reference
+// (rust/crates/cpu/src/phase/limit_time_step.rs:25-223) as device
+// reductions over the live particles of the engine's current state:
+//   by_sound     min h / c, c = sqrt(kappa / rho), kappa the largest
+//                diagonal-space stiffness of the stress over J (:36-130)
+//   by_isolated  min of the isolated-particle bounds (:132-185)
+//   by_velocity  0.5 h / max |v| (:199-208)
+//   by_deformation min 0.2 / max(|C_ij|, 1e-8) (:210-222)
+// Solids take the Neo-Hookean formulas and fluids the inviscid ones, as the
+// reference does for its two parameter variants (the engine's fixed-corotated
+// and snow solids are Solid there too).  A limit with no particle is +inf
+// (the reference's None).
+#include "device_math.hpp"
+#include "internal.hpp"
+
+namespace svamd {
+namespace {
+
+constexpr float SEPARATION = 1e-5f;  // SINGULAR_VALUE_SEPARATION, rust/crates/util/src/consts.rs:14
+
+// h / c for one particle, or +inf when kappa / rho is not a positive number.
+__device__ float sound_limit(bool fluid, float p0, float p1, float m, float vol, const float F[9], float h) {
+    float U[9], s[3], V[9];
+    svd3(F, U, s, V);
+    // nalgebra's singular values: non-negative, in descending order
+    float a = fabsf(s[0]), b = fabsf(s[1]), c = fabsf(s[2]);
+    if (a < b) { const float t = a; a = b; b = t; }
+    if (b < c) { const float t = b; b = c; c = t; }
+    if (a < b) { const float t = a; a = b; b = t; }
+    const float x = a, y = b, z = c;
+    const float J = x * y * z;
+    const float g[3] = {y * z, x * z, x * y};  // dJ/ds
+    float d1, d2, diag;                          // dPsi/dJ, d2Psi/dJ2, the identity's coefficient
+    if (fluid) {
+        // inviscid: Psi'(J) = K (1 - J^-gamma), Psi''(J) = gamma K / J^(gamma + 1) (elastic.rs:555-570)
+        const float K = p0;
+        const int gamma = (int)p1;
+        float Jg = 1.f;
+        for (int k = 0; k < gamma; ++k) Jg *= J;
+        d1 = K * (1.f - 1.f / Jg);
+        d2 = (float)gamma * K / (Jg * J);
+        diag = 0.f;
+    } else {
+        // Neo-Hookean: dPsi/dI2 = mu / 2, dPsi/dJ = (lambda ln J - mu) / J,
+        // d2Psi/dJ2 = (lambda (1 - ln J) + mu) / J^2 (elastic.rs:276-295)
+        const float mu = p0, lambda = p1;
+        const float lnJ = logf(J);
+        d1 = (lambda * lnJ - mu) / J;
+        d2 = (lambda * (1.f - lnJ) + mu) / (J * J);
+        diag = mu;  // (mu / 2) * d2I2/ds2 = (mu / 2) * 2 I
+    }
+    // first[i] = dPsi/ds_i; M = second derivative in diagonal space
+    float P[3];
+    const float sv[3] = {x, y, z};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) P[i] = (fluid ? 0.f : diag * sv[i]) + d1 * g[i];
+    // d2J/ds2 = [[0, z, y], [z, 0, x], [y, x, 0]]
+    const float M11 = diag + d2 * g[0] * g[0], M22 = diag + d2 * g[1] * g[1], M33 = diag + d2 * g[2] * g[2];
+    const float M21 = d2 * g[1] * g[0] + d1 * z, M32 = d2 * g[2] * g[1] + d1 * x, M13 = d2 * g[0] * g[2] + d1 * y;
+    const bool xy = fabsf(x - y) < SEPARATION, yz = fabsf(y - z) < SEPARATION, zx = fabsf(z - x) < SEPARATION;
+    float k[6];
+    k[0] = x * x * M11;
+    k[1] = y * y * M22;
+    k[2] = z * z * M33;
+    k[3] = y * y * (xy ? (P[0] + x * M11 - y * M21) / 2.f / x : (x * P[0] - y * P[1]) / (x * x - y * y));
+    k[4] = y * z * (yz ? (P[1] + y * M22 - z * M32) / 2.f / y : (y * P[1] - z * P[2]) / (y * y - z * z));
+    k[5] = z * x * (zx ? (P[2] + x * M33 - x * M13) / 2.f / z : (z * P[2] - x * P[0]) / (z * z - x * x));
+    float kappa = k[0];
+#pragma unroll
+    for (int i = 1; i < 6; ++i) kappa = fmaxf(kappa, k[i]);
+    kappa /= J;
+    const float rho = (m / vol) / J;
+    const float c2 = kappa / rho;
+    if (!(c2 > 0.f) || !isfinite(c2)) return INFINITY;
+    return h / sqrtf(c2);
+}
+
+__device__ float isolated_limit(bool fluid, float p0, float p1, float m, float vol, const float F[9], float h) {
+    if (!fluid) {
+        // Stability analysis of explicit MPM 3.12: xi = 3 / h^2, R = 1, K = 1, D = 3
+        const float xi = 3.f / h / h;
+        const float r = m / (vol * xi * (1.f - 0.5f) * (p0 + 1.5f * p1));
+        return r > 0.f ? sqrtf(r) : INFINITY;
+    }
+    const float rho0 = m / vol;
+    const float j = det3(F);
+    const float K = p0;
+    const int gamma = (int)p1;
+    float jg = 1.f;
+    for (int k = 0; k < gamma; ++k) jg *= j;
+    const float first = K * (1.f - 1.f / jg);
+    float r;
+    if (fabsf(j - 1.f) > SEPARATION) {
+        r = h / j * sqrtf(rho0 * (j - 1.f) / (6.f * first * 3.f));
+    } else {
+        const float second = (float)gamma * K / (jg * j);
+        r = h * sqrtf(rho0 / (6.f * second * 3.f));
+    }
+    return r == r ? r : INFINITY;
+}
+
+// Per work-group: {min by_sound, min by_isolated, max |v|, min by_deformation}.
+__global__ __launch_bounds__(256) void k_step_limits(ParticleArrays P, uint64_t n, float h, float4 *__restrict__ part) {
+    float snd = INFINITY, iso = INFINITY, vmax = 0.f, def = INFINITY;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t flags = P.flags(i);
+        if (flags & F_TOMB) continue;
+        float F[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) F[k] = P.F(k, i);
+        const bool fluid = (flags & F_FLUID) != 0;
+        const float m = P.m(i), vol = P.vol(i), p0 = P.p0(i), p1 = P.p1(i);
+        snd = fminf(snd, sound_limit(fluid, p0, p1, m, vol, F, h));
+        iso = fminf(iso, isolated_limit(fluid, p0, p1, m, vol, F, h));
+        const float v0 = P.v(0, i), v1 = P.v(1, i), v2 = P.v(2, i);
+        vmax = fmaxf(vmax, sqrtf(v0 * v0 + v1 * v1 + v2 * v2));
+#pragma unroll
+        for (int k = 0; k < 9; ++k) def = fminf(def, 0.2f / fmaxf(fabsf(P.at(i)[(FA + k) * TILE]), 1e-8f));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        snd = fminf(snd, __shfl_xor(snd, o, 64));
+        iso = fminf(iso, __shfl_xor(iso, o, 64));
+        vmax = fmaxf(vmax, __shfl_xor(vmax, o, 64));
+        def = fminf(def, __shfl_xor(def, o, 64));
+    }
+    __shared__ float4 w[4];
+    if ((threadIdx.x & 63u) == 0) w[threadIdx.x >> 6] = make_float4(snd, iso, vmax, def);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float4 r = w[0];
+        for (int k = 1; k < 4; ++k) {
+            r.x = fminf(r.x, w[k].x);
+            r.y = fminf(r.y, w[k].y);
+            r.z = fmaxf(r.z, w[k].z);
+            r.w = fminf(r.w, w[k].w);
+        }
+        part[blockIdx.x] = r;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_step_limits_final(const float4 *__restrict__ part, uint32_t groups,
+                                                           float4 *__restrict__ out) {
+    float snd = INFINITY, iso = INFINITY, vmax = 0.f, def = INFINITY;
+    for (uint32_t i = threadIdx.x; i < groups; i += blockDim.x) {
+        const float4 r = part[i];
+        snd = fminf(snd, r.x);
+        iso = fminf(iso, r.y);
+        vmax = fmaxf(vmax, r.z);
+        def = fminf(def, r.w);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        snd = fminf(snd, __shfl_xor(snd, o, 64));
+        iso = fminf(iso, __shfl_xor(iso, o, 64));
+        vmax = fmaxf(vmax, __shfl_xor(vmax, o, 64));
+        def = fminf(def, __shfl_xor(def, o, 64));
+    }
+    __shared__ float4 w[4];
+    if ((threadIdx.x & 63u) == 0) w[threadIdx.x >> 6] = make_float4(snd, iso, vmax, def);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float4 r = w[0];
+        for (int k = 1; k < 4; ++k) {
+            r.x = fminf(r.x, w[k].x);
+            r.y = fminf(r.y, w[k].y);
+            r.z = fmaxf(r.z, w[k].z);
+            r.w = fminf(r.w, w[k].w);
+        }
+        *out = r;
+    }
+}
+
+}  // namespace
+
+uint32_t step_limit_groups(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 2048)); }
+
+hipError_t launch_step_limits(const ParticleArrays &P, uint64_t n, float h, float4 *part, float4 *out,
+                              hipStream_t stream) {
+    const uint32_t groups = step_limit_groups(n);
+    hipLaunchKernelGGL(k_step_limits, dim3(groups), dim3(256), 0, stream, P, n, h, part);
+    hipLaunchKernelGGL(k_step_limits_final, dim3(1), dim3(256), 0, stream, part, groups, out);
+    return hipGetLastError();
+}
+
+}  // namespace svamd

@@ -209,6 +209,25 @@ class GpuState:
         c.judge_stride = int(judge_stride)
         _check(self._lib, self._lib.mpm_set_closure(self._h, C.byref(c)))
 
+    def time_step_limits(self) -> dict:
+        """The reference's adaptive time-step limits over the current state
+        (mpm_read_time_step_limits); +inf where a limit has no particle."""
+        out = abi.MpmTimeStepLimits()
+        _check(self._lib, self._lib.mpm_read_time_step_limits(self._h, C.byref(out)))
+        return {k: float(getattr(out, k)) for k, _ in abi.MpmTimeStepLimits._fields_}
+
+    def set_time_step(self, time_step: float):
+        _check(self._lib, self._lib.mpm_set_time_step(self._h, float(time_step)))
+
+    def adapt_time_step(self, max_time_step: float) -> float:
+        """Sets the step of the following calls to the smallest of
+        max_time_step and the current limits (allowed_time_step_without_prior,
+        rust/crates/cpu/src/adaptive_time_step_state.rs:36-47, taken once per
+        call instead of twice per substep); returns it."""
+        dt = float(np.float32(min([max_time_step, *self.time_step_limits().values()])))
+        self.set_time_step(dt)
+        return dt
+
     def closure_pairs(self, stride: int, phase: int = 0, capacity: int = 1 << 20):
         """Training pairs (x [k][6], t [k][3], float32 device tensors) of the
         live snow particles with id % stride == phase (mpm_closure_pairs)."""
