@@ -340,6 +340,7 @@ struct ClosureArgs {
     float hardening;
     uint32_t judge_stride, judge_phase;
     ClosureStats *stats;
+    const float *svd;  // U, s by P slot (12 rows of stride), or null
 };
 
 __global__ __launch_bounds__(256) void k_closure(ClosureArgs a) {
@@ -362,10 +363,18 @@ __global__ __launch_bounds__(256) void k_closure(ClosureArgs a) {
             const uint32_t flags = a.P.flags(src);
             snow = (flags & F_SNOW) && !(flags & F_TOMB);
             if (snow) {
-                float F[9], V[9];
+                if (a.svd) {
+                    // the factors the gather formed F from
 #pragma unroll
-                for (int k = 0; k < 9; ++k) F[k] = a.P.F(k, src);
-                svd3(F, U, s, V);
+                    for (int k = 0; k < 9; ++k) U[k] = a.svd[(uint64_t)k * a.stride + src];
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) s[q] = a.svd[(uint64_t)(9 + q) * a.stride + src];
+                } else {
+                    float F[9], V[9];
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) F[k] = a.P.F(k, src);
+                    svd3(F, U, s, V);
+                }
                 mu = a.P.p0(src);
                 lam = a.P.p1(src);
                 id = a.P.id(src);
@@ -552,6 +561,7 @@ hipError_t launch_closure(const ClosureLaunch &c, hipStream_t stream) {
     a.judge_stride = c.judge_stride;
     a.judge_phase = c.judge_phase;
     a.stats = c.stats;
+    a.svd = c.svd;
     const uint32_t blocks = (uint32_t)std::min<uint64_t>((c.n + 255) / 256, CLOSURE_STATS_SLOTS);
     hipLaunchKernelGGL(k_closure, dim3(blocks), dim3(256), 0, stream, a);
     return hipGetLastError();

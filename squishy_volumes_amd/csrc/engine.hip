@@ -100,6 +100,8 @@ struct mpm_engine {
     bool closure_on = false;
     float *closure_params = nullptr;  // MLP_NPARAM
     float *closure_tau = nullptr;     // 9 rows of n
+    float *closure_svd = nullptr;     // 12 rows of n: U, s of snow particles from the last gather
+    bool svd_fresh = false;           // closure_svd holds the current F's factors
     ClosureStats *closure_stats = nullptr;
     float hull_lo[MLP_IN] = {0}, hull_hi[MLP_IN] = {0};
     uint32_t judge_stride = 0, judge_phase = 0;
@@ -396,6 +398,12 @@ int run_transfer(mpm_engine *e, TransferKind kind, bool goals, bool want_keys) {
         t.tau_ext = e->closure_tau;
         t.tau_stride = e->n;
     }
+    // In the learned loop the gather hands the closure its SVD factors.
+    const bool svd_out = kind == TransferKind::G2P && e->closure_on && !t.permute && e->closure_svd;
+    if (svd_out) {
+        t.svd_out = e->closure_svd;
+        t.tau_stride = e->n;
+    }
     const int stage = kind == TransferKind::P2G ? ST_P2G : (kind == TransferKind::G2P2G ? ST_G2P2G : ST_G2P);
     e->begin(stage, r);
     HIP_TRY(launch_transfer(e->launch, t, e->stream));
@@ -410,6 +418,7 @@ int run_transfer(mpm_engine *e, TransferKind kind, bool goals, bool want_keys) {
     if (t.permute) e->cur = 1 - e->cur;
     e->pending_permute = false;
     e->keys_valid = want_keys;
+    e->svd_fresh = svd_out;
     return MPM_OK;
 }
 
@@ -445,6 +454,9 @@ int run_closure(mpm_engine *e) {
     c.judge_stride = e->judge_stride;
     c.judge_phase = e->judge_phase;
     c.stats = e->closure_stats;
+    // the factors of the last gather, by the same slots (a sort since then
+    // only adds the order that c.order applies)
+    c.svd = e->svd_fresh ? e->closure_svd : nullptr;
     e->begin(ST_CLOSURE, r);
     HIP_TRY(launch_closure(c, e->stream));
     e->end(r);
@@ -994,6 +1006,7 @@ int mpm_set_closure(mpm_engine *e, const mpm_closure *c) {
     if (!e->closure_params) {
         if (e->alloc(e->closure_params, MLP_NPARAM) != hipSuccess ||
             e->alloc(e->closure_tau, 9 * std::max<uint64_t>(e->n, 1)) != hipSuccess ||
+            e->alloc(e->closure_svd, 12 * std::max<uint64_t>(e->n, 1)) != hipSuccess ||
             e->alloc(e->closure_stats, CLOSURE_STATS_SLOTS) != hipSuccess)
             return MPM_ERR_OUT_OF_MEMORY;
         HIP_TRY(hipMemsetAsync(e->closure_stats, 0, CLOSURE_STATS_SLOTS * sizeof(ClosureStats), e->stream));

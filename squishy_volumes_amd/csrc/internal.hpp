@@ -210,6 +210,7 @@ struct TransferCall {
     DeviceScalars *ds;
     const float *tau_ext = nullptr;  // P2G: snow stress from the learned closure (rows of tau_stride)
     uint64_t tau_stride = 0;
+    float *svd_out = nullptr;        // G2P: U and s of each snow particle's new F (12 rows of tau_stride)
 };
 hipError_t launch_transfer(const TransferLaunch &L, const TransferCall &t, hipStream_t stream);
 uint32_t grid_update_waves(const TransferLaunch &L);
@@ -277,6 +278,9 @@ struct ClosureLaunch {
     float hardening;
     uint32_t judge_stride, judge_phase;
     ClosureStats *stats;    // CLOSURE_STATS_SLOTS slots
+    const float *svd = nullptr;  // U (rows 0-8) and s (rows 9-11) of each snow particle's F by
+                                 // P slot (stride rows), written by the gather; null: the
+                                 // closure takes the SVD itself
 };
 hipError_t launch_closure(const ClosureLaunch &c, hipStream_t stream);
 uint32_t step_limit_groups(uint64_t n);

@@ -307,6 +307,8 @@ struct XferArgs {
     DeviceScalars *ds;
     const float *tau_ext;        // P2G mode: Kirchhoff stress of snow particles by slot (9 rows
     uint64_t tau_stride;         // of tau_stride floats), from the learned closure; or null
+    float *svd_out;              // G2P mode: U and s of snow particles' new F by slot (12 rows of
+                                 // tau_stride floats), for the learned closure; or null
 };
 
 // The substep parameters do not change while a kernel runs: read through the
@@ -730,6 +732,15 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
                         a.X.jp[id] = jp;
                         recompose(U, sv, V, F);
                         J = sv[0] * sv[1] * sv[2];
+                        if constexpr (LAST) {
+                            // the factors of the new F, for the closure that follows
+                            if (a.svd_out) {
+#pragma unroll
+                                for (int k = 0; k < 9; ++k) a.svd_out[(uint64_t)k * a.tau_stride + i] = U[k];
+#pragma unroll
+                                for (int q = 0; q < 3; ++q) a.svd_out[(uint64_t)(9 + q) * a.tau_stride + i] = sv[q];
+                            }
+                        }
                         hard = expf(S1.snow_hardening * (1.0f - jp));
                         if (LAST) energy = energy_fixed_corotated(p0 * hard, p1 * hard, sv);
                     } else {
@@ -1309,6 +1320,7 @@ hipError_t launch_transfer(const TransferLaunch &L, const TransferCall &t, hipSt
     a.ds = t.ds;
     a.tau_ext = t.tau_ext;
     a.tau_stride = t.tau_stride;
+    a.svd_out = t.svd_out;
     const bool goals = t.goals;
     switch (t.kind) {
         case TransferKind::P2G: return dispatch_set<XM_P2G>(L, t.permute, goals, a, stream);
