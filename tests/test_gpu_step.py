@@ -296,6 +296,24 @@ def test_spheres_and_slab_invariants_at_full_size():
     assert np.abs(Ff @ np.transpose(Ff, (0, 2, 1)) - (J ** (2 / 3))[:, None, None] * np.eye(3)).max() < 1e-4
 
 
+def test_spheres_and_slab_full_size_against_fp64():
+    """The benchmark scene itself, all 4.2M particles, 4 substeps against the
+    FP64 oracle (a few seconds of host time): x, v, F within 1e-4.  C is held
+    to the round-off floor: the bodies still move almost uniformly, so the
+    FP64 C is near zero and its relative error says nothing."""
+    sc = scenes.spheres_and_slab()
+    T = scenes.advance_time(0.0, sc.time_step, 4)
+    g = engine(sc)
+    st, err = g.produce_next_state(sc.frame_input, GpuRunParameters(target_time=T))
+    assert err is None
+    pyoracle.set_threads(16)
+    o64 = pyoracle.Oracle(sc.consts, sc.time_step, sc.io_state, sc.options, 8)
+    o64.step_to(T, sc.frame_input)
+    ref = o64.particles_f64()
+    o64.close()
+    compare_states(st.particles, ref, sc.consts.grid_node_size, "spheres and slab, 4.2M particles, 4 substeps vs fp64")
+
+
 def spin(parts: Particles, omega):
     """Adds the rigid rotation omega x (x - centre) to the velocities and its
     gradient to the velocity gradients (column-major)."""
