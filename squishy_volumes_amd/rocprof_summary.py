@@ -17,8 +17,8 @@ Two reports:
 
 Usage::
 
-    python -m squishy_volumes_amd.rocprof_summary stats gpurun_out/trace --command "..."
-    python -m squishy_volumes_amd.rocprof_summary traffic gpurun_out/fetch gpurun_out/write \\
+    python -m squishy_volumes_amd.rocprof_summary stats TRACE_DIR --command "..."
+    python -m squishy_volumes_amd.rocprof_summary traffic FETCH_DIR WRITE_DIR \\
         --copy-mib 1024 --workload spheres_and_slab --write-json profiles/traffic_
 """
 from __future__ import annotations
