@@ -26,10 +26,15 @@ class GpuError(RuntimeError):
 
 @dataclass
 class GpuRunParameters:
-    """GpuRunParameters, rust/crates/gpu/src/gpu_state.rs:331-334."""
+    """GpuRunParameters, rust/crates/gpu/src/gpu_state.rs:331-334, with the
+    time-step fields of CpuRunParameters (rust/crates/cpu/src/cpu_state.rs:138-143):
+    with adaptive_time_steps the call's step is the smallest of max_time_step
+    and the limits of the state it starts from (taken once per call)."""
 
     target_time: float
     store_grid: bool = False
+    max_time_step: Optional[float] = None
+    adaptive_time_steps: bool = False
 
 
 def _check(lib, rc: int):
@@ -99,6 +104,10 @@ class GpuState:
     def produce_next_state(
         self, frame_input: FrameInput, params: GpuRunParameters, cancel: Optional[C.c_int32] = None
     ) -> tuple[IoState, Optional[GpuError]]:
+        if params.adaptive_time_steps:
+            self.adapt_time_step(params.max_time_step if params.max_time_step is not None else float("inf"))
+        elif params.max_time_step is not None:
+            self.set_time_step(params.max_time_step)
         fi, keep = frame_input.to_c()
         status = abi.MpmStatus()
         rc = self._lib.mpm_step_to(

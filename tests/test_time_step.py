@@ -125,3 +125,9 @@ def test_adapting_the_step_between_calls():
     assert g.adapt_time_step(max_time_step=1e-5) == pytest.approx(1e-5)
     with pytest.raises(Exception):
         g.set_time_step(float("nan"))
+    # through the run parameters, as CpuRunParameters carries them
+    lim = min(g.time_step_limits().values())
+    target = g.time + 5.5 * lim
+    st, err = g.produce_next_state(sc.frame_input, GpuRunParameters(target_time=target, max_time_step=1.0,
+                                                                    adaptive_time_steps=True))
+    assert err is None and g.last_status.substeps == 6
