@@ -50,18 +50,28 @@ def cpu_baseline(scene, substeps: int, threads: int) -> dict:
 
     pyoracle.set_threads(threads)
     o = pyoracle.Oracle(scene.consts, scene.time_step, scene.io_state, scene.options, 4)
-    o.step_to(scenes.advance_time(o.time, scene.time_step, 1), scene.frame_input)  # warm-up substep
+    o.step_to(scenes.advance_time(o.time, scene.time_step, 2), scene.frame_input)  # 2 warm-up substeps
     t0 = time.perf_counter()
     o.step_to(scenes.advance_time(o.time, scene.time_step, substeps), scene.frame_input)
     dt = time.perf_counter() - t0
     n = scene.io_state.particles.count
     o.close()
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {
         "value": n * substeps / dt,
         "unit": "particle-substeps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{scene.name}: all {n} particles, {substeps} substeps after 1 warm-up substep, "
+        "cpu_model": model,
+        "nproc": os.cpu_count(),
+        "sample": f"{scene.name}: all {n} particles, {substeps} substeps after 2 warm-up substeps, "
         f"float oracle (C++ restatement of the reference CPU step), {threads} OpenMP threads, {dt:.2f} s",
     }
 
