@@ -212,7 +212,8 @@ typedef struct mpm_tuning {
                                             (default) each whole sort cycle inside a call is one
                                             launch of a captured HIP graph (not while profiling,
                                             not with the learned closure) */
-    int32_t reserved;
+    int32_t particles_per_lane;          /* particles each thread of the transfer kernel takes per
+                                            pass of a block, 1..3 (default 3: passes of 768) */
 } mpm_tuning;
 
 /* ---- engine API (host pointers) ------------------------------------------ */

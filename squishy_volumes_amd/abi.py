@@ -156,7 +156,7 @@ class MpmTuning(C.Structure):
         ("resort_interval", C.c_int32),
         ("persistent_blocks", C.c_int32),
         ("graph", C.c_int32),
-        ("reserved", C.c_int32),
+        ("particles_per_lane", C.c_int32),
     ]
 
 

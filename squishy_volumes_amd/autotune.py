@@ -5,8 +5,10 @@ interval (substeps between particle sorts: a longer interval saves sorts, a
 shorter one keeps particles inside their blocks' staged node regions), the
 work-group count of the persistent transfer kernels (all resident work-groups
 by default; fewer leave CUs to the grid update's tail, more over-subscribe
-the CUs and balance uneven blocks) and whether whole sort cycles run as
-captured HIP graphs.  Each candidate runs the scene on a fresh
+the CUs and balance uneven blocks), the particles each lane takes per pass of
+a block (fewer make more, smaller passes: less idle tail, more flushes) and
+whether whole sort cycles run as captured HIP graphs (they measure alike at
+these sizes, so the default search keeps them on).  Each candidate runs the scene on a fresh
 engine for a few warm-up and a few timed substeps; the fastest wins.  Results
 are cached in a JSON file keyed by the workload's signature (device, particle
 count, node spacing, number of batched scenes), so a workload is tuned once.
@@ -22,8 +24,9 @@ from pathlib import Path
 from typing import Optional
 
 RESORT_INTERVALS = (4, 8, 16)
-PERSISTENT_BLOCKS = (0, 512, 2048)
-GRAPH = (0, -1)  # 0: sort cycles as HIP graphs (the engine's default), -1: every kernel launched
+PERSISTENT_BLOCKS = (0, 2048)
+PARTICLES_PER_LANE = (1, 2, 3)
+GRAPH = (0,)  # 0: sort cycles as HIP graphs (the engine's default), -1: every kernel launched
 
 
 @dataclass
@@ -62,11 +65,12 @@ def time_tuning(scene, tuning: dict, warmup: int, substeps: int, device: int = 0
         g.close()
 
 
-def autotune(scene, *, resort_intervals=RESORT_INTERVALS, persistent_blocks=PERSISTENT_BLOCKS, graph=GRAPH,
-             warmup: int = 8, substeps: int = 48, cache: Optional[Path] = None, device: int = 0,
-             log=None) -> TuneResult:
-    """The fastest (resort_interval, persistent_blocks, graph) for `scene`: a
-    grid search over the candidates, timing each on the scene itself."""
+def autotune(scene, *, resort_intervals=RESORT_INTERVALS, persistent_blocks=PERSISTENT_BLOCKS,
+             particles_per_lane=PARTICLES_PER_LANE, graph=GRAPH, warmup: int = 8, substeps: int = 48,
+             cache: Optional[Path] = None, device: int = 0, log=None) -> TuneResult:
+    """The fastest (resort_interval, persistent_blocks, particles_per_lane,
+    graph) for `scene`: a grid search over the candidates, timing each on the
+    scene itself."""
     device_name = ""
     try:
         import torch
@@ -81,8 +85,8 @@ def autotune(scene, *, resort_intervals=RESORT_INTERVALS, persistent_blocks=PERS
             e = table[key]
             return TuneResult(e["best"], e["ms_per_substep"], e.get("trials", []), cached=True)
     trials = []
-    for ri, pb, gr in itertools.product(resort_intervals, persistent_blocks, graph):
-        tuning = {"resort_interval": ri, "persistent_blocks": pb, "graph": gr}
+    for ri, pb, pl, gr in itertools.product(resort_intervals, persistent_blocks, particles_per_lane, graph):
+        tuning = {"resort_interval": ri, "persistent_blocks": pb, "particles_per_lane": pl, "graph": gr}
         ms = time_tuning(scene, tuning, warmup, substeps, device)
         trials.append((tuning, ms))
         if log:

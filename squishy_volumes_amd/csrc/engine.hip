@@ -592,10 +592,13 @@ int mpm_create(const mpm_consts *consts, const mpm_particles_view *pv, const mpm
     if (hipGetDeviceProperties(&prop, e->device) != hipSuccess) return fail(MPM_ERR_HIP);
     e->launch.persistent_blocks = 0;
     e->launch.cu_count = prop.multiProcessorCount;
+    e->launch.particles_per_lane = 3;
     if (tuning) {
         if (tuning->persistent_blocks > 0) e->launch.persistent_blocks = tuning->persistent_blocks;
         if (tuning->resort_interval > 0) e->resort_interval = tuning->resort_interval;
         if (tuning->graph < 0) e->use_graph = false;
+        if (tuning->particles_per_lane < 0 || tuning->particles_per_lane > 3) return fail(MPM_ERR_INVALID_ARGUMENT);
+        if (tuning->particles_per_lane > 0) e->launch.particles_per_lane = tuning->particles_per_lane;
     }
     // Kernels are specialised on the constitutive models present: snow and
     // sand need the full SVD, every other model a polar decomposition at most.

@@ -175,6 +175,7 @@ struct TransferLaunch {
     uint32_t model_set;     // SET_POLAR or SET_ALL
     bool batched;           // batched scenes (mpm_consts.scene_repeat): the M_BATCH kernels
     int cu_count;
+    int particles_per_lane; // particles per thread per pass of a block, 1..3 (pass = 256 x this)
 };
 
 // The three modes of the transfer kernel: the first scatter of a call (from

@@ -309,6 +309,7 @@ struct XferArgs {
     uint64_t tau_stride;         // of tau_stride floats), from the learned closure; or null
     float *svd_out;              // G2P mode: U and s of snow particles' new F by slot (12 rows of
                                  // tau_stride floats), for the learned closure; or null
+    uint32_t pass_cap;           // particles per pass of a block: XF_THREADS x 1..3
 };
 
 // The substep parameters do not change while a kernel runs: read through the
@@ -584,7 +585,7 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
     // SIMDs (wave w of every work-group runs on SIMD w) share the load.
     uint32_t npass = 0;
     auto issue = [&](const Item &w, uint32_t flip) {
-        const uint32_t ce = min(w.end, w.cs + (uint32_t)XF_CAP);
+        const uint32_t ce = min(w.end, w.cs + a.pass_cap);
         if constexpr (HAS_G2P) {
 #pragma unroll
             for (int t = 0; t < 2; ++t) {
@@ -611,7 +612,7 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
     }
     while (have) {
         const uint32_t bi = w.bi, start = w.start, end = w.end, cs = w.cs;
-        const uint32_t ce = min(end, cs + (uint32_t)XF_CAP);
+        const uint32_t ce = min(end, cs + a.pass_cap);
         const uint32_t flip = npass & 1u ? 255u : 0u;
         const uint32_t b[3] = {w.b[0], w.b[1], w.b[2]};
         int cell0[3];  // global cell of the block's first cell
@@ -905,7 +906,7 @@ __global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
                 if (lane == 0) atomicOr(&blk_touch, touch);
             }
 
-            // The next pass: the block's next XF_CAP particles, or the next block.
+            // The next pass: the block's next pass_cap particles, or the next block.
             if (ce < end) {
                 wn = w;
                 wn.cs = ce;
@@ -1335,6 +1336,7 @@ hipError_t launch_transfer(const TransferLaunch &L, const TransferCall &t, hipSt
     a.tau_ext = t.tau_ext;
     a.tau_stride = t.tau_stride;
     a.svd_out = t.svd_out;
+    a.pass_cap = (uint32_t)XF_THREADS * (uint32_t)std::min(std::max(L.particles_per_lane, 1), XF_PER_THREAD);
     const bool goals = t.goals;
     switch (t.kind) {
         case TransferKind::P2G: return dispatch_set<XM_P2G>(L, t.permute, goals, a, stream);

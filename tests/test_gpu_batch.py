@@ -73,8 +73,8 @@ def test_autotuner_picks_the_fastest_measured_tuning_and_caches_it(tmp_path):
 
     b = Batch(uneven([500, 4000, 1728]))
     cache = tmp_path / "tune.json"
-    r = autotune.autotune(b.scene, resort_intervals=(4, 8), persistent_blocks=(0, 64), graph=(0,), warmup=2,
-                          substeps=6, cache=cache)
+    r = autotune.autotune(b.scene, resort_intervals=(4, 8), persistent_blocks=(0, 64), particles_per_lane=(3,),
+                          graph=(0,), warmup=2, substeps=6, cache=cache)
     assert len(r.trials) == 4 and not r.cached
     assert r.ms_per_substep == min(ms for _, ms in r.trials)
     assert r.best in [t for t, _ in r.trials]

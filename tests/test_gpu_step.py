@@ -342,11 +342,13 @@ def test_fast_particles_drift_out_of_their_blocks(interval):
     compare_states(st.particles, o64.particles_f64(), sc.consts.grid_node_size, f"fast cube, resort interval {interval}")
 
 
-def test_dense_blocks_take_several_passes_and_overflow_ranks():
+@pytest.mark.parametrize("per_lane", [3, 1])
+def test_dense_blocks_take_several_passes_and_overflow_ranks(per_lane):
     """Two jittered lattices (16 particles per cell) in an 8^3-cell cube, spun
     about its centre: blocks hold up to 1024 particles (more than one pass of
-    the transfer kernel) and cells more than the 12 ranked payload slots (the
-    overflow list)."""
+    the transfer kernel: 2 passes of 768, or 4 of 256 with one particle per
+    lane) and cells more than the 12 ranked payload slots (the overflow
+    list)."""
     a = scenes.cube_scene(n_cells=8, corner_cells=(20, 20, 20), velocity=(0.5, -0.3, 0.2), seed=1,
                           domain=((-0.5, -0.5, -0.5), (1.5, 1.5, 1.5)))
     b = scenes.cube_scene(n_cells=8, corner_cells=(20, 20, 20), velocity=(0.5, -0.3, 0.2), seed=2,
@@ -356,12 +358,12 @@ def test_dense_blocks_take_several_passes_and_overflow_ranks():
     sc = scenes.Scene(name="dense_cube", consts=a.consts, time_step=a.time_step, options=a.options,
                       io_state=IoState(time=0.0, particles=parts), frame_input=a.frame_input, substeps=8)
     T = scenes.advance_time(0.0, sc.time_step, 8)
-    g = engine(sc, tuning={"resort_interval": 3})
+    g = engine(sc, tuning={"resort_interval": 3, "particles_per_lane": per_lane})
     st, err = g.produce_next_state(sc.frame_input, GpuRunParameters(target_time=T))
     assert err is None
     o32, o64 = oracles(sc)
     o64.step_to(T, sc.frame_input)
-    compare_states(st.particles, o64.particles_f64(), sc.consts.grid_node_size, "dense cube")
+    compare_states(st.particles, o64.particles_f64(), sc.consts.grid_node_size, f"dense cube, {per_lane} per lane")
 
 
 def test_profiling_times_the_selected_stages_only():
