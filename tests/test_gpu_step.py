@@ -388,3 +388,26 @@ def test_profiling_times_the_selected_stages_only():
     step(2)
     assert not any(v[1] for v in g.kernel_times(reset=True).values())
     g.close()
+
+
+def test_empty_and_all_culled_states():
+    """No particles, and particles that are all tombstoned: every entry point
+    runs, time advances, nothing changes."""
+    consts = InputConsts(grid_node_size=1.0 / 32, frames_per_second=24, domain_min=(0, 0, 0), domain_max=(1, 1, 1))
+    fi = FrameInput(frame=0, gravity_a=(0, 0, -9.81))
+    empty = GpuState.from_io_state(0, consts, 1e-3, IoState(0.0, Particles.empty(0)))
+    T = scenes.advance_time(0.0, 1e-3, 12)
+    st, err = empty.produce_next_state(fi, GpuRunParameters(target_time=T, store_grid=True))
+    assert err is None and st.particles.count == 0 and st.time == pytest.approx(T)
+    assert empty.last_status.substeps == 12 and st.grid_nodes.masses.size == 0
+    lim = empty.time_step_limits()
+    assert all(np.isinf(v) for v in lim.values())
+    x, t = empty.closure_pairs(stride=1)
+    assert x.shape[0] == 0
+    parts = lattice_particles()
+    parts.flags[:] |= abi.TOMBSTONED
+    dead = GpuState.from_io_state(0, consts, 1e-3, IoState(0.0, parts))
+    st, err = dead.produce_next_state(fi, GpuRunParameters(target_time=T))
+    assert err is None and dead.last_status.substeps == 12
+    assert np.array_equal(st.particles.positions, parts.positions)
+    assert np.array_equal(st.particles.velocities, parts.velocities)
