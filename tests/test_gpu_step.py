@@ -411,3 +411,26 @@ def test_empty_and_all_culled_states():
     assert err is None and dead.last_status.substeps == 12
     assert np.array_equal(st.particles.positions, parts.positions)
     assert np.array_equal(st.particles.velocities, parts.velocities)
+
+
+def test_cancellation_before_the_first_substep_is_fatal_and_harmless():
+    """A set cancel flag (Harness::check, rust/crates/cpu/src/cpu_state.rs:168)
+    ends the call before its first substep completes with MPM_ERR_CANCELED (an
+    outer, fatal error); the state is untouched and a later call runs as if
+    the cancelled one had not happened."""
+    import ctypes
+
+    from squishy_volumes_amd.gpu_state import GpuError
+
+    sc = perturbed_cube(n_cells=4)
+    T = scenes.advance_time(0.0, sc.time_step, 9)
+    g = engine(sc, tuning={"resort_interval": 2})
+    with pytest.raises(GpuError) as e:
+        g.produce_next_state(sc.frame_input, GpuRunParameters(target_time=T), cancel=ctypes.c_int32(1))
+    assert e.value.code == abi.ERR_CANCELED
+    assert g.last_status.substeps == 0 and g.time == 0.0
+    st, err = g.produce_next_state(sc.frame_input, GpuRunParameters(target_time=T), cancel=ctypes.c_int32(0))
+    assert err is None and g.last_status.substeps == 9
+    ref, _ = engine(sc, tuning={"resort_interval": 2}).produce_next_state(sc.frame_input, GpuRunParameters(target_time=T))
+    assert rel_norm_err(st.particles.positions, ref.particles.positions) <= 1e-6
+    assert rel_norm_err(st.particles.velocities, ref.particles.velocities) <= 1e-5
