@@ -434,3 +434,35 @@ def test_cancellation_before_the_first_substep_is_fatal_and_harmless():
     ref, _ = engine(sc, tuning={"resort_interval": 2}).produce_next_state(sc.frame_input, GpuRunParameters(target_time=T))
     assert rel_norm_err(st.particles.positions, ref.particles.positions) <= 1e-6
     assert rel_norm_err(st.particles.velocities, ref.particles.velocities) <= 1e-5
+
+
+def test_error_paths():
+    """Fatal errors as the reference raises them: a frame input that does not
+    hold the current time (FrameInput::frame_factor,
+    rust/crates/xpu/src/frame_input.rs:266-278), a zero time step
+    (Error::ZeroTimeStep, rust/crates/cpu/src/cpu_state.rs:171-173), a domain
+    whose block grid does not fit, and invalid constants."""
+    from squishy_volumes_amd.gpu_state import GpuError
+
+    sc = perturbed_cube(n_cells=3)
+    g = engine(sc)
+    with pytest.raises(GpuError) as e:
+        g.produce_next_state(FrameInput(frame=3, gravity_a=(0, 0, -9.81)),
+                             GpuRunParameters(scenes.advance_time(0.0, sc.time_step, 2)))
+    assert e.value.code == abi.ERR_WRONG_FRAME
+    z = GpuState.from_io_state(0, sc.consts, 0.0, sc.io_state, sc.options)
+    with pytest.raises(GpuError) as e:
+        z.produce_next_state(sc.frame_input, GpuRunParameters(1e-3))
+    assert e.value.code == abi.ERR_ZERO_TIME_STEP
+    huge = InputConsts(grid_node_size=1.0 / 256, frames_per_second=24, domain_min=(-100, -100, -100),
+                       domain_max=(100, 100, 100))
+    with pytest.raises(GpuError) as e:
+        GpuState.from_io_state(0, huge, 1e-4, sc.io_state)
+    assert e.value.code == abi.ERR_GRID_TOO_LARGE
+    bad = InputConsts(grid_node_size=-1.0, frames_per_second=24)
+    with pytest.raises(GpuError) as e:
+        GpuState.from_io_state(0, bad, 1e-4, sc.io_state)
+    assert e.value.code == abi.ERR_INVALID_ARGUMENT
+    # the engine that saw the wrong frame still steps with the right one
+    st, err = g.produce_next_state(sc.frame_input, GpuRunParameters(scenes.advance_time(0.0, sc.time_step, 2)))
+    assert err is None and g.last_status.substeps == 2
