@@ -96,6 +96,45 @@ def copy_calibration(mib: int):
     log(f"bench: copy calibration, 3 x {mib} MiB")
 
 
+def copy_bandwidth(mib: int = 2048, launches: int = 5) -> dict:
+    """The on-box copy bandwidth over `mib` MiB, read + written bytes over the
+    average HIP-event time (after one untimed launch), of the engine's plain
+    copy kernel and of torch's tensor copy; `achieved` is the larger."""
+    import ctypes as C
+
+    import torch
+
+    from squishy_volumes_amd import abi
+
+    lib = abi.load_library()
+    src = torch.ones(mib << 18, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    stream = torch.cuda.current_stream()
+    copy = lambda: lib.mpm_dev_copy(abi.ptr(src), abi.ptr(dst), src.numel() * 4, C.c_void_p(stream.cuda_stream))
+    copy()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(launches):
+        if copy() != 0:
+            raise RuntimeError("mpm_dev_copy failed")
+    b.record(stream)
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / launches
+    dst.copy_(src)
+    a.record(stream)
+    for _ in range(launches):
+        dst.copy_(src)
+    b.record(stream)
+    torch.cuda.synchronize()
+    ms_torch = a.elapsed_time(b) / launches
+    del src, dst
+    torch.cuda.empty_cache()
+    rate = lambda t: 2.0 * (mib << 20) / (t * 1e-3) / 1e9
+    gbs = max(rate(ms), rate(ms_torch))
+    return {"bytes_per_launch": 2 * (mib << 20), "engine_copy_ms": ms, "torch_copy_ms": ms_torch,
+            "achieved": gbs, "unit": "GB/s", "frac_of_peak": gbs / HBM_PEAK_GBS}
+
+
 def traffic_for(kernel: str, workload: str):
     f = ROOT / "profiles" / f"traffic_{kernel}_{workload}.json"
     if f.exists():
@@ -511,6 +550,8 @@ def main():
     gpu.step_to(scenes.advance_time(gpu.time, scene.time_step, breakdown_steps), fi)
     gpu.set_profiling(False)
     breakdown = gpu.kernel_times(reset=True)
+    copy_bw = copy_bandwidth()
+    log(f"bench: copy kernel {copy_bw['achieved']:.0f} GB/s")
 
     nodes = int(status.active_nodes)
     value = n * args.steps / elapsed
@@ -554,12 +595,15 @@ def main():
             "traffic": traffic_for(dominant, "spheres_and_slab"),
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": stage_ms[dominant],
+            "frac_of_copy": achieved / copy_bw["achieved"],
         },
         "substep_roofline": {
             "bytes_per_substep": substep_bytes,
             "achieved": substep_bytes * args.steps / elapsed / 1e9,
             "frac": substep_bytes * args.steps / elapsed / 1e9 / HBM_PEAK_GBS,
+            "frac_of_copy": substep_bytes * args.steps / elapsed / 1e9 / copy_bw["achieved"],
         },
+        "copy_bandwidth": copy_bw,
         "stage_ms_per_launch": {k: v[0] / max(v[1], 1) for k, v in breakdown.items() if v[1]},
         "stage_launches_per_substep": {k: v[1] / breakdown_steps for k, v in breakdown.items() if v[1]},
         "svd_fp64_fallbacks": int(status.svd_fp64_fallbacks),
