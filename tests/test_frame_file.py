@@ -150,3 +150,33 @@ def test_empty_state(tmp_path):
     back = IoState.read(tmp_path / "e.bin")
     assert back.particles.count == 0 and back.time == 0.5
     assert (tmp_path / "e.bin").read_bytes() == restated(0.5, Particles.empty(0))
+
+
+def test_corrupted_files_are_refused_or_decoded_never_read_out_of_bounds(tmp_path):
+    """Random byte flips, truncations and extensions of a valid frame: the
+    decoder either refuses the file or decodes it, and never reads past it
+    (every read is bounds-checked against the file size)."""
+    path = tmp_path / "f.bin"
+    p = three_particles()
+    IoState(time=1.0, particles=p, grid_nodes=some_grid(3)).write(path)
+    good = path.read_bytes()
+    rng = np.random.default_rng(11)
+    outcomes = {"refused": 0, "decoded": 0}
+    for trial in range(300):
+        bad = bytearray(good)
+        kind = trial % 3
+        if kind == 0:
+            for _ in range(int(rng.integers(1, 4))):
+                bad[int(rng.integers(96, len(bad)))] = int(rng.integers(0, 256))
+        elif kind == 1:
+            del bad[int(rng.integers(96, len(bad))):]
+        else:
+            bad.extend(rng.integers(0, 256, int(rng.integers(1, 64))).astype(np.uint8).tobytes())
+        (tmp_path / "bad.bin").write_bytes(bytes(bad))
+        try:
+            IoState.read(tmp_path / "bad.bin")
+            outcomes["decoded"] += 1
+        except ValueError:
+            outcomes["refused"] += 1
+    print(outcomes)
+    assert outcomes["refused"] >= 200  # truncations and extensions are always refused
