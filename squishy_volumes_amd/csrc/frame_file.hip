@@ -14,6 +14,7 @@
 // parameters are Fluid iff MPM_IS_FLUID, sand_alpha is Some iff
 // MPM_USE_SAND_ALPHA.  Jp (plastic_jp) has no place in the format.
 #include <cstring>
+#include <initializer_list>
 
 #include "../../include/mpm_engine.h"
 

@@ -180,3 +180,27 @@ def test_corrupted_files_are_refused_or_decoded_never_read_out_of_bounds(tmp_pat
             outcomes["refused"] += 1
     print(outcomes)
     assert outcomes["refused"] >= 200  # truncations and extensions are always refused
+
+
+def test_decoder_under_address_sanitizer(tmp_path):
+    """tests/frame_fuzz.cpp with the decoder, built by g++ with ASan and UBSan
+    (host code only) and run: no sanitizer report, and a verdict for every
+    mutated file."""
+    import shutil
+    import subprocess
+    from pathlib import Path
+
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    root = Path(__file__).resolve().parents[1]
+    exe = tmp_path / "frame_fuzz"
+    build = subprocess.run([gxx, "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+                            "-I", str(root / "include"), "-x", "c++",
+                            str(root / "squishy_volumes_amd" / "csrc" / "frame_file.hip"), "-x", "none",
+                            str(root / "tests" / "frame_fuzz.cpp"), "-o", str(exe)], capture_output=True, text=True)
+    assert build.returncode == 0, build.stderr[-2000:]
+    run = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    print(run.stdout.strip())
+    assert run.returncode == 0, run.stderr[-2000:]
+    assert "refused" in run.stdout and "ERROR" not in run.stderr
