@@ -242,13 +242,16 @@ int mpm_read_grid(mpm_engine *engine, mpm_grid_view *out);
 /* Drop of GpuState. */
 void mpm_destroy(mpm_engine *engine);
 
-/* Text of an mpm_result. */
+/* Text of an mpm_result (the Display of rust/crates/gpu/src/errors.rs and
+ * rust/crates/cpu/src/errors.rs). */
 const char *mpm_error_string(int result);
 
-/* Simulation time of the engine. */
+/* Simulation time of the engine (CpuState::time, rust/crates/cpu/src/cpu_state.rs). */
 double mpm_time(const mpm_engine *engine);
 
-/* Per-kernel timing of the substeps run since the last call with reset=1:
+/* Per-kernel timing (in place of the profiling_output_file of
+ * GpuState::from_io_state, rust/crates/gpu/src/gpu_state.rs:38-190) of the
+ * substeps run since the last call with reset=1:
  * names[i] / total_ms[i] / launches[i] for i < *count (at most `capacity`).
  * Timing uses HIP events on the engine's stream and is enabled by
  * mpm_set_profiling: enabled = 0 off, 1 every stage, otherwise a bit mask
@@ -258,7 +261,8 @@ int mpm_set_profiling(mpm_engine *engine, int32_t enabled);
 int mpm_kernel_times(mpm_engine *engine, int32_t reset, int32_t capacity, const char **names,
                      double *total_ms, uint64_t *launches, int32_t *count);
 
-/* The engine's HIP stream (hipStream_t) so that callers can order work. */
+/* The engine's HIP stream (hipStream_t) so that callers can order work (no
+ * counterpart in the reference). */
 void *mpm_stream(mpm_engine *engine);
 
 /* Number of HIP devices (GpuContext::available_gpus, rust/crates/gpu/src/context.rs). */
@@ -360,7 +364,8 @@ int mpm_read_closure_stats(mpm_engine *engine, int32_t reset, mpm_closure_stats 
 int mpm_closure_pairs(mpm_engine *engine, uint32_t stride, uint32_t phase, float *x, float *t, uint64_t capacity,
                       uint64_t *count);
 
-/* One substep stage at a time on an engine, for the per-kernel parity tests:
+/* One substep stage at a time on an engine, for the per-kernel parity tests
+ * (as the reference tests its shaders one by one, rust/crates/gpu/src/*/test.rs):
  * stage 0 = sort, 1 = P2G from the stored state (stress from F and C),
  * 2 = grid update, 3 = G2P writing the state (x, v, F, C), as the last
  * substep of a call does.  Host-side grid/particle readback goes through
