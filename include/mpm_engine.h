@@ -365,7 +365,8 @@ int mpm_closure_pairs(mpm_engine *engine, uint32_t stride, uint32_t phase, float
                       uint64_t *count);
 
 /* One substep stage at a time on an engine, for the per-kernel parity tests
- * (as the reference tests its shaders one by one, rust/crates/gpu/src/*/test.rs):
+ * (as the reference tests its shaders one by one, in the test.rs of each
+ * shader's directory under rust/crates/gpu/src):
  * stage 0 = sort, 1 = P2G from the stored state (stress from F and C),
  * 2 = grid update, 3 = G2P writing the state (x, v, F, C), as the last
  * substep of a call does.  Host-side grid/particle readback goes through
