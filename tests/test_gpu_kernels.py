@@ -186,3 +186,41 @@ def test_svd_against_numpy(engine_lib):
     eye = np.eye(3)[None]
     assert np.abs(U @ np.transpose(U, (0, 2, 1)) - eye).max() < 1e-5
     assert np.abs(V @ np.transpose(V, (0, 2, 1)) - eye).max() < 1e-5
+
+
+@pytest.mark.parametrize("model", ["neo_hookean", "fixed_corotated", "inviscid"])
+def test_stress_over_the_reference_parameter_sets(engine_lib, model):
+    """The stress kernel on the reference's own parameter sets
+    (test_lame_parameters and test_inviscid_parameters,
+    rust/crates/util/src/elastic.rs:679-692), dealt round-robin over the
+    particles, against the FP64 oracle at the tolerances above."""
+    from squishy_volumes_amd.io_state import lame_lambda, lame_mu
+
+    fx = reference_fixtures()
+    if model == "inviscid":
+        flags_value = abi.IS_FLUID
+        sets = [dict(bulk_modulus=K, exponent=e) for K, e in fx["inviscid_parameters_K_exponent"]]
+    else:
+        flags_value = abi.IS_SOLID | (abi.FIXED_COROTATED if model == "fixed_corotated" else 0)
+        sets = [dict(mu=lame_mu(E, nu), **{"lambda": lame_lambda(E, nu)}) for E, nu in fx["lame_parameters_E_nu"]]
+    F, flags, p = _stress_case(flags_value)
+    n = len(F)
+    for i in range(n):
+        for k, v in sets[i % len(sets)].items():
+            p[k][i] = v
+    jp = np.ones(n, np.float32)
+    F32 = F.astype(np.float32)
+    Fc = np.ascontiguousarray(np.transpose(F32, (0, 2, 1)))
+    d_F, d_flags, d_p, d_jp = dev(Fc), dev(flags.view(np.int32)), dev(p.view(np.uint8)), dev(jp)
+    d_tau = torch.zeros((n, 3, 3), device="cuda")
+    d_e = torch.zeros(n, device="cuda")
+    assert engine_lib.mpm_dev_stress(abi.ptr(d_flags), abi.ptr(d_p), abi.ptr(d_F), abi.ptr(d_jp), 10.0,
+                                     abi.ptr(d_tau), abi.ptr(d_e), n, None) == 0
+    torch.cuda.synchronize()
+    tau = np.transpose(d_tau.cpu().numpy(), (0, 2, 1)).astype(np.float64)
+    rtau, _, _ = pyoracle.stress(F32.astype(np.float64), flags, p, jp.astype(np.float64), hardening=10.0)
+    tol = 1e-3 if flags_value & abi.IS_FLUID else 1e-4
+    scale = np.abs(rtau).reshape(n, -1).max(1) + 1e-3
+    err = np.abs(tau - rtau).reshape(n, -1).max(1)
+    print(model, f"max relative error {(err / scale).max():.2e} over {len(sets)} parameter sets")
+    assert np.all(err <= tol * scale), (err / scale).max()
