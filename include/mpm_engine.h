@@ -227,7 +227,10 @@ int mpm_create(const mpm_consts *consts, const mpm_particles_view *particles,
 /* GpuState::produce_next_state (rust/crates/gpu/src/gpu_state.rs:337-675):
  * runs substeps of the fixed time step while time < target_time.  `cancel`
  * (may be NULL) is polled between substeps like Harness::check
- * (rust/crates/xpu/src/harness.rs:65-76). */
+ * (rust/crates/xpu/src/harness.rs:65-76).  A call runs at most
+ * MPM_MAX_SUBSTEPS_PER_CALL substeps: a longer span, or a step too small to
+ * advance the f64 time, returns MPM_ERR_INVALID_ARGUMENT before any work. */
+#define MPM_MAX_SUBSTEPS_PER_CALL (1u << 20)
 int mpm_step_to(mpm_engine *engine, double target_time, const mpm_frame_input *input,
                 const volatile int32_t *cancel, mpm_status *status);
 

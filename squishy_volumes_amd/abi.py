@@ -52,6 +52,9 @@ ERR_ZERO_TIME_STEP = -8
 ERR_OUT_OF_GRID = -9
 ERR_CAPACITY = -10
 
+# Substeps one mpm_step_to call runs at most (MPM_MAX_SUBSTEPS_PER_CALL).
+MAX_SUBSTEPS_PER_CALL = 1 << 20
+
 # mpm_particle_parameters as a numpy record (36 bytes, no padding).
 PARAMS_DTYPE = np.dtype(
     [

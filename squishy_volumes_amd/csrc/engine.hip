@@ -712,7 +712,10 @@ int mpm_step_to(mpm_engine *e, double target_time, const mpm_frame_input *input,
     // The substep times of the call, accumulated as the reference does
     // (f32 time step added to an f64 time, cpu_state.rs:187-190).
     std::vector<double> times;
-    for (double t = e->time; t < target_time; t += (double)e->consts.time_step) times.push_back(t);
+    for (double t = e->time; t < target_time; t += (double)e->consts.time_step) {
+        if (times.size() >= MPM_MAX_SUBSTEPS_PER_CALL) return MPM_ERR_INVALID_ARGUMENT;
+        times.push_back(t);
+    }
     std::vector<SubstepParams> params;
     params.reserve(times.size());
     bool goals = false;
@@ -937,7 +940,7 @@ int mpm_set_profiling(mpm_engine *e, int32_t enabled) {
 
 int mpm_kernel_times(mpm_engine *e, int32_t reset, int32_t capacity, const char **names, double *total_ms,
                      uint64_t *launches, int32_t *count) {
-    if (!e) return MPM_ERR_INVALID_ARGUMENT;
+    if (!e || capacity < 0) return MPM_ERR_INVALID_ARGUMENT;
     int c = std::min<int>(capacity, ST_COUNT);
     for (int i = 0; i < c; ++i) {
         if (names) names[i] = kStageNames[i];
@@ -976,7 +979,7 @@ int mpm_dev_exclusive_scan(const uint32_t *in, uint32_t *out, uint32_t *total, u
 }
 
 int mpm_dev_copy(const void *src, void *dst, uint64_t bytes, void *stream) {
-    if (bytes % 16 != 0) return MPM_ERR_INVALID_ARGUMENT;
+    if (bytes % 16 != 0 || (bytes > 0 && (!src || !dst))) return MPM_ERR_INVALID_ARGUMENT;
     HIP_TRY(launch_copy(src, dst, bytes, static_cast<hipStream_t>(stream)));
     return MPM_OK;
 }

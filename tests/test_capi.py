@@ -48,3 +48,15 @@ def test_struct_layouts_match_header_sizes():
 def test_missing_library_fails_loudly(tmp_path):
     with pytest.raises(abi.MissingEngineLibrary):
         abi.load_library(tmp_path / "nope.so")
+
+
+def test_python_mirror_holds_the_header_constants():
+    """Return codes, option bits and the per-call substep cap in abi.py match
+    include/mpm_engine.h."""
+    text = re.sub(r"/\*.*?\*/", "", (ROOT / "include" / "mpm_engine.h").read_text(), flags=re.S)
+    codes = dict(re.findall(r"\bMPM_((?:OK|ERR_\w+))\s*=\s*(-?\d+)", text))
+    assert len(codes) >= 12
+    for name, value in codes.items():
+        assert getattr(abi, name) == int(value), name
+    cap = re.search(r"#define MPM_MAX_SUBSTEPS_PER_CALL \(1u << (\d+)\)", text)
+    assert cap and abi.MAX_SUBSTEPS_PER_CALL == 1 << int(cap.group(1))

@@ -463,6 +463,11 @@ def test_error_paths():
     with pytest.raises(GpuError) as e:
         GpuState.from_io_state(0, bad, 1e-4, sc.io_state)
     assert e.value.code == abi.ERR_INVALID_ARGUMENT
+    # a span beyond the per-call substep cap is refused before any work
+    t0 = g.time
+    with pytest.raises(GpuError) as e:
+        g.produce_next_state(sc.frame_input, GpuRunParameters(t0 + sc.time_step * (abi.MAX_SUBSTEPS_PER_CALL + 8)))
+    assert e.value.code == abi.ERR_INVALID_ARGUMENT and g.time == t0
     # the engine that saw the wrong frame still steps with the right one
     st, err = g.produce_next_state(sc.frame_input, GpuRunParameters(scenes.advance_time(0.0, sc.time_step, 2)))
     assert err is None and g.last_status.substeps == 2
