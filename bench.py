@@ -606,6 +606,8 @@ def main():
         "copy_bandwidth": copy_bw,
         "stage_ms_per_launch": {k: v[0] / max(v[1], 1) for k, v in breakdown.items() if v[1]},
         "stage_launches_per_substep": {k: v[1] / breakdown_steps for k, v in breakdown.items() if v[1]},
+        "stage_breakdown": f"the {breakdown_steps} substeps after the timed region, every stage event-timed "
+                           "(an event pair per launch); roofline.avg_launch_ms is from the timed region",
         "svd_fp64_fallbacks": int(status.svd_fp64_fallbacks),
     }
     gpu.close()
