@@ -209,6 +209,8 @@ def learned_form(args) -> dict:
         g.close()
         return el, stats, parts, stages
 
+    mlp_rates = mlp_kernel_rates(net)
+    log(f"closure: MLP kernels {mlp_rates}")
     el_plain, _, parts_plain, stages_plain = timed(False)
     el_learned, stats, parts_learned, stages_learned = timed(True)
 
@@ -246,7 +248,46 @@ def learned_form(args) -> dict:
         "position_error_vs_plain": ex,
         "training": {"pairs": len(data), "epochs": args.epochs, "seconds": train_s,
                      "mse_first_epoch": tlog.epochs[0], "mse_last_epoch": tlog.epochs[-1]},
+        "mlp_kernels": mlp_rates,
     }
+
+
+# Algorithmic FLOP per sample of the 6-64-64-3 network (the MFMA tiles pad the
+# input and output widths to 16; the padding is not counted).
+MLP_FORWARD_FLOP = 2 * (6 * 64 + 64 * 64 + 64 * 3)
+# Forward, weight gradients of all three layers, input gradients of layers 2 and 3.
+MLP_GRAD_FLOP = 2 * MLP_FORWARD_FLOP + 2 * (64 * 64 + 64 * 3)
+BF16_DENSE_PEAK_TFLOPS = 2500.0
+
+
+def mlp_kernel_rates(net, samples: int = 1 << 22, reps: int = 20) -> dict:
+    """The network kernels alone on one large batch, against the BF16 dense
+    MFMA peak: the forward pass and the gradient pass (forward and backward),
+    timed with events on the stream they run on."""
+    import torch
+
+    g = torch.Generator(device="cuda").manual_seed(0)
+    x = torch.rand((samples, 6), generator=g, device="cuda") * 2.0 - 1.0
+    t = torch.rand((samples, 3), generator=g, device="cuda")
+    out = {}
+    for name, flop, run in (
+        ("forward", MLP_FORWARD_FLOP, lambda: net.forward(x)),
+        ("grad", MLP_GRAD_FLOP, lambda: net.accumulate(x, t, 1.0 / samples)),
+    ):
+        run()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(reps):
+            run()
+        e.record()
+        e.synchronize()
+        ms = s.elapsed_time(e) / reps
+        tflops = flop * samples / (ms * 1e-3) / 1e12
+        out[name] = {"samples": samples, "flop_per_sample": flop, "avg_launch_ms": ms, "achieved": tflops,
+                     "peak": BF16_DENSE_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tflops / BF16_DENSE_PEAK_TFLOPS}
+    net.grad.zero_()
+    net.loss.zero_()
+    return out
 
 
 def batched_form(args) -> dict:
