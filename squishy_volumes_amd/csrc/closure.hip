@@ -186,13 +186,17 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ P
 // Each wave keeps its weight-gradient tiles in registers over all of its
 // sample groups and adds them to grad once.
 constexpr int GRAD_WAVES = 4;
+// Rows are padded (16 -> 20 and 64 -> 68 floats) so that the 16 lanes that
+// write one column of a 16-sample tile, and the four lane groups that read
+// four rows of it, fall on different LDS banks.
+constexpr int XS = 20, HS = 68;
 struct WaveStage {
-    float x[16 * 16];   // [sample][feature]
-    float h1[16 * 64];  // [sample][unit]
-    float h2[16 * 64];
-    float dy[16 * 16];  // [sample][output], BF16-rounded
-    float d2[16 * 64];  // dL/dz2, BF16-rounded
-    float d1[16 * 64];  // dL/dz1, BF16-rounded
+    float x[16 * XS];   // [sample][feature]
+    float h1[16 * HS];  // [sample][unit]
+    float h2[16 * HS];
+    float dy[16 * XS];  // [sample][output], BF16-rounded
+    float d2[16 * HS];  // dL/dz2, BF16-rounded
+    float d1[16 * HS];  // dL/dz1, BF16-rounded
 };
 
 __global__ __launch_bounds__(64 * GRAD_WAVES) void k_mlp_grad(const float *__restrict__ P, const float *__restrict__ x,
@@ -217,7 +221,7 @@ __global__ __launch_bounds__(64 * GRAD_WAVES) void k_mlp_grad(const float *__res
     for (uint64_t g = wave; g * 16 < n; g += nwaves) {
         const bf16x4 xop = feature_op(x, g * 16, n);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) S.x[c * 16 + r0 + i] = __uint_as_float(((uint32_t)(uint16_t)xop[i]) << 16);
+        for (int i = 0; i < 4; ++i) S.x[c * XS + r0 + i] = __uint_as_float(((uint32_t)(uint16_t)xop[i]) << 16);
         Tiles h1, h2;
         f32x4 y;
         forward16(P, xop, h1, h2, y);
@@ -225,8 +229,8 @@ __global__ __launch_bounds__(64 * GRAD_WAVES) void k_mlp_grad(const float *__res
         for (int t = 0; t < 4; ++t)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                S.h1[c * 64 + 16 * t + r0 + i] = h1.t[t][i];
-                S.h2[c * 64 + 16 * t + r0 + i] = h2.t[t][i];
+                S.h1[c * HS + 16 * t + r0 + i] = h1.t[t][i];
+                S.h2[c * HS + 16 * t + r0 + i] = h2.t[t][i];
             }
         // Output error: lanes l < 16 hold sample l's outputs (rows r0 + i = i).
         const uint64_t s = g * 16 + (uint64_t)c;
@@ -237,16 +241,16 @@ __global__ __launch_bounds__(64 * GRAD_WAVES) void k_mlp_grad(const float *__res
                 d = y[i] - tgt[s * MLP_OUT + i];
                 lsum += d * d;
             }
-            S.dy[c * 16 + r0 + i] = bf16_round(2.f * scale * d);
+            S.dy[c * XS + r0 + i] = bf16_round(2.f * scale * d);
         }
         __builtin_amdgcn_wave_barrier();
         // dz2^T = (W3^T dy^T) masked by relu'(z2).
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            const f32x4 acc = mfma(op4(P + MLP_OFF_W3 + r0 * MLP_H + 16 * t + c, MLP_H), op4(S.dy + c * 16 + r0, 1), zero4());
+            const f32x4 acc = mfma(op4(P + MLP_OFF_W3 + r0 * MLP_H + 16 * t + c, MLP_H), op4(S.dy + c * XS + r0, 1), zero4());
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int k = c * 64 + 16 * t + r0 + i;
+                const int k = c * HS + 16 * t + r0 + i;
                 S.d2[k] = S.h2[k] > 0.f ? bf16_round(acc[i]) : 0.f;
             }
         }
@@ -257,11 +261,11 @@ __global__ __launch_bounds__(64 * GRAD_WAVES) void k_mlp_grad(const float *__res
             f32x4 acc = zero4();
 #pragma unroll
             for (int kk = 0; kk < 4; ++kk)
-                acc = mfma(op4(P + MLP_OFF_W2 + (16 * kk + r0) * MLP_H + 16 * t + c, MLP_H), op4(S.d2 + c * 64 + 16 * kk + r0, 1),
+                acc = mfma(op4(P + MLP_OFF_W2 + (16 * kk + r0) * MLP_H + 16 * t + c, MLP_H), op4(S.d2 + c * HS + 16 * kk + r0, 1),
                            acc);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int k = c * 64 + 16 * t + r0 + i;
+                const int k = c * HS + 16 * t + r0 + i;
                 S.d1[k] = S.h1[k] > 0.f ? bf16_round(acc[i]) : 0.f;
             }
         }
@@ -269,17 +273,17 @@ __global__ __launch_bounds__(64 * GRAD_WAVES) void k_mlp_grad(const float *__res
         // Weight gradients, contracted over the 16 samples.
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            g3[t] = mfma(op4(S.dy + r0 * 16 + c, 16), op4(S.h2 + r0 * 64 + 16 * t + c, 64), g3[t]);
-            g1[t] = mfma(op4(S.d1 + r0 * 64 + 16 * t + c, 64), op4(S.x + r0 * 16 + c, 16), g1[t]);
+            g3[t] = mfma(op4(S.dy + r0 * XS + c, XS), op4(S.h2 + r0 * HS + 16 * t + c, HS), g3[t]);
+            g1[t] = mfma(op4(S.d1 + r0 * HS + 16 * t + c, HS), op4(S.x + r0 * XS + c, XS), g1[t]);
 #pragma unroll
             for (int u = 0; u < 4; ++u)
-                g2[t][u] = mfma(op4(S.d2 + r0 * 64 + 16 * t + c, 64), op4(S.h1 + r0 * 64 + 16 * u + c, 64), g2[t][u]);
+                g2[t][u] = mfma(op4(S.d2 + r0 * HS + 16 * t + c, HS), op4(S.h1 + r0 * HS + 16 * u + c, HS), g2[t][u]);
         }
 #pragma unroll 4
         for (int q = 0; q < 16; ++q) {
-            gb1 += S.d1[q * 64 + l];
-            gb2 += S.d2[q * 64 + l];
-            if (l < MLP_OUT) gb3 += S.dy[q * 16 + l];
+            gb1 += S.d1[q * HS + l];
+            gb2 += S.d2[q * HS + l];
+            if (l < MLP_OUT) gb3 += S.dy[q * XS + l];
         }
         __builtin_amdgcn_wave_barrier();
     }
