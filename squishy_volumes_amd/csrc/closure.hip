@@ -166,13 +166,16 @@ __device__ __forceinline__ bf16x4 feature_op(const float *__restrict__ x, uint64
 
 __global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ P, const float *__restrict__ x,
                                                     float *__restrict__ y, uint64_t n) {
+    // The weights are staged in LDS once per work-group, as BF16 MFMA operands.
+    __shared__ StagedNet net;
+    stage_net(P, net);
+    __syncthreads();
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
     const int l = (int)(threadIdx.x & 63u);
     for (uint64_t g = wave; g * 16 < n; g += nwaves) {
-        Tiles h1, h2;
         f32x4 out;
-        forward16(P, feature_op(x, g * 16, n), h1, h2, out);
+        forward16_staged(net, feature_op(x, g * 16, n), out);
         const uint64_t s = g * 16 + (uint64_t)l;
         if (l < 16 && s < n) {
 #pragma unroll
