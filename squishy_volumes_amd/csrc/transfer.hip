@@ -483,8 +483,13 @@ __device__ __forceinline__ void accumulate_plane(int i, uint32_t cnt, int cx, in
     }
 }
 
+// The gather-only mode has no scatter arenas (10 KB of LDS): asking for three
+// waves per SIMD fits it into 168 registers (no spills without the permute),
+// which takes the learned substep's gather from 0.148 to 0.124 ms per 1M
+// particles.  The other modes stay at the compiler's choice (2 waves, bound
+// by their 78 KB of LDS per work-group anyway).
 template <int MODE, bool PERMUTE, bool GOALS, uint32_t SET>
-__global__ __launch_bounds__(XF_THREADS) void k_transfer(XferArgs a) {
+__global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE == XM_G2P ? 3 : 1))) void k_transfer(XferArgs a) {
     constexpr bool HAS_G2P = MODE != XM_P2G;
     constexpr bool HAS_P2G = MODE != XM_G2P;
     constexpr bool LAST = MODE == XM_G2P;
