@@ -67,7 +67,7 @@ typedef enum mpm_result {
      * it (the inner Result of produce_next_state,
      * rust/core/src/compute_thread.rs:165-169). */
     MPM_ERR_ENERGY = 1,            /* a particle's det F went <= 0; it is flagged FAILED */
-    MPM_ERR_CLOSE_TO_INVERTED = 2, /* GpuShaderError::ParticleCloseToInverted */
+    MPM_ERR_CLOSE_TO_INVERTED = 2, /* GpuShaderError::ParticleCloseToInverted: det F < 1e-7 */
     /* Fatal errors (the outer Err). */
     MPM_ERR_INVALID_ARGUMENT = -1,
     MPM_ERR_HIP = -2,
@@ -227,9 +227,30 @@ int mpm_create(const mpm_consts *consts, const mpm_particles_view *particles,
 /* GpuState::produce_next_state (rust/crates/gpu/src/gpu_state.rs:337-675):
  * runs substeps of the fixed time step while time < target_time.  `cancel`
  * (may be NULL) is polled between substeps like Harness::check
- * (rust/crates/xpu/src/harness.rs:65-76).  A call runs at most
- * MPM_MAX_SUBSTEPS_PER_CALL substeps: a longer span, or a step too small to
- * advance the f64 time, returns MPM_ERR_INVALID_ARGUMENT before any work. */
+ * (rust/crates/xpu/src/harness.rs:65-76), and between sort cycles where they
+ * run as HIP graphs: set before the first substep, the call does nothing;
+ * later, the substep whose grid update has run is finished (its gather
+ * writes x, v, F, C) and the call returns MPM_ERR_CANCELED with that
+ * consistent state.
+ * A particle that fails stops the call at the failing substep
+ * (CpuState::produce_next_state, rust/crates/cpu/src/cpu_state.rs:178-183):
+ *   - det F <= 0 after a Neo-Hookean (or sand) particle's advance
+ *     (try_elastic_energy_neo_hookean, advance_particles.rs:73-74):
+ *     MPM_ERR_ENERGY; the state holds x, F after that advance and v, C of
+ *     that substep, the time stays at the substep's start;
+ *   - det F < 1e-7 after any live particle's advance, i.e. before its next
+ *     stress evaluation (prepare_tmp.wesl:74-78): MPM_ERR_CLOSE_TO_INVERTED;
+ *     the state and the time are at the end of that substep (the wgpu
+ *     backend reports the frame's target time instead);
+ *   - det F < 1e-7 in the stored state: MPM_ERR_CLOSE_TO_INVERTED before any
+ *     substep, the state unchanged.
+ * The particle is flagged MPM_FAILED and takes no part in any scatter.  The
+ * engine finds the failing substep by running the call again from its start
+ * (it keeps the starting state for that), so such a call costs up to twice
+ * its substeps.
+ * A call runs at most MPM_MAX_SUBSTEPS_PER_CALL substeps: a longer span, or a
+ * step too small to advance the f64 time, returns MPM_ERR_INVALID_ARGUMENT
+ * before any work. */
 #define MPM_MAX_SUBSTEPS_PER_CALL (1u << 20)
 int mpm_step_to(mpm_engine *engine, double target_time, const mpm_frame_input *input,
                 const volatile int32_t *cancel, mpm_status *status);
@@ -375,6 +396,12 @@ int mpm_closure_pairs(mpm_engine *engine, uint32_t stride, uint32_t phase, float
  * substep of a call does.  Host-side grid/particle readback goes through
  * mpm_read_grid / mpm_read_particles. */
 int mpm_run_stage(mpm_engine *engine, int32_t stage, const mpm_frame_input *input);
+
+/* Test hook: the following mpm_step_to calls behave as if their cancel flag
+ * were set before substep `substep` of the call (where sort cycles run as
+ * HIP graphs, before the first cycle boundary at or after it); a negative
+ * value turns it off.  Stands in for a Harness cancel arriving mid-call. */
+int mpm_debug_cancel_after(mpm_engine *engine, int64_t substep);
 
 /* Raw P2G result (mass, momentum before normalisation) of the last P2G stage
  * for the active nodes, same layout as mpm_read_grid (velocities hold momentum). */

@@ -52,9 +52,15 @@ struct mpm_engine {
     int key_bits = 0;
     TransferLaunch launch{};
 
-    ParticleArrays P[2]{};
+    // Three particle sets.  A call never writes the set that held its
+    // starting state (`start`): its permuting transfers alternate between the
+    // other two, so that the call can be run again from its start (a
+    // particle that fails stops the call at the failing substep).
+    ParticleArrays P[3]{};
     int cur = 0;
+    int start = 0;
     AuxArrays X{};
+    float *jp_start = nullptr;    // Jp at the start of the call (scenes with snow)
 
     uint32_t *keys = nullptr, *keys_sorted = nullptr, *order = nullptr;
     void *sort_temp = nullptr;
@@ -80,7 +86,7 @@ struct mpm_engine {
     SubstepParams *param_host = nullptr;   // pinned staging
     uint32_t param_capacity = 0;
     bool use_graph = true;
-    hipGraphExec_t cycle_graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [cur][goals]
+    hipGraphExec_t cycle_graph[3][3][2] = {};  // [set read][set written][goals]
     int graph_interval = 0;                // resort interval the graphs were captured with
 
     double time = 0;
@@ -107,6 +113,10 @@ struct mpm_engine {
     uint32_t judge_stride = 0, judge_phase = 0;
     unsigned long long *pair_count = nullptr;  // mpm_closure_pairs
     float4 *limit_part = nullptr, *limit_out = nullptr;  // mpm_time_step_limits
+    // mpm_read_particles staging in original order (allocated on first use)
+    float *rb_x = nullptr, *rb_v = nullptr, *rb_F = nullptr, *rb_C = nullptr;
+    uint32_t *rb_flags = nullptr;
+    int64_t cancel_after = -1;  // test hook: calls behave as cancelled before this substep
 
     bool profiling = false;
     uint32_t profile_mask = 0;  // stages timed while profiling (bit = Stage)
@@ -134,12 +144,13 @@ struct mpm_engine {
     }
 
     void drop_graphs() {
-        for (auto &row : cycle_graph)
-            for (auto &g : row)
-                if (g) {
-                    (void)hipGraphExecDestroy(g);
-                    g = nullptr;
-                }
+        for (auto &plane : cycle_graph)
+            for (auto &row : plane)
+                for (auto &g : row)
+                    if (g) {
+                        (void)hipGraphExecDestroy(g);
+                        g = nullptr;
+                    }
     }
 
     template <class T>
@@ -374,16 +385,25 @@ int run_sort(mpm_engine *e) {
     return MPM_OK;
 }
 
+// The set a permuting transfer writes: neither the one it reads nor the one
+// holding the call's starting state.
+int out_set(const mpm_engine *e) {
+    for (int b = 0; b < 3; ++b)
+        if (b != e->cur && b != e->start) return b;
+    return (e->cur + 1) % 3;
+}
+
 // One transfer launch.  After a sort the transfer reads slot i from
-// order[i] and writes the other particle set; otherwise it works in place.
+// order[i] and writes another particle set; otherwise it works in place.
 // want_keys: write the block keys of the new positions for the next sort.
 int run_transfer(mpm_engine *e, TransferKind kind, bool goals, bool want_keys) {
     EventRecord r{};
     TransferCall t{};
     t.kind = kind;
     t.permute = e->pending_permute;
+    const int out = out_set(e);
     t.Pin = e->P[e->cur];
-    t.Pout = e->P[1 - e->cur];
+    t.Pout = e->P[out];
     t.X = e->X;
     t.order = e->order;
     t.blk_start = e->blk_start;
@@ -415,7 +435,7 @@ int run_transfer(mpm_engine *e, TransferKind kind, bool goals, bool want_keys) {
                                  e->ds, e->stream));
         e->end(r);
     }
-    if (t.permute) e->cur = 1 - e->cur;
+    if (t.permute) e->cur = out;
     e->pending_permute = false;
     e->keys_valid = want_keys;
     e->svd_fresh = svd_out;
@@ -479,9 +499,9 @@ int run_cycle_graph(mpm_engine *e, bool goals, Iter iterations) {
     }
     // The state after the cycle: one permuting transfer, K grid updates, the
     // next cycle's keys written.
-    const int cur_after = 1 - e->cur;
+    const int cur_after = out_set(e);
     const uint32_t epoch_after = e->epoch + (uint32_t)K;
-    hipGraphExec_t &exec = e->cycle_graph[e->cur][goals ? 1 : 0];
+    hipGraphExec_t &exec = e->cycle_graph[e->cur][cur_after][goals ? 1 : 0];
     if (!exec) {
         HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
         int rc = MPM_OK;
@@ -516,6 +536,127 @@ int fetch_scalars(mpm_engine *e) {
     if (e->grid_valid) HIP_TRY(launch_sum_partials(e->launch, e->partial, e->ds, e->stream));
     HIP_TRY(hipMemcpyAsync(e->ds_host, e->ds, sizeof(DeviceScalars), hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
+    return MPM_OK;
+}
+
+// Where a call started: the particle set holding its starting state (which
+// the call does not write), the host-side bookkeeping and the time.
+struct CallStart {
+    int set;
+    uint32_t epoch, judge_phase;
+    double time;
+    bool stepped;
+};
+
+CallStart begin_call(mpm_engine *e) {
+    e->start = e->cur;
+    return CallStart{e->cur, e->epoch, e->judge_phase, e->time, e->stepped};
+}
+
+// Back to the start of the call: its particle set and Jp, and an empty grid
+// (the accumulation grid, block flags and overflow lists as after creation).
+int restore_call_start(mpm_engine *e, const CallStart &c) {
+    e->cur = c.set;
+    e->start = c.set;
+    e->keys_valid = false;
+    e->pending_permute = false;
+    e->svd_fresh = false;
+    e->epoch = c.epoch;
+    e->judge_phase = c.judge_phase;
+    e->time = c.time;
+    e->stepped = c.stepped;
+    e->grid_valid = false;
+    if (e->jp_start && e->n > 0)
+        HIP_TRY(hipMemcpyAsync(e->X.jp, e->jp_start, e->n * 4, hipMemcpyDeviceToDevice, e->stream));
+    const size_t nblk = e->geom.num_blocks_total;
+    HIP_TRY(hipMemsetAsync(e->grid, 0, nblk * 64 * sizeof(float4), e->stream));
+    HIP_TRY(hipMemsetAsync(e->blk_flag, 0, nblk * 4, e->stream));
+    HIP_TRY(hipMemsetAsync(e->blk_epoch, 0xff, nblk * 4, e->stream));
+    HIP_TRY(hipMemsetAsync(e->ds->ovf_count, 0, sizeof(e->ds->ovf_count), e->stream));
+    HIP_TRY(hipMemsetAsync(&e->ds->status, 0, 4, e->stream));
+    HIP_TRY(hipMemsetAsync(&e->ds->fail_index, 0xff, 4, e->stream));
+    return MPM_OK;
+}
+
+struct LoopReport {
+    uint64_t done = 0;            // substeps advanced
+    uint64_t transfers = 0;       // index of the last transfer (a gather writing the state) + 1
+    uint64_t graph_launches = 0;
+    bool canceled = false;
+};
+
+// Substeps 0 .. k - 1 of the uploaded parameter table (times[t] is the start
+// of substep t).  A set cancel flag is seen before a substep's work: before
+// the first one the call does nothing; later, the substep whose grid update
+// has run is finished with a gather that writes the state, so that x, v, F
+// and C stay one consistent state.
+int run_loop(mpm_engine *e, const std::vector<double> &times, size_t k, bool goals, const volatile int32_t *cancel,
+             bool cancellable, LoopReport &rep) {
+    int rc;
+    // the cancel flag, or the test hook's substep (mpm_debug_cancel_after)
+    auto canceled = [&](size_t t) {
+        return cancellable && ((cancel && *cancel) || (e->cancel_after >= 0 && t >= (uint64_t)e->cancel_after));
+    };
+    const size_t K = (size_t)std::max(1, e->resort_interval);
+    auto sorts_before = [&](size_t t) { return t == 0 || (t < k && t % K == 0); };
+    auto advanced = [&](size_t t) {
+        e->time = times[t - 1] + (double)e->consts.time_step;
+        ++rep.done;
+        rep.transfers = t + 1;
+    };
+    // With the learned closure, substep t is a gather (of substep t - 1, it
+    // writes the state), the sort when due, the closure, a scatter from the
+    // state and the grid update.
+    for (size_t t = 0; e->closure_on && k > 0 && t <= k; ++t) {
+        if (t > 0) {
+            if ((rc = run_transfer(e, TransferKind::G2P, false, t == k || sorts_before(t))) != MPM_OK) return rc;
+            advanced(t);
+        }
+        if (t == k) break;
+        if (canceled(t)) {
+            rep.canceled = true;
+            break;
+        }
+        if (sorts_before(t) && (rc = run_sort(e)) != MPM_OK) return rc;
+        if ((rc = run_closure(e)) != MPM_OK) return rc;
+        if ((rc = run_transfer(e, TransferKind::P2G, goals, false)) != MPM_OK) return rc;
+        if ((rc = run_grid(e)) != MPM_OK) return rc;
+    }
+    // Plain loop.  Iteration t: the sort when due, transfer t, grid update t.
+    auto iteration = [&](size_t t) -> int {
+        int r;
+        if (sorts_before(t) && (r = run_sort(e)) != MPM_OK) return r;
+        const bool want_keys = t == k || sorts_before(t + 1);
+        const TransferKind kind = t == 0 ? TransferKind::P2G : (t == k ? TransferKind::G2P : TransferKind::G2P2G);
+        if ((r = run_transfer(e, kind, goals, want_keys)) != MPM_OK) return r;
+        if (t < k && (r = run_grid(e)) != MPM_OK) return r;
+        return MPM_OK;
+    };
+    // A sort cycle t .. t + K - 1 that starts after a transfer which wrote
+    // the keys and ends with one that writes the next cycle's keys has the
+    // same launches every time: it runs as one HIP graph launch.
+    const bool graphs = e->use_graph && !e->profiling;
+    for (size_t t = 0; !e->closure_on && k > 0 && t <= k;) {
+        if (canceled(t)) {
+            if (t > 0) {
+                if ((rc = run_transfer(e, TransferKind::G2P, false, true)) != MPM_OK) return rc;
+                advanced(t);
+            }
+            rep.canceled = true;
+            break;
+        }
+        const bool cycle = graphs && t > 0 && t % K == 0 && t + K < k && e->keys_valid && !e->pending_permute;
+        const size_t steps = cycle ? K : 1;
+        if (cycle) {
+            if ((rc = run_cycle_graph(e, goals, [&](size_t u) { return iteration(t + u); })) != MPM_OK) return rc;
+            ++rep.graph_launches;
+        } else if ((rc = iteration(t)) != MPM_OK) {
+            return rc;
+        }
+        for (size_t u = t; u < t + steps; ++u)
+            if (u > 0) advanced(u);
+        t += steps;
+    }
     return MPM_OK;
 }
 
@@ -614,8 +755,10 @@ int mpm_create(const mpm_consts *consts, const mpm_particles_view *pv, const mpm
     const uint64_t max_blocks = std::max<uint64_t>(std::min<uint64_t>(n, e->geom.num_blocks_total), 1);
     e->blk_capacity = (uint32_t)max_blocks;
     const uint64_t nodes = (uint64_t)e->geom.num_blocks_total * 64ull;
-    if (alloc_particles(e, e->P[0]) != hipSuccess || alloc_particles(e, e->P[1]) != hipSuccess)
+    if (alloc_particles(e, e->P[0]) != hipSuccess || alloc_particles(e, e->P[1]) != hipSuccess ||
+        alloc_particles(e, e->P[2]) != hipSuccess)
         return fail(MPM_ERR_OUT_OF_MEMORY);
+    if (e->launch.model_set == SET_ALL && e->alloc(e->jp_start, n) != hipSuccess) return fail(MPM_ERR_OUT_OF_MEMORY);
     if (e->alloc(e->X.visc_dyn, n) || e->alloc(e->X.visc_bulk, n) || e->alloc(e->X.sand_alpha, n) ||
         e->alloc(e->X.jp, n) || e->alloc(e->X.energy, n))
         return fail(MPM_ERR_OUT_OF_MEMORY);
@@ -717,7 +860,7 @@ int mpm_step_to(mpm_engine *e, double target_time, const mpm_frame_input *input,
         times.push_back(t);
     }
     std::vector<SubstepParams> params;
-    params.reserve(times.size());
+    params.reserve(times.size() + 1);
     bool goals = false;
     int rc = upload_goals(e, input, goals);
     if (rc != MPM_OK) return rc;
@@ -727,92 +870,75 @@ int mpm_step_to(mpm_engine *e, double target_time, const mpm_frame_input *input,
         if (rc != MPM_OK) return rc;
         params.push_back(make_params(e, g, fb, goals, input && input->has_b));
     }
-    HIP_TRY(hipMemsetAsync(&e->ds->status, 0, 4, e->stream));
     // Entry k (the last gather's) repeats entry k - 1.
     if (!params.empty()) params.push_back(params.back());
+
+    // The call's starting point, kept so that the call can be run again up to
+    // a failing substep: the particle set (never written by the call), Jp,
+    // the epoch and the time.
+    const CallStart cs = begin_call(e);
+    HIP_TRY(hipMemsetAsync(&e->ds->status, 0, 4, e->stream));
+    HIP_TRY(hipMemsetAsync(&e->ds->fail_index, 0xff, 4, e->stream));
+    if (e->jp_start && e->n > 0)
+        HIP_TRY(hipMemcpyAsync(e->jp_start, e->X.jp, e->n * 4, hipMemcpyDeviceToDevice, e->stream));
     if ((rc = upload_params(e, params)) != MPM_OK) return rc;
 
-    // Transfer t gathers substep t-1 (t >= 1) and scatters substep t (t < k):
-    // a scatter from the stored state, k-1 fused gather-scatters, a last
-    // gather.  The particles are sorted before the first transfer and then
-    // every resort_interval substeps; in between they keep their slots.
-    const size_t k = times.size();
-    const size_t K = (size_t)std::max(1, e->resort_interval);
-    auto sorts_before = [&](size_t t) { return t == 0 || (t < k && t % K == 0); };
-    uint64_t done = 0;
-    int result = MPM_OK;
-    // With the learned closure, substep t is a gather (of substep t - 1, it
-    // writes the state), the sort when due, the closure, a scatter from the
-    // state and the grid update.
-    for (size_t t = 0; e->closure_on && k > 0 && t <= k; ++t) {
-        if (t > 0 && cancel && *cancel) {
-            result = MPM_ERR_CANCELED;
-            break;
-        }
-        if (t > 0) {
-            if ((rc = run_transfer(e, TransferKind::G2P, false, t == k || sorts_before(t))) != MPM_OK) return rc;
-            e->time = times[t - 1] + (double)e->consts.time_step;
-            ++done;
-        }
-        if (t == k) break;
-        if (sorts_before(t) && (rc = run_sort(e)) != MPM_OK) return rc;
-        if ((rc = run_closure(e)) != MPM_OK) return rc;
-        if ((rc = run_transfer(e, TransferKind::P2G, goals, false)) != MPM_OK) return rc;
-        if ((rc = run_grid(e)) != MPM_OK) return rc;
-    }
-    // Plain loop.  Iteration t: the sort when due, transfer t, grid update t.
-    auto iteration = [&](size_t t) -> int {
-        int r;
-        if (sorts_before(t) && (r = run_sort(e)) != MPM_OK) return r;
-        const bool want_keys = t == k || sorts_before(t + 1);
-        const TransferKind kind = t == 0 ? TransferKind::P2G : (t == k ? TransferKind::G2P : TransferKind::G2P2G);
-        if ((r = run_transfer(e, kind, goals, want_keys)) != MPM_OK) return r;
-        if (t < k && (r = run_grid(e)) != MPM_OK) return r;
-        return MPM_OK;
-    };
-    // A sort cycle t .. t + K - 1 that starts after a transfer which wrote
-    // the keys and ends with one that writes the next cycle's keys has the
-    // same launches every time: it runs as one HIP graph launch.
-    const bool graphs = e->use_graph && !e->profiling;
-    uint64_t graph_launches = 0;
-    for (size_t t = 0; !e->closure_on && k > 0 && t <= k;) {
-        if (t > 0 && cancel && *cancel) {
-            result = MPM_ERR_CANCELED;
-            break;
-        }
-        const bool cycle = graphs && t > 0 && t % K == 0 && t + K < k && e->keys_valid && !e->pending_permute;
-        const size_t steps = cycle ? K : 1;
-        if (cycle) {
-            if ((rc = run_cycle_graph(e, goals, [&](size_t u) { return iteration(t + u); })) != MPM_OK) return rc;
-            ++graph_launches;
-        } else if ((rc = iteration(t)) != MPM_OK) {
-            return rc;
-        }
-        for (size_t u = t; u < t + steps; ++u)
-            if (u > 0) {
-                e->time = times[u - 1] + (double)e->consts.time_step;
-                ++done;
-            }
-        t += steps;
-    }
-    if (done > 0) e->stepped = true;
+    LoopReport rep;
+    if ((rc = run_loop(e, times, times.size(), goals, cancel, true, rep)) != MPM_OK) return rc;
     if ((rc = fetch_scalars(e)) != MPM_OK) return rc;
-    if (e->profiling) e->harvest();
-    const uint32_t st = e->ds_host->status & 0xffffu;
-    if (result == MPM_OK) {
-        if (st & MPM_STATUS_INDIRECT_LIMIT_EXCEEDED) result = MPM_ERR_CAPACITY;
-        else if (st & MPM_STATUS_ENERGY_ERROR) result = MPM_ERR_ENERGY;
-        else if (st & MPM_STATUS_OUT_OF_GRID) result = MPM_ERR_OUT_OF_GRID;
+    uint32_t st = e->ds_host->status;
+    uint64_t done = rep.done;
+
+    // A particle flagged FAILED stops the call at the substep whose advance
+    // failed (CpuState::produce_next_state returns the state stored there,
+    // cpu_state.rs:178-183; the wgpu backend ends the frame's work at the
+    // first error, prepare_tmp.wesl:74-78).  Transfer f gathered and advanced
+    // substep f - 1.  Unless it was the call's last gather, which wrote the
+    // whole state, the call runs again from its start to a last gather at f.
+    const uint32_t f = e->ds_host->fail_index;
+    if (f != 0xffffffffu && rep.transfers > 0 && f < rep.transfers - 1) {
+        if ((rc = restore_call_start(e, cs)) != MPM_OK) return rc;
+        uint32_t stop_bits = 0;
+        if (f == 0) {
+            // The stored state itself is too close to inversion: it stays,
+            // with the particles flagged.
+            HIP_TRY(launch_flag_inverted(e->P[e->cur], e->n, e->stream));
+            stop_bits = MPM_STATUS_PARTICLE_CLOSE_TO_INVERTED | (KID_P2G << 16);
+            done = 0;
+        } else {
+            std::vector<SubstepParams> redo(params.begin(), params.begin() + f);
+            redo.push_back(redo.back());
+            if ((rc = upload_params(e, redo)) != MPM_OK) return rc;
+            LoopReport rep2;
+            if ((rc = run_loop(e, times, f, goals, nullptr, false, rep2)) != MPM_OK) return rc;
+            done = rep2.done;
+        }
+        if ((rc = fetch_scalars(e)) != MPM_OK) return rc;
+        // The failure bits of the run that stopped there (an FP32 rerun
+        // reproduces them unless the particle sat on the threshold).
+        const uint32_t fail_bits = MPM_STATUS_ENERGY_ERROR | MPM_STATUS_PARTICLE_CLOSE_TO_INVERTED;
+        const uint32_t st2 = e->ds_host->status | stop_bits;
+        st = (st2 & fail_bits) ? st2 : (st2 | (st & fail_bits));
     }
+    // An energy error leaves the time at the start of the failing substep:
+    // the reference returns before adding the step (cpu_state.rs:178-190).
+    if (f != 0xffffffffu && f >= 1 && (st & MPM_STATUS_ENERGY_ERROR) && done > 0) e->time = times[done - 1];
+    if (done > 0) e->stepped = true;
+    if (e->profiling) e->harvest();
+    int result = rep.canceled ? MPM_ERR_CANCELED : MPM_OK;
+    if (st & MPM_STATUS_INDIRECT_LIMIT_EXCEEDED) result = MPM_ERR_CAPACITY;
+    else if (st & MPM_STATUS_ENERGY_ERROR) result = MPM_ERR_ENERGY;
+    else if (st & MPM_STATUS_PARTICLE_CLOSE_TO_INVERTED) result = MPM_ERR_CLOSE_TO_INVERTED;
+    else if (result == MPM_OK && (st & MPM_STATUS_OUT_OF_GRID)) result = MPM_ERR_OUT_OF_GRID;
     if (status) {
-        status->device_status = e->ds_host->status;
+        status->device_status = st;
         status->result = result;
         status->substeps = done;
         status->time = e->time;
         status->active_blocks = e->ds_host->num_blocks;
         status->active_nodes = e->ds_host->active_nodes;
-        status->svd_fp64_fallbacks = (e->ds_host->status & MPM_STATUS_SVD_FP64) ? 1 : 0;
-        status->graph_launches = graph_launches;
+        status->svd_fp64_fallbacks = (st & MPM_STATUS_SVD_FP64) ? 1 : 0;
+        status->graph_launches = rep.graph_launches;
     }
     return result;
 }
@@ -824,13 +950,18 @@ int mpm_read_particles(mpm_engine *e, mpm_particles_view *out) {
     const size_t n = e->n;
     out->time = e->time;
     if (n == 0) return MPM_OK;
-    float *x, *v, *F, *C;
-    uint32_t *fl;
-    HIP_TRY(hipMalloc((void **)&x, n * 3 * 4));
-    HIP_TRY(hipMalloc((void **)&v, n * 3 * 4));
-    HIP_TRY(hipMalloc((void **)&F, n * 9 * 4));
-    HIP_TRY(hipMalloc((void **)&C, n * 9 * 4));
-    HIP_TRY(hipMalloc((void **)&fl, n * 4));
+    // Staging buffers of the engine, allocated once (a failed allocation
+    // leaves nothing half-made: the engine frees what it holds).
+    if (!e->rb_flags) {
+        if (e->alloc(e->rb_x, n * 3) != hipSuccess || e->alloc(e->rb_v, n * 3) != hipSuccess ||
+            e->alloc(e->rb_F, n * 9) != hipSuccess || e->alloc(e->rb_C, n * 9) != hipSuccess ||
+            e->alloc(e->rb_flags, n) != hipSuccess) {
+            e->rb_flags = nullptr;
+            return MPM_ERR_OUT_OF_MEMORY;
+        }
+    }
+    float *x = e->rb_x, *v = e->rb_v, *F = e->rb_F, *C = e->rb_C;
+    uint32_t *fl = e->rb_flags;
     int rc = MPM_OK;
     if (launch_unpermute(e->P[e->cur], n, x, v, F, C, fl, e->stream) != hipSuccess ||
         hipStreamSynchronize(e->stream) != hipSuccess) {
@@ -847,11 +978,6 @@ int mpm_read_particles(mpm_engine *e, mpm_particles_view *out) {
         down(out->elastic_energies, e->X.energy, n * 4);
         down(out->plastic_jp, e->X.jp, n * 4);
     }
-    (void)hipFree(x);
-    (void)hipFree(v);
-    (void)hipFree(F);
-    (void)hipFree(C);
-    (void)hipFree(fl);
     if (rc != MPM_OK) return rc;
     if (out->parameters) std::copy(e->parameters.begin(), e->parameters.end(), out->parameters);
     if (out->initial_positions) std::copy(e->initial_positions.begin(), e->initial_positions.end(), out->initial_positions);
@@ -1060,6 +1186,12 @@ int mpm_read_time_step_limits(mpm_engine *e, mpm_time_step_limits *out) {
     out->by_isolated = r.y;
     out->by_velocity = r.z > 0.f ? 0.5f * e->consts.grid_node_size / r.z : INFINITY;
     out->by_deformation = r.w;
+    return MPM_OK;
+}
+
+int mpm_debug_cancel_after(mpm_engine *e, int64_t substep) {
+    if (!e) return MPM_ERR_INVALID_ARGUMENT;
+    e->cancel_after = substep < 0 ? -1 : substep;
     return MPM_OK;
 }
 

@@ -83,7 +83,14 @@ struct DeviceScalars {
     uint32_t param_index;   // entry of the substep parameter table the next transfer reads
                             // (the grid update sets it to its own entry + 1)
     uint32_t grid_param;    // entry the next grid update reads (the transfer sets it)
+    uint32_t fail_index;    // smallest parameter-table entry (transfer index) of a transfer that
+                            // flagged a particle FAILED in the call (all ones: none)
 };
+
+// A particle whose deformation gradient has det F below this before its
+// stress evaluation is flagged FAILED (INVERSE_EPS, rust/crates/util/src/consts.rs:12;
+// rust/crates/gpu/src/shaders/prepare_tmp.wesl:74-78).
+constexpr float INVERSE_EPS = 1e-7f;
 
 // Candidate blocks listed per block-list entry for the grid update: the 27
 // blocks around the entry's block that its region reaches.
@@ -221,6 +228,7 @@ hipError_t launch_sum_partials(const TransferLaunch &L, const uint32_t *partial,
                                hipStream_t stream);
 hipError_t launch_copy_dead(const ParticleArrays &Pin, const ParticleArrays &Pout, const uint32_t *order,
                             uint64_t n, uint32_t *keys_out, uint32_t tomb, DeviceScalars *ds, hipStream_t stream);
+hipError_t launch_flag_inverted(const ParticleArrays &P, uint64_t n, hipStream_t stream);
 hipError_t launch_unpermute(const ParticleArrays &P, uint64_t n, float *x, float *v, float *F, float *C,
                             uint32_t *flags, hipStream_t stream);
 hipError_t launch_stress(const uint32_t *flags, const mpm_particle_parameters *params, const float *F,

@@ -171,15 +171,18 @@ __device__ __forceinline__ void fused_affine(float m, float vol, float scaling, 
 }
 
 // Stress inputs of a particle whose state holds F and C (first substep of a
-// call): polar factor / SVD, close-to-inverted check, A.  With tau_ext, a
-// snow particle's Kirchhoff stress is read from it (row k at tau_ext[k *
-// stride + slot]) instead of being evaluated.
+// call): polar factor / SVD, A.  With tau_ext, a snow particle's Kirchhoff
+// stress is read from it (row k at tau_ext[k * stride + slot]) instead of
+// being evaluated.  Returns false, with A untouched, for a particle too close
+// to inversion (det F < INVERSE_EPS): it takes no part in the scatter
+// (prepare_tmp.wesl:74-78).
 template <uint32_t SET>
-__device__ __forceinline__ void affine_from_state(uint32_t flags, const float F[9], const float C[9], float m, float vol,
+__device__ __forceinline__ bool affine_from_state(uint32_t flags, const float F[9], const float C[9], float m, float vol,
                                                   float p0, float p1, uint32_t id, const AuxArrays &X, float scaling,
                                                   float hardening, DeviceScalars *ds, const float *tau_ext,
                                                   uint64_t stride, uint64_t slot, float A[9]) {
     float J = det3(F);
+    if (!(J >= INVERSE_EPS)) return false;
     float R[9], U[9], s[3] = {1.f, 1.f, 1.f};
     float hard = 1.0f;
     const bool ext = (SET & M_SNOW) && (flags & F_SNOW) && tau_ext;
@@ -195,8 +198,6 @@ __device__ __forceinline__ void affine_from_state(uint32_t flags, const float F[
             matmul3_bt(U, V, R);
         }
     }
-    if (!(J >= 1e-7f) && !(flags & F_FLUID))
-        atomicOr(&ds->status, MPM_STATUS_PARTICLE_CLOSE_TO_INVERTED | (KID_P2G << 16));
     float tau[9];
     if (ext) {
 #pragma unroll
@@ -211,6 +212,15 @@ __device__ __forceinline__ void affine_from_state(uint32_t flags, const float F[
         vb = X.visc_bulk[id];
     }
     fused_affine<SET>(m, vol, scaling, C, tau, J, visc, vd, vb, A);
+    return true;
+}
+
+// A particle flagged FAILED in this transfer: the status bits, and the
+// transfer's parameter-table entry as the call's first failing transfer if
+// no earlier one failed.
+__device__ __forceinline__ void report_failure(DeviceScalars *ds, uint32_t bits, uint32_t transfer) {
+    atomicOr(&ds->status, bits);
+    atomicMin(&ds->fail_index, transfer);
 }
 
 // Separable gather of v = sum w v_i and C = 4/h sum w v_i (i - f)^T over the
@@ -667,6 +677,7 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                 float R[9], U[9], sv[3] = {1.f, 1.f, 1.f};
                 float hard = 1.0f;
                 bool inside = true;
+                uint32_t failed = 0;  // status bits of a failure detected in this transfer
                 float A[9];
 
                 if constexpr (HAS_G2P) {
@@ -712,15 +723,20 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                     float energy = 0.f;
                     J = det3(F);
                     if ((SET & M_FLUID) && (flags & F_FLUID)) {
-                        // advance_particles.rs:76-86: singular values <- (s1 s2 s3)^(1/3),
-                        // i.e. F <- J^(1/3) R with R the polar factor of F.
+                        // advance_particles.rs:76-86: singular values <- (s1 s2 s3)^(1/3)
+                        // (nalgebra's, >= 0), i.e. F <- |J|^(1/3) R with R the orthogonal
+                        // polar factor of F (a reflection when det F < 0).
                         if (!(J > 0.f) || !polar3(F, R)) {
                             float V[9];
                             if (svd3_call(F, U, sv, V)) atomicOr(&ds->status, MPM_STATUS_SVD_FP64);
+                            if (sv[2] < 0.f) {
+#pragma unroll
+                                for (int k = 6; k < 9; ++k) V[k] = -V[k];  // U diag(1, 1, -1) V^T
+                            }
                             matmul3_bt(U, V, R);
                             J = sv[0] * sv[1] * sv[2];
                         }
-                        const float g = cbrtf(J);
+                        const float g = cbrtf(fabsf(J));
 #pragma unroll
                         for (int k = 0; k < 9; ++k) F[k] = g * R[k];
                         J = det3(F);
@@ -755,7 +771,10 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                             float V[9];
                             if (svd3_call(F, U, sv, V)) atomicOr(&ds->status, MPM_STATUS_SVD_FP64);
                             const float alpha = a.X.sand_alpha[id];
-                            const float e[3] = {logf(sv[0]), logf(sv[1]), logf(sv[2])};
+                            // nalgebra's singular values are >= 0: the log strain of |s|, the
+                            // sign of det F kept in the recomposition (a reflection for det F < 0)
+                            const float sg = sv[2] < 0.f ? -1.f : 1.f;
+                            const float e[3] = {logf(fabsf(sv[0])), logf(fabsf(sv[1])), logf(fabsf(sv[2]))};
                             const float etr = e[0] + e[1] + e[2];
                             const float eh[3] = {e[0] - etr / 3.f, e[1] - etr / 3.f, e[2] - etr / 3.f};
                             const float ehn = sqrtf(eh[0] * eh[0] + eh[1] * eh[1] + eh[2] * eh[2]);
@@ -765,10 +784,11 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                                     float d[3];
 #pragma unroll
                                     for (int q = 0; q < 3; ++q) d[q] = expf(e[q] - dg / ehn * eh[q]);
+                                    d[2] *= sg;
                                     recompose(U, d, V, F);
                                 }
                             } else {
-                                const float one[3] = {1.f, 1.f, 1.f};
+                                const float one[3] = {1.f, 1.f, sg};
                                 recompose(U, one, V, F);
                             }
                             J = det3(F);
@@ -782,9 +802,9 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                             if (LAST) energy = energy_fixed_corotated_polar(p0, p1, F, R, J);
                         } else {
                             if (!(J > 0.f)) {
-                                // try_elastic_energy_neo_hookean fails: EnergyError, particle FAILED.
-                                flags |= F_FAILED;
-                                atomicOr(&ds->status, MPM_STATUS_ENERGY_ERROR | (KID_G2P << 16));
+                                // try_elastic_energy_neo_hookean fails: EnergyError, particle FAILED
+                                // (advance_particles.rs:73-74).
+                                failed = MPM_STATUS_ENERGY_ERROR;
                             } else if (LAST) {
                                 energy = energy_neo_hookean(p0, p1, F, J);
                             }
@@ -799,6 +819,14 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                         inside = inside && xl > S1.domain_min[q] && xl < S1.domain_max[q];
                     }
                     if (!inside) flags |= F_TOMB;
+                    // The next substep's stress needs det F >= INVERSE_EPS
+                    // (prepare_tmp.wesl:74-78): a particle below it is FAILED
+                    // and takes no part in any later scatter of the call.
+                    if (inside && !(J >= INVERSE_EPS)) failed |= MPM_STATUS_PARTICLE_CLOSE_TO_INVERTED;
+                    if (failed) {
+                        flags |= F_FAILED;
+                        report_failure(ds, failed | (KID_G2P << 16), pidx);
+                    }
 
                     if (a.keys_out) a.keys_out[i] = inside ? particle_block_key(xp, G, ds) : G.tomb_key;
 #pragma unroll
@@ -810,7 +838,8 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                         for (int q = 0; q < 3; ++q) Pout.v(q, i) = v[q];
 #pragma unroll
                         for (int k = 0; k < 9; ++k) Pout.A(k, i) = C[k];
-                        a.X.energy[id] = energy;
+                        // a failed energy evaluation leaves the stored energy
+                        if (!(failed & MPM_STATUS_ENERGY_ERROR)) a.X.energy[id] = energy;
                     }
                     if (PERMUTE || flags != flags0) Pout.flags(i) = flags;
                     if (PERMUTE) {
@@ -820,7 +849,7 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                         Pout.p1(i) = p1;
                         Pout.id(i) = id;
                     }
-                    if (LAST || !inside) continue;
+                    if (LAST || !inside || failed) continue;
                     // Stress of the new F for the next substep's scatter.
                     float tau[9];
                     kirchhoff<SET>(flags, p0, p1, hard, F, J, R, U, sv, tau);
@@ -837,10 +866,16 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                     for (int q = 0; q < 3; ++q) v[q] = Pin.v(q, src);
 #pragma unroll
                     for (int k = 0; k < 9; ++k) C[k] = Pin.A(k, src);
-                    affine_from_state<SET>(flags, F, C, m, vol, p0, p1, id, a.X, scaling, S1.snow_hardening, ds,
-                                           a.tau_ext, a.tau_stride, i, A);
+                    const bool ok = affine_from_state<SET>(flags, F, C, m, vol, p0, p1, id, a.X, scaling,
+                                                           S1.snow_hardening, ds, a.tau_ext, a.tau_stride, i, A);
                     if (PERMUTE) {
                         for (int f = 0; f < NFIELDS; ++f) Pout.at(i)[f * TILE] = Pin.at(src)[f * TILE];
+                    }
+                    if (!ok) {
+                        flags |= F_FAILED;
+                        Pout.flags(i) = flags;
+                        report_failure(ds, MPM_STATUS_PARTICLE_CLOSE_TO_INVERTED | (KID_P2G << 16), pidx);
+                        continue;
                     }
                 }
 
@@ -1197,6 +1232,20 @@ __global__ __launch_bounds__(256) void k_copy_dead(ParticleArrays Pin, ParticleA
     }
 }
 
+// The live particles of a stored state whose det F is below INVERSE_EPS are
+// flagged FAILED (the check of prepare_tmp.wesl:74-78 on a state that a call
+// stops at before its first substep).
+__global__ __launch_bounds__(256) void k_flag_inverted(ParticleArrays P, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t fl = P.flags(i);
+        if (fl & F_TOMB) continue;
+        float F[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) F[k] = P.F(k, i);
+        if (!(det3(F) >= INVERSE_EPS)) P.flags(i) = fl | F_FAILED;
+    }
+}
+
 // to_io_state's reverse permutation (rust/crates/cpu/src/cpu_state.rs:84-93):
 // out[id] = state[i], vectors as xyz triples, matrices column-major.
 __global__ __launch_bounds__(256) void k_unpermute(ParticleArrays P, uint64_t n, float *__restrict__ x,
@@ -1373,6 +1422,13 @@ hipError_t launch_copy_dead(const ParticleArrays &Pin, const ParticleArrays &Pou
     if (n == 0) return hipSuccess;
     const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 1024);
     hipLaunchKernelGGL(k_copy_dead, dim3(grid), dim3(256), 0, stream, Pin, Pout, order, n, keys_out, tomb, ds);
+    return hipGetLastError();
+}
+
+hipError_t launch_flag_inverted(const ParticleArrays &P, uint64_t n, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_flag_inverted, dim3(grid), dim3(256), 0, stream, P, n);
     return hipGetLastError();
 }
 

@@ -225,6 +225,11 @@ class GpuState:
         _check(self._lib, self._lib.mpm_read_time_step_limits(self._h, C.byref(out)))
         return {k: float(getattr(out, k)) for k, _ in abi.MpmTimeStepLimits._fields_}
 
+    def debug_cancel_after(self, substep: int):
+        """Test hook: the following calls behave as cancelled before their
+        substep `substep` (mpm_debug_cancel_after); negative turns it off."""
+        _check(self._lib, self._lib.mpm_debug_cancel_after(self._h, int(substep)))
+
     def set_time_step(self, time_step: float):
         _check(self._lib, self._lib.mpm_set_time_step(self._h, float(time_step)))
 
@@ -233,7 +238,9 @@ class GpuState:
         max_time_step and the current limits (allowed_time_step_without_prior,
         rust/crates/cpu/src/adaptive_time_step_state.rs:36-47, taken once per
         call instead of twice per substep); returns it."""
-        dt = float(np.float32(min([max_time_step, *self.time_step_limits().values()])))
+        # a limit without particles (None in the reference) counts as f32::MAX
+        f32_max = float(np.finfo(np.float32).max)
+        dt = float(np.float32(min([min(max_time_step, f32_max), *(min(v, f32_max) for v in self.time_step_limits().values())])))
         self.set_time_step(dt)
         return dt
 
