@@ -179,9 +179,11 @@ template <class R> M3<R> cauchy_viscosity(R dynamic, R bulk, const M3<R> &C) {
 
 // ---------------------------------------------------------------- 3x3 SVD
 // Signed SVD F = U diag(s) V^T with U, V proper rotations; s[0] >= s[1] >= |s[2]|,
-// s[2] < 0 iff det F < 0.  Cyclic Jacobi on F^T F, then a Givens QR of F V.
-// Written for the oracle from the textbook method (Golub & Van Loan, 8.5);
-// it is deliberately not the engine's algorithm.
+// s[2] < 0 iff det F < 0.  Cyclic Jacobi on F^T F, then a Givens QR of F V,
+// written from the textbook method (Golub & Van Loan, 8.5).  The engine's SVD
+// follows the same method (in float, with an FP64 fallback); what makes this
+// one an independent check is its comparison with numpy.linalg.svd on the
+// reference's sampling of F (tests/test_oracle_math.py).
 template <class R> void jacobi_rotate(M3<R> &A, M3<R> &V, int p, int q) {
     R apq = A.m[p][q];
     if (apq == 0) return;
@@ -263,6 +265,57 @@ template <class R> void svd3(const M3<R> &F, M3<R> &U, V3<R> &s, M3<R> &V) {
             U.set_col(j, U.col(j) * R(-1));
             U.set_col(2, U.col(2) * R(-1));
         }
+}
+
+// ---- plasticity of the advance (advance_particles.rs:43-86) ------------------
+// nalgebra's SVD has non-negative singular values; for det F < 0 its U V^T is
+// a reflection.  In terms of the signed SVD: |s|, and the sign g = sgn s[2]
+// folded back as diag(1, 1, g).  Both projections are functions of the
+// singular values, so the result does not depend on which SVD was taken.
+
+// Drucker-Prager return map in log strain (advance_particles.rs:50-71).
+template <class R> M3<R> sand_project(const M3<R> &F, R mu, R lambda, R alpha) {
+    M3<R> U, V;
+    V3<R> s;
+    svd3(F, U, s, V);
+    const R g = s[2] < 0 ? R(-1) : R(1);
+    V3<R> e(std::log(std::abs(s[0])), std::log(std::abs(s[1])), std::log(std::abs(s[2])));
+    R e_tr = e.x + e.y + e.z;
+    V3<R> e_hat = e - V3<R>(e_tr / 3, e_tr / 3, e_tr / 3);
+    R e_hat_norm = std::sqrt(dot(e_hat, e_hat));
+    if (e_tr < 0 && e_hat_norm > 0) {
+        R delta_gamma = e_hat_norm + (R(3) * lambda + R(2) * mu) / R(2) / mu * e_tr * alpha;
+        if (delta_gamma > 0) {
+            V3<R> H = e - e_hat * (delta_gamma / e_hat_norm);
+            return U * M3<R>::diag(std::exp(H.x), std::exp(H.y), g * std::exp(H.z)) * V.transpose();
+        }
+        return F;
+    }
+    return U * M3<R>::diag(R(1), R(1), g) * V.transpose();
+}
+
+// Fluid: every singular value <- (s1 s2 s3)^(1/3) (advance_particles.rs:76-86),
+// i.e. |det F|^(1/3) times the orthogonal polar factor of F.
+template <class R> M3<R> fluid_project(const M3<R> &F) {
+    M3<R> U, V;
+    V3<R> s;
+    svd3(F, U, s, V);
+    const R g = s[2] < 0 ? R(-1) : R(1);
+    const R c = std::cbrt(std::abs(s[0] * s[1] * s[2]));
+    return U * M3<R>::diag(c, c, g * c) * V.transpose();
+}
+
+// Extension (no reference counterpart): separating slip on a sphere.  A node
+// at xi inside the sphere loses the velocity component along the inward
+// normal; its tangential component and any outward motion stay.
+template <class R> V3<R> sphere_slip(const V3<R> &xi, const V3<R> &center, R radius, V3<R> v) {
+    const V3<R> d = xi - center;
+    const R r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+    if (r2 < radius * radius && r2 > 0) {
+        const R vn = (v.x * d.x + v.y * d.y + v.z * d.z) / r2;
+        if (vn < 0) v = v - d * vn;
+    }
+    return v;
 }
 
 // ---- extensions: fixed-corotated and snow (no reference counterpart) --------
@@ -593,16 +646,9 @@ struct State {
                 if (xi[a] > R(consts.wall_max[a]) && v[a] > 0) v[a] = 0;
             }
         }
-        if (consts.options & MPM_OPT_SPHERE) {
-            const V3<R> d(xi.x - R(consts.sphere_center[0]), xi.y - R(consts.sphere_center[1]),
-                          xi.z - R(consts.sphere_center[2]));
-            const R r2 = d.x * d.x + d.y * d.y + d.z * d.z;
-            const R rad = R(consts.sphere_radius);
-            if (r2 < rad * rad && r2 > 0) {
-                const R vn = (v.x * d.x + v.y * d.y + v.z * d.z) / r2;
-                if (vn < 0) v = v - d * vn;
-            }
-        }
+        if (consts.options & MPM_OPT_SPHERE)
+            v = sphere_slip<R>(xi, V3<R>(R(consts.sphere_center[0]), R(consts.sphere_center[1]), R(consts.sphere_center[2])),
+                               R(consts.sphere_radius), v);
     }
 
     // ---- CollectVelocity, rust/crates/cpu/src/phase/collect_velocity.rs:19-75.
@@ -657,12 +703,7 @@ struct State {
             F += velocity_gradients[i] * F * t;
             const mpm_particle_parameters &p = parameters[i];
             if (flags[i] & MPM_IS_FLUID) {
-                // advance_particles.rs:76-86: singular values <- (s1 s2 s3)^(1/3).
-                M3<R> U, V;
-                V3<R> s;
-                svd3(F, U, s, V);
-                R g = std::cbrt(s[0] * s[1] * s[2]);
-                F = U * M3<R>::diag(g, g, g) * V.transpose();
+                F = fluid_project<R>(F);
                 elastic_energies[i] = energy_inviscid<R>(p.bulk_modulus, p.exponent, F);
                 continue;
             }
@@ -685,26 +726,7 @@ struct State {
                 elastic_energies[i] = e;
                 continue;
             }
-            if ((flags[i] & MPM_USE_SAND_ALPHA)) {
-                // advance_particles.rs:50-71: Drucker-Prager return map in log strain.
-                M3<R> U, V;
-                V3<R> s;
-                svd3(F, U, s, V);
-                R mu = p.mu, lambda = p.lambda, alpha = p.sand_alpha;
-                V3<R> e(std::log(s[0]), std::log(s[1]), std::log(s[2]));
-                R e_tr = e.x + e.y + e.z;
-                V3<R> e_hat = e - V3<R>(e_tr / 3, e_tr / 3, e_tr / 3);
-                R e_hat_norm = std::sqrt(dot(e_hat, e_hat));
-                if (e_tr < 0 && e_hat_norm > 0) {
-                    R delta_gamma = e_hat_norm + (R(3) * lambda + R(2) * mu) / R(2) / mu * e_tr * alpha;
-                    if (delta_gamma > 0) {
-                        V3<R> H = e - e_hat * (delta_gamma / e_hat_norm);
-                        F = U * M3<R>::diag(std::exp(H.x), std::exp(H.y), std::exp(H.z)) * V.transpose();
-                    }
-                } else {
-                    F = U * V.transpose();
-                }
-            }
+            if ((flags[i] & MPM_USE_SAND_ALPHA)) F = sand_project<R>(F, R(p.mu), R(p.lambda), R(p.sand_alpha));
             if (flags[i] & MPM_FIXED_COROTATED) {
                 R e;
                 kirchhoff_fixed_corotated<R>(R(p.mu), R(p.lambda), F, e);

@@ -182,9 +182,52 @@ void svd_impl(const double *F, double *U, double *s, double *V, uint64_t n) {
     }
 }
 
+template <class R> M3<R> mat_in(const double *a) {
+    M3<R> f;
+    for (int col = 0; col < 3; ++col)
+        for (int r = 0; r < 3; ++r) f.m[r][col] = R(a[3 * col + r]);
+    return f;
+}
+template <class R> void mat_out(const M3<R> &f, double *a) {
+    for (int col = 0; col < 3; ++col)
+        for (int r = 0; r < 3; ++r) a[3 * col + r] = (double)f.m[r][col];
+}
+
+template <class R>
+void plasticity_impl(int kind, const double *F, double mu, double lambda, double alpha, double *out, uint64_t n) {
+    for (uint64_t i = 0; i < n; ++i) {
+        const M3<R> f = mat_in<R>(F + 9 * i);
+        mat_out<R>(kind == 0 ? sand_project<R>(f, R(mu), R(lambda), R(alpha)) : fluid_project<R>(f), out + 9 * i);
+    }
+}
+
 }  // namespace
 
 extern "C" {
+
+// The advance's plasticity projection of n deformation gradients (column-major):
+// kind 0 = sand (Drucker-Prager with mu, lambda, alpha), 1 = fluid.
+void oracle_plasticity(int precision, int kind, const double *F, double mu, double lambda, double alpha, double *out,
+                       uint64_t n) {
+    if (precision == 8) plasticity_impl<double>(kind, F, mu, lambda, alpha, out, n);
+    else plasticity_impl<float>(kind, F, mu, lambda, alpha, out, n);
+}
+
+// cauchy_stress_general_viscosity of n velocity gradients (column-major), FP64.
+void oracle_cauchy_viscosity(double dynamic, double bulk, const double *C, double *out, uint64_t n) {
+    for (uint64_t i = 0; i < n; ++i) mat_out<double>(cauchy_viscosity<double>(dynamic, bulk, mat_in<double>(C + 9 * i)), out + 9 * i);
+}
+
+// The sphere collider's slip on n node velocities v at node positions xi, FP64.
+void oracle_sphere_slip(const double *xi, const double *center, double radius, const double *v, double *out,
+                        uint64_t n) {
+    const V3<double> c(center[0], center[1], center[2]);
+    for (uint64_t i = 0; i < n; ++i) {
+        const V3<double> r = sphere_slip<double>(V3<double>(xi[3 * i], xi[3 * i + 1], xi[3 * i + 2]), c, radius,
+                                                 V3<double>(v[3 * i], v[3 * i + 1], v[3 * i + 2]));
+        for (int a = 0; a < 3; ++a) out[3 * i + a] = r[a];
+    }
+}
 
 void oracle_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
 int oracle_max_threads(void) { return omp_get_max_threads(); }

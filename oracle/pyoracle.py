@@ -32,38 +32,49 @@ def build(verbose: bool = False) -> Path:
     return LIB_PATH
 
 
-def load():
+def load(path: Optional[Path] = None):
+    """The oracle library (built on first use), or the one at `path` (a
+    separately compiled variant, for the tests that check that the pinning
+    tests fail on a planted mistake)."""
     global _LIB
+    if path is not None:
+        return _configure(C.CDLL(str(path)))
     if _LIB is None:
         if not LIB_PATH.exists():
             build()
-        lib = C.CDLL(str(LIB_PATH))
-        V = C.c_void_p
-        lib.oracle_create.restype = V
-        lib.oracle_create.argtypes = [C.c_int, C.POINTER(abi.MpmConsts), C.POINTER(abi.MpmParticlesView)]
-        lib.oracle_destroy.argtypes = [V]
-        lib.oracle_time.restype = C.c_double
-        lib.oracle_time.argtypes = [V]
-        lib.oracle_step_to.argtypes = [V, C.c_double, C.POINTER(abi.MpmFrameInput), V, C.POINTER(abi.MpmStatus)]
-        lib.oracle_run_to_scatter.argtypes = [V, C.POINTER(abi.MpmFrameInput)]
-        lib.oracle_run_from_meld.argtypes = [V]
-        lib.oracle_read_particles.argtypes = [V, C.POINTER(abi.MpmParticlesView)]
-        lib.oracle_read_particles_f64.argtypes = [V, V, V, V, V]
-        lib.oracle_read_grid.restype = C.c_int64
-        lib.oracle_read_grid.argtypes = [V, V, V, V, C.c_int64]
-        lib.oracle_read_grid_f64.argtypes = [V, V, V]
-        lib.oracle_stress.argtypes = [C.c_int, V, V, V, V, C.c_double, V, V, V, C.c_uint64]
-        lib.oracle_svd.argtypes = [C.c_int, V, V, V, V, C.c_uint64]
-        lib.oracle_kernel_quadratic.restype = C.c_double
-        lib.oracle_kernel_quadratic.argtypes = [C.c_double]
-        lib.oracle_lame_mu.restype = C.c_double
-        lib.oracle_lame_mu.argtypes = [C.c_double, C.c_double]
-        lib.oracle_lame_lambda.restype = C.c_double
-        lib.oracle_lame_lambda.argtypes = [C.c_double, C.c_double]
-        lib.oracle_set_threads.argtypes = [C.c_int]
-        lib.oracle_max_threads.restype = C.c_int
-        _LIB = lib
+        _LIB = _configure(C.CDLL(str(LIB_PATH)))
     return _LIB
+
+
+def _configure(lib):
+    V = C.c_void_p
+    lib.oracle_create.restype = V
+    lib.oracle_create.argtypes = [C.c_int, C.POINTER(abi.MpmConsts), C.POINTER(abi.MpmParticlesView)]
+    lib.oracle_destroy.argtypes = [V]
+    lib.oracle_time.restype = C.c_double
+    lib.oracle_time.argtypes = [V]
+    lib.oracle_step_to.argtypes = [V, C.c_double, C.POINTER(abi.MpmFrameInput), V, C.POINTER(abi.MpmStatus)]
+    lib.oracle_run_to_scatter.argtypes = [V, C.POINTER(abi.MpmFrameInput)]
+    lib.oracle_run_from_meld.argtypes = [V]
+    lib.oracle_read_particles.argtypes = [V, C.POINTER(abi.MpmParticlesView)]
+    lib.oracle_read_particles_f64.argtypes = [V, V, V, V, V]
+    lib.oracle_read_grid.restype = C.c_int64
+    lib.oracle_read_grid.argtypes = [V, V, V, V, C.c_int64]
+    lib.oracle_read_grid_f64.argtypes = [V, V, V]
+    lib.oracle_stress.argtypes = [C.c_int, V, V, V, V, C.c_double, V, V, V, C.c_uint64]
+    lib.oracle_svd.argtypes = [C.c_int, V, V, V, V, C.c_uint64]
+    lib.oracle_kernel_quadratic.restype = C.c_double
+    lib.oracle_kernel_quadratic.argtypes = [C.c_double]
+    lib.oracle_lame_mu.restype = C.c_double
+    lib.oracle_lame_mu.argtypes = [C.c_double, C.c_double]
+    lib.oracle_lame_lambda.restype = C.c_double
+    lib.oracle_lame_lambda.argtypes = [C.c_double, C.c_double]
+    lib.oracle_plasticity.argtypes = [C.c_int, C.c_int, V, C.c_double, C.c_double, C.c_double, V, C.c_uint64]
+    lib.oracle_cauchy_viscosity.argtypes = [C.c_double, C.c_double, V, V, C.c_uint64]
+    lib.oracle_sphere_slip.argtypes = [V, V, C.c_double, V, V, C.c_uint64]
+    lib.oracle_set_threads.argtypes = [C.c_int]
+    lib.oracle_max_threads.restype = C.c_int
+    return lib
 
 
 def set_threads(n: int):
@@ -178,3 +189,50 @@ def svd(F: np.ndarray, precision: int = 8):
 
 def kernel_quadratic(x: float) -> float:
     return load().oracle_kernel_quadratic(x)
+
+
+def _mats_in(M: np.ndarray) -> np.ndarray:
+    """(N, row, col) -> column-major rows of 9."""
+    return np.ascontiguousarray(np.transpose(np.asarray(M, np.float64).reshape(-1, 3, 3), (0, 2, 1)))
+
+
+def _mats_out(a: np.ndarray) -> np.ndarray:
+    return np.transpose(a.reshape(-1, 3, 3), (0, 2, 1))
+
+
+def sand_project(F: np.ndarray, mu: float, lam: float, alpha: float, precision: int = 8, lib=None) -> np.ndarray:
+    """The advance's Drucker-Prager projection of deformation gradients F (N, row, col)."""
+    lib = lib or load()
+    Fc = _mats_in(F)
+    out = np.zeros_like(Fc)
+    lib.oracle_plasticity(precision, 0, abi.ptr(Fc), mu, lam, alpha, abi.ptr(out), Fc.shape[0])
+    return _mats_out(out)
+
+
+def fluid_project(F: np.ndarray, precision: int = 8, lib=None) -> np.ndarray:
+    """The advance's fluid projection (singular values to their geometric mean)."""
+    lib = lib or load()
+    Fc = _mats_in(F)
+    out = np.zeros_like(Fc)
+    lib.oracle_plasticity(precision, 1, abi.ptr(Fc), 0.0, 0.0, 0.0, abi.ptr(out), Fc.shape[0])
+    return _mats_out(out)
+
+
+def cauchy_viscosity(dynamic: float, bulk: float, C: np.ndarray, lib=None) -> np.ndarray:
+    """cauchy_stress_general_viscosity of velocity gradients C (N, row, col), FP64."""
+    lib = lib or load()
+    Cc = _mats_in(C)
+    out = np.zeros_like(Cc)
+    lib.oracle_cauchy_viscosity(dynamic, bulk, abi.ptr(Cc), abi.ptr(out), Cc.shape[0])
+    return _mats_out(out)
+
+
+def sphere_slip(xi: np.ndarray, center, radius: float, v: np.ndarray, lib=None) -> np.ndarray:
+    """The sphere collider's slip of node velocities v (N, 3) at node positions xi (N, 3), FP64."""
+    lib = lib or load()
+    x = np.ascontiguousarray(xi, np.float64)
+    vv = np.ascontiguousarray(v, np.float64)
+    c = np.ascontiguousarray(center, np.float64)
+    out = np.zeros_like(vv)
+    lib.oracle_sphere_slip(abi.ptr(x), abi.ptr(c), radius, abi.ptr(vv), abi.ptr(out), x.shape[0])
+    return out

@@ -57,6 +57,12 @@ lame = [[float(a), float(b)] for a, b in re.findall(rf"\[({num})\.?,\s*({num})\.
 inv_block = re.search(r"fn test_inviscid_parameters.*?\[(\(.*?\))\]\s*\.into_iter", el, re.S).group(1)
 inviscid = [[float(a), int(b)] for a, b in re.findall(rf"\(({num})\.?,\s*(\d+)\)", inv_block)]
 
+# test_viscosity_parameters: (a..b).flat_map(|i| (c..d).map(|j| [i, j])), the
+# (dynamic, bulk) pairs of a product of two integer ranges
+visc_body = body_of(el, "test_viscosity_parameters")
+(i0, i1), (j0, j1) = [(int(a), int(b)) for a, b in re.findall(r"\((\d+)\.\.(\d+)\)", visc_body)]
+viscosity = [[float(i), float(j)] for i in range(i0, i1) for j in range(j0, j1)]
+
 rs = (REF / "rust/crates/gpu/src/radix_sort/test.rs").read_text()
 sort_simple = [int(k) for k in re.search(r"let keys = \[([\d,\s]+)\];", rs).group(1).split(",") if k.strip()]
 sort_specific = [int(k) for k in re.findall(r"let keys = \[([\d,\s]+)\];", rs)[1].split(",") if k.strip()]
@@ -68,12 +74,13 @@ data = {
     "specific_positions_and_collider_bits": specific,
     "lame_parameters_E_nu": lame,
     "inviscid_parameters_K_exponent": inviscid,
+    "viscosity_parameters_dynamic_bulk": viscosity,
     "radix_sort_simple_keys": sort_simple,
     "radix_sort_specific_keys": sort_specific,
 }
 assert len(many) == 512, len(many)
 assert len(simple) == 9, len(simple)
 assert len(specific) == 512, len(specific)
-assert len(lame) == 4 and len(inviscid) == 4
+assert len(lame) == 4 and len(inviscid) == 4 and len(viscosity) == 64
 OUT.write_text(json.dumps(data))
 print(f"wrote {OUT}: {len(many)} positions, {len(simple)} F, {len(specific)} specific, lame {lame}, inviscid {inviscid}, sort {sort_simple} / {sort_specific}")

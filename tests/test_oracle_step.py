@@ -7,6 +7,7 @@ rust/crates/cpu/src/phase/*.rs fails it: a dropped or mis-scaled stress term
 gravity placement, culling strictness, goal velocities, the energy-error path.
 """
 import numpy as np
+import scipy.linalg
 import pytest
 
 from oracle import pyoracle
@@ -218,12 +219,21 @@ def test_fluid_projection_keeps_only_volume():
     parts.parameters["exponent"] = 7
     rng = np.random.default_rng(2)
     parts.velocity_gradients[:] = rng.uniform(-1, 1, (parts.count, 3, 3))
-    o, st = run_one(parts, 0.5, 1e-2)
-    F = o.particles_f64()["F"]
+    dt = 1e-2
+    o, st = run_one(parts, 0.5, dt)
+    r = o.particles_f64()
+    F = r["F"]
     J = np.linalg.det(F)
     # F = J^(1/3) R  =>  F F^T = J^(2/3) I
     assert np.allclose(F @ np.transpose(F, (0, 2, 1)), (J ** (2 / 3))[:, None, None] * np.eye(3), atol=1e-10)
     assert np.any(np.abs(J - 1) > 1e-4)
+    # the projection keeps the advanced volume: J = det(F0 + dt C F0), F0 = I,
+    # with C the gathered velocity gradient the advance used
+    F_pre = np.eye(3) + float(np.float32(dt)) * r["C"]  # the step is an f32
+    assert np.allclose(J, np.linalg.det(F_pre), rtol=1e-12, atol=0)
+    # and the rotation: F / J^(1/3) is the orthogonal polar factor of F_pre
+    Q = np.array([scipy.linalg.polar(f)[0] for f in F_pre])
+    assert np.allclose(F / np.cbrt(J)[:, None, None], Q, atol=1e-12)
 
 
 def test_snow_clamps_singular_values_and_tracks_plastic_volume():
