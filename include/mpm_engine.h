@@ -304,6 +304,19 @@ int mpm_dev_radix_sort(const uint32_t *keys_in, const uint32_t *values_in, uint3
                        uint32_t *values_out, uint64_t n, int32_t key_bits, void *temp,
                        void *stream);
 
+/* The same sort without a temp buffer, for callers that cannot allocate
+ * device memory (code under stream capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  Its working
+ * memory is a device array of 34 MiB that each GPU holds from the library's
+ * load to the process's end; inputs of more than
+ * mpm_dev_radix_sort_noalloc_max_keys() keys (2^22) return MPM_ERR_CAPACITY.
+ * Calls on one GPU share that array: calls that overlap in time on one GPU
+ * (other streams, other threads) race and must be serialised by the caller.
+ * The call allocates nothing. */
+uint64_t mpm_dev_radix_sort_noalloc_max_keys(void);
+int mpm_dev_radix_sort_noalloc(const uint32_t *keys_in, const uint32_t *values_in, uint32_t *keys_out,
+                               uint32_t *values_out, uint64_t n, int32_t key_bits, void *stream);
+
 /* Exclusive prefix sum of u32 (prefix_sum_on_cpu, rust/crates/gpu/src/util.rs:54-63);
  * *total receives the sum (device pointer). */
 size_t mpm_dev_scan_temp_bytes(uint64_t n);

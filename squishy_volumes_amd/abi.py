@@ -212,6 +212,10 @@ SIGNATURES = {
         C.c_int,
         [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_int32, C.c_void_p, C.c_void_p],
     ),
+    "mpm_dev_radix_sort_noalloc_max_keys": (C.c_uint64, []),
+    "mpm_dev_radix_sort_noalloc": (
+        C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_int32, C.c_void_p],
+    ),
     "mpm_dev_scan_temp_bytes": (C.c_size_t, [C.c_uint64]),
     "mpm_dev_exclusive_scan": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]),
     "mpm_dev_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
