@@ -51,8 +51,10 @@ enum {
  * GpuStatus (rust/crates/gpu/src/status.rs:18-77).  Bits 1-8 keep the
  * reference values; 16.. are errors this engine adds. */
 enum {
-    MPM_STATUS_TABLE_TRIES_EXCEEDED = 1u,      /* never set: the grid is not hashed */
-    MPM_STATUS_TABLE_ENTRY_MISSING = 2u,       /* never set: the grid is not hashed */
+    MPM_STATUS_TABLE_TRIES_EXCEEDED = 1u,      /* the table of collider twin nodes overflowed (the call
+                                                  is redone with a larger one, as the reference redoes a
+                                                  frame, rust/crates/gpu/src/gpu_state.rs:562-599) */
+    MPM_STATUS_TABLE_ENTRY_MISSING = 2u,       /* a gather found no twin node of its collider bits */
     MPM_STATUS_INDIRECT_LIMIT_EXCEEDED = 4u,   /* a block held more particles than a work list can */
     MPM_STATUS_PARTICLE_CLOSE_TO_INVERTED = 8u,/* det F < 1e-7 before the stress evaluation */
     MPM_STATUS_ENERGY_ERROR = 16u,             /* det F <= 0 after the F update (CPU EnergyError) */
@@ -164,8 +166,15 @@ typedef struct mpm_particles_view {
 } mpm_particles_view;
 
 /* Per-frame inputs: the parts of FrameInput (rust/crates/xpu/src/frame_input.rs:33-278)
- * the substep reads without mesh colliders.  `a` is the frame that contains
- * the current time, `b` the next one (or absent at the end of the input). */
+ * the substep reads.  `a` is the frame that contains the current time, `b`
+ * the next one (or absent at the end of the input).  The collider fields are
+ * the frames' InputFrame data of the mesh set with mpm_set_collider_mesh
+ * (vertex positions, and the per-triangle friction and damping); they are
+ * required while a mesh is set and ignored otherwise.  The mesh is
+ * interpolated per substep with the frame factor, as the reference's
+ * interpolate_input does (rust/crates/cpu/src/phase/interpolate_input.rs:18-106),
+ * and its vertex velocities are (b - a) * frames_per_second, or 0 without b
+ * (linear_vertex_velocities, rust/crates/xpu/src/frame_input.rs:333-347). */
 typedef struct mpm_frame_input {
     uint64_t frame;                      /* index of frame a */
     float gravity_a[3];
@@ -175,7 +184,26 @@ typedef struct mpm_frame_input {
     const uint32_t *particle_flags_b;
     const float *goal_positions_a;       /* count*3 */
     const float *goal_positions_b;
+    const float *vertex_positions_a;     /* num_vertices*3 of the collider mesh */
+    const float *vertex_positions_b;
+    const float *triangle_frictions_a;   /* num_triangles */
+    const float *triangle_frictions_b;
+    const float *triangle_dampings_a;    /* num_triangles */
+    const float *triangle_dampings_b;
 } mpm_frame_input;
+
+/* The collider mesh's topology (Topology, rust/crates/mesh_util/src/mesh.rs;
+ * set up once per simulation like the reference's FrameInput): triangles as
+ * vertex index triples, and the collider (0..15) each triangle belongs to.
+ * The engine derives the triangles across each edge (triangle_opposites) and
+ * the triangles at each vertex itself.  Collider bits per particle hold, for
+ * collider c, bit 16 + c "near" and bit c "side" (collider_bits.rs:9-38). */
+typedef struct mpm_collider_mesh {
+    uint32_t num_vertices;
+    uint32_t num_triangles;
+    const uint32_t *triangles;           /* num_triangles*3 */
+    const uint32_t *triangle_colliders;  /* num_triangles, each < 16 */
+} mpm_collider_mesh;
 
 /* Report of one mpm_step_to call. */
 typedef struct mpm_status {
@@ -262,6 +290,16 @@ int mpm_read_particles(mpm_engine *engine, mpm_particles_view *out);
 
 /* The grid half of to_io_state (store_grid = true). */
 int mpm_read_grid(mpm_engine *engine, mpm_grid_view *out);
+
+/* Mesh colliders (the collide phase, rust/crates/cpu/src/phase/collide.rs:21-206,
+ * collider bits and the grid's twin nodes, meld_grid.rs:16-69): from the next
+ * call on, every substep sets each live particle's collider bits from the
+ * closest triangle of each collider within 2.2 h, corrects the velocity of a
+ * particle that crossed to the other side (friction, damping, push-back), and
+ * particles whose bits are incompatible exchange no momentum through the
+ * grid.  NULL removes the mesh.  Not with batched scenes
+ * (MPM_ERR_INVALID_ARGUMENT). */
+int mpm_set_collider_mesh(mpm_engine *engine, const mpm_collider_mesh *mesh);
 
 /* Drop of GpuState. */
 void mpm_destroy(mpm_engine *engine);

@@ -318,6 +318,106 @@ template <class R> V3<R> sphere_slip(const V3<R> &xi, const V3<R> &center, R rad
     return v;
 }
 
+// ---- mesh geometry (rust/crates/mesh_util/src/mesh.rs:228-309) -------------
+// NORMALIZATION_EPS, rust/crates/util/src/consts.rs:11.
+constexpr double NORMALIZATION_EPS = 1e-5;
+
+template <class R> R norm3(const V3<R> &v) { return std::sqrt(dot(v, v)); }
+
+// nalgebra's try_normalize(eps): None (here: zero) when |v| <= eps.
+template <class R> V3<R> try_normalize(const V3<R> &v, R eps) {
+    const R n = norm3(v);
+    return n <= eps ? V3<R>() : v / n;
+}
+
+// nalgebra's Vector::angle: the clamped arc cosine of the normalised dot
+// product, 0 when either vector is zero.
+template <class R> R angle_between(const V3<R> &a, const V3<R> &b) {
+    const R n1 = norm3(a), n2 = norm3(b);
+    if (n1 == R(0) || n2 == R(0)) return R(0);
+    R c = dot(a, b) / (n1 * n2);
+    c = std::min(std::max(c, R(-1)), R(1));
+    return std::acos(c);
+}
+
+template <class R> struct DistanceResult {
+    R distance;
+    V3<R> to_p, normal;
+};
+
+// segment_distance_result, mesh.rs:234-262.
+template <class R>
+DistanceResult<R> segment_distance_result(const V3<R> &p, const V3<R> &start, const V3<R> &end,
+                                          const V3<R> &start_normal, const V3<R> &segment_normal,
+                                          const V3<R> &end_normal) {
+    const V3<R> segment = end - start;
+    const R along = dot(p - start, segment) / dot(segment, segment);
+    if (along < R(0)) return {norm3(p - start), p - start, start_normal};
+    if (along < R(1)) {
+        const V3<R> d = p - start - segment * along;
+        return {norm3(d), d, segment_normal};
+    }
+    return {norm3(p - end), p - end, end_normal};
+}
+
+// distance_to_segment, mesh.rs:264-275.
+template <class R> R distance_to_segment(const V3<R> &p, const V3<R> &start, const V3<R> &end) {
+    const V3<R> segment = end - start;
+    const R along = dot(p - start, segment) / dot(segment, segment);
+    if (along < R(0)) return norm3(p - start);
+    if (along < R(1)) return norm3(p - start - segment * along);
+    return norm3(p - end);
+}
+
+// distance_to_triangle, mesh.rs:277-309: the distance to the plane inside the
+// triangle's prism, else to the nearest edge facing the point.
+template <class R>
+R distance_to_triangle(const V3<R> &p, const V3<R> &a, const V3<R> &b, const V3<R> &c, const V3<R> &n) {
+    const V3<R> ab = a - b, bc = b - c, ca = c - a;
+    const bool sa = dot(n, cross(bc, c - p)) > R(0);
+    const bool sb = dot(n, cross(ca, a - p)) > R(0);
+    const bool sc = dot(n, cross(ab, b - p)) > R(0);
+    if (sa && sb && sc) return std::abs(dot(p - a, n));
+    R distance = std::numeric_limits<R>::max();
+    if (!sa) distance = std::min(distance, distance_to_segment(p, b, c));
+    if (!sb) distance = std::min(distance, distance_to_segment(p, c, a));
+    if (!sc) distance = std::min(distance, distance_to_segment(p, a, b));
+    return distance;
+}
+
+// collider_bits, rust/crates/util/src/collider_bits.rs:9-32.
+inline bool bits_near(uint32_t bits, int c) { return (bits & (0x00010000u << c)) != 0; }
+inline bool bits_side(uint32_t bits, int c) { return (bits & (0x00000001u << c)) != 0; }
+inline void bits_set(uint32_t &bits, int c, int side /* -1: None, 0: false, 1: true */) {
+    bits &= ~(0x00010001u << c);
+    if (side == 1) bits |= 0x00010001u << c;
+    else if (side == 0) bits |= 0x00010000u << c;
+}
+
+// Topology::triangle_opposites (mesh.rs:183-226): per edge (ab, bc, ca) the
+// other triangle sharing it, or all ones.
+inline std::vector<uint32_t> triangle_opposites(const std::vector<uint32_t> &tri) {
+    const size_t nt = tri.size() / 3;
+    std::vector<std::pair<uint64_t, uint32_t>> edges;
+    edges.reserve(3 * nt);
+    auto key = [](uint32_t a, uint32_t b) { return a < b ? ((uint64_t)a << 32) | b : ((uint64_t)b << 32) | a; };
+    for (size_t t = 0; t < nt; ++t)
+        for (int e = 0; e < 3; ++e) edges.push_back({key(tri[3 * t + e], tri[3 * t + (e + 1) % 3]), (uint32_t)t});
+    std::sort(edges.begin(), edges.end());
+    std::vector<uint32_t> opp(3 * nt, 0xffffffffu);
+    for (size_t t = 0; t < nt; ++t)
+        for (int e = 0; e < 3; ++e) {
+            const uint64_t k = key(tri[3 * t + e], tri[3 * t + (e + 1) % 3]);
+            auto it = std::lower_bound(edges.begin(), edges.end(), std::make_pair(k, 0u));
+            for (; it != edges.end() && it->first == k; ++it)
+                if (it->second != (uint32_t)t) {
+                    opp[3 * t + e] = it->second;
+                    break;
+                }
+        }
+    return opp;
+}
+
 // ---- extensions: fixed-corotated and snow (no reference counterpart) --------
 // Fixed-corotated (Stomakhin et al. 2012): Psi = mu |F - R|^2 + lambda/2 (J-1)^2,
 // tau = P F^T = 2 mu (F - R) F^T + lambda (J - 1) J I.
@@ -374,7 +474,43 @@ struct FrameInputCopy {
     bool has_b = false;
     std::vector<uint32_t> flags_a, flags_b;
     std::vector<float> goals_a, goals_b;
+    // the collider mesh's frames a and b (b = a when absent)
+    std::vector<float> vertices_a, vertices_b, frictions_a, frictions_b, dampings_a, dampings_b;
 };
+
+// The collider mesh's topology (Topology, rust/crates/mesh_util/src/mesh.rs).
+struct MeshTopology {
+    std::vector<uint32_t> triangles;           // 3 per triangle
+    std::vector<uint32_t> triangle_colliders;
+    std::vector<uint32_t> opposites;           // 3 per triangle (ab, bc, ca)
+    std::vector<uint32_t> vertex_offsets, vertex_triangles;  // triangles at each vertex, ascending
+    uint32_t num_vertices = 0;
+    size_t num_triangles() const { return triangle_colliders.size(); }
+};
+
+// The topology from triangles and their colliders: opposites, and the
+// triangles at each vertex in ascending order (Topology::vertex_triangle_lists).
+inline MeshTopology make_topology(uint32_t num_vertices, const uint32_t *triangles, const uint32_t *colliders,
+                                  uint32_t num_triangles) {
+    MeshTopology m;
+    m.num_vertices = num_vertices;
+    m.triangles.assign(triangles, triangles + 3 * (size_t)num_triangles);
+    m.triangle_colliders.assign(colliders, colliders + num_triangles);
+    m.opposites = triangle_opposites(m.triangles);
+    m.vertex_offsets.assign(num_vertices + 1, 0);
+    for (uint32_t t = 0; t < num_triangles; ++t)
+        for (int e = 0; e < 3; ++e) ++m.vertex_offsets[m.triangles[3 * t + e] + 1];
+    for (uint32_t v = 0; v < num_vertices; ++v) m.vertex_offsets[v + 1] += m.vertex_offsets[v];
+    m.vertex_triangles.assign(m.vertex_offsets[num_vertices], 0);
+    std::vector<uint32_t> fill(num_vertices, 0);
+    for (uint32_t t = 0; t < num_triangles; ++t)
+        for (int e = 0; e < 3; ++e) {
+            const uint32_t v = m.triangles[3 * t + e];
+            m.vertex_triangles[m.vertex_offsets[v] + fill[v]++] = t;
+        }
+    return m;
+}
+
 
 template <class R>
 struct State {
@@ -396,20 +532,32 @@ struct State {
     std::vector<int32_t> lookup;              // dense node id -> node index (the FxHashMap)
     int lookup_lo[3] = {0, 0, 0}, lookup_dim[3] = {0, 0, 0};
 
+    std::vector<uint32_t> lookup_count;       // twins (keys) at each dense node id
+
     // InterpolatedInput (rust/crates/cpu/src/interpolated_input.rs:12-22).
     V3<R> gravity;
     std::vector<V3<R>> goal_positions;
     bool have_goals = false;
     FrameInputCopy input;
 
-    int32_t lookup_node(const int id[3]) const {
+    // The collider mesh and its interpolation for the current substep.
+    MeshTopology mesh;
+    std::vector<V3<R>> vertex_positions, vertex_normals, vertex_velocities, triangle_normals;
+    std::vector<R> triangle_frictions, triangle_dampings;
+
+    // The node with id and collider bits (GridNodes::map), or -1.
+    int32_t lookup_node(const int id[3], uint32_t bits = 0) const {
         int64_t idx = 0;
         for (int a = 2; a >= 0; --a) {
             int c = id[a] - lookup_lo[a];
             if (c < 0 || c >= lookup_dim[a]) return -1;
             idx = idx * lookup_dim[a] + c;
         }
-        return lookup[idx];
+        const int32_t first = lookup[idx];
+        if (first < 0) return -1;
+        for (uint32_t k = 0; k < lookup_count[idx]; ++k)
+            if (keys[first + k].collider_bits == bits) return first + (int32_t)k;
+        return -1;
     }
 
     R h() const { return R(consts.grid_node_size); }
@@ -439,7 +587,54 @@ struct State {
                                           R(factor_a * input.goals_a[3 * i + 1] + factor_b * b[3 * i + 1]),
                                           R(factor_a * input.goals_a[3 * i + 2] + factor_b * b[3 * i + 2]));
         }
+        interpolate_mesh(factor_a, factor_b);
         return MPM_OK;
+    }
+
+    // The mesh half of interpolate_input (interpolate_input.rs:37-94): vertex
+    // positions, frictions and dampings interpolated in f32 as the reference
+    // does; triangle normals; vertex normals as the angle-weighted sums of
+    // their triangles' normals (interpolate_input.rs:57-81); and the vertex
+    // velocities (b - a) * fps (frame_input.rs:333-347).
+    void interpolate_mesh(float factor_a, float factor_b) {
+        const size_t nv = mesh.num_vertices, nt = mesh.num_triangles();
+        if (nt == 0) return;
+        if (input.vertices_a.size() != 3 * nv) throw std::runtime_error("the frame input holds no collider mesh");
+        const std::vector<float> &pb = input.vertices_b.empty() ? input.vertices_a : input.vertices_b;
+        vertex_positions.resize(nv);
+        vertex_velocities.resize(nv);
+        for (size_t v = 0; v < nv; ++v)
+            for (int a = 0; a < 3; ++a) {
+                vertex_positions[v][a] = R(factor_a * input.vertices_a[3 * v + a] + factor_b * pb[3 * v + a]);
+                vertex_velocities[v][a] = input.vertices_b.empty() ? R(0)
+                    : R((pb[3 * v + a] - input.vertices_a[3 * v + a]) * (float)consts.frames_per_second);
+            }
+        const std::vector<float> &fb = input.frictions_b.empty() ? input.frictions_a : input.frictions_b;
+        const std::vector<float> &db = input.dampings_b.empty() ? input.dampings_a : input.dampings_b;
+        triangle_normals.resize(nt);
+        triangle_frictions.resize(nt);
+        triangle_dampings.resize(nt);
+        for (size_t t = 0; t < nt; ++t) {
+            const V3<R> &a = vertex_positions[mesh.triangles[3 * t]];
+            const V3<R> &b = vertex_positions[mesh.triangles[3 * t + 1]];
+            const V3<R> &c = vertex_positions[mesh.triangles[3 * t + 2]];
+            triangle_normals[t] = try_normalize(cross(b - a, c - a), R(NORMALIZATION_EPS));
+            triangle_frictions[t] = R(factor_a * input.frictions_a[t] + factor_b * fb[t]);
+            triangle_dampings[t] = R(factor_a * input.dampings_a[t] + factor_b * db[t]);
+        }
+        vertex_normals.assign(nv, V3<R>());
+        for (size_t v = 0; v < nv; ++v) {
+            V3<R> sum;
+            for (uint32_t k = mesh.vertex_offsets[v]; k < mesh.vertex_offsets[v + 1]; ++k) {
+                const uint32_t t = mesh.vertex_triangles[k];
+                uint32_t others[2], no = 0;
+                for (int e = 0; e < 3; ++e)
+                    if (mesh.triangles[3 * t + e] != (uint32_t)v) others[no++] = mesh.triangles[3 * t + e];
+                const V3<R> &p = vertex_positions[v];
+                sum += triangle_normals[t] * angle_between(vertex_positions[others[0]] - p, vertex_positions[others[1]] - p);
+            }
+            vertex_normals[v] = try_normalize(sum, R(NORMALIZATION_EPS));
+        }
     }
 
     // ---- Sort, rust/crates/cpu/src/phase/sort.rs:17-114.
@@ -479,13 +674,106 @@ struct State {
         for (size_t cur = 0; cur < n; ++cur) reverse_sort_map[sort_map[cur]] = (uint32_t)cur;
     }
 
-    // ---- Collide without meshes: rust/crates/cpu/src/phase/collide.rs:53-58
-    // sets collider_bits = 0 for every live particle when no triangle is near.
+    // ---- Collide, rust/crates/cpu/src/phase/collide.rs:21-206.  Candidate
+    // triangles are all triangles in ascending order (the reference's BVH
+    // query returns a superset of the triangles within the forget distance,
+    // in ascending order, and the distance test discards the rest).
     void collide() {
         const size_t n = positions.size();
-#pragma omp parallel for schedule(static)
-        for (int64_t i = 0; i < (int64_t)n; ++i)
-            if (!(flags[i] & MPM_TOMBSTONED)) collider_bits[i] = 0;
+        const size_t nt = mesh.num_triangles();
+        const R hh = h(), t = dt();
+        const R forget = R(consts.grid_node_size * 2.2f), accept = R(consts.grid_node_size * 2.0f);
+#pragma omp parallel for schedule(dynamic, 256)
+        for (int64_t i = 0; i < (int64_t)n; ++i) {
+            if (flags[i] & MPM_TOMBSTONED) continue;
+            if (nt == 0) {
+                collider_bits[i] = 0;
+                continue;
+            }
+            (void)hh;
+            const V3<R> p = positions[i];
+            V3<R> velocity = velocities[i];
+            uint32_t bits = collider_bits[i];
+            uint32_t closest[16];
+            R min_distance[16];
+            for (int c = 0; c < 16; ++c) {
+                closest[c] = 0xffffffffu;
+                min_distance[c] = std::numeric_limits<R>::max();
+            }
+            for (size_t tri = 0; tri < nt; ++tri) {
+                const V3<R> &nrm = triangle_normals[tri];
+                if (nrm.x == R(0) && nrm.y == R(0) && nrm.z == R(0)) continue;
+                const V3<R> &a = vertex_positions[mesh.triangles[3 * tri]];
+                const V3<R> &b = vertex_positions[mesh.triangles[3 * tri + 1]];
+                const V3<R> &c = vertex_positions[mesh.triangles[3 * tri + 2]];
+                const R distance = distance_to_triangle(p, a, b, c, nrm);
+                if (distance >= forget) continue;
+                const uint32_t col = mesh.triangle_colliders[tri];
+                if (distance < min_distance[col]) {
+                    min_distance[col] = distance;
+                    closest[col] = (uint32_t)tri;
+                }
+            }
+            for (int col = 0; col < 16; ++col) {
+                if (closest[col] == 0xffffffffu) {
+                    bits_set(bits, col, -1);
+                    continue;
+                }
+                const uint32_t ct = closest[col];
+                const uint32_t ia = mesh.triangles[3 * ct], ib = mesh.triangles[3 * ct + 1], ic = mesh.triangles[3 * ct + 2];
+                const V3<R> &nrm = triangle_normals[ct];
+                const V3<R> &a = vertex_positions[ia], &b = vertex_positions[ib], &c = vertex_positions[ic];
+                const uint32_t o_ab = mesh.opposites[3 * ct], o_bc = mesh.opposites[3 * ct + 1], o_ca = mesh.opposites[3 * ct + 2];
+                const V3<R> ab_n = o_ab != 0xffffffffu ? nrm + triangle_normals[o_ab] : V3<R>();
+                const V3<R> bc_n = o_bc != 0xffffffffu ? nrm + triangle_normals[o_bc] : V3<R>();
+                const V3<R> ca_n = o_ca != 0xffffffffu ? nrm + triangle_normals[o_ca] : V3<R>();
+                const V3<R> ab = a - b, bc = b - c, ca = c - a;
+                const R area2 = dot(nrm, cross(ca, ab));
+                const R a_bary = dot(nrm, cross(bc, c - p)) / area2;
+                const R b_bary = dot(nrm, cross(ca, a - p)) / area2;
+                const R c_bary = dot(nrm, cross(ab, b - p)) / area2;
+                DistanceResult<R> res;
+                if (a_bary > R(0) && b_bary > R(0) && c_bary > R(0)) {
+                    res = {std::abs(dot(p - a, nrm)), nrm * dot(p - a, nrm), nrm};
+                } else {
+                    // min_by over ab, bc, ca: the first of equal distances
+                    DistanceResult<R> r3[3] = {
+                        segment_distance_result(p, a, b, vertex_normals[ia], ab_n, vertex_normals[ib]),
+                        segment_distance_result(p, b, c, vertex_normals[ib], bc_n, vertex_normals[ic]),
+                        segment_distance_result(p, c, a, vertex_normals[ic], ca_n, vertex_normals[ia])};
+                    res = r3[0];
+                    for (int k = 1; k < 3; ++k)
+                        if (r3[k].distance < res.distance) res = r3[k];
+                }
+                if (res.normal.x == R(0) && res.normal.y == R(0) && res.normal.z == R(0)) {
+                    bits_set(bits, col, -1);
+                    continue;
+                }
+                const bool new_side = R(0) <= dot(res.to_p, res.normal);
+                if (!bits_near(bits, col)) {
+                    if (res.distance < accept) bits_set(bits, col, new_side ? 1 : 0);
+                    continue;
+                }
+                if (bits_side(bits, col) == new_side) continue;
+                if (res.distance > R(NORMALIZATION_EPS)) {
+                    const V3<R> collider_velocity = vertex_velocities[ia] * a_bary + vertex_velocities[ib] * b_bary +
+                                                    vertex_velocities[ic] * c_bary;
+                    const V3<R> relative = velocity - collider_velocity;
+                    const V3<R> contact_normal = res.to_p / res.distance;
+                    const V3<R> normal_velocity = contact_normal * dot(relative, contact_normal);
+                    const V3<R> tangential = relative - normal_velocity;
+                    const R tn = norm3(tangential);
+                    if (tn > R(NORMALIZATION_EPS)) {
+                        const V3<R> tangent = tangential / tn;
+                        velocity -= tangent * std::min(triangle_frictions[ct] * res.distance / t, tn);
+                    }
+                    velocity -= normal_velocity * std::min(triangle_dampings[ct], R(1));
+                }
+                velocity -= res.to_p / t;
+            }
+            velocities[i] = velocity;
+            collider_bits[i] = bits;
+        }
     }
 
     // ---- ExternalForce, rust/crates/cpu/src/phase/external_force.rs:17-50.
@@ -536,7 +824,18 @@ struct State {
             total *= lookup_dim[a];
         }
         if (total > (int64_t)1 << 31) throw std::runtime_error("oracle grid box too large");
+        // Keys are (node id, collider bits) (GridKey, grid_nodes.rs:15-19):
+        // contributions with bits 0 are counted per node id, the rarer ones
+        // with other bits are listed and sorted.
         std::vector<uint32_t> count((size_t)total, 0);
+        struct Tagged {
+            int64_t f;
+            uint32_t bits, particle;
+            bool operator<(const Tagged &o) const {
+                return f != o.f ? f < o.f : (bits != o.bits ? bits < o.bits : particle < o.particle);
+            }
+        };
+        std::vector<Tagged> tagged;
         auto flat = [&](const int id[3]) {
             return ((int64_t)(id[2] - lo[2]) * lookup_dim[1] + (id[1] - lo[1])) * lookup_dim[0] + (id[0] - lo[0]);
         };
@@ -546,21 +845,39 @@ struct State {
                 for (int b = 0; b < 3; ++b)
                     for (int c = 0; c < 3; ++c) {
                         int id[3] = {shift[3 * i] + a, shift[3 * i + 1] + b, shift[3 * i + 2] + c};
-                        ++count[flat(id)];
+                        if (collider_bits[i] == 0) ++count[flat(id)];
+                        else tagged.push_back({flat(id), collider_bits[i], (uint32_t)i});
                     }
         }
+        std::sort(tagged.begin(), tagged.end());
         lookup.assign((size_t)total, -1);
-        // Node order: node id order (the reference's order is the hash map's).
+        lookup_count.assign((size_t)total, 0);
+        // Node order: node id, then collider bits (the reference's order is the hash map's).
+        size_t ti = 0;
         for (int64_t f = 0; f < total; ++f) {
-            if (!count[f]) continue;
+            const bool plain = count[f] > 0;
+            const bool twins = ti < tagged.size() && tagged[ti].f == f;
+            if (!plain && !twins) continue;
             GridKey<R> k;
             k.id[0] = (int)(f % lookup_dim[0]) + lo[0];
             k.id[1] = (int)((f / lookup_dim[0]) % lookup_dim[1]) + lo[1];
             k.id[2] = (int)(f / ((int64_t)lookup_dim[0] * lookup_dim[1])) + lo[2];
-            k.collider_bits = 0;
             lookup[f] = (int32_t)keys.size();
-            keys.push_back(k);
-            contrib_offsets.push_back(contrib_offsets.back() + count[f]);
+            if (plain) {
+                k.collider_bits = 0;
+                keys.push_back(k);
+                contrib_offsets.push_back(contrib_offsets.back() + count[f]);
+                ++lookup_count[f];
+            }
+            while (ti < tagged.size() && tagged[ti].f == f) {
+                const uint32_t bits = tagged[ti].bits;
+                uint32_t cnt = 0;
+                while (ti < tagged.size() && tagged[ti].f == f && tagged[ti].bits == bits) ++cnt, ++ti;
+                k.collider_bits = bits;
+                keys.push_back(k);
+                contrib_offsets.push_back(contrib_offsets.back() + cnt);
+                ++lookup_count[f];
+            }
         }
         contributors.assign(contrib_offsets.back(), 0);
         std::vector<uint32_t> fill(keys.size(), 0);
@@ -570,7 +887,7 @@ struct State {
                 for (int b = 0; b < 3; ++b)
                     for (int c = 0; c < 3; ++c) {
                         int id[3] = {shift[3 * i] + a, shift[3 * i + 1] + b, shift[3 * i + 2] + c};
-                        int32_t node = lookup[flat(id)];
+                        int32_t node = lookup_node(id, collider_bits[i]);
                         contributors[contrib_offsets[node] + fill[node]++] = (uint32_t)i;
                     }
         }
@@ -615,22 +932,42 @@ struct State {
         }
     }
 
-    // ---- MeldGrid, rust/crates/cpu/src/phase/meld_grid.rs:16-69.  Every key
-    // carries collider bits 0 here, so a node's only twin is itself.  The grid
-    // extensions (grid gravity and boundary conditions) follow the division.
+    // ---- MeldGrid, rust/crates/cpu/src/phase/meld_grid.rs:16-69: each key
+    // adds the mass and momentum of the compatible twins at its node id (the
+    // keys of one id are adjacent here), then divides.  The grid extensions
+    // (grid gravity and boundary conditions) follow the division.
     void meld_grid() {
         const size_t nn = keys.size();
         const R hh = h(), t = dt();
         const bool grid_gravity = consts.options & MPM_OPT_GRID_GRAVITY;
+        const std::vector<R> m0 = masses;
+        const std::vector<V3<R>> p0 = node_velocities;
 #pragma omp parallel for schedule(static)
         for (int64_t node = 0; node < (int64_t)nn; ++node) {
-            R mass = masses[node];
-            V3<R> v = node_velocities[node];
+            R mass = m0[node];
+            V3<R> v = p0[node];
+            for (int64_t other = first_twin_bits(node); other < (int64_t)nn; ++other) {
+                if (keys[other].id[0] != keys[node].id[0] || keys[other].id[1] != keys[node].id[1] ||
+                    keys[other].id[2] != keys[node].id[2])
+                    break;
+                if (other == node || !collider_compatible(keys[node].collider_bits, keys[other].collider_bits)) continue;
+                mass += m0[other];
+                v += p0[other];
+            }
             if (mass > 0) v = v / mass;
             else v = V3<R>();
             if (mass > 0) apply_grid_extensions(keys[node].id, v, hh, t, grid_gravity);
             node_velocities[node] = v;
+            masses[node] = mass;
         }
+    }
+
+    // The first key (twin) with the same node id as key `node`.
+    int64_t first_twin_bits(int64_t node) const {
+        while (node > 0 && keys[node - 1].id[0] == keys[node].id[0] && keys[node - 1].id[1] == keys[node].id[1] &&
+               keys[node - 1].id[2] == keys[node].id[2])
+            --node;
+        return node;
     }
 
     // Extension (no reference counterpart): grid gravity, sticky plane, box
@@ -675,7 +1012,7 @@ struct State {
                         int id[3] = {(int)shift.x + a, (int)shift.y + b, (int)shift.z + c};
                         V3<R> node_pos(R(id[0]) * hh, R(id[1]) * hh, R(id[2]) * hh);
                         V3<R> to_node = node_pos - position;
-                        int32_t node = lookup_node(id);
+                        int32_t node = lookup_node(id, collider_bits[i]);
                         if (node < 0) throw std::runtime_error("missing node");
                         V3<R> gv = node_velocities[node];
                         velocity += gv * weight;

@@ -96,9 +96,20 @@ void store_f64(const State<R> &s, double *x, double *v, double *F, double *C) {
     }
 }
 
-void copy_input(FrameInputCopy &dst, const mpm_frame_input *in, size_t n) {
+void copy_input(FrameInputCopy &dst, const mpm_frame_input *in, size_t n, const MeshTopology &mesh) {
     dst = FrameInputCopy();
     if (!in) return;
+    const size_t nv = mesh.num_vertices, nt = mesh.num_triangles();
+    if (nt > 0 && in->vertex_positions_a && in->triangle_frictions_a && in->triangle_dampings_a) {
+        dst.vertices_a.assign(in->vertex_positions_a, in->vertex_positions_a + 3 * nv);
+        dst.frictions_a.assign(in->triangle_frictions_a, in->triangle_frictions_a + nt);
+        dst.dampings_a.assign(in->triangle_dampings_a, in->triangle_dampings_a + nt);
+        if (in->has_b && in->vertex_positions_b) {
+            dst.vertices_b.assign(in->vertex_positions_b, in->vertex_positions_b + 3 * nv);
+            dst.frictions_b.assign(in->triangle_frictions_b, in->triangle_frictions_b + nt);
+            dst.dampings_b.assign(in->triangle_dampings_b, in->triangle_dampings_b + nt);
+        }
+    }
     dst.frame = in->frame;
     for (int a = 0; a < 3; ++a) {
         dst.gravity_a[a] = in->gravity_a[a];
@@ -218,6 +229,19 @@ void oracle_cauchy_viscosity(double dynamic, double bulk, const double *C, doubl
     for (uint64_t i = 0; i < n; ++i) mat_out<double>(cauchy_viscosity<double>(dynamic, bulk, mat_in<double>(C + 9 * i)), out + 9 * i);
 }
 
+// distance_to_triangle of n points p to triangles (a, b, c) with unit normals n, FP64.
+void oracle_distance_to_triangle(const double *p, const double *tri, double *out, uint64_t count) {
+    for (uint64_t i = 0; i < count; ++i) {
+        const double *t = tri + 12 * i;
+        out[i] = distance_to_triangle<double>(V3<double>(p[3 * i], p[3 * i + 1], p[3 * i + 2]), V3<double>(t[0], t[1], t[2]),
+                                              V3<double>(t[3], t[4], t[5]), V3<double>(t[6], t[7], t[8]),
+                                              V3<double>(t[9], t[10], t[11]));
+    }
+}
+
+// collider_bits::compatible.
+int oracle_collider_compatible(uint32_t a, uint32_t b) { return collider_compatible(a, b) ? 1 : 0; }
+
 // The sphere collider's slip on n node velocities v at node positions xi, FP64.
 void oracle_sphere_slip(const double *xi, const double *center, double radius, const double *v, double *out,
                         uint64_t n) {
@@ -249,6 +273,20 @@ void *oracle_create(int precision, const mpm_consts *consts, const mpm_particles
 
 void oracle_destroy(void *handle) { delete static_cast<Handle *>(handle); }
 
+// The collider mesh (mpm_set_collider_mesh); NULL removes it.
+int oracle_set_collider_mesh(void *handle, const mpm_collider_mesh *mesh) {
+    Handle *h = static_cast<Handle *>(handle);
+    MeshTopology m;
+    if (mesh && mesh->num_triangles > 0) {
+        for (uint32_t t = 0; t < mesh->num_triangles; ++t)
+            if (mesh->triangle_colliders[t] >= 16) return MPM_ERR_INVALID_ARGUMENT;
+        m = make_topology(mesh->num_vertices, mesh->triangles, mesh->triangle_colliders, mesh->num_triangles);
+    }
+    if (h->precision == 8) h->d->mesh = m;
+    else h->f->mesh = m;
+    return MPM_OK;
+}
+
 double oracle_time(void *handle) {
     Handle *h = static_cast<Handle *>(handle);
     return h->precision == 8 ? h->d->time : h->f->time;
@@ -261,10 +299,10 @@ int oracle_step_to(void *handle, double target_time, const mpm_frame_input *inpu
     int r;
     try {
         if (h->precision == 8) {
-            copy_input(h->d->input, input, h->d->positions.size());
+            copy_input(h->d->input, input, h->d->positions.size(), h->d->mesh);
             r = h->d->step_to(target_time, cancel, substeps);
         } else {
-            copy_input(h->f->input, input, h->f->positions.size());
+            copy_input(h->f->input, input, h->f->positions.size(), h->f->mesh);
             r = h->f->step_to(target_time, cancel, substeps);
         }
     } catch (...) {
@@ -287,10 +325,10 @@ int oracle_run_to_scatter(void *handle, const mpm_frame_input *input) {
     Handle *h = static_cast<Handle *>(handle);
     try {
         if (h->precision == 8) {
-            copy_input(h->d->input, input, h->d->positions.size());
+            copy_input(h->d->input, input, h->d->positions.size(), h->d->mesh);
             return h->d->run_to_scatter();
         }
-        copy_input(h->f->input, input, h->f->positions.size());
+        copy_input(h->f->input, input, h->f->positions.size(), h->f->mesh);
         return h->f->run_to_scatter();
     } catch (...) {
         return MPM_ERR_INVALID_ARGUMENT;

@@ -72,6 +72,10 @@ def _configure(lib):
     lib.oracle_plasticity.argtypes = [C.c_int, C.c_int, V, C.c_double, C.c_double, C.c_double, V, C.c_uint64]
     lib.oracle_cauchy_viscosity.argtypes = [C.c_double, C.c_double, V, V, C.c_uint64]
     lib.oracle_sphere_slip.argtypes = [V, V, C.c_double, V, V, C.c_uint64]
+    lib.oracle_set_collider_mesh.argtypes = [V, C.POINTER(abi.MpmColliderMesh)]
+    lib.oracle_distance_to_triangle.argtypes = [V, V, V, C.c_uint64]
+    lib.oracle_collider_compatible.restype = C.c_int
+    lib.oracle_collider_compatible.argtypes = [C.c_uint32, C.c_uint32]
     lib.oracle_set_threads.argtypes = [C.c_int]
     lib.oracle_max_threads.restype = C.c_int
     return lib
@@ -112,6 +116,16 @@ class Oracle:
     @property
     def time(self) -> float:
         return self.lib.oracle_time(self.h)
+
+    def set_collider_mesh(self, mesh):
+        """The collider mesh (an io_state.ColliderMesh), or None."""
+        if mesh is None:
+            rc = self.lib.oracle_set_collider_mesh(self.h, None)
+        else:
+            m, keep = mesh.to_c()
+            rc = self.lib.oracle_set_collider_mesh(self.h, C.byref(m))
+        if rc != 0:
+            raise ValueError("invalid collider mesh")
 
     def step_to(self, target_time: float, frame_input: FrameInput) -> abi.MpmStatus:
         fi, keep = frame_input.to_c()
@@ -236,3 +250,17 @@ def sphere_slip(xi: np.ndarray, center, radius: float, v: np.ndarray, lib=None) 
     out = np.zeros_like(vv)
     lib.oracle_sphere_slip(abi.ptr(x), abi.ptr(c), radius, abi.ptr(vv), abi.ptr(out), x.shape[0])
     return out
+
+
+def distance_to_triangle(p: np.ndarray, a: np.ndarray, b: np.ndarray, c: np.ndarray, n: np.ndarray, lib=None) -> np.ndarray:
+    """The oracle's distance_to_triangle of points p (N, 3) to triangles (N, 3 each) with normals n, FP64."""
+    lib = lib or load()
+    pp = np.ascontiguousarray(p, np.float64)
+    tri = np.ascontiguousarray(np.concatenate([a, b, c, n], axis=1), np.float64)
+    out = np.zeros(pp.shape[0])
+    lib.oracle_distance_to_triangle(abi.ptr(pp), abi.ptr(tri), abi.ptr(out), pp.shape[0])
+    return out
+
+
+def collider_compatible(a: int, b: int, lib=None) -> bool:
+    return bool((lib or load()).oracle_collider_compatible(a, b))

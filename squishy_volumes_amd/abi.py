@@ -127,6 +127,21 @@ class MpmFrameInput(C.Structure):
         ("particle_flags_b", C.c_void_p),
         ("goal_positions_a", C.c_void_p),
         ("goal_positions_b", C.c_void_p),
+        ("vertex_positions_a", C.c_void_p),
+        ("vertex_positions_b", C.c_void_p),
+        ("triangle_frictions_a", C.c_void_p),
+        ("triangle_frictions_b", C.c_void_p),
+        ("triangle_dampings_a", C.c_void_p),
+        ("triangle_dampings_b", C.c_void_p),
+    ]
+
+
+class MpmColliderMesh(C.Structure):
+    _fields_ = [
+        ("num_vertices", C.c_uint32),
+        ("num_triangles", C.c_uint32),
+        ("triangles", C.c_void_p),
+        ("triangle_colliders", C.c_void_p),
     ]
 
 
@@ -242,6 +257,7 @@ SIGNATURES = {
     "mpm_read_time_step_limits": (C.c_int, [C.c_void_p, C.POINTER(MpmTimeStepLimits)]),
     "mpm_set_time_step": (C.c_int, [C.c_void_p, C.c_float]),
     "mpm_debug_cancel_after": (C.c_int, [C.c_void_p, C.c_int64]),
+    "mpm_set_collider_mesh": (C.c_int, [C.c_void_p, C.POINTER(MpmColliderMesh)]),
     "mpm_frame_file_size": (C.c_uint64, [C.POINTER(MpmParticlesView), C.POINTER(MpmGridView)]),
     "mpm_frame_encode": (C.c_int, [C.POINTER(MpmParticlesView), C.POINTER(MpmGridView), C.c_void_p, C.c_uint64,
                                    C.POINTER(C.c_uint64)]),

@@ -11,7 +11,7 @@ PACKAGE_DIR = Path(__file__).resolve().parent
 CSRC = PACKAGE_DIR / "csrc"
 LIB_NAME = "libsquishy_volumes_amd.so"
 LIB_PATH = PACKAGE_DIR / LIB_NAME
-SOURCES = ["sort.hip", "transfer.hip", "closure.hip", "engine.hip", "frame_file.hip", "time_step.hip", "noalloc.hip"]
+SOURCES = ["sort.hip", "transfer.hip", "closure.hip", "engine.hip", "frame_file.hip", "time_step.hip", "noalloc.hip", "collide.hip"]
 
 # -packed-fp32-ops keeps the compiler from emitting packed FP32 instructions;
 # -munsafe-fp-atomics lets float atomicAdd lower to the hardware atomics.

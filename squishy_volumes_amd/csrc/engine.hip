@@ -118,6 +118,15 @@ struct mpm_engine {
     uint32_t *rb_flags = nullptr;
     int64_t cancel_after = -1;  // test hook: calls behave as cancelled before this substep
 
+    // Mesh colliders (mpm_set_collider_mesh).
+    bool mesh_set = false;
+    ColliderCall coll{};                   // device view: mesh and twin tables
+    std::vector<uint32_t> mesh_tri;        // host copy of the triangles
+    std::vector<float> mesh_pos_a, mesh_pos_b;  // the call's frames (host), for the candidate lists
+    std::vector<void *> mesh_allocs;       // freed when the mesh changes
+    uint32_t blk_tri_capacity = 0;
+    uint32_t twin_capacity = 0;
+
     bool profiling = false;
     uint32_t profile_mask = 0;  // stages timed while profiling (bit = Stage)
     std::vector<EventRecord> records;
@@ -131,6 +140,7 @@ struct mpm_engine {
         if (device >= 0) (void)hipSetDevice(device);
         if (stream) (void)hipStreamSynchronize(stream);
         drop_graphs();
+        free_mesh();
         for (void *p : allocations) (void)hipFree(p);
         if (param_table) (void)hipFree(param_table);
         if (param_host) (void)hipHostFree(param_host);
@@ -151,6 +161,32 @@ struct mpm_engine {
                         (void)hipGraphExecDestroy(g);
                         g = nullptr;
                     }
+    }
+
+    void free_mesh() {
+        for (void *p : mesh_allocs) (void)hipFree(p);
+        mesh_allocs.clear();
+        coll = ColliderCall{};
+        blk_tri_capacity = 0;
+        twin_capacity = 0;
+    }
+
+    // Device memory of the collider mesh (freed with the mesh).
+    template <class T>
+    hipError_t mesh_alloc(T *&ptr, size_t count) {
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, std::max<size_t>(count * sizeof(T), 256));
+        if (e == hipSuccess) mesh_allocs.push_back(p);
+        ptr = static_cast<T *>(p);
+        return e;
+    }
+    template <class T>
+    void mesh_free(T *ptr) {
+        auto it = std::find(mesh_allocs.begin(), mesh_allocs.end(), (void *)ptr);
+        if (it != mesh_allocs.end()) {
+            (void)hipFree(*it);
+            mesh_allocs.erase(it);
+        }
     }
 
     template <class T>
@@ -425,6 +461,11 @@ int run_transfer(mpm_engine *e, TransferKind kind, bool goals, bool want_keys) {
         t.tau_stride = e->n;
     }
     const int stage = kind == TransferKind::P2G ? ST_P2G : (kind == TransferKind::G2P2G ? ST_G2P2G : ST_G2P);
+    if (e->launch.colliders) {
+        // the mesh of the substep whose scatter this transfer runs
+        if (kind != TransferKind::G2P) HIP_TRY(launch_mesh_interp(e->coll.mesh, e->param_table, e->ds, e->stream));
+        t.coll = &e->coll;
+    }
     e->begin(stage, r);
     HIP_TRY(launch_transfer(e->launch, t, e->stream));
     e->end(r);
@@ -446,6 +487,8 @@ int run_transfer(mpm_engine *e, TransferKind kind, bool goals, bool want_keys) {
 int run_grid(mpm_engine *e) {
     EventRecord r{};
     e->begin(ST_GRID, r);
+    if (e->launch.colliders)
+        HIP_TRY(launch_twin_meld(e->coll.twins, e->grid, e->geom, e->param_table, e->ds, e->stream));
     HIP_TRY(launch_grid_update(e->launch, grid_buffers(e), e->geom, e->param_table, e->partial, e->ds, e->stream));
     e->end(r);
     e->grid_epoch = e->epoch;
@@ -575,6 +618,7 @@ int restore_call_start(mpm_engine *e, const CallStart &c) {
     HIP_TRY(hipMemsetAsync(e->ds->ovf_count, 0, sizeof(e->ds->ovf_count), e->stream));
     HIP_TRY(hipMemsetAsync(&e->ds->status, 0, 4, e->stream));
     HIP_TRY(hipMemsetAsync(&e->ds->fail_index, 0xff, 4, e->stream));
+    if (e->launch.colliders) HIP_TRY(launch_twin_clear(e->coll.twins, e->stream));
     return MPM_OK;
 }
 
@@ -660,9 +704,216 @@ int run_loop(mpm_engine *e, const std::vector<double> &times, size_t k, bool goa
     return MPM_OK;
 }
 
+// Twin tables of `cap` entries each (a power of two), cleared.
+int set_twin_capacity(mpm_engine *e, uint32_t cap) {
+    for (int k = 0; k < 2; ++k)
+        if (e->coll.twins.tab[k]) e->mesh_free(e->coll.twins.tab[k]);
+    e->coll.twins = TwinTables{};
+    e->twin_capacity = 0;
+    e->drop_graphs();  // they hold the old tables' addresses
+    for (int k = 0; k < 2; ++k)
+        if (e->mesh_alloc(e->coll.twins.tab[k], cap) != hipSuccess) return MPM_ERR_OUT_OF_MEMORY;
+    e->coll.twins.mask = cap - 1u;
+    e->twin_capacity = cap;
+    HIP_TRY(launch_twin_clear(e->coll.twins, e->stream));
+    return MPM_OK;
+}
+
+// The call's collider frames: vertex positions, frictions and dampings of
+// frames a and b (b = a without it), the vertex velocities (b - a) * fps or 0
+// (linear_vertex_velocities, rust/crates/xpu/src/frame_input.rs:333-347), and
+// the candidate triangles of each grid block: every triangle whose bounding
+// box over both frames, grown by the forget distance (update_bvh,
+// frame_input.rs:349-389), reaches the block, in ascending order.
+int upload_mesh_frames(mpm_engine *e, const mpm_frame_input *in) {
+    if (!e->mesh_set) return MPM_OK;
+    MeshDev &M = e->coll.mesh;
+    if (!in || !in->vertex_positions_a || !in->triangle_frictions_a || !in->triangle_dampings_a)
+        return MPM_ERR_INVALID_ARGUMENT;
+    const bool b = in->has_b && in->vertex_positions_b && in->triangle_frictions_b && in->triangle_dampings_b;
+    const size_t nv = M.nvert, nt = M.ntri;
+    e->mesh_pos_a.assign(in->vertex_positions_a, in->vertex_positions_a + 3 * nv);
+    const float *pb = b ? in->vertex_positions_b : in->vertex_positions_a;
+    e->mesh_pos_b.assign(pb, pb + 3 * nv);
+    HIP_TRY(hipMemcpyAsync((void *)M.pos_a, e->mesh_pos_a.data(), nv * 12, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync((void *)M.pos_b, e->mesh_pos_b.data(), nv * 12, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync((void *)M.fric_a, in->triangle_frictions_a, nt * 4, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync((void *)M.fric_b, b ? in->triangle_frictions_b : in->triangle_frictions_a, nt * 4,
+                           hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync((void *)M.damp_a, in->triangle_dampings_a, nt * 4, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync((void *)M.damp_b, b ? in->triangle_dampings_b : in->triangle_dampings_a, nt * 4,
+                           hipMemcpyHostToDevice, e->stream));
+    std::vector<float4> vel(nv);
+    const float fps = (float)e->consts.frames_per_second;
+    for (size_t v = 0; v < nv; ++v) {
+        float q[3];
+        for (int a = 0; a < 3; ++a) q[a] = b ? (e->mesh_pos_b[3 * v + a] - e->mesh_pos_a[3 * v + a]) * fps : 0.f;
+        vel[v] = make_float4(q[0], q[1], q[2], 0.f);
+    }
+    HIP_TRY(hipMemcpyAsync(M.vvel, vel.data(), nv * 16, hipMemcpyHostToDevice, e->stream));
+    // Candidate lists per block of the dense grid.
+    const GridGeom &G = e->geom;
+    const double h = G.h, margin = (double)M.forget + 0.01 * h;
+    const size_t nblk = G.num_blocks_total;
+    std::vector<uint32_t> off(nblk + 1, 0);
+    std::vector<int> range(6 * nt);
+    for (size_t t = 0; t < nt; ++t) {
+        double lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = 1e300;
+            hi[a] = -1e300;
+        }
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t v = e->mesh_tri[3 * t + k];
+            for (const std::vector<float> *P : {&e->mesh_pos_a, &e->mesh_pos_b})
+                for (int a = 0; a < 3; ++a) {
+                    lo[a] = std::min(lo[a], (double)(*P)[3 * v + a]);
+                    hi[a] = std::max(hi[a], (double)(*P)[3 * v + a]);
+                }
+        }
+        bool empty = false;
+        for (int a = 0; a < 3; ++a) {
+            const double c0 = std::floor((lo[a] - margin) / h) - G.origin[a], c1 = std::floor((hi[a] + margin) / h) - G.origin[a];
+            const double b0 = std::max(0.0, std::floor(c0 / 4.0)), b1 = std::min((double)G.nb[a] - 1, std::floor(c1 / 4.0));
+            if (!(b0 <= b1)) empty = true;
+            range[6 * t + 2 * a] = (int)b0;
+            range[6 * t + 2 * a + 1] = (int)b1;
+        }
+        if (empty) range[6 * t] = 1, range[6 * t + 1] = 0;
+    }
+    auto for_blocks = [&](size_t t, auto fn) {
+        const int *r = &range[6 * t];
+        for (int z = r[4]; z <= r[5]; ++z)
+            for (int y = r[2]; y <= r[3]; ++y)
+                for (int x = r[0]; x <= r[1]; ++x) fn((size_t)x + (size_t)G.nb[0] * ((size_t)y + (size_t)G.nb[1] * z));
+    };
+    for (size_t t = 0; t < nt; ++t) for_blocks(t, [&](size_t B) { ++off[B + 1]; });
+    for (size_t B = 0; B < nblk; ++B) off[B + 1] += off[B];
+    const uint32_t total = off[nblk];
+    std::vector<uint32_t> list(std::max<uint32_t>(total, 1));
+    {
+        std::vector<uint32_t> fill(off.begin(), off.end() - 1);
+        for (size_t t = 0; t < nt; ++t) for_blocks(t, [&](size_t B) { list[fill[B]++] = (uint32_t)t; });
+    }
+    if (total > e->blk_tri_capacity) {
+        if (M.blk_tri) e->mesh_free(M.blk_tri);
+        uint32_t *p = nullptr;
+        const uint32_t cap = std::max<uint32_t>(total + total / 2, 1024);
+        if (e->mesh_alloc(p, cap) != hipSuccess) return MPM_ERR_OUT_OF_MEMORY;
+        M.blk_tri = p;
+        e->blk_tri_capacity = cap;
+        e->drop_graphs();
+    }
+    HIP_TRY(hipMemcpyAsync((void *)M.blk_off, off.data(), (nblk + 1) * 4, hipMemcpyHostToDevice, e->stream));
+    if (total > 0)
+        HIP_TRY(hipMemcpyAsync((void *)M.blk_tri, list.data(), (size_t)total * 4, hipMemcpyHostToDevice, e->stream));
+    // the host vectors are freed when this returns: the copies complete first
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return MPM_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int mpm_set_collider_mesh(mpm_engine *e, const mpm_collider_mesh *mesh) {
+    if (!e) return MPM_ERR_INVALID_ARGUMENT;
+    HIP_TRY(hipSetDevice(e->device));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->drop_graphs();
+    e->free_mesh();
+    e->mesh_set = false;
+    e->launch.colliders = false;
+    e->mesh_tri.clear();
+    if (!mesh || mesh->num_triangles == 0) return MPM_OK;
+    if (e->launch.batched || !mesh->triangles || !mesh->triangle_colliders) return MPM_ERR_INVALID_ARGUMENT;
+    const uint32_t nv = mesh->num_vertices, nt = mesh->num_triangles;
+    uint32_t ncol = 0;
+    for (uint32_t t = 0; t < nt; ++t) {
+        if (mesh->triangle_colliders[t] >= 16) return MPM_ERR_INVALID_ARGUMENT;
+        ncol = std::max(ncol, mesh->triangle_colliders[t] + 1);
+        for (int k = 0; k < 3; ++k)
+            if (mesh->triangles[3 * t + k] >= nv) return MPM_ERR_INVALID_ARGUMENT;
+    }
+    // Topology: the triangle across each edge (ab, bc, ca) and the triangles at
+    // each vertex in ascending order (Topology, rust/crates/mesh_util/src/mesh.rs).
+    std::vector<uint32_t> tri(mesh->triangles, mesh->triangles + 3 * (size_t)nt);
+    std::vector<std::pair<uint64_t, uint32_t>> edges;
+    edges.reserve(3 * (size_t)nt);
+    auto ekey = [](uint32_t a, uint32_t b) { return a < b ? ((uint64_t)a << 32) | b : ((uint64_t)b << 32) | a; };
+    for (uint32_t t = 0; t < nt; ++t)
+        for (int k = 0; k < 3; ++k) edges.push_back({ekey(tri[3 * t + k], tri[3 * t + (k + 1) % 3]), t});
+    std::sort(edges.begin(), edges.end());
+    std::vector<uint32_t> opp(3 * (size_t)nt, 0xffffffffu);
+    for (uint32_t t = 0; t < nt; ++t)
+        for (int k = 0; k < 3; ++k) {
+            const uint64_t key = ekey(tri[3 * t + k], tri[3 * t + (k + 1) % 3]);
+            for (auto it = std::lower_bound(edges.begin(), edges.end(), std::make_pair(key, 0u));
+                 it != edges.end() && it->first == key; ++it)
+                if (it->second != t) {
+                    opp[3 * t + k] = it->second;
+                    break;
+                }
+        }
+    std::vector<uint32_t> voff(nv + 1, 0), vtri(3 * (size_t)nt);
+    for (uint32_t t = 0; t < nt; ++t)
+        for (int k = 0; k < 3; ++k) ++voff[tri[3 * t + k] + 1];
+    for (uint32_t v = 0; v < nv; ++v) voff[v + 1] += voff[v];
+    {
+        std::vector<uint32_t> fill(voff.begin(), voff.end() - 1);
+        for (uint32_t t = 0; t < nt; ++t)
+            for (int k = 0; k < 3; ++k) vtri[fill[tri[3 * t + k]]++] = t;
+    }
+    MeshDev &M = e->coll.mesh;
+    M.nvert = nv;
+    M.ntri = nt;
+    M.ncolliders = ncol;
+    M.forget = e->consts.grid_node_size * 2.2f;  // InputConsts::forget_distance
+    M.accept = e->consts.grid_node_size * 2.0f;  // InputConsts::accept_distance
+    uint32_t *d_tri, *d_coll, *d_opp, *d_voff, *d_vtri, *d_boff;
+    float *pa, *pb, *fa, *fb, *da, *db;
+    const size_t nblk = e->geom.num_blocks_total;
+    if (e->mesh_alloc(d_tri, 3 * (size_t)nt) || e->mesh_alloc(d_coll, nt) || e->mesh_alloc(d_opp, 3 * (size_t)nt) ||
+        e->mesh_alloc(d_voff, nv + 1) || e->mesh_alloc(d_vtri, 3 * (size_t)nt) || e->mesh_alloc(pa, 3 * (size_t)nv) ||
+        e->mesh_alloc(pb, 3 * (size_t)nv) || e->mesh_alloc(fa, nt) || e->mesh_alloc(fb, nt) || e->mesh_alloc(da, nt) ||
+        e->mesh_alloc(db, nt) || e->mesh_alloc(M.vvel, nv) || e->mesh_alloc(M.vpos, nv) || e->mesh_alloc(M.vnorm, nv) ||
+        e->mesh_alloc(M.tnorm, nt) || e->mesh_alloc(M.tfric, nt) || e->mesh_alloc(M.tdamp, nt) ||
+        e->mesh_alloc(d_boff, nblk + 1)) {
+        e->free_mesh();
+        return MPM_ERR_OUT_OF_MEMORY;
+    }
+    M.tri = d_tri;
+    M.tri_coll = d_coll;
+    M.opp = d_opp;
+    M.vtx_off = d_voff;
+    M.vtx_tri = d_vtri;
+    M.pos_a = pa;
+    M.pos_b = pb;
+    M.fric_a = fa;
+    M.fric_b = fb;
+    M.damp_a = da;
+    M.damp_b = db;
+    M.blk_off = d_boff;
+    auto up = [&](void *dst, const void *src, size_t bytes) { return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice); };
+    HIP_TRY(up(d_tri, tri.data(), tri.size() * 4));
+    HIP_TRY(up(d_coll, mesh->triangle_colliders, (size_t)nt * 4));
+    HIP_TRY(up(d_opp, opp.data(), opp.size() * 4));
+    HIP_TRY(up(d_voff, voff.data(), voff.size() * 4));
+    HIP_TRY(up(d_vtri, vtri.data(), vtri.size() * 4));
+    e->mesh_tri = std::move(tri);
+    // twin tables sized for a shell of near-collider nodes; grown on overflow
+    uint32_t cap = 1u << 14;
+    while (cap < std::min<uint64_t>(e->n / 2, 1ull << 26)) cap <<= 1;
+    int rc = set_twin_capacity(e, cap);
+    if (rc != MPM_OK) {
+        e->free_mesh();
+        return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->mesh_set = true;
+    e->launch.colliders = true;
+    return MPM_OK;
+}
 
 const char *mpm_error_string(int result) {
     switch (result) {
@@ -760,7 +1011,7 @@ int mpm_create(const mpm_consts *consts, const mpm_particles_view *pv, const mpm
         return fail(MPM_ERR_OUT_OF_MEMORY);
     if (e->launch.model_set == SET_ALL && e->alloc(e->jp_start, n) != hipSuccess) return fail(MPM_ERR_OUT_OF_MEMORY);
     if (e->alloc(e->X.visc_dyn, n) || e->alloc(e->X.visc_bulk, n) || e->alloc(e->X.sand_alpha, n) ||
-        e->alloc(e->X.jp, n) || e->alloc(e->X.energy, n))
+        e->alloc(e->X.jp, n) || e->alloc(e->X.energy, n) || e->alloc(e->X.collider_bits, n))
         return fail(MPM_ERR_OUT_OF_MEMORY);
     char *st = nullptr, *bt = nullptr;
     if (e->alloc(e->keys, n) || e->alloc(e->keys_sorted, n) || e->alloc(e->order, n) ||
@@ -833,6 +1084,7 @@ int mpm_create(const mpm_consts *consts, const mpm_particles_view *pv, const mpm
     ok = ok && upf(e->X.sand_alpha, [&](size_t i) { return pp[i].sand_alpha; });
     ok = ok && upf(e->X.jp, [&](size_t i) { return pv->plastic_jp ? pv->plastic_jp[i] : 1.0f; });
     ok = ok && upf(e->X.energy, [&](size_t i) { return pv->elastic_energies ? pv->elastic_energies[i] : 0.0f; });
+    ok = ok && up(e->X.collider_bits, e->collider_bits.data(), n * 4);
     if (!ok) return fail(MPM_ERR_HIP);
     if (hipStreamSynchronize(e->stream) != hipSuccess) return fail(MPM_ERR_HIP);
     *out_engine = e;
@@ -864,6 +1116,7 @@ int mpm_step_to(mpm_engine *e, double target_time, const mpm_frame_input *input,
     bool goals = false;
     int rc = upload_goals(e, input, goals);
     if (rc != MPM_OK) return rc;
+    if ((rc = upload_mesh_frames(e, input)) != MPM_OK) return rc;
     for (double t : times) {
         float g[3], fb;
         rc = frame_params(e, input, t, g, fb);
@@ -886,6 +1139,18 @@ int mpm_step_to(mpm_engine *e, double target_time, const mpm_frame_input *input,
     LoopReport rep;
     if ((rc = run_loop(e, times, times.size(), goals, cancel, true, rep)) != MPM_OK) return rc;
     if ((rc = fetch_scalars(e)) != MPM_OK) return rc;
+    // The table of collider twin nodes overflowed: the call runs again from
+    // its start with a larger table, as the reference redoes a frame whose
+    // hash table was too small (rust/crates/gpu/src/gpu_state.rs:562-599).
+    for (int redo = 0; e->launch.colliders && (e->ds_host->status & MPM_STATUS_TABLE_TRIES_EXCEEDED); ++redo) {
+        if (redo >= 6 || e->twin_capacity >= (1u << 28)) return MPM_ERR_CAPACITY;
+        if ((rc = set_twin_capacity(e, e->twin_capacity * 4)) != MPM_OK) return rc;
+        if ((rc = restore_call_start(e, cs)) != MPM_OK) return rc;
+        if ((rc = upload_params(e, params)) != MPM_OK) return rc;
+        rep = LoopReport{};
+        if ((rc = run_loop(e, times, times.size(), goals, cancel, true, rep)) != MPM_OK) return rc;
+        if ((rc = fetch_scalars(e)) != MPM_OK) return rc;
+    }
     uint32_t st = e->ds_host->status;
     uint64_t done = rep.done;
 
@@ -982,10 +1247,14 @@ int mpm_read_particles(mpm_engine *e, mpm_particles_view *out) {
     if (out->parameters) std::copy(e->parameters.begin(), e->parameters.end(), out->parameters);
     if (out->initial_positions) std::copy(e->initial_positions.begin(), e->initial_positions.end(), out->initial_positions);
     if (out->collider_bits) {
-        // Without mesh colliders every live particle's collider bits are set
-        // to 0 in each substep (rust/crates/cpu/src/phase/collide.rs:53-58).
-        for (size_t i = 0; i < n; ++i)
-            out->collider_bits[i] = (e->stepped && !e->dead_at_create[i]) ? 0u : e->collider_bits[i];
+        if (e->mesh_set) {
+            HIP_TRY(hipMemcpy(out->collider_bits, e->X.collider_bits, n * 4, hipMemcpyDeviceToHost));
+        } else {
+            // Without mesh colliders every live particle's collider bits are set
+            // to 0 in each substep (rust/crates/cpu/src/phase/collide.rs:53-58).
+            for (size_t i = 0; i < n; ++i)
+                out->collider_bits[i] = (e->stepped && !e->dead_at_create[i]) ? 0u : e->collider_bits[i];
+        }
     }
     return MPM_OK;
 }
@@ -1038,6 +1307,7 @@ int mpm_run_stage(mpm_engine *e, int32_t stage, const mpm_frame_input *input) {
     if (rc != MPM_OK) return rc;
     bool goals = false;
     if ((rc = upload_goals(e, input, goals)) != MPM_OK) return rc;
+    if ((rc = upload_mesh_frames(e, input)) != MPM_OK) return rc;
     std::vector<SubstepParams> one{make_params(e, g, fb, goals, input && input->has_b)};
     if ((rc = upload_params(e, one)) != MPM_OK) return rc;
     switch (stage) {

@@ -58,6 +58,50 @@ struct AuxArrays {
     float *goal_b[3];
     uint32_t *goal_flags_a;
     uint32_t *goal_flags_b;
+    uint32_t *collider_bits;   // per particle (the reference keeps them with the position)
+};
+
+// ---- mesh colliders (collide.hip, collide.hpp)
+// The mesh on the device: topology, the per-call frames and their
+// interpolation for the current substep, and the candidate triangles of each
+// grid block (every triangle within the forget distance of a point of the
+// block, in ascending order).
+struct MeshDev {
+    uint32_t ntri = 0, nvert = 0, ncolliders = 0;
+    const uint32_t *tri = nullptr;       // 3 vertex indices per triangle
+    const uint32_t *tri_coll = nullptr;  // collider of each triangle
+    const uint32_t *opp = nullptr;       // 3 opposite triangles per triangle (ab, bc, ca)
+    const uint32_t *vtx_off = nullptr;   // triangles at each vertex: vtx_tri[vtx_off[v] .. vtx_off[v + 1])
+    const uint32_t *vtx_tri = nullptr;
+    const float *pos_a = nullptr, *pos_b = nullptr;    // 3 per vertex, frames a and b
+    const float *fric_a = nullptr, *fric_b = nullptr;  // per triangle
+    const float *damp_a = nullptr, *damp_b = nullptr;
+    float4 *vvel = nullptr;              // vertex velocities of the call
+    float4 *vpos = nullptr;              // interpolated for the substep
+    float4 *vnorm = nullptr;
+    float4 *tnorm = nullptr;
+    float *tfric = nullptr, *tdamp = nullptr;
+    const uint32_t *blk_off = nullptr;   // num_blocks_total + 1
+    const uint32_t *blk_tri = nullptr;
+    float forget = 0.f, accept = 0.f;    // 2.2 h and 2 h (InputConsts::forget_distance / accept_distance)
+};
+
+// Twin nodes: the nodes that particles near a collider scatter to under their
+// collider bits, beside the plain grid that holds every particle's
+// contribution.  A hash table by node index; an entry holds up to TWIN_SLOTS
+// distinct bit patterns.  After the meld, mp holds (m, v) of the twin.
+constexpr int TWIN_SLOTS = 4;
+constexpr uint32_t TWIN_EMPTY = 0xffffffffu;
+constexpr int TWIN_PROBES = 64;
+struct __align__(16) TwinEntry {
+    uint32_t node;
+    uint32_t bits[TWIN_SLOTS];
+    uint32_t pad[3];
+    float4 mp[TWIN_SLOTS];
+};
+struct TwinTables {
+    TwinEntry *tab[2] = {nullptr, nullptr};  // by parity of the substep's epoch
+    uint32_t mask = 0;                        // capacity - 1 (a power of two)
 };
 
 // Dense block grid.  Local node (x, y, z) = global node - origin lives in
@@ -150,6 +194,41 @@ __device__ __forceinline__ float scene_local(float x, int a, const SubstepParams
     return x - k * S.scene_period[a];
 }
 
+// Grid-side extensions without a reference counterpart, applied to the
+// velocity v of a node with mass at local node coordinates (lx, ly, lz): grid
+// gravity, the sticky plane, box walls and the sphere (separating slip).
+__device__ __forceinline__ void grid_options(const SubstepParams &S, const GridGeom &G, int lx, int ly, int lz,
+                                             float v[3]) {
+    if (S.options & MPM_OPT_GRID_GRAVITY) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) v[q] += S.gravity[q] * S.dt;
+    }
+    if (S.options & (MPM_OPT_STICKY_PLANE | MPM_OPT_BOX_WALLS | MPM_OPT_SPHERE)) {
+        const float xi[3] = {scene_local((float)(lx + G.origin[0]) * S.h, 0, S),
+                             scene_local((float)(ly + G.origin[1]) * S.h, 1, S),
+                             scene_local((float)(lz + G.origin[2]) * S.h, 2, S)};
+        if ((S.options & MPM_OPT_STICKY_PLANE) && xi[2] < S.plane_z) v[0] = v[1] = v[2] = 0.f;
+        if (S.options & MPM_OPT_BOX_WALLS) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                if (xi[q] < S.wall_min[q] && v[q] < 0.f) v[q] = 0.f;
+                if (xi[q] > S.wall_max[q] && v[q] > 0.f) v[q] = 0.f;
+            }
+        }
+        if (S.options & MPM_OPT_SPHERE) {
+            const float d[3] = {xi[0] - S.sphere_center[0], xi[1] - S.sphere_center[1], xi[2] - S.sphere_center[2]};
+            const float r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            if (r2 < S.sphere_radius * S.sphere_radius && r2 > 0.f) {
+                const float vn = (v[0] * d[0] + v[1] * d[1] + v[2] * d[2]) / r2;
+                if (vn < 0.f) {
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) v[q] -= vn * d[q];
+                }
+            }
+        }
+    }
+}
+
 enum KernelId : uint32_t {
     KID_KEYS = 1,
     KID_SORT = 2,
@@ -183,6 +262,7 @@ struct TransferLaunch {
     bool batched;           // batched scenes (mpm_consts.scene_repeat): the M_BATCH kernels
     int cu_count;
     int particles_per_lane; // particles per thread per pass of a block, 1..3 (pass = 256 x this)
+    bool colliders = false; // a collider mesh is set: the M_COLL kernels
 };
 
 // The three modes of the transfer kernel: the first scatter of a call (from
@@ -198,6 +278,12 @@ struct GridBuffers {
     uint32_t *cand;             // NCAND candidate blocks per list entry (a block may be listed
                                 // by several entries)
     uint32_t *ovf[2];           // overflow lists of claimed blocks, by parity of the epoch
+};
+
+// Collider state a transfer needs (mesh set: the M_COLL kernels).
+struct ColliderCall {
+    MeshDev mesh;
+    TwinTables twins;
 };
 
 struct TransferCall {
@@ -219,6 +305,7 @@ struct TransferCall {
     const float *tau_ext = nullptr;  // P2G: snow stress from the learned closure (rows of tau_stride)
     uint64_t tau_stride = 0;
     float *svd_out = nullptr;        // G2P: U and s of each snow particle's new F (12 rows of tau_stride)
+    const ColliderCall *coll = nullptr;  // mesh colliders, or null
 };
 hipError_t launch_transfer(const TransferLaunch &L, const TransferCall &t, hipStream_t stream);
 uint32_t grid_update_waves(const TransferLaunch &L);
@@ -229,6 +316,16 @@ hipError_t launch_sum_partials(const TransferLaunch &L, const uint32_t *partial,
 hipError_t launch_copy_dead(const ParticleArrays &Pin, const ParticleArrays &Pout, const uint32_t *order,
                             uint64_t n, uint32_t *keys_out, uint32_t tomb, DeviceScalars *ds, hipStream_t stream);
 hipError_t launch_flag_inverted(const ParticleArrays &P, uint64_t n, hipStream_t stream);
+
+// ---- collide.hip
+// The mesh for the substep of parameter entry ds->param_index.
+hipError_t launch_mesh_interp(const MeshDev &m, const SubstepParams *params, DeviceScalars *ds, hipStream_t stream);
+// The twins of the substep just scattered melded against the plain grid's
+// totals (before the grid update clears them); the other parity's table
+// cleared for the next scatter.
+hipError_t launch_twin_meld(const TwinTables &t, const float4 *grid, const GridGeom &G, const SubstepParams *params,
+                            DeviceScalars *ds, hipStream_t stream);
+hipError_t launch_twin_clear(const TwinTables &t, hipStream_t stream);
 hipError_t launch_unpermute(const ParticleArrays &P, uint64_t n, float *x, float *v, float *F, float *C,
                             uint32_t *flags, hipStream_t stream);
 hipError_t launch_stress(const uint32_t *flags, const mpm_particle_parameters *params, const float *F,

@@ -36,6 +36,7 @@
 // polar decomposition; snow and sand need the full SVD.
 #include <algorithm>
 
+#include "collide.hpp"
 #include "device_math.hpp"
 #include "internal.hpp"
 
@@ -320,6 +321,7 @@ struct XferArgs {
     float *svd_out;              // G2P mode: U and s of snow particles' new F by slot (12 rows of
                                  // tau_stride floats), for the learned closure; or null
     uint32_t pass_cap;           // particles per pass of a block: XF_THREADS x 1..3
+    ColliderCall coll;           // M_COLL kernels: the mesh and the twin tables
 };
 
 // The substep parameters do not change while a kernel runs: read through the
@@ -503,6 +505,7 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
     constexpr bool HAS_G2P = MODE != XM_P2G;
     constexpr bool HAS_P2G = MODE != XM_G2P;
     constexpr bool LAST = MODE == XM_G2P;
+    constexpr bool COLL = (SET & M_COLL) != 0;
     // The region's node velocities (gather) and the arenas (scatter: one
     // 256-slot arena per core plane wave, one 512-slot arena for the overflow
     // wave) share LDS; a pass restages the velocities after the previous
@@ -672,6 +675,10 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                 for (int k = 0; k < 9; ++k) F[k] = cur.F[k];
                 const float m = cur.m, vol = cur.vol, p0 = cur.p0, p1 = cur.p1;
                 const uint32_t id = cur.id;
+                // Collider bits: a particle near a collider gathers from and
+                // scatters to its twin nodes as well (collide.hpp).
+                uint32_t cbits = 0;
+                if constexpr (COLL) cbits = a.X.collider_bits[id];
                 float v[3], C[9];
                 float J = 1.f;
                 float R[9], U[9], sv[3] = {1.f, 1.f, 1.f};
@@ -694,7 +701,24 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                         in_region = in_region && c[q] >= -1 && c[q] <= 4;
                         bspline_weights(f[q], w[q]);
                     }
-                    if (in_region) {
+                    if (COLL && (cbits >> 16)) {
+                        // the melded velocities of this particle's twins
+                        const int g0[3] = {c[0] + 4 * (int)b[0], c[1] + 4 * (int)b[1], c[2] + 4 * (int)b[2]};
+                        const bool ok = node_in_grid(G, g0[0], g0[1], g0[2]) &&
+                                        node_in_grid(G, g0[0] + 2, g0[1] + 2, g0[2] + 2);
+                        if (!ok) atomicOr(&ds->status, MPM_STATUS_OUT_OF_GRID | (KID_G2P << 16));
+                        const TwinEntry *tab = a.coll.twins.tab[(S1.epoch + 1u) & 1u];
+                        gather_stencil(
+                            w, f, apic,
+                            [&](int i3, int j3, int k3) {
+                                return ok ? twin_velocity(tab, a.coll.twins.mask,
+                                                          node_index(G, (uint32_t)(g0[0] + i3), (uint32_t)(g0[1] + j3),
+                                                                     (uint32_t)(g0[2] + k3)),
+                                                          cbits, ds)
+                                          : make_float4(0.f, 0.f, 0.f, 0.f);
+                            },
+                            v, C);
+                    } else if (in_region) {
                         const float4 *base = vstage + (c[0] + 1) + VS_Y * (c[1] + 1) + VS_Z * (c[2] + 1);
                         gather_stencil(
                             w, f, apic, [&](int i3, int j3, int k3) { return base[i3 + VS_Y * j3 + VS_Z * k3]; }, v, C);
@@ -883,6 +907,13 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                     // external_force of the substep that starts, then the payload
                     //   (f, m | q | h A columns): contribution at stencil node (i,j,k)
                     //   w_ijk (m v + A d) = w_ijk (q + i c0 + j c1 + k c2).
+                    if constexpr (COLL) {
+                        // collide (collide.rs:21-206) runs after the sort and before
+                        // the external force of the substep that starts
+                        const uint32_t b0 = cbits;
+                        collide_particle(a.coll.mesh, G, dt, xp, v, cbits);
+                        if (cbits != b0) a.X.collider_bits[id] = cbits;
+                    }
                     external_velocity<GOALS>(S1, a.X, id, xp, v);
                     float pl[16];
                     int c[3];
@@ -936,6 +967,12 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                     } else {
                         const int gc[3] = {c[0] + 4 * (int)b[0], c[1] + 4 * (int)b[1], c[2] + 4 * (int)b[2]};
                         scatter_global(a.B, G, S1.epoch, ds, gc, pl);
+                    }
+                    if constexpr (COLL) {
+                        if (cbits >> 16) {
+                            const int gc[3] = {c[0] + 4 * (int)b[0], c[1] + 4 * (int)b[1], c[2] + 4 * (int)b[2]};
+                            twin_scatter(a.coll.twins.tab[S1.epoch & 1u], a.coll.twins.mask, G, ds, gc, pl, cbits);
+                        }
                     }
                 }
             }
@@ -1141,37 +1178,7 @@ __global__ __launch_bounds__(256) void k_grid_update(GridBuffers B, GridGeom G, 
                 v[0] = g.y / m;
                 v[1] = g.z / m;
                 v[2] = g.w / m;
-                // Extensions without a reference counterpart.
-                if (S.options & MPM_OPT_GRID_GRAVITY) {
-#pragma unroll
-                    for (int q = 0; q < 3; ++q) v[q] += S.gravity[q] * S.dt;
-                }
-                if (S.options & (MPM_OPT_STICKY_PLANE | MPM_OPT_BOX_WALLS | MPM_OPT_SPHERE)) {
-                    const int lx = 4 * bx + (int)x, ly = 4 * by + (int)y, lz = 4 * bz + (int)z;
-                    const float xi[3] = {scene_local((float)(lx + G.origin[0]) * S.h, 0, S),
-                                         scene_local((float)(ly + G.origin[1]) * S.h, 1, S),
-                                         scene_local((float)(lz + G.origin[2]) * S.h, 2, S)};
-                    if ((S.options & MPM_OPT_STICKY_PLANE) && xi[2] < S.plane_z) v[0] = v[1] = v[2] = 0.f;
-                    if (S.options & MPM_OPT_BOX_WALLS) {
-#pragma unroll
-                        for (int q = 0; q < 3; ++q) {
-                            if (xi[q] < S.wall_min[q] && v[q] < 0.f) v[q] = 0.f;
-                            if (xi[q] > S.wall_max[q] && v[q] > 0.f) v[q] = 0.f;
-                        }
-                    }
-                    if (S.options & MPM_OPT_SPHERE) {
-                        const float d[3] = {xi[0] - S.sphere_center[0], xi[1] - S.sphere_center[1],
-                                            xi[2] - S.sphere_center[2]};
-                        const float r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-                        if (r2 < S.sphere_radius * S.sphere_radius && r2 > 0.f) {
-                            const float vn = (v[0] * d[0] + v[1] * d[1] + v[2] * d[2]) / r2;
-                            if (vn < 0.f) {
-#pragma unroll
-                                for (int q = 0; q < 3; ++q) v[q] -= vn * d[q];
-                            }
-                        }
-                    }
-                }
+                grid_options(S, G, 4 * bx + (int)x, 4 * by + (int)y, 4 * bz + (int)z, v);
                 ++active;
             }
             B.vgrid[node] = make_float4(m, v[0], v[1], v[2]);
@@ -1363,6 +1370,10 @@ hipError_t dispatch_flags(const TransferLaunch &L, bool permute, bool goals, con
 
 template <int MODE>
 hipError_t dispatch_set(const TransferLaunch &L, bool permute, bool goals, const XferArgs &a, hipStream_t stream) {
+    if (L.colliders) {
+        if (L.model_set == SET_POLAR) return dispatch_flags<MODE, SET_POLAR | M_COLL>(L, permute, goals, a, stream);
+        return dispatch_flags<MODE, SET_ALL | M_COLL>(L, permute, goals, a, stream);
+    }
     if (L.batched) {
         if (L.model_set == SET_POLAR) return dispatch_flags<MODE, SET_POLAR | M_BATCH>(L, permute, goals, a, stream);
         return dispatch_flags<MODE, SET_ALL | M_BATCH>(L, permute, goals, a, stream);
@@ -1391,6 +1402,10 @@ hipError_t launch_transfer(const TransferLaunch &L, const TransferCall &t, hipSt
     a.tau_stride = t.tau_stride;
     a.svd_out = t.svd_out;
     a.pass_cap = (uint32_t)XF_THREADS * (uint32_t)std::min(std::max(L.particles_per_lane, 1), XF_PER_THREAD);
+    if (L.colliders) {
+        if (!t.coll) return hipErrorInvalidValue;
+        a.coll = *t.coll;
+    }
     const bool goals = t.goals;
     switch (t.kind) {
         case TransferKind::P2G: return dispatch_set<XM_P2G>(L, t.permute, goals, a, stream);

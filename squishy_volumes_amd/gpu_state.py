@@ -225,6 +225,16 @@ class GpuState:
         _check(self._lib, self._lib.mpm_read_time_step_limits(self._h, C.byref(out)))
         return {k: float(getattr(out, k)) for k, _ in abi.MpmTimeStepLimits._fields_}
 
+    def set_collider_mesh(self, mesh):
+        """Mesh colliders (an io_state.ColliderMesh) from the next call on, or
+        none with mesh=None (mpm_set_collider_mesh); the frame inputs then
+        carry the mesh's vertex positions, frictions and dampings."""
+        if mesh is None:
+            _check(self._lib, self._lib.mpm_set_collider_mesh(self._h, None))
+            return
+        m, keep = mesh.to_c()
+        _check(self._lib, self._lib.mpm_set_collider_mesh(self._h, C.byref(m)))
+
     def debug_cancel_after(self, substep: int):
         """Test hook: the following calls behave as cancelled before their
         substep `substep` (mpm_debug_cancel_after); negative turns it off."""

@@ -259,6 +259,13 @@ class FrameInput:
     particle_flags_b: Optional[np.ndarray] = None
     goal_positions_a: Optional[np.ndarray] = None
     goal_positions_b: Optional[np.ndarray] = None
+    # the collider mesh's frames (InputFrame vertex positions, triangle frictions and dampings)
+    vertex_positions_a: Optional[np.ndarray] = None
+    vertex_positions_b: Optional[np.ndarray] = None
+    triangle_frictions_a: Optional[np.ndarray] = None
+    triangle_frictions_b: Optional[np.ndarray] = None
+    triangle_dampings_a: Optional[np.ndarray] = None
+    triangle_dampings_b: Optional[np.ndarray] = None
 
     def frame_factor(self, time: float, fps: int) -> float:
         """FrameInput::frame_factor, rust/crates/xpu/src/frame_input.rs:266-278."""
@@ -275,10 +282,32 @@ class FrameInput:
         fi.gravity_a[:] = self.gravity_a
         fi.has_b = 1 if self.gravity_b is not None else 0
         fi.gravity_b[:] = self.gravity_b if self.gravity_b is not None else self.gravity_a
-        for name in ("particle_flags_a", "particle_flags_b", "goal_positions_a", "goal_positions_b"):
+        for name in ("particle_flags_a", "particle_flags_b", "goal_positions_a", "goal_positions_b",
+                     "vertex_positions_a", "vertex_positions_b", "triangle_frictions_a", "triangle_frictions_b",
+                     "triangle_dampings_a", "triangle_dampings_b"):
             a = getattr(self, name)
             if a is not None:
                 a = np.ascontiguousarray(a, dtype=np.uint32 if "flags" in name else np.float32)
                 keep.append(a)
                 setattr(fi, name, a.ctypes.data)
         return fi, keep
+
+
+@dataclass
+class ColliderMesh:
+    """The collider mesh's topology (Topology, rust/crates/mesh_util/src/mesh.rs):
+    triangles as vertex index triples and the collider (0..15) of each triangle."""
+
+    num_vertices: int
+    triangles: np.ndarray            # (num_triangles, 3) uint32
+    triangle_colliders: np.ndarray   # (num_triangles,) uint32
+
+    def to_c(self) -> tuple[abi.MpmColliderMesh, list]:
+        tri = np.ascontiguousarray(self.triangles, np.uint32)
+        col = np.ascontiguousarray(self.triangle_colliders, np.uint32)
+        m = abi.MpmColliderMesh()
+        m.num_vertices = int(self.num_vertices)
+        m.num_triangles = int(tri.shape[0])
+        m.triangles = tri.ctypes.data
+        m.triangle_colliders = col.ctypes.data
+        return m, [tri, col]

@@ -14,7 +14,7 @@ from typing import Optional
 import numpy as np
 
 from . import abi
-from .io_state import EngineOptions, FrameInput, InputConsts, IoState, Particles, lame_lambda, lame_mu
+from .io_state import ColliderMesh, EngineOptions, FrameInput, InputConsts, IoState, Particles, lame_lambda, lame_mu
 
 
 @dataclass
@@ -294,3 +294,52 @@ def mixed_scene_set(count: int = 128, lo: int = 10_000, hi: int = 1_000_000, see
     """The combined workload: `count` scenes of elastic, snow and fluid
     objects, log-uniform sizes in [lo, hi] scaled to about `total`."""
     return uneven_scene_set(count, lo, hi, seed, total, materials=("elastic", "snow", "fluid"), **kw)
+
+
+def quad_mesh(corners, collider: int = 0, subdivisions: int = 1) -> tuple[ColliderMesh, np.ndarray]:
+    """A planar quadrilateral (corners in counter-clockwise order seen from
+    the side its normal points to) cut into subdivisions^2 cells of two
+    triangles each: the topology and the vertex positions (float32)."""
+    c = np.asarray(corners, np.float64)
+    k = subdivisions
+    uu, vv = np.meshgrid(np.linspace(0, 1, k + 1), np.linspace(0, 1, k + 1), indexing="ij")
+    uu, vv = uu.ravel(), vv.ravel()
+    verts = ((1 - uu) * (1 - vv))[:, None] * c[0] + (uu * (1 - vv))[:, None] * c[1] + (uu * vv)[:, None] * c[2] + \
+            ((1 - uu) * vv)[:, None] * c[3]
+    idx = lambda i, j: i * (k + 1) + j
+    tris = []
+    for i in range(k):
+        for j in range(k):
+            tris.append((idx(i, j), idx(i + 1, j), idx(i + 1, j + 1)))
+            tris.append((idx(i, j), idx(i + 1, j + 1), idx(i, j + 1)))
+    tris = np.array(tris, np.uint32)
+    return ColliderMesh(verts.shape[0], tris, np.full(tris.shape[0], collider, np.uint32)), verts.astype(np.float32)
+
+
+def box_mesh(lo, hi, collider: int = 0) -> tuple[ColliderMesh, np.ndarray]:
+    """The closed surface of an axis-aligned box (12 triangles, outward normals)."""
+    lo, hi = np.asarray(lo, np.float64), np.asarray(hi, np.float64)
+    verts = np.array([[lo[0] if i & 1 == 0 else hi[0], lo[1] if i & 2 == 0 else hi[1], lo[2] if i & 4 == 0 else hi[2]]
+                      for i in range(8)], np.float32)
+    quads = [(0, 2, 3, 1), (4, 5, 7, 6), (0, 1, 5, 4), (2, 6, 7, 3), (0, 4, 6, 2), (1, 3, 7, 5)]
+    tris = []
+    for a, b, c, d in quads:
+        tris.append((a, b, c))
+        tris.append((a, c, d))
+    tris = np.array(tris, np.uint32)
+    return ColliderMesh(8, tris, np.full(12, collider, np.uint32)), verts
+
+
+def mesh_frame(fi: FrameInput, vertices_a, vertices_b=None, friction: float = 0.0, damping: float = 0.0,
+               num_triangles: int = 0) -> FrameInput:
+    """fi with a collider mesh's frames: vertex positions of frames a (and b),
+    and a uniform friction and damping per triangle."""
+    import dataclasses
+
+    fa = np.full(num_triangles, friction, np.float32)
+    da = np.full(num_triangles, damping, np.float32)
+    return dataclasses.replace(
+        fi, vertex_positions_a=np.asarray(vertices_a, np.float32),
+        vertex_positions_b=None if vertices_b is None else np.asarray(vertices_b, np.float32),
+        triangle_frictions_a=fa, triangle_frictions_b=fa if vertices_b is not None else None,
+        triangle_dampings_a=da, triangle_dampings_b=da if vertices_b is not None else None)

@@ -41,7 +41,8 @@ def test_struct_layouts_match_header_sizes():
     # Sizes of the C structs as laid out by the C compiler of the platform.
     assert ctypes.sizeof(abi.MpmParticleParameters) == 36
     assert ctypes.sizeof(abi.MpmParticlesView) == 8 * 12
-    assert ctypes.sizeof(abi.MpmFrameInput) == 8 + 12 + 12 + 4 + 4 + 32
+    assert ctypes.sizeof(abi.MpmFrameInput) == 8 + 12 + 12 + 4 + 4 + 32 + 48
+    assert ctypes.sizeof(abi.MpmColliderMesh) == 8 + 16
     assert ctypes.sizeof(abi.MpmStatus) == 56
 
 
