@@ -100,7 +100,7 @@ enum {
  * (rust/crates/core/src/simulation.rs:242-250). */
 typedef struct mpm_consts {
     float grid_node_size;        /* scaled_grid_node_size() */
-    float time_step;             /* fixed dt (adaptive time steps are not supported) */
+    float time_step;             /* fixed dt (see mpm_set_adaptive_time_steps) */
     float domain_min[3];         /* scaled_domain_min(); particles are culled outside */
     float domain_max[3];
     uint32_t frames_per_second;
@@ -480,6 +480,28 @@ int mpm_read_time_step_limits(mpm_engine *engine, mpm_time_step_limits *out);
 /* The time step of the following mpm_step_to calls (>= 0; 0 makes them
  * return MPM_ERR_ZERO_TIME_STEP). */
 int mpm_set_time_step(mpm_engine *engine, float time_step);
+
+/* Per-substep adaptive time steps: CpuRunParameters { adaptive_time_steps,
+ * max_time_step } of CpuState::produce_next_state (rust/crates/cpu/src/
+ * cpu_state.rs:138-197), with the phases LimitTimeStepBeforeForce and
+ * LimitTimeStepBeforeIntegrate (rust/crates/cpu/src/phase/mod.rs:27-75,
+ * limit_time_step.rs:25-223).  With adaptive != 0 every following
+ * mpm_step_to substep takes two limits as device reductions: the sound-speed
+ * and isolated-particle limits of the state (after the grid is laid out,
+ * before the scatter) and the velocity and deformation limits of the
+ * gathered v and C (before the advance).  Each is pushed into the engine's
+ * AdaptiveTimeStepState (adaptive_time_step_state.rs:13-70: a history of
+ * the last 11 pushes, a missing limit counting as f32::MAX), whose allowed
+ * step (their minimum and max_time_step) is the step of the phases that
+ * follow: the collide and external force take the step allowed at the
+ * substep's start, the scatter and the grid update the step after the first
+ * push, the advance and the time the step after the second.  A step of 0
+ * returns MPM_ERR_ZERO_TIME_STEP.  The state lives as long as the engine.
+ * With adaptive == 0 the calls take consts.time_step (mpm_set_time_step).
+ * max_time_step must be > 0 (FLT_MAX or +inf: no maximum).  Not with a
+ * batched engine.  An adaptive substep synchronizes the stream twice and is
+ * not captured in a graph. */
+int mpm_set_adaptive_time_steps(mpm_engine *engine, int32_t adaptive, float max_time_step);
 
 /* ---- frame files (host only, no device) --------------------------------- */
 

@@ -512,6 +512,92 @@ inline MeshTopology make_topology(uint32_t num_vertices, const uint32_t *triangl
 }
 
 
+// ---- adaptive time steps (rust/crates/cpu/src/phase/limit_time_step.rs:25-223)
+constexpr double SINGULAR_VALUE_SEPARATION = 1e-5;  // rust/crates/util/src/consts.rs:14
+
+// limit_time_step_by_speed_of_sound for one particle (:36-130): h / c with
+// c^2 = kappa / rho, kappa the largest diagonal-space stiffness over J
+// (Neo-Hookean or inviscid, elastic.rs:276-295, :314-339, :555-570, :630-656),
+// from nalgebra's singular values (non-negative, descending).
+template <class R> R sound_limit(bool fluid, const mpm_particle_parameters &p, const M3<R> &F, R h) {
+    M3<R> U, V;
+    V3<R> sv;
+    svd3(F, U, sv, V);
+    R s[3] = {std::abs(sv[0]), std::abs(sv[1]), std::abs(sv[2])};
+    std::sort(s, s + 3, [](R a, R b) { return a > b; });
+    const R x = s[0], y = s[1], z = s[2];
+    const R J = x * y * z;
+    const R g[3] = {y * z, x * z, x * y};
+    const R Hm[3][3] = {{0, z, y}, {z, 0, x}, {y, x, 0}};
+    R d1, d2, diag, first[3], M[3][3];
+    if (fluid) {
+        const R K = R(p.bulk_modulus);
+        const int gamma = p.exponent;
+        d1 = K * (R(1) - R(1) / powi(J, gamma));
+        d2 = R(gamma) * K / powi(J, gamma + 1);
+        diag = 0;
+        for (int i = 0; i < 3; ++i) first[i] = d1 * g[i];
+    } else {
+        const R mu = R(p.mu), lambda = R(p.lambda);
+        d1 = (lambda * std::log(J) - mu) / J;
+        d2 = (lambda * (R(1) - std::log(J)) + mu) / (J * J);
+        diag = mu;
+        for (int i = 0; i < 3; ++i) first[i] = (mu / R(2)) * (R(2) * s[i]) + d1 * g[i];
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) M[i][j] = (i == j ? diag : R(0)) + d2 * g[i] * g[j] + d1 * Hm[i][j];
+    const R sep = R(SINGULAR_VALUE_SEPARATION);
+    const R kx = std::abs(x - y) < sep ? (first[0] + x * M[0][0] - y * M[1][0]) / R(2) / x
+                                       : (x * first[0] - y * first[1]) / (x * x - y * y);
+    const R ky = std::abs(y - z) < sep ? (first[1] + y * M[1][1] - z * M[2][1]) / R(2) / y
+                                       : (y * first[1] - z * first[2]) / (y * y - z * z);
+    const R kz = std::abs(z - x) < sep ? (first[2] + x * M[2][2] - x * M[0][2]) / R(2) / z
+                                       : (z * first[2] - x * first[0]) / (z * z - x * x);
+    const R kappa = std::max({x * x * M[0][0], y * y * M[1][1], z * z * M[2][2], y * y * kx, y * z * ky, z * x * kz}) / J;
+    const R rho = R(p.mass) / R(p.initial_volume) / J;
+    const R c2 = kappa / rho;
+    return c2 > R(0) ? h / std::sqrt(c2) : std::numeric_limits<R>::infinity();
+}
+
+// limit_time_step_by_isolated_particles for one particle (:132-185).
+template <class R> R isolated_limit(bool fluid, const mpm_particle_parameters &p, const M3<R> &F, R h) {
+    const R m = R(p.mass), vol = R(p.initial_volume);
+    if (!fluid) {
+        const R xi = R(3) / h / h;
+        return std::sqrt(m / (vol * xi * (R(1) - R(1) / R(2)) * (R(p.mu) + R(3) / R(2) * R(p.lambda))));
+    }
+    const R rho0 = m / vol;
+    const R j = F.determinant();
+    const R K = R(p.bulk_modulus);
+    const int gamma = p.exponent;
+    const R first = K * (R(1) - R(1) / powi(j, gamma));
+    if (std::abs(j - R(1)) > R(SINGULAR_VALUE_SEPARATION)) return h / j * std::sqrt(rho0 * (j - R(1)) / (R(6) * first * R(3)));
+    const R second = R(gamma) * K / powi(j, gamma + 1);
+    return h * std::sqrt(rho0 / (R(6) * second * R(3)));
+}
+
+// AdaptiveTimeStepState (rust/crates/cpu/src/adaptive_time_step_state.rs:13-70), f32
+// as in the reference; a missing limit is f32::MAX.
+struct AdaptiveState {
+    float max_time_step = std::numeric_limits<float>::max();
+    float by_velocity = std::numeric_limits<float>::max(), by_deformation = std::numeric_limits<float>::max();
+    float by_sound = std::numeric_limits<float>::max(), by_isolated = std::numeric_limits<float>::max();
+    std::vector<float> prior;
+    float without_prior() const { return std::min({max_time_step, by_velocity, by_deformation, by_sound, by_isolated}); }
+    float allowed() const {
+        float a = without_prior();
+        for (float v : prior) a = std::min(a, v);
+        return a;
+    }
+    void push_current_limit() {
+        if (prior.size() > 10) prior.erase(prior.begin());
+        prior.push_back(without_prior());
+    }
+};
+inline float limit_value(double v) {
+    return std::isfinite(v) ? (float)v : std::numeric_limits<float>::max();
+}
+
 template <class R>
 struct State {
     // CpuState (rust/crates/cpu/src/cpu_state.rs:15-23) and Particles
@@ -560,8 +646,47 @@ struct State {
         return -1;
     }
 
+    // Adaptive time steps (CpuRunParameters.adaptive_time_steps): the step of
+    // each phase is the adaptive state's allowed step at that phase.
+    bool adaptive = false;
+    AdaptiveState ats;
+
     R h() const { return R(consts.grid_node_size); }
-    R dt() const { return R(consts.time_step); }
+    float step() const { return adaptive ? ats.allowed() : consts.time_step; }
+    R dt() const { return R(step()); }
+
+    // limit_time_step_before_force (limit_time_step.rs:25-34).
+    void limit_time_step_before_force() {
+        const size_t n = positions.size();
+        const R hh = h();
+        R snd = std::numeric_limits<R>::infinity(), iso = std::numeric_limits<R>::infinity();
+        for (size_t i = 0; i < n; ++i) {
+            if (flags[i] & MPM_TOMBSTONED) continue;
+            const bool fluid = flags[i] & MPM_IS_FLUID;
+            snd = std::min(snd, sound_limit<R>(fluid, parameters[i], position_gradients[i], hh));
+            iso = std::min(iso, isolated_limit<R>(fluid, parameters[i], position_gradients[i], hh));
+        }
+        ats.by_sound = limit_value((double)snd);
+        ats.by_isolated = limit_value((double)iso);
+        ats.push_current_limit();
+    }
+
+    // limit_time_step_before_integrate (limit_time_step.rs:187-222).
+    void limit_time_step_before_integrate() {
+        const size_t n = positions.size();
+        R vmax = 0, cmax = 0;
+        bool any = false;
+        for (size_t i = 0; i < n; ++i) {
+            if (flags[i] & MPM_TOMBSTONED) continue;
+            any = true;
+            vmax = std::max(vmax, std::sqrt(dot(velocities[i], velocities[i])));
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) cmax = std::max(cmax, std::abs(velocity_gradients[i].m[r][c]));
+        }
+        ats.by_velocity = any && vmax != R(0) ? limit_value((double)(R(0.5) * h() / vmax)) : std::numeric_limits<float>::max();
+        ats.by_deformation = any ? limit_value((double)(R(0.2) / std::max(cmax, R(1e-8)))) : std::numeric_limits<float>::max();
+        ats.push_current_limit();
+    }
 
     // ---- InterpolateInput, rust/crates/cpu/src/phase/interpolate_input.rs:18-35
     // with FrameInput::frame_factor, rust/crates/xpu/src/frame_input.rs:266-278.
@@ -1097,7 +1222,9 @@ struct State {
         }
     }
 
-    // Phases up to and including ScatterMomentum (the P2G half of a substep).
+    // Phases up to and including ScatterMomentum (the P2G half of a substep),
+    // with the limits phase between UpdateGridNodes and ScatterMomentum when
+    // the step adapts (phase/mod.rs:27-75).
     int run_to_scatter() {
         int r = interpolate_input();
         if (r) return r;
@@ -1105,6 +1232,10 @@ struct State {
         collide();
         external_force();
         update_grid_nodes();
+        if (adaptive) {
+            limit_time_step_before_force();
+            if (step() == 0.f) return MPM_ERR_ZERO_TIME_STEP;
+        }
         scatter_momentum();
         return MPM_OK;
     }
@@ -1113,17 +1244,22 @@ struct State {
     int run_from_meld() {
         meld_grid();
         collect_velocity();
+        if (adaptive) {
+            limit_time_step_before_integrate();
+            if (step() == 0.f) return MPM_ERR_ZERO_TIME_STEP;
+        }
         if (!advance_particles()) return MPM_ERR_ENERGY;
         cull_particles();
-        time += (double)consts.time_step;  // cpu_state.rs:187-190, f32 dt added in f64
+        time += (double)step();  // cpu_state.rs:187-190, f32 dt added in f64
         return MPM_OK;
     }
 
-    // produce_next_state, rust/crates/cpu/src/cpu_state.rs:146-197 with fixed dt.
+    // produce_next_state, rust/crates/cpu/src/cpu_state.rs:146-197.
     int step_to(double target_time, const volatile int32_t *cancel, uint64_t &substeps) {
         substeps = 0;
-        if (consts.time_step == 0.f) return MPM_ERR_ZERO_TIME_STEP;
+        if (!adaptive && step() == 0.f) return MPM_ERR_ZERO_TIME_STEP;
         while (time < target_time) {
+            if (step() == 0.f) return MPM_ERR_ZERO_TIME_STEP;
             if (cancel && *cancel) return MPM_ERR_CANCELED;
             int r = run_to_scatter();
             if (r) return r;

@@ -287,6 +287,18 @@ int oracle_set_collider_mesh(void *handle, const mpm_collider_mesh *mesh) {
     return MPM_OK;
 }
 
+// Adaptive time steps with the given max_time_step (0: the fixed step).
+void oracle_set_adaptive(void *handle, int adaptive, float max_time_step) {
+    Handle *h = static_cast<Handle *>(handle);
+    if (h->precision == 8) {
+        h->d->adaptive = adaptive != 0;
+        h->d->ats.max_time_step = max_time_step;
+    } else {
+        h->f->adaptive = adaptive != 0;
+        h->f->ats.max_time_step = max_time_step;
+    }
+}
+
 double oracle_time(void *handle) {
     Handle *h = static_cast<Handle *>(handle);
     return h->precision == 8 ? h->d->time : h->f->time;

@@ -73,6 +73,7 @@ def _configure(lib):
     lib.oracle_cauchy_viscosity.argtypes = [C.c_double, C.c_double, V, V, C.c_uint64]
     lib.oracle_sphere_slip.argtypes = [V, V, C.c_double, V, V, C.c_uint64]
     lib.oracle_set_collider_mesh.argtypes = [V, C.POINTER(abi.MpmColliderMesh)]
+    lib.oracle_set_adaptive.argtypes = [V, C.c_int, C.c_float]
     lib.oracle_distance_to_triangle.argtypes = [V, V, V, C.c_uint64]
     lib.oracle_collider_compatible.restype = C.c_int
     lib.oracle_collider_compatible.argtypes = [C.c_uint32, C.c_uint32]
@@ -126,6 +127,10 @@ class Oracle:
             rc = self.lib.oracle_set_collider_mesh(self.h, C.byref(m))
         if rc != 0:
             raise ValueError("invalid collider mesh")
+
+    def set_adaptive(self, adaptive: bool, max_time_step: float):
+        """Per-substep adaptive steps (CpuRunParameters.adaptive_time_steps)."""
+        self.lib.oracle_set_adaptive(self.h, 1 if adaptive else 0, float(max_time_step))
 
     def step_to(self, target_time: float, frame_input: FrameInput) -> abi.MpmStatus:
         fi, keep = frame_input.to_c()

@@ -256,6 +256,7 @@ SIGNATURES = {
     "mpm_run_stage": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(MpmFrameInput)]),
     "mpm_read_time_step_limits": (C.c_int, [C.c_void_p, C.POINTER(MpmTimeStepLimits)]),
     "mpm_set_time_step": (C.c_int, [C.c_void_p, C.c_float]),
+    "mpm_set_adaptive_time_steps": (C.c_int, [C.c_void_p, C.c_int32, C.c_float]),
     "mpm_debug_cancel_after": (C.c_int, [C.c_void_p, C.c_int64]),
     "mpm_set_collider_mesh": (C.c_int, [C.c_void_p, C.POINTER(MpmColliderMesh)]),
     "mpm_frame_file_size": (C.c_uint64, [C.POINTER(MpmParticlesView), C.POINTER(MpmGridView)]),

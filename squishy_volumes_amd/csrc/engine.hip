@@ -7,7 +7,9 @@
 // (rust/crates/cpu/src/cpu_state.rs:146-197) and FrameInput::frame_factor
 // (rust/crates/xpu/src/frame_input.rs:266-278).
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
+#include <deque>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -42,6 +44,29 @@ struct EventRecord {
 };
 
 }  // namespace
+
+// AdaptiveTimeStepState (rust/crates/cpu/src/adaptive_time_step_state.rs:13-70):
+// the current limits (a missing limit is f32::MAX, as unwrap_or gives it) and
+// the history of up to 11 pushed limits; the allowed step is the smallest of
+// them and max_time_step.  It lives as long as the engine, as the reference's
+// lives as long as its CpuState.
+struct AdaptiveStep {
+    float max_time_step = FLT_MAX;
+    float by_velocity = FLT_MAX, by_deformation = FLT_MAX, by_isolated = FLT_MAX, by_sound = FLT_MAX;
+    std::deque<float> prior;
+    float without_prior() const {
+        return std::min({max_time_step, by_velocity, by_deformation, by_sound, by_isolated});
+    }
+    float allowed() const {
+        float a = without_prior();
+        for (float p : prior) a = std::min(a, p);
+        return a;
+    }
+    void push_current_limit() {
+        if (prior.size() > 10) prior.pop_front();  // TIME_STEP_HISTORY_LENGTH
+        prior.push_back(without_prior());
+    }
+};
 
 struct mpm_engine {
     int device = 0;
@@ -117,6 +142,10 @@ struct mpm_engine {
     float *rb_x = nullptr, *rb_v = nullptr, *rb_F = nullptr, *rb_C = nullptr;
     uint32_t *rb_flags = nullptr;
     int64_t cancel_after = -1;  // test hook: calls behave as cancelled before this substep
+
+    // Adaptive time steps (mpm_set_adaptive_time_steps): two limits per substep.
+    bool adaptive = false;
+    AdaptiveStep ats;
 
     // Mesh colliders (mpm_set_collider_mesh).
     bool mesh_set = false;
@@ -286,6 +315,7 @@ SubstepParams make_params(const mpm_engine *e, const float gravity[3], float fac
     SubstepParams S{};
     const mpm_consts &c = e->consts;
     S.dt = c.time_step;
+    S.dt_force = c.time_step;
     S.h = c.grid_node_size;
     for (int a = 0; a < 3; ++a) {
         S.gravity[a] = gravity[a];
@@ -461,6 +491,7 @@ int run_transfer(mpm_engine *e, TransferKind kind, bool goals, bool want_keys) {
         t.tau_stride = e->n;
     }
     const int stage = kind == TransferKind::P2G ? ST_P2G : (kind == TransferKind::G2P2G ? ST_G2P2G : ST_G2P);
+    if (kind == TransferKind::LIMITS && t.permute) return MPM_ERR_INVALID_ARGUMENT;
     if (e->launch.colliders) {
         // the mesh of the substep whose scatter this transfer runs
         if (kind != TransferKind::G2P) HIP_TRY(launch_mesh_interp(e->coll.mesh, e->param_table, e->ds, e->stream));
@@ -469,6 +500,7 @@ int run_transfer(mpm_engine *e, TransferKind kind, bool goals, bool want_keys) {
     e->begin(stage, r);
     HIP_TRY(launch_transfer(e->launch, t, e->stream));
     e->end(r);
+    if (kind == TransferKind::LIMITS) return MPM_OK;  // reads the state, writes only the reduction
     if (t.permute || want_keys) {
         // Slots past the live particles: copied when permuting, tombstone keys.
         e->begin(ST_COPY, r);
@@ -812,9 +844,153 @@ int upload_mesh_frames(mpm_engine *e, const mpm_frame_input *in) {
     return MPM_OK;
 }
 
+// Entry k of the parameter table rewritten (pinned staging slot k; the
+// caller has synchronized since the slot's previous copy).
+int upload_param_entry(mpm_engine *e, uint32_t k, SubstepParams p) {
+    p.index = k;
+    std::memcpy(e->param_host + k, &p, sizeof(SubstepParams));
+    HIP_TRY(hipMemcpyAsync(e->param_table + k, e->param_host + k, sizeof(SubstepParams), hipMemcpyHostToDevice,
+                           e->stream));
+    return MPM_OK;
+}
+
+// A limit of the device reductions: +inf (no particle, or none finite) is the
+// reference's None, which the adaptive state counts as f32::MAX.
+float as_limit(float v) { return std::isinf(v) || std::isnan(v) ? FLT_MAX : v; }
+
+// The substep loop with adaptive time steps, following the phase order of
+// CpuState::run_phase (rust/crates/cpu/src/phase/mod.rs:27-75) with
+// adaptive_time_steps: per substep the external force and collide take the
+// allowed step, the sound-speed and isolated-particle limits of the state
+// (limit_time_step_before_force) set the step of the scatter, and the
+// velocity and deformation limits of the gathered v and C
+// (limit_time_step_before_integrate) the step of the advance and of the time
+// (limit_time_step.rs:25-223).  The limits are device reductions; the
+// AdaptiveTimeStepState is host code, read after two synchronizations per
+// substep.  Substep t uses parameter entries 0 (its scatter) and 1 (its
+// gather).  A particle that fails stops the call after the substep in which
+// it failed, with the state its gather wrote; `stop` receives the failure's
+// status bits.
+int run_adaptive(mpm_engine *e, double target_time, const mpm_frame_input *input, bool goals,
+                 const volatile int32_t *cancel, LoopReport &rep, uint32_t &stop) {
+    int rc;
+    stop = 0;
+    const size_t K = (size_t)std::max(1, e->resort_interval);
+    const float h = e->consts.grid_node_size;
+    const uint32_t fail_bits = MPM_STATUS_ENERGY_ERROR | MPM_STATUS_PARTICLE_CLOSE_TO_INVERTED;
+    if (!e->limit_part &&
+        (e->alloc(e->limit_part, step_limit_groups(e->n)) != hipSuccess || e->alloc(e->limit_out, 1) != hipSuccess))
+        return MPM_ERR_OUT_OF_MEMORY;
+    float4 *lim_host = nullptr;
+    HIP_TRY(hipHostMalloc((void **)&lim_host, sizeof(float4), hipHostMallocDefault));
+    struct PinnedFree {
+        float4 *p;
+        ~PinnedFree() { (void)hipHostFree(p); }
+    } pinned_guard{lim_host};
+    auto sync_read = [&]() -> int {
+        HIP_TRY(hipMemcpyAsync(lim_host, e->limit_out, sizeof(float4), hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipMemcpyAsync(e->ds_host, e->ds, sizeof(DeviceScalars), hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        return MPM_OK;
+    };
+    bool force_sort = false;
+    float last_advance = 0.f;
+    for (size_t t = 0; e->time < target_time;) {
+        if (rep.done >= MPM_MAX_SUBSTEPS_PER_CALL) return MPM_ERR_INVALID_ARGUMENT;
+        if ((cancel && *cancel) || (e->cancel_after >= 0 && t >= (uint64_t)e->cancel_after)) {
+            rep.canceled = true;
+            break;
+        }
+        if (e->ats.allowed() == 0.f) return MPM_ERR_ZERO_TIME_STEP;
+        const AdaptiveStep saved = e->ats;
+        const uint32_t epoch0 = e->epoch;
+        const float dt_force = e->ats.allowed();
+        // limit_time_step_before_force, over the state the last gather wrote
+        HIP_TRY(launch_step_limits(e->P[e->cur], e->n, h, e->limit_part, e->limit_out, e->stream));
+        if ((rc = sync_read()) != MPM_OK) return rc;
+        if (e->ds_host->status & fail_bits) {
+            // the last gather failed a particle: its substep is the last
+            stop = e->ds_host->status;
+            if ((stop & MPM_STATUS_ENERGY_ERROR) && rep.done > 0) e->time -= (double)last_advance;
+            break;
+        }
+        e->ats.by_sound = as_limit(lim_host->x);
+        e->ats.by_isolated = as_limit(lim_host->y);
+        e->ats.push_current_limit();
+        if (e->ats.allowed() == 0.f) return MPM_ERR_ZERO_TIME_STEP;
+        const float dt_scatter = e->ats.allowed();
+        float g[3], fb;
+        if ((rc = frame_params(e, input, e->time, g, fb)) != MPM_OK) return rc;
+        SubstepParams s0 = make_params(e, g, fb, goals, input && input->has_b);
+        s0.dt = dt_scatter;
+        s0.dt_force = dt_force;
+        std::vector<SubstepParams> two{s0, s0};
+        two[1].dt_force = dt_scatter;  // rewritten with the step of the advance below
+        if ((rc = upload_params(e, two)) != MPM_OK) return rc;
+        if ((t % K == 0 || force_sort) && (rc = run_sort(e)) != MPM_OK) return rc;
+        force_sort = false;
+        if (e->closure_on && (rc = run_closure(e)) != MPM_OK) return rc;
+        if ((rc = run_transfer(e, TransferKind::P2G, goals, false)) != MPM_OK) return rc;
+        if ((rc = run_grid(e)) != MPM_OK) return rc;
+        // limit_time_step_before_integrate, over the gathered v and C
+        HIP_TRY(hipMemsetAsync(e->ds->limit_bits, 0, sizeof(e->ds->limit_bits), e->stream));
+        if ((rc = run_transfer(e, TransferKind::LIMITS, false, false)) != MPM_OK) return rc;
+        if ((rc = sync_read()) != MPM_OK) return rc;
+        if (e->launch.colliders && (e->ds_host->status & MPM_STATUS_TABLE_TRIES_EXCEEDED)) {
+            // the substep's twin table overflowed: again with a larger table
+            if (e->twin_capacity >= (1u << 28)) return MPM_ERR_CAPACITY;
+            if ((rc = set_twin_capacity(e, e->twin_capacity * 4)) != MPM_OK) return rc;
+            HIP_TRY(hipMemsetAsync(&e->ds->status, 0, 4, e->stream));
+            e->ats = saved;
+            e->epoch = epoch0;
+            force_sort = true;
+            e->keys_valid = false;
+            continue;
+        }
+        if (e->ds_host->status & fail_bits) {
+            // the stored state failed before the scatter: nothing advanced
+            stop = e->ds_host->status;
+            break;
+        }
+        float vmax, cmax;
+        std::memcpy(&vmax, &e->ds_host->limit_bits[0], 4);
+        std::memcpy(&cmax, &e->ds_host->limit_bits[1], 4);
+        const bool any = e->ds_host->limit_bits[2] != 0;
+        e->ats.by_velocity = any && vmax != 0.f ? 0.5f * h / vmax : FLT_MAX;
+        e->ats.by_deformation = any ? 0.2f / std::max(cmax, 1e-8f) : FLT_MAX;
+        e->ats.push_current_limit();
+        if (e->ats.allowed() == 0.f) return MPM_ERR_ZERO_TIME_STEP;
+        const float dt_advance = e->ats.allowed();
+        SubstepParams s1 = two[1];
+        s1.epoch = epoch0 + 1;
+        s1.dt_force = dt_advance;
+        if ((rc = upload_param_entry(e, 1, s1)) != MPM_OK) return rc;
+        if ((rc = run_transfer(e, TransferKind::G2P, false, (t + 1) % K == 0)) != MPM_OK) return rc;
+        e->time += (double)dt_advance;  // cpu_state.rs:187-190
+        last_advance = dt_advance;
+        ++rep.done;
+        ++t;
+    }
+    if (!stop) {
+        if ((rc = sync_read()) != MPM_OK) return rc;
+        if (e->ds_host->status & fail_bits) {
+            stop = e->ds_host->status;
+            if ((stop & MPM_STATUS_ENERGY_ERROR) && rep.done > 0) e->time -= (double)last_advance;
+        }
+    }
+    return MPM_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int mpm_set_adaptive_time_steps(mpm_engine *e, int32_t adaptive, float max_time_step) {
+    if (!e || !(max_time_step > 0.f) || (adaptive && e->launch.batched)) return MPM_ERR_INVALID_ARGUMENT;
+    e->adaptive = adaptive != 0;
+    e->ats.max_time_step = std::min(max_time_step, FLT_MAX);
+    return MPM_OK;
+}
 
 int mpm_set_collider_mesh(mpm_engine *e, const mpm_collider_mesh *mesh) {
     if (!e) return MPM_ERR_INVALID_ARGUMENT;
@@ -1102,6 +1278,42 @@ int mpm_step_to(mpm_engine *e, double target_time, const mpm_frame_input *input,
     if (!e) return MPM_ERR_INVALID_ARGUMENT;
     if (status) std::memset(status, 0, sizeof(*status));
     HIP_TRY(hipSetDevice(e->device));
+    if (e->adaptive) {
+        bool goals = false;
+        int rc = upload_goals(e, input, goals);
+        if (rc != MPM_OK) return rc;
+        if ((rc = upload_mesh_frames(e, input)) != MPM_OK) return rc;
+        {
+            float g[3], fb;
+            if (e->time < target_time && (rc = frame_params(e, input, e->time, g, fb)) != MPM_OK) return rc;
+        }
+        e->start = e->cur;
+        HIP_TRY(hipMemsetAsync(&e->ds->status, 0, 4, e->stream));
+        HIP_TRY(hipMemsetAsync(&e->ds->fail_index, 0xff, 4, e->stream));
+        LoopReport rep;
+        uint32_t stop = 0;
+        rc = run_adaptive(e, target_time, input, goals, cancel, rep, stop);
+        if (rc != MPM_OK) return rc;
+        if (rep.done > 0) e->stepped = true;
+        if ((rc = fetch_scalars(e)) != MPM_OK) return rc;
+        if (e->profiling) e->harvest();
+        const uint32_t st = e->ds_host->status | stop;
+        int result = rep.canceled ? MPM_ERR_CANCELED : MPM_OK;
+        if (st & MPM_STATUS_INDIRECT_LIMIT_EXCEEDED) result = MPM_ERR_CAPACITY;
+        else if (st & MPM_STATUS_ENERGY_ERROR) result = MPM_ERR_ENERGY;
+        else if (st & MPM_STATUS_PARTICLE_CLOSE_TO_INVERTED) result = MPM_ERR_CLOSE_TO_INVERTED;
+        else if (result == MPM_OK && (st & MPM_STATUS_OUT_OF_GRID)) result = MPM_ERR_OUT_OF_GRID;
+        if (status) {
+            status->device_status = st;
+            status->result = result;
+            status->substeps = rep.done;
+            status->time = e->time;
+            status->active_blocks = e->ds_host->num_blocks;
+            status->active_nodes = e->ds_host->active_nodes;
+            status->svd_fp64_fallbacks = (st & MPM_STATUS_SVD_FP64) ? 1 : 0;
+        }
+        return result;
+    }
     if (e->consts.time_step == 0.0f) return MPM_ERR_ZERO_TIME_STEP;
 
     // The substep times of the call, accumulated as the reference does

@@ -129,6 +129,8 @@ struct DeviceScalars {
     uint32_t grid_param;    // entry the next grid update reads (the transfer sets it)
     uint32_t fail_index;    // smallest parameter-table entry (transfer index) of a transfer that
                             // flagged a particle FAILED in the call (all ones: none)
+    uint32_t limit_bits[3]; // the LIMITS transfer's largest |v| and |C_ij| (float bits), and 1 when
+                            // it saw a live particle
 };
 
 // A particle whose deformation gradient has det F below this before its
@@ -165,7 +167,10 @@ __device__ __forceinline__ uint32_t particle_block_key(const float x[3], const G
 }
 
 struct SubstepParams {
-    float dt;
+    float dt;          // the step of the scatter and the grid update
+    float dt_force;    // the step of the external force and collide of this substep, and of the
+                       // advance that a gather of this entry completes (equal to dt unless the
+                       // step adapts within the substep)
     float h;
     float gravity[3];  // interpolated gravity of the substep
     float factor_b;    // frame factor (goal interpolation)
@@ -267,7 +272,7 @@ struct TransferLaunch {
 
 // The three modes of the transfer kernel: the first scatter of a call (from
 // the stored v, C, F), gather + advance + next scatter, and the last gather.
-enum class TransferKind { P2G, G2P2G, G2P };
+enum class TransferKind { P2G, G2P2G, G2P, LIMITS };
 
 // Grid-side buffers shared by the transfer kernel and the grid update.
 struct GridBuffers {

@@ -15,6 +15,8 @@
 //             substep n+1 from the registers.  The fused affine momentum
 //             A = m C - dt 4/h^2 V0 (tau + J sigma_visc) and the particle
 //             velocity never leave the chip.
+//   XM_LIMITS the gather alone, reduced to the largest |v| and |C_ij| for the
+//             adaptive step's limits before the advance (limit_time_step.rs:187-222).
 //   XM_G2P    last substep of a call: the gather, advance and cull; writes the
 //             reference's state x, v, F, C and the elastic energies.
 //
@@ -60,7 +62,7 @@ constexpr int VS_Z = 84;
 constexpr int VS_SLOTS = 7 + 7 * VS_Y + 7 * VS_Z + 1;
 constexpr uint32_t NO_BLOCK = 0xffffffffu;
 
-enum : int { XM_P2G = 0, XM_G2P2G = 1, XM_G2P = 2 };
+enum : int { XM_P2G = 0, XM_G2P2G = 1, XM_G2P = 2, XM_LIMITS = 3 };
 
 // Arena slot of region node (x, y, z), 0 <= x, y, z < 8 (region node = block
 // node + 1).  The low six bits are (x&3) | (y&3)<<2 | (z&3)<<4: for one
@@ -291,14 +293,14 @@ __device__ __forceinline__ void external_velocity(const SubstepParams &S, const 
             for (int a = 0; a < 3; ++a) {
                 const float gb = S.has_b ? X.goal_b[a][id] : X.goal_a[a][id];
                 const float gp = fa * X.goal_a[a][id] + S.factor_b * gb;
-                v[a] = (gp - xp[a]) / S.dt;
+                v[a] = (gp - xp[a]) / S.dt_force;
             }
             return;
         }
     }
     if (!(S.options & MPM_OPT_GRID_GRAVITY)) {
 #pragma unroll
-        for (int a = 0; a < 3; ++a) v[a] += S.dt * S.gravity[a];
+        for (int a = 0; a < 3; ++a) v[a] += S.dt_force * S.gravity[a];
     }
 }
 
@@ -501,10 +503,13 @@ __device__ __forceinline__ void accumulate_plane(int i, uint32_t cnt, int cx, in
 // particles.  The other modes stay at the compiler's choice (2 waves, bound
 // by their 78 KB of LDS per work-group anyway).
 template <int MODE, bool PERMUTE, bool GOALS, uint32_t SET>
-__global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE == XM_G2P ? 3 : 1))) void k_transfer(XferArgs a) {
+__global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE == XM_G2P || MODE == XM_LIMITS ? 3 : 1))) void k_transfer(XferArgs a) {
     constexpr bool HAS_G2P = MODE != XM_P2G;
-    constexpr bool HAS_P2G = MODE != XM_G2P;
+    constexpr bool HAS_P2G = MODE == XM_P2G || MODE == XM_G2P2G;
     constexpr bool LAST = MODE == XM_G2P;
+    constexpr bool LIMITS = MODE == XM_LIMITS;
+    float lim_v = 0.f, lim_c = 0.f;  // LIMITS: largest |v| and |C_ij| of this thread's particles
+    uint32_t lim_any = 0;
     constexpr bool COLL = (SET & M_COLL) != 0;
     // The region's node velocities (gather) and the arenas (scatter: one
     // 256-slot arena per core plane wave, one 512-slot arena for the overflow
@@ -543,7 +548,11 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
     // kernels hands the index on to the other in a slot the other kernel
     // alone reads, so no work-group races a reader of its own launch.
     if (blockIdx.x == 0 && tid == 0) ds->grid_param = pidx;
-    const float dt = S1.dt;
+    // The step of the advance that the gather precedes, and of the external
+    // force and collide of the substep that starts (the same value: the
+    // adaptive step does not change between them), and of the scatter (after
+    // the sound and isolated-particle limits).  Equal with a fixed step.
+    const float dt = S1.dt_force;
     const float scaling = S1.dt * 4.0f / (h * h);
     const float apic = 4.0f / h;
 
@@ -737,6 +746,14 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                             v, C);
                     }
 
+                    if constexpr (LIMITS) {
+                        // limit_time_step_by_velocity / _by_deformation (limit_time_step.rs:199-222)
+                        lim_v = fmaxf(lim_v, sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+#pragma unroll
+                        for (int k = 0; k < 9; ++k) lim_c = fmaxf(lim_c, fabsf(C[k]));
+                        lim_any = 1u;
+                        continue;
+                    }
                     // advance_particles: x += dt v, F += (C F) dt, plasticity, energy.
                     float CF[9];
                     matmul3(C, F, CF);
@@ -1137,6 +1154,20 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
         have = have_next;
         ++npass;
     }
+    if constexpr (LIMITS) {
+        // non-negative floats order as their bit patterns
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            lim_v = fmaxf(lim_v, __shfl_xor(lim_v, o, 64));
+            lim_c = fmaxf(lim_c, __shfl_xor(lim_c, o, 64));
+            lim_any |= (uint32_t)__shfl_xor((int)lim_any, o, 64);
+        }
+        if (lane == 0) {
+            atomicMax(&ds->limit_bits[0], __float_as_uint(lim_v));
+            atomicMax(&ds->limit_bits[1], __float_as_uint(lim_c));
+            if (lim_any) atomicMax(&ds->limit_bits[2], 1u);
+        }
+    }
 }
 
 // ------------------------------------------------------------ grid update
@@ -1359,7 +1390,10 @@ hipError_t launch_one(const TransferLaunch &L, const XferArgs &a, hipStream_t st
 
 template <int MODE, uint32_t SET>
 hipError_t dispatch_flags(const TransferLaunch &L, bool permute, bool goals, const XferArgs &a, hipStream_t stream) {
-    if constexpr (MODE == XM_G2P) {
+    if constexpr (MODE == XM_LIMITS) {
+        if (permute) return hipErrorInvalidValue;
+        return launch_one<MODE, false, false, SET>(L, a, stream);
+    } else if constexpr (MODE == XM_G2P) {
         return permute ? launch_one<MODE, true, false, SET>(L, a, stream) : launch_one<MODE, false, false, SET>(L, a, stream);
     } else {
         if (permute)
@@ -1411,6 +1445,7 @@ hipError_t launch_transfer(const TransferLaunch &L, const TransferCall &t, hipSt
         case TransferKind::P2G: return dispatch_set<XM_P2G>(L, t.permute, goals, a, stream);
         case TransferKind::G2P2G: return dispatch_set<XM_G2P2G>(L, t.permute, goals, a, stream);
         case TransferKind::G2P: return dispatch_set<XM_G2P>(L, t.permute, false, a, stream);
+        case TransferKind::LIMITS: return dispatch_set<XM_LIMITS>(L, t.permute, false, a, stream);
     }
     return hipErrorInvalidValue;
 }

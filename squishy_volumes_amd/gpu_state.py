@@ -28,8 +28,10 @@ class GpuError(RuntimeError):
 class GpuRunParameters:
     """GpuRunParameters, rust/crates/gpu/src/gpu_state.rs:331-334, with the
     time-step fields of CpuRunParameters (rust/crates/cpu/src/cpu_state.rs:138-143):
-    with adaptive_time_steps the call's step is the smallest of max_time_step
-    and the limits of the state it starts from (taken once per call)."""
+    with adaptive_time_steps every substep takes the reference's two limits
+    and steps by the allowed step of its AdaptiveTimeStepState, at most
+    max_time_step (mpm_set_adaptive_time_steps); without it, max_time_step
+    (when given) is the fixed step of the call."""
 
     target_time: float
     store_grid: bool = False
@@ -105,9 +107,11 @@ class GpuState:
         self, frame_input: FrameInput, params: GpuRunParameters, cancel: Optional[C.c_int32] = None
     ) -> tuple[IoState, Optional[GpuError]]:
         if params.adaptive_time_steps:
-            self.adapt_time_step(params.max_time_step if params.max_time_step is not None else float("inf"))
-        elif params.max_time_step is not None:
-            self.set_time_step(params.max_time_step)
+            self.set_adaptive_time_steps(True, params.max_time_step if params.max_time_step is not None else float("inf"))
+        else:
+            self.set_adaptive_time_steps(False)
+            if params.max_time_step is not None:
+                self.set_time_step(params.max_time_step)
         fi, keep = frame_input.to_c()
         status = abi.MpmStatus()
         rc = self._lib.mpm_step_to(
@@ -242,6 +246,10 @@ class GpuState:
 
     def set_time_step(self, time_step: float):
         _check(self._lib, self._lib.mpm_set_time_step(self._h, float(time_step)))
+
+    def set_adaptive_time_steps(self, adaptive: bool, max_time_step: float = float("inf")):
+        """Per-substep adaptive steps (mpm_set_adaptive_time_steps)."""
+        _check(self._lib, self._lib.mpm_set_adaptive_time_steps(self._h, 1 if adaptive else 0, float(max_time_step)))
 
     def adapt_time_step(self, max_time_step: float) -> float:
         """Sets the step of the following calls to the smallest of

@@ -156,10 +156,8 @@ def test_moving_mesh_with_friction_and_damping():
     assert max(errs.values()) <= 1e-4
 
 
-def test_twin_table_overflow_redoes_the_call():
-    """2,000 particles 3 cells apart, all near a floor: 27 twins each, more
-    than the engine's first table holds; the call is redone with a larger
-    table and matches the oracle."""
+def separate_particles_on_a_floor():
+    """2,000 particles 3 cells apart, all near a floor: 27 twins each."""
     rng = np.random.default_rng(7)
     ij = np.array([(i, j) for i in range(50) for j in range(40)])
     pos = np.zeros((ij.shape[0], 3), np.float32)
@@ -177,9 +175,15 @@ def test_twin_table_overflow_redoes_the_call():
     p.parameters["mu"] = 10.0
     p.parameters["lambda"] = 10.0
     c = consts(lo=(0.0, 0.0, 0.0), hi=(5.0, 4.0, 1.0))
-    mesh, fi = floor(0.2)
     mesh, verts = scenes.quad_mesh([(-0.5, -0.5, 0.2), (5.5, -0.5, 0.2), (5.5, 4.5, 0.2), (-0.5, 4.5, 0.2)], subdivisions=8)
     fi = scenes.mesh_frame(FrameInput(frame=0, gravity_a=(0.0, 0.0, -9.81)), verts, num_triangles=mesh.triangles.shape[0])
+    return c, p, mesh, fi
+
+
+def test_twin_table_overflow_redoes_the_call():
+    """More twins than the engine's first table holds: the call is redone
+    with a larger table and matches the oracle."""
+    c, p, mesh, fi = separate_particles_on_a_floor()
     g, st, [(o64, r64), (o32, r32)] = run_both(c, p, mesh, fi, 30)
     assert np.all(st.particles.collider_bits != 0)
     errs = report("2000 separate particles on a floor, 30 substeps vs fp64", st, r64)
