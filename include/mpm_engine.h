@@ -215,6 +215,9 @@ typedef struct mpm_status {
     uint64_t active_nodes;               /* grid nodes with mass > 0 in the last substep */
     uint64_t svd_fp64_fallbacks;         /* 1 when an SVD was recomputed in FP64 during the call */
     uint64_t graph_launches;             /* sort cycles run as one HIP graph launch by the call */
+    uint64_t grid_blocks;                /* 4^3 blocks of the block grid (the window's, with
+                                            mpm_tuning.grid_window) */
+    uint64_t window_redos;               /* times the call ran again with a larger grid window */
 } mpm_status;
 
 /* Grid nodes, as GridNodes (rust/crates/file_frame/src/grid_nodes.rs:9-15):
@@ -242,13 +245,22 @@ typedef struct mpm_tuning {
                                             not with the learned closure) */
     int32_t particles_per_lane;          /* particles each thread of the transfer kernel takes per
                                             pass of a block, 1..3 (default 3: passes of 768) */
+    int32_t grid_window;                 /* 1: the block grid is a window fitted to the particles
+                                            at the start of every call (grown and the call run
+                                            again when a particle leaves it), so that its memory
+                                            follows the particles, not the domain; -1: the grid
+                                            covers the whole domain; 0 (default): a window when
+                                            the whole domain would take more than 4 GiB of grid
+                                            or more than 1023 blocks along an axis */
 } mpm_tuning;
 
 /* ---- engine API (host pointers) ------------------------------------------ */
 
 /* GpuState::from_io_state (rust/crates/gpu/src/gpu_state.rs:38-190).  Copies
- * the particles to the device.  Fails with MPM_ERR_GRID_TOO_LARGE when the
- * domain's dense block table does not fit. */
+ * the particles to the device.  The block grid covers the domain, or a
+ * window fitted to the particles (mpm_tuning.grid_window); fails with
+ * MPM_ERR_GRID_TOO_LARGE when the grid it needs has more than 1023 blocks
+ * along an axis or more than 96 GiB of nodes (a window: at mpm_step_to). */
 int mpm_create(const mpm_consts *consts, const mpm_particles_view *particles,
                const mpm_tuning *tuning, mpm_engine **out_engine);
 

@@ -155,6 +155,8 @@ class MpmStatus(C.Structure):
         ("active_nodes", C.c_uint64),
         ("svd_fp64_fallbacks", C.c_uint64),
         ("graph_launches", C.c_uint64),
+        ("grid_blocks", C.c_uint64),
+        ("window_redos", C.c_uint64),
     ]
 
 
@@ -175,6 +177,7 @@ class MpmTuning(C.Structure):
         ("persistent_blocks", C.c_int32),
         ("graph", C.c_int32),
         ("particles_per_lane", C.c_int32),
+        ("grid_window", C.c_int32),
     ]
 
 

@@ -45,6 +45,10 @@ struct EventRecord {
 
 }  // namespace
 
+// A domain whose dense grid would take more than this (two float4 node
+// grids) gets a grid window fitted to its particles instead.
+constexpr uint64_t WINDOW_FROM_BYTES = 4ull << 30;
+
 // AdaptiveTimeStepState (rust/crates/cpu/src/adaptive_time_step_state.rs:13-70):
 // the current limits (a missing limit is f32::MAX, as unwrap_or gives it) and
 // the history of up to 11 pushed limits; the allowed step is the smallest of
@@ -143,6 +147,17 @@ struct mpm_engine {
     uint32_t *rb_flags = nullptr;
     int64_t cancel_after = -1;  // test hook: calls behave as cancelled before this substep
 
+    // Grid window (mpm_tuning.grid_window): the block grid covers the base
+    // cells win_lo..win_hi of the domain's dom_lo..dom_hi, refitted to the
+    // particles at the start of a call, instead of the whole domain.
+    bool windowed = false;
+    long long dom_lo[3] = {0, 0, 0}, dom_hi[3] = {0, 0, 0};
+    long long win_lo[3] = {0, 0, 0}, win_hi[3] = {0, 0, 0};
+    uint32_t grid_block_cap = 0;     // blocks the grid buffers hold
+    uint32_t blk_off_capacity = 0;   // entries of the collider block offsets
+    float4 *bounds_part = nullptr;
+    float *bounds_out = nullptr;
+
     // Adaptive time steps (mpm_set_adaptive_time_steps): two limits per substep.
     bool adaptive = false;
     AdaptiveStep ats;
@@ -198,6 +213,7 @@ struct mpm_engine {
         coll = ColliderCall{};
         blk_tri_capacity = 0;
         twin_capacity = 0;
+        blk_off_capacity = 0;
     }
 
     // Device memory of the collider mesh (freed with the mesh).
@@ -225,6 +241,17 @@ struct mpm_engine {
         if (e == hipSuccess) allocations.push_back(p);
         ptr = static_cast<T *>(p);
         return e;
+    }
+
+    template <class T>
+    void release(T *&ptr) {
+        if (!ptr) return;
+        auto it = std::find(allocations.begin(), allocations.end(), (void *)ptr);
+        if (it != allocations.end()) {
+            (void)hipFree(*it);
+            allocations.erase(it);
+        }
+        ptr = nullptr;
     }
 
     hipEvent_t take_event() {
@@ -271,19 +298,29 @@ namespace {
 
 bool finite3(const float *v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
 
-int compute_geometry(const mpm_consts &c, GridGeom &g, int &key_bits) {
+// The cells of the domain along each axis (base cells floor(x / h - 0.5) of
+// the positions inside it; with batched scenes, of every copy).
+void domain_cells(const mpm_consts &c, long long lo[3], long long hi[3]) {
     const double h = c.grid_node_size;
-    int nbmax = 0;
     for (int a = 0; a < 3; ++a) {
         double extent_max = c.domain_max[a];
         if (c.scene_repeat[a] > 1) extent_max += (double)c.scene_period[a] * (c.scene_repeat[a] - 1);
-        double lo = std::floor((double)c.domain_min[a] / h - 0.5);
-        double hi = std::floor(extent_max / h - 0.5);
-        if (!(hi - lo < 4.0 * 1023)) return MPM_ERR_GRID_TOO_LARGE;
-        long long origin = 4 * (long long)std::floor((lo - 2.0) / 4.0);
-        long long cmax = (long long)hi + 2 - origin;
-        long long nb = (cmax >> 2) + 2;
-        if (origin < -(1ll << 30) || origin > (1ll << 30)) return MPM_ERR_GRID_TOO_LARGE;
+        lo[a] = (long long)std::floor((double)c.domain_min[a] / h - 0.5);
+        hi[a] = (long long)std::floor(extent_max / h - 0.5);
+    }
+}
+
+// The block grid whose particles' base cells may lie in lo..hi (two cells of
+// stencil around them, whole 4^3 blocks): at most 1023 blocks per axis (30-bit
+// Morton keys) and 96 GiB for the two node grids.
+int geometry_for(const long long lo[3], const long long hi[3], double h, GridGeom &g, int &key_bits) {
+    int nbmax = 0;
+    for (int a = 0; a < 3; ++a) {
+        if (!(hi[a] - lo[a] < 4 * 1023ll)) return MPM_ERR_GRID_TOO_LARGE;
+        const long long origin = 4 * (long long)std::floor(((double)lo[a] - 2.0) / 4.0);
+        const long long cmax = hi[a] + 2 - origin;
+        const long long nb = (cmax >> 2) + 2;
+        if (origin < -(1ll << 30) || origin > (1ll << 30) || nb > 1023) return MPM_ERR_GRID_TOO_LARGE;
         g.origin[a] = (int)origin;
         g.nb[a] = (int)nb;
         nbmax = std::max<int>(nbmax, (int)nb);
@@ -294,13 +331,19 @@ int compute_geometry(const mpm_consts &c, GridGeom &g, int &key_bits) {
     g.axis_bits = k;
     key_bits = (int)(3 * k);
     g.tomb_key = (key_bits == 32) ? 0xffffffffu : ((1u << key_bits) - 1u);
-    g.h = c.grid_node_size;
+    g.h = (float)h;
     g.inv_h = 1.0f / g.h;
     uint64_t total = (uint64_t)g.nb[0] * g.nb[1] * g.nb[2];
     // Two float4 node grids: accumulation and velocity.
     if (total * 64ull * 16ull * 2ull > (96ull << 30)) return MPM_ERR_GRID_TOO_LARGE;
     g.num_blocks_total = (uint32_t)total;
     return MPM_OK;
+}
+
+int compute_geometry(const mpm_consts &c, GridGeom &g, int &key_bits) {
+    long long lo[3], hi[3];
+    domain_cells(c, lo, hi);
+    return geometry_for(lo, hi, c.grid_node_size, g, key_bits);
 }
 
 // Whole 32-particle tiles, rounded to an even count so a wave of 64 never
@@ -457,6 +500,140 @@ int out_set(const mpm_engine *e) {
     for (int b = 0; b < 3; ++b)
         if (b != e->cur && b != e->start) return b;
     return (e->cur + 1) % 3;
+}
+
+// The grid buffers for `blocks` blocks (freed first when held): the two node
+// grids, the block flags and epochs, the overflow lists, and the block list
+// and candidate slots for min(n, blocks) occupied blocks.
+int alloc_grid(mpm_engine *e, uint32_t blocks) {
+    e->release(e->grid);
+    e->release(e->vgrid);
+    e->release(e->blk_flag);
+    e->release(e->blk_epoch);
+    e->release(e->ovf[0]);
+    e->release(e->ovf[1]);
+    e->release(e->blk_start);
+    e->release(e->blk_key);
+    e->release(e->cand);
+    e->grid_block_cap = 0;
+    const uint64_t max_blocks = std::max<uint64_t>(std::min<uint64_t>(e->n, blocks), 1);
+    const uint64_t nodes = (uint64_t)blocks * 64ull;
+    if (e->alloc(e->grid, nodes) || e->alloc(e->vgrid, nodes) || e->alloc(e->blk_flag, blocks) ||
+        e->alloc(e->blk_epoch, blocks) || e->alloc(e->ovf[0], blocks) || e->alloc(e->ovf[1], blocks) ||
+        e->alloc(e->blk_start, max_blocks) || e->alloc(e->blk_key, max_blocks) || e->alloc(e->cand, NCAND * max_blocks))
+        return MPM_ERR_OUT_OF_MEMORY;
+    e->grid_block_cap = blocks;
+    e->blk_capacity = (uint32_t)max_blocks;
+    return MPM_OK;
+}
+
+// An empty grid of the current geometry (as after creation).
+int clear_grid(mpm_engine *e) {
+    const size_t nblk = e->geom.num_blocks_total;
+    HIP_TRY(hipMemsetAsync(e->grid, 0, nblk * 64 * sizeof(float4), e->stream));
+    HIP_TRY(hipMemsetAsync(e->vgrid, 0, nblk * 64 * sizeof(float4), e->stream));
+    HIP_TRY(hipMemsetAsync(e->blk_flag, 0, nblk * 4, e->stream));
+    HIP_TRY(hipMemsetAsync(e->blk_epoch, 0xff, nblk * 4, e->stream));
+    return MPM_OK;
+}
+
+bool window_is_domain(const mpm_engine *e) {
+    for (int a = 0; a < 3; ++a)
+        if (e->win_lo[a] != e->dom_lo[a] || e->win_hi[a] != e->dom_hi[a]) return false;
+    return true;
+}
+
+// The grid window over base cells lo..hi: a new geometry, buffers grown when
+// it needs more blocks than they hold, an empty grid, and everything built
+// on the old geometry dropped (captured graphs, block keys, collider block
+// lists and twin tables).
+int set_window(mpm_engine *e, const long long lo[3], const long long hi[3]) {
+    GridGeom g{};
+    int kb = 0;
+    int rc = geometry_for(lo, hi, e->consts.grid_node_size, g, kb);
+    if (rc != MPM_OK) return rc;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->drop_graphs();
+    if (g.num_blocks_total > e->grid_block_cap) {
+        const uint64_t want = std::min<uint64_t>((uint64_t)g.num_blocks_total * 5 / 4, 1023ull * 1023 * 1023);
+        if ((rc = alloc_grid(e, (uint32_t)std::max<uint64_t>(want, g.num_blocks_total))) != MPM_OK) return rc;
+    }
+    e->geom = g;
+    e->key_bits = kb;
+    for (int a = 0; a < 3; ++a) {
+        e->win_lo[a] = lo[a];
+        e->win_hi[a] = hi[a];
+    }
+    if ((rc = clear_grid(e)) != MPM_OK) return rc;
+    e->keys_valid = false;
+    e->grid_valid = false;
+    if (e->mesh_set && g.num_blocks_total + 1 > e->blk_off_capacity) {
+        MeshDev &M = e->coll.mesh;
+        uint32_t *p = nullptr;
+        if (M.blk_off) e->mesh_free(M.blk_off);
+        if (e->mesh_alloc(p, (size_t)g.num_blocks_total + 1) != hipSuccess) return MPM_ERR_OUT_OF_MEMORY;
+        M.blk_off = p;
+        e->blk_off_capacity = g.num_blocks_total + 1;
+    }
+    if (e->launch.colliders) HIP_TRY(launch_twin_clear(e->coll.twins, e->stream));
+    return MPM_OK;
+}
+
+// The live particles' bounding box and largest speed (out[0..7], see
+// launch_particle_bounds).  Synchronizes the stream.
+int particle_bounds(mpm_engine *e, float out[8]) {
+    if (!e->bounds_part) {
+        if (e->alloc(e->bounds_part, 2 * (size_t)step_limit_groups(e->n)) != hipSuccess ||
+            e->alloc(e->bounds_out, 8) != hipSuccess)
+            return MPM_ERR_OUT_OF_MEMORY;
+    }
+    HIP_TRY(launch_particle_bounds(e->P[e->cur], e->n, e->bounds_part, e->bounds_out, e->stream));
+    HIP_TRY(hipMemcpyAsync(out, e->bounds_out, 8 * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return MPM_OK;
+}
+
+// The window a call to `target` needs: the particles' base cells grown by
+// how far they can travel in the call at their largest speed under the
+// frame's gravity (times `scale`, plus two blocks), within the domain.  The
+// window is kept when it holds that box and is not 64 times larger; else it
+// is refitted with a quarter of the box's extent as slack on each side.
+int fit_window(mpm_engine *e, double target, const mpm_frame_input *in, double scale) {
+    float b[8];
+    int rc = particle_bounds(e, b);
+    if (rc != MPM_OK) return rc;
+    if (!(b[7] > 0.f)) return MPM_OK;  // no live particle
+    const double h = e->consts.grid_node_size;
+    const double T = std::max(0.0, target - e->time);
+    double g = 0.0;
+    if (in) {
+        for (const float *gv : {in->gravity_a, in->gravity_b})
+            g = std::max(g, std::sqrt((double)gv[0] * gv[0] + (double)gv[1] * gv[1] + (double)gv[2] * gv[2]));
+    }
+    const double disp = (std::isfinite(b[6]) ? (double)b[6] : 0.0) * T + 0.5 * g * T * T;
+    const double mcells = std::min(scale * (1.25 * disp / h + 8.0), 1e12);
+    const long long m = (long long)std::ceil(mcells);
+    long long need_lo[3], need_hi[3];
+    double vol_need = 1.0, vol_win = 1.0;
+    bool covered = true;
+    for (int a = 0; a < 3; ++a) {
+        const double cl = std::floor((double)b[a] / h - 0.5), ch = std::floor((double)b[3 + a] / h - 0.5);
+        const double lo = std::max((double)e->dom_lo[a], std::min(cl, (double)e->dom_hi[a]) - (double)m);
+        const double hi = std::min((double)e->dom_hi[a], std::max(ch, (double)e->dom_lo[a]) + (double)m);
+        need_lo[a] = (long long)lo;
+        need_hi[a] = std::max((long long)hi, need_lo[a]);
+        covered = covered && need_lo[a] >= e->win_lo[a] && need_hi[a] <= e->win_hi[a];
+        vol_need *= (double)(need_hi[a] - need_lo[a] + 1);
+        vol_win *= (double)(e->win_hi[a] - e->win_lo[a] + 1);
+    }
+    if (covered && vol_win <= 64.0 * vol_need) return MPM_OK;
+    long long lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        const long long slack = std::max<long long>(8, (need_hi[a] - need_lo[a]) / 4);
+        lo[a] = std::max(e->dom_lo[a], need_lo[a] - slack);
+        hi[a] = std::min(e->dom_hi[a], need_hi[a] + slack);
+    }
+    return set_window(e, lo, hi);
 }
 
 // One transfer launch.  After a sort the transfer reads slot i from
@@ -1070,6 +1247,7 @@ int mpm_set_collider_mesh(mpm_engine *e, const mpm_collider_mesh *mesh) {
     M.damp_a = da;
     M.damp_b = db;
     M.blk_off = d_boff;
+    e->blk_off_capacity = (uint32_t)(nblk + 1);
     auto up = [&](void *dst, const void *src, size_t bytes) { return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice); };
     HIP_TRY(up(d_tri, tri.data(), tri.size() * 4));
     HIP_TRY(up(d_coll, mesh->triangle_colliders, (size_t)nt * 4));
@@ -1144,15 +1322,50 @@ int mpm_create(const mpm_consts *consts, const mpm_particles_view *pv, const mpm
     e->consts = *consts;
     e->n = pv->count;
     e->time = pv->time;
-    int rc = compute_geometry(*consts, e->geom, e->key_bits);
-    if (rc != MPM_OK) {
-        delete e;
-        return rc;
-    }
     auto fail = [&](int code) {
         delete e;
         return code;
     };
+    // The grid covers the whole domain, or (mpm_tuning.grid_window, or a
+    // domain too large for one dense grid) a window fitted to the particles.
+    domain_cells(*consts, e->dom_lo, e->dom_hi);
+    int rc = geometry_for(e->dom_lo, e->dom_hi, consts->grid_node_size, e->geom, e->key_bits);
+    const int window_mode = tuning ? tuning->grid_window : 0;
+    e->windowed = window_mode > 0 ||
+                  (window_mode == 0 && (rc != MPM_OK || (uint64_t)e->geom.num_blocks_total * 64ull * 32ull > WINDOW_FROM_BYTES));
+    if (!e->windowed && rc != MPM_OK) return fail(rc);
+    if (e->windowed) {
+        // the first window: the live particles' cells, two blocks around them
+        // and slack, as fit_window would choose it
+        double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+        const double h = consts->grid_node_size;
+        for (uint64_t i = 0; i < pv->count; ++i) {
+            if (pv->flags[i] & MPM_TOMBSTONED) continue;
+            for (int a = 0; a < 3; ++a) {
+                const double c = std::floor((double)pv->positions[3 * i + a] / h - 0.5);
+                if (!std::isfinite(c)) continue;
+                lo[a] = std::min(lo[a], c);
+                hi[a] = std::max(hi[a], c);
+            }
+        }
+        long long wl[3], wh[3];
+        for (int a = 0; a < 3; ++a) {
+            if (!(lo[a] <= hi[a])) lo[a] = hi[a] = (double)e->dom_lo[a];
+            const double need_lo = std::max((double)e->dom_lo[a], std::min(lo[a], (double)e->dom_hi[a]) - 8.0);
+            const double need_hi = std::min((double)e->dom_hi[a], std::max(hi[a], (double)e->dom_lo[a]) + 8.0);
+            const long long slack = std::max<long long>(8, (long long)(need_hi - need_lo) / 4);
+            wl[a] = std::max(e->dom_lo[a], (long long)need_lo - slack);
+            wh[a] = std::min(e->dom_hi[a], std::max((long long)need_hi, (long long)need_lo) + slack);
+            e->win_lo[a] = wl[a];
+            e->win_hi[a] = wh[a];
+        }
+        if ((rc = geometry_for(wl, wh, consts->grid_node_size, e->geom, e->key_bits)) != MPM_OK) return fail(rc);
+    } else {
+        for (int a = 0; a < 3; ++a) {
+            e->win_lo[a] = e->dom_lo[a];
+            e->win_hi[a] = e->dom_hi[a];
+        }
+    }
     if (hipSetDevice(e->device) != hipSuccess) return fail(MPM_ERR_HIP);
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) return fail(MPM_ERR_HIP);
 
@@ -1167,6 +1380,7 @@ int mpm_create(const mpm_consts *consts, const mpm_particles_view *pv, const mpm
         if (tuning->graph < 0) e->use_graph = false;
         if (tuning->particles_per_lane < 0 || tuning->particles_per_lane > 3) return fail(MPM_ERR_INVALID_ARGUMENT);
         if (tuning->particles_per_lane > 0) e->launch.particles_per_lane = tuning->particles_per_lane;
+        if (tuning->grid_window < -1 || tuning->grid_window > 1) return fail(MPM_ERR_INVALID_ARGUMENT);
     }
     // Kernels are specialised on the constitutive models present: snow and
     // sand need the full SVD, every other model a polar decomposition at most.
@@ -1179,9 +1393,6 @@ int mpm_create(const mpm_consts *consts, const mpm_particles_view *pv, const mpm
         }
 
     const size_t n = e->n;
-    const uint64_t max_blocks = std::max<uint64_t>(std::min<uint64_t>(n, e->geom.num_blocks_total), 1);
-    e->blk_capacity = (uint32_t)max_blocks;
-    const uint64_t nodes = (uint64_t)e->geom.num_blocks_total * 64ull;
     if (alloc_particles(e, e->P[0]) != hipSuccess || alloc_particles(e, e->P[1]) != hipSuccess ||
         alloc_particles(e, e->P[2]) != hipSuccess)
         return fail(MPM_ERR_OUT_OF_MEMORY);
@@ -1191,24 +1402,16 @@ int mpm_create(const mpm_consts *consts, const mpm_particles_view *pv, const mpm
         return fail(MPM_ERR_OUT_OF_MEMORY);
     char *st = nullptr, *bt = nullptr;
     if (e->alloc(e->keys, n) || e->alloc(e->keys_sorted, n) || e->alloc(e->order, n) ||
-        e->alloc(st, radix_sort_temp_bytes(n)) || e->alloc(e->blk_start, max_blocks) ||
-        e->alloc(e->blk_key, max_blocks) || e->alloc(bt, block_list_temp_bytes(n)) || e->alloc(e->grid, nodes) ||
-        e->alloc(e->vgrid, nodes) || e->alloc(e->blk_flag, e->geom.num_blocks_total) ||
-        e->alloc(e->blk_epoch, e->geom.num_blocks_total) || e->alloc(e->ds, 1) ||
-        e->alloc(e->cand, NCAND * max_blocks) || e->alloc(e->ovf[0], e->geom.num_blocks_total) ||
-        e->alloc(e->ovf[1], e->geom.num_blocks_total) || e->alloc(e->partial, grid_update_waves(e->launch)))
+        e->alloc(st, radix_sort_temp_bytes(n)) || e->alloc(bt, block_list_temp_bytes(n)) || e->alloc(e->ds, 1) ||
+        e->alloc(e->partial, grid_update_waves(e->launch)))
         return fail(MPM_ERR_OUT_OF_MEMORY);
+    if ((rc = alloc_grid(e, e->geom.num_blocks_total)) != MPM_OK) return fail(rc);
     e->sort_temp = st;
     e->blk_temp = bt;
 
     if (hipHostMalloc((void **)&e->ds_host, sizeof(DeviceScalars), hipHostMallocDefault) != hipSuccess)
         return fail(MPM_ERR_OUT_OF_MEMORY);
-    if (hipMemsetAsync(e->grid, 0, nodes * 16, e->stream) != hipSuccess ||
-        hipMemsetAsync(e->vgrid, 0, nodes * 16, e->stream) != hipSuccess ||
-        hipMemsetAsync(e->blk_flag, 0, (size_t)e->geom.num_blocks_total * 4, e->stream) != hipSuccess ||
-
-        hipMemsetAsync(e->blk_epoch, 0xff, (size_t)e->geom.num_blocks_total * 4, e->stream) != hipSuccess ||
-        hipMemsetAsync(e->ds, 0, sizeof(DeviceScalars), e->stream) != hipSuccess)
+    if (clear_grid(e) != MPM_OK || hipMemsetAsync(e->ds, 0, sizeof(DeviceScalars), e->stream) != hipSuccess)
         return fail(MPM_ERR_HIP);
 
     // Host AoS -> device SoA.
@@ -1273,8 +1476,13 @@ double mpm_time(const mpm_engine *engine) { return engine ? engine->time : 0.0; 
 
 void *mpm_stream(mpm_engine *engine) { return engine ? (void *)engine->stream : nullptr; }
 
-int mpm_step_to(mpm_engine *e, double target_time, const mpm_frame_input *input, const volatile int32_t *cancel,
-                mpm_status *status) {
+}  // extern "C"
+
+namespace {
+
+// One call on the current grid (see mpm_step_to).
+int step_call(mpm_engine *e, double target_time, const mpm_frame_input *input, const volatile int32_t *cancel,
+              mpm_status *status) {
     if (!e) return MPM_ERR_INVALID_ARGUMENT;
     if (status) std::memset(status, 0, sizeof(*status));
     HIP_TRY(hipSetDevice(e->device));
@@ -1311,6 +1519,7 @@ int mpm_step_to(mpm_engine *e, double target_time, const mpm_frame_input *input,
             status->active_blocks = e->ds_host->num_blocks;
             status->active_nodes = e->ds_host->active_nodes;
             status->svd_fp64_fallbacks = (st & MPM_STATUS_SVD_FP64) ? 1 : 0;
+            status->grid_blocks = e->geom.num_blocks_total;
         }
         return result;
     }
@@ -1416,8 +1625,45 @@ int mpm_step_to(mpm_engine *e, double target_time, const mpm_frame_input *input,
         status->active_nodes = e->ds_host->active_nodes;
         status->svd_fp64_fallbacks = (st & MPM_STATUS_SVD_FP64) ? 1 : 0;
         status->graph_launches = rep.graph_launches;
+        status->grid_blocks = e->geom.num_blocks_total;
     }
     return result;
+}
+
+}  // namespace
+
+extern "C" {
+
+// With a grid window, the window is fitted to the particles first; a call
+// in which a particle left it (MPM_STATUS_OUT_OF_GRID while the window is
+// not yet the whole domain) runs again from its start with a window grown
+// by a factor of four each time.
+int mpm_step_to(mpm_engine *e, double target_time, const mpm_frame_input *input, const volatile int32_t *cancel,
+                mpm_status *status) {
+    if (!e) return MPM_ERR_INVALID_ARGUMENT;
+    if (!e->windowed) return step_call(e, target_time, input, cancel, status);
+    if (status) std::memset(status, 0, sizeof(*status));
+    HIP_TRY(hipSetDevice(e->device));
+    int rc = fit_window(e, target_time, input, 1.0);
+    if (rc != MPM_OK) return rc;
+    const CallStart cs = begin_call(e);
+    const AdaptiveStep ats0 = e->ats;
+    if (e->jp_start && e->n > 0)
+        HIP_TRY(hipMemcpyAsync(e->jp_start, e->X.jp, e->n * 4, hipMemcpyDeviceToDevice, e->stream));
+    for (int attempt = 1;; ++attempt) {
+        mpm_status st{};
+        rc = step_call(e, target_time, input, cancel, &st);
+        const bool short_grid = (st.device_status & MPM_STATUS_OUT_OF_GRID) && !window_is_domain(e);
+        if (!short_grid || attempt >= 8 || (rc < 0 && rc != MPM_ERR_OUT_OF_GRID && rc != MPM_ERR_CANCELED)) {
+            st.grid_blocks = e->geom.num_blocks_total;
+            st.window_redos = (uint64_t)(attempt - 1);
+            if (status) *status = st;
+            return rc;
+        }
+        if ((rc = restore_call_start(e, cs)) != MPM_OK) return rc;
+        e->ats = ats0;
+        if ((rc = fit_window(e, target_time, input, std::pow(4.0, attempt))) != MPM_OK) return rc;
+    }
 }
 
 int mpm_read_particles(mpm_engine *e, mpm_particles_view *out) {

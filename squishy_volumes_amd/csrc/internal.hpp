@@ -395,6 +395,10 @@ struct ClosureLaunch {
 };
 hipError_t launch_closure(const ClosureLaunch &c, hipStream_t stream);
 uint32_t step_limit_groups(uint64_t n);
+// The live particles' bounding box and largest speed into out[0..7]
+// ({lo xyz, hi xyz, max |v|, 1 if any live particle}); part holds
+// 2 step_limit_groups(n) float4.
+hipError_t launch_particle_bounds(const ParticleArrays &P, uint64_t n, float4 *part, float *out, hipStream_t stream);
 hipError_t launch_step_limits(const ParticleArrays &P, uint64_t n, float h, float4 *part, float4 *out,
                               hipStream_t stream);
 hipError_t launch_closure_pairs(const ParticleArrays &P, const AuxArrays &X, uint64_t n, float hardening,

@@ -171,7 +171,91 @@ __global__ __launch_bounds__(256) void k_step_limits_final(const float4 *__restr
     }
 }
 
+// The bounding box of the live particles' positions and their largest
+// speed, for the grid window: per work-group {min x, max x, max |v|}, then one
+// work-group over the partials.  NaN positions are skipped (fminf/fmaxf).
+struct Bounds {
+    float lo[3], hi[3], vmax, live;
+};
+__device__ __forceinline__ void bounds_merge(Bounds &r, const Bounds &o) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        r.lo[a] = fminf(r.lo[a], o.lo[a]);
+        r.hi[a] = fmaxf(r.hi[a], o.hi[a]);
+    }
+    r.vmax = fmaxf(r.vmax, o.vmax);
+    r.live = fmaxf(r.live, o.live);
+}
+__device__ __forceinline__ Bounds bounds_empty() {
+    Bounds r;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        r.lo[a] = INFINITY;
+        r.hi[a] = -INFINITY;
+    }
+    r.vmax = 0.f;
+    r.live = 0.f;
+    return r;
+}
+__device__ Bounds bounds_block_reduce(Bounds r) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        Bounds q;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            q.lo[a] = __shfl_xor(r.lo[a], o, 64);
+            q.hi[a] = __shfl_xor(r.hi[a], o, 64);
+        }
+        q.vmax = __shfl_xor(r.vmax, o, 64);
+        q.live = __shfl_xor(r.live, o, 64);
+        bounds_merge(r, q);
+    }
+    __shared__ Bounds w[4];
+    if ((threadIdx.x & 63u) == 0) w[threadIdx.x >> 6] = r;
+    __syncthreads();
+    Bounds t = w[0];
+    for (int k = 1; k < 4; ++k) bounds_merge(t, w[k]);
+    return t;
+}
+
+__global__ __launch_bounds__(256) void k_bounds(ParticleArrays P, uint64_t n, Bounds *__restrict__ part) {
+    Bounds r = bounds_empty();
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        if (P.flags(i) & F_TOMB) continue;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            r.lo[a] = fminf(r.lo[a], P.x(a, i));
+            r.hi[a] = fmaxf(r.hi[a], P.x(a, i));
+        }
+        const float v0 = P.v(0, i), v1 = P.v(1, i), v2 = P.v(2, i);
+        r.vmax = fmaxf(r.vmax, sqrtf(v0 * v0 + v1 * v1 + v2 * v2));
+        r.live = 1.f;
+    }
+    const Bounds t = bounds_block_reduce(r);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(256) void k_bounds_final(const Bounds *__restrict__ part, uint32_t groups,
+                                                      float *__restrict__ out) {
+    Bounds r = bounds_empty();
+    for (uint32_t i = threadIdx.x; i < groups; i += blockDim.x) bounds_merge(r, part[i]);
+    const Bounds t = bounds_block_reduce(r);
+    if (threadIdx.x < 8u) {
+        const float v[8] = {t.lo[0], t.lo[1], t.lo[2], t.hi[0], t.hi[1], t.hi[2], t.vmax, t.live};
+        out[threadIdx.x] = v[threadIdx.x];
+    }
+}
+
 }  // namespace
+
+hipError_t launch_particle_bounds(const ParticleArrays &P, uint64_t n, float4 *part, float *out, hipStream_t stream) {
+    static_assert(sizeof(Bounds) == 2 * sizeof(float4), "two float4 per partial");
+    const uint32_t groups = step_limit_groups(n);
+    hipLaunchKernelGGL(k_bounds, dim3(groups), dim3(256), 0, stream, P, n, reinterpret_cast<Bounds *>(part));
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, stream, reinterpret_cast<const Bounds *>(part), groups,
+                       out);
+    return hipGetLastError();
+}
 
 uint32_t step_limit_groups(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 2048)); }
 

@@ -43,7 +43,7 @@ def test_struct_layouts_match_header_sizes():
     assert ctypes.sizeof(abi.MpmParticlesView) == 8 * 12
     assert ctypes.sizeof(abi.MpmFrameInput) == 8 + 12 + 12 + 4 + 4 + 32 + 48
     assert ctypes.sizeof(abi.MpmColliderMesh) == 8 + 16
-    assert ctypes.sizeof(abi.MpmStatus) == 56
+    assert ctypes.sizeof(abi.MpmStatus) == 72
 
 
 def test_missing_library_fails_loudly(tmp_path):
