@@ -580,10 +580,17 @@ def main():
     assert status.result == 0, status.result
     graph_launches = int(status.graph_launches)
     log(f"bench: {args.steps} substeps in {elapsed:.3f} s ({graph_launches} graph launches)")
-    # Kernel times from separate runs after the timed region: HIP events on the
-    # engine's stream around the transfer kernels (the dominant kernel is one
-    # of them), then around every stage.
-    kernel_steps = min(args.steps, 100)
+    nodes = int(status.active_nodes)
+    active_blocks = int(status.active_blocks)
+    gpu.close()
+    # Kernel times from a separate run over the same substeps: a second engine
+    # from the same state, the same warm-up, then the timed region's substeps
+    # again with HIP events on the engine's stream around the transfer kernels
+    # (the dominant kernel is one of them); then around every stage.
+    gpu = GpuState.from_io_state(0, scene.consts, scene.time_step, scene.io_state, scene.options, tuning)
+    if args.warmup:
+        gpu.step_to(scenes.advance_time(gpu.time, scene.time_step, args.warmup), fi)
+    kernel_steps = args.steps
     gpu.kernel_times(reset=True)
     gpu.set_profiling(True, stages=TIMED_STAGES)
     gpu.step_to(scenes.advance_time(gpu.time, scene.time_step, kernel_steps), fi)
@@ -597,7 +604,6 @@ def main():
     copy_bw = copy_bandwidth()
     log(f"bench: copy kernel {copy_bw['achieved']:.0f} GB/s")
 
-    nodes = int(status.active_nodes)
     value = n * args.steps / elapsed
     stage_ms = {k: v[0] / max(v[1], 1) for k, v in stages.items()}
     dominant = max(TIMED_STAGES, key=lambda k: stages.get(k, (0, 1))[0])
@@ -627,7 +633,7 @@ def main():
             "materials": "3 Neo-Hookean spheres (E log-uniform 1e4..1e6, nu 0.3) + inviscid fluid slab (K 1e5, gamma 7)",
             "boundary": "box walls (separating slip)",
             "active_nodes": nodes,
-            "active_blocks": int(status.active_blocks),
+            "active_blocks": active_blocks,
         },
         "roofline": {
             "kernel": dominant,
@@ -651,9 +657,10 @@ def main():
         "stage_ms_per_launch": {k: v[0] / max(v[1], 1) for k, v in breakdown.items() if v[1]},
         "stage_launches_per_substep": {k: v[1] / breakdown_steps for k, v in breakdown.items() if v[1]},
         "timed_path": f"default: {graph_launches} captured re-sort cycles replayed as HIP graphs, no events",
-        "stage_breakdown": f"roofline.avg_launch_ms: {kernel_steps} substeps after the timed region with events "
-                           f"around the transfer kernels; stage_ms_per_launch: {breakdown_steps} further substeps "
-                           "with an event pair around every launch",
+        "stage_breakdown": f"roofline.avg_launch_ms: the timed region's {kernel_steps} substeps run again by a second "
+                           "engine from the same state after the same warm-up, with events around the transfer "
+                           f"kernels; stage_ms_per_launch: {breakdown_steps} further substeps with an event pair "
+                           "around every launch",
         "svd_fp64_fallbacks": int(status.svd_fp64_fallbacks),
     }
     gpu.close()

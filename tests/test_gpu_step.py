@@ -426,7 +426,7 @@ def test_error_paths():
     hold the current time (FrameInput::frame_factor,
     rust/crates/xpu/src/frame_input.rs:266-278), a zero time step
     (Error::ZeroTimeStep, rust/crates/cpu/src/cpu_state.rs:171-173), a domain
-    whose block grid does not fit, and invalid constants."""
+    whose whole-domain block grid does not fit, and invalid constants."""
     from squishy_volumes_amd.gpu_state import GpuError
 
     sc = perturbed_cube(n_cells=3)
@@ -442,8 +442,10 @@ def test_error_paths():
     huge = InputConsts(grid_node_size=1.0 / 256, frames_per_second=24, domain_min=(-100, -100, -100),
                        domain_max=(100, 100, 100))
     with pytest.raises(GpuError) as e:
-        GpuState.from_io_state(0, huge, 1e-4, sc.io_state)
+        GpuState.from_io_state(0, huge, 1e-4, sc.io_state, None, {"grid_window": -1})
     assert e.value.code == abi.ERR_GRID_TOO_LARGE
+    # by default such a domain gets a grid window (tests/test_gpu_window.py)
+    GpuState.from_io_state(0, huge, 1e-4, sc.io_state).close()
     bad = InputConsts(grid_node_size=-1.0, frames_per_second=24)
     with pytest.raises(GpuError) as e:
         GpuState.from_io_state(0, bad, 1e-4, sc.io_state)
