@@ -601,8 +601,10 @@ def main():
     gpu.step_to(scenes.advance_time(gpu.time, scene.time_step, breakdown_steps), fi)
     gpu.set_profiling(False)
     breakdown = gpu.kernel_times(reset=True)
-    copy_bw = copy_bandwidth()
-    log(f"bench: copy kernel {copy_bw['achieved']:.0f} GB/s")
+    # (not in counter runs: their copy-kernel dispatches are the calibration's alone)
+    copy_bw = None if args.copy_mib else copy_bandwidth()
+    if copy_bw:
+        log(f"bench: copy kernel {copy_bw['achieved']:.0f} GB/s")
 
     value = n * args.steps / elapsed
     stage_ms = {k: v[0] / max(v[1], 1) for k, v in stages.items()}
@@ -645,13 +647,13 @@ def main():
             "traffic": traffic_for(dominant, "spheres_and_slab"),
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": stage_ms[dominant],
-            "frac_of_copy": achieved / copy_bw["achieved"],
+            "frac_of_copy": achieved / copy_bw["achieved"] if copy_bw else None,
         },
         "substep_roofline": {
             "bytes_per_substep": substep_bytes,
             "achieved": substep_bytes * args.steps / elapsed / 1e9,
             "frac": substep_bytes * args.steps / elapsed / 1e9 / HBM_PEAK_GBS,
-            "frac_of_copy": substep_bytes * args.steps / elapsed / 1e9 / copy_bw["achieved"],
+            "frac_of_copy": substep_bytes * args.steps / elapsed / 1e9 / copy_bw["achieved"] if copy_bw else None,
         },
         "copy_bandwidth": copy_bw,
         "stage_ms_per_launch": {k: v[0] / max(v[1], 1) for k, v in breakdown.items() if v[1]},

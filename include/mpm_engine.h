@@ -245,6 +245,8 @@ typedef struct mpm_tuning {
                                             not with the learned closure) */
     int32_t particles_per_lane;          /* particles each thread of the transfer kernel takes per
                                             pass of a block, 1..3 (default 3: passes of 768) */
+    int32_t grid_update_threads;         /* work-group size of the grid update kernel: 64, 128 or
+                                            256 threads (0: 256); its wave count stays 16 per CU */
     int32_t grid_window;                 /* 1: the block grid is a window fitted to the particles
                                             at the start of every call (grown and the call run
                                             again when a particle leaves it), so that its memory

@@ -177,6 +177,7 @@ class MpmTuning(C.Structure):
         ("persistent_blocks", C.c_int32),
         ("graph", C.c_int32),
         ("particles_per_lane", C.c_int32),
+        ("grid_update_threads", C.c_int32),
         ("grid_window", C.c_int32),
     ]
 

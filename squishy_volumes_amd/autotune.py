@@ -6,7 +6,9 @@ shorter one keeps particles inside their blocks' staged node regions), the
 work-group count of the persistent transfer kernels (all resident work-groups
 by default; fewer leave CUs to the grid update's tail, more over-subscribe
 the CUs and balance uneven blocks), the particles each lane takes per pass of
-a block (fewer make more, smaller passes: less idle tail, more flushes) and
+a block (fewer make more, smaller passes: less idle tail, more flushes), the
+work-group size of the grid update (64, 128 or 256 threads over the same 16
+waves per CU: smaller groups retire unevenly loaded waves independently) and
 whether whole sort cycles run as captured HIP graphs (they measure alike at
 these sizes, so the default search keeps them on).  Each candidate runs the scene on a fresh
 engine for a few warm-up and a few timed substeps; the fastest wins.  Results
@@ -26,6 +28,7 @@ from typing import Optional
 RESORT_INTERVALS = (4, 8, 16)
 PERSISTENT_BLOCKS = (0, 2048)
 PARTICLES_PER_LANE = (1, 2, 3)
+GRID_UPDATE_THREADS = (64, 256)
 GRAPH = (0,)  # 0: sort cycles as HIP graphs (the engine's default), -1: every kernel launched
 
 
@@ -66,11 +69,12 @@ def time_tuning(scene, tuning: dict, warmup: int, substeps: int, device: int = 0
 
 
 def autotune(scene, *, resort_intervals=RESORT_INTERVALS, persistent_blocks=PERSISTENT_BLOCKS,
-             particles_per_lane=PARTICLES_PER_LANE, graph=GRAPH, warmup: int = 8, substeps: int = 48,
+             particles_per_lane=PARTICLES_PER_LANE, grid_update_threads=GRID_UPDATE_THREADS, graph=GRAPH,
+             warmup: int = 8, substeps: int = 48,
              cache: Optional[Path] = None, device: int = 0, log=None) -> TuneResult:
     """The fastest (resort_interval, persistent_blocks, particles_per_lane,
-    graph) for `scene`: a grid search over the candidates, timing each on the
-    scene itself."""
+    grid_update_threads, graph) for `scene`: a grid search over the
+    candidates, timing each on the scene itself."""
     device_name = ""
     try:
         import torch
@@ -85,8 +89,10 @@ def autotune(scene, *, resort_intervals=RESORT_INTERVALS, persistent_blocks=PERS
             e = table[key]
             return TuneResult(e["best"], e["ms_per_substep"], e.get("trials", []), cached=True)
     trials = []
-    for ri, pb, pl, gr in itertools.product(resort_intervals, persistent_blocks, particles_per_lane, graph):
-        tuning = {"resort_interval": ri, "persistent_blocks": pb, "particles_per_lane": pl, "graph": gr}
+    for ri, pb, pl, gu, gr in itertools.product(resort_intervals, persistent_blocks, particles_per_lane,
+                                                 grid_update_threads, graph):
+        tuning = {"resort_interval": ri, "persistent_blocks": pb, "particles_per_lane": pl,
+                  "grid_update_threads": gu, "graph": gr}
         ms = time_tuning(scene, tuning, warmup, substeps, device)
         trials.append((tuning, ms))
         if log:

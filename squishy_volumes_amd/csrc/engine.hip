@@ -1380,6 +1380,9 @@ int mpm_create(const mpm_consts *consts, const mpm_particles_view *pv, const mpm
         if (tuning->graph < 0) e->use_graph = false;
         if (tuning->particles_per_lane < 0 || tuning->particles_per_lane > 3) return fail(MPM_ERR_INVALID_ARGUMENT);
         if (tuning->particles_per_lane > 0) e->launch.particles_per_lane = tuning->particles_per_lane;
+        const int gut = tuning->grid_update_threads;
+        if (gut != 0 && gut != 64 && gut != 128 && gut != 256) return fail(MPM_ERR_INVALID_ARGUMENT);
+        if (gut) e->launch.grid_update_threads = gut;
         if (tuning->grid_window < -1 || tuning->grid_window > 1) return fail(MPM_ERR_INVALID_ARGUMENT);
     }
     // Kernels are specialised on the constitutive models present: snow and

@@ -267,6 +267,7 @@ struct TransferLaunch {
     bool batched;           // batched scenes (mpm_consts.scene_repeat): the M_BATCH kernels
     int cu_count;
     int particles_per_lane; // particles per thread per pass of a block, 1..3 (pass = 256 x this)
+    int grid_update_threads = 256;  // work-group size of the grid update: 64, 128 or 256
     bool colliders = false; // a collider mesh is set: the M_COLL kernels
 };
 

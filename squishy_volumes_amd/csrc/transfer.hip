@@ -1450,14 +1450,18 @@ hipError_t launch_transfer(const TransferLaunch &L, const TransferCall &t, hipSt
     return hipErrorInvalidValue;
 }
 
-// The grid update runs on a fixed grid so that its per-wave partial counts
-// have a fixed size.
-uint32_t grid_update_groups(const TransferLaunch &L) { return (uint32_t)std::max(8, 4 * L.cu_count); }
-uint32_t grid_update_waves(const TransferLaunch &L) { return grid_update_groups(L) * 4u; }
+// The grid update runs on a fixed number of waves (16 per CU), so that its
+// per-wave partial counts have a fixed size, in work-groups of
+// L.grid_update_threads (a tuning knob: 1, 2 or 4 waves).
+uint32_t grid_update_waves(const TransferLaunch &L) { return (uint32_t)std::max(32, 16 * L.cu_count); }
+uint32_t grid_update_groups(const TransferLaunch &L) {
+    return grid_update_waves(L) / (uint32_t)(L.grid_update_threads / 64);
+}
 
 hipError_t launch_grid_update(const TransferLaunch &L, const GridBuffers &B, const GridGeom &G,
                               const SubstepParams *params, uint32_t *partial, DeviceScalars *ds, hipStream_t stream) {
-    hipLaunchKernelGGL(k_grid_update, dim3(grid_update_groups(L)), dim3(256), 0, stream, B, G, params, partial, ds);
+    hipLaunchKernelGGL(k_grid_update, dim3(grid_update_groups(L)), dim3(L.grid_update_threads), 0, stream, B, G, params,
+                       partial, ds);
     return hipGetLastError();
 }
 

@@ -74,8 +74,8 @@ def test_autotuner_picks_the_fastest_measured_tuning_and_caches_it(tmp_path):
     b = Batch(uneven([500, 4000, 1728]))
     cache = tmp_path / "tune.json"
     r = autotune.autotune(b.scene, resort_intervals=(4, 8), persistent_blocks=(0, 64), particles_per_lane=(3,),
-                          graph=(0,), warmup=2, substeps=6, cache=cache)
-    assert len(r.trials) == 4 and not r.cached
+                          grid_update_threads=(64, 256), graph=(0,), warmup=2, substeps=6, cache=cache)
+    assert len(r.trials) == 8 and not r.cached
     assert r.ms_per_substep == min(ms for _, ms in r.trials)
     assert r.best in [t for t, _ in r.trials]
     again = autotune.autotune(b.scene, cache=cache)
@@ -83,7 +83,8 @@ def test_autotuner_picks_the_fastest_measured_tuning_and_caches_it(tmp_path):
     # any tuning gives the same physics up to round-off
     T = scenes.advance_time(0.0, b.scene.time_step, 30)
     runs = []
-    for tuning in ({"resort_interval": 4, "persistent_blocks": 64}, {"resort_interval": 16, "persistent_blocks": 2048}):
+    for tuning in ({"resort_interval": 4, "persistent_blocks": 64, "grid_update_threads": 64},
+                   {"resort_interval": 16, "persistent_blocks": 2048, "grid_update_threads": 128}):
         st, err = engine(b.scene, tuning=tuning).produce_next_state(b.frame_input, GpuRunParameters(target_time=T))
         assert err is None
         runs.append(st.particles)
