@@ -352,9 +352,11 @@ struct ClosureArgs {
     uint32_t judge_stride, judge_phase;
     ClosureStats *stats;
     const float *svd;  // U, s by P slot (12 rows of stride), or null
+    const DeviceScalars *ds;  // halt flag, or null
 };
 
 __global__ __launch_bounds__(256) void k_closure(ClosureArgs a) {
+    if (a.ds && a.ds->halt) return;  // a substep past the end of an adaptive call's loop
     __shared__ StagedNet net;
     __shared__ float feat[4][64 * 8];
     __shared__ float outs[4][64 * 4];
@@ -573,6 +575,7 @@ hipError_t launch_closure(const ClosureLaunch &c, hipStream_t stream) {
     a.judge_phase = c.judge_phase;
     a.stats = c.stats;
     a.svd = c.svd;
+    a.ds = c.ds;
     // Each work-group stages the network once and then takes 256 particles
     // at a time: 2048 work-groups (8 per CU) amortize the staging (one
     // work-group per 256 particles measured 10 % slower on 1M particles;

@@ -21,6 +21,7 @@ __device__ __forceinline__ float3 try_normalize3(float3 v, float eps) {
 // factor_a a + factor_b b (f32, as the reference), triangle normals from the
 // interpolated vertices, frictions and dampings.
 __global__ __launch_bounds__(256) void k_mesh_interp(MeshDev m, const SubstepParams *params, DeviceScalars *ds) {
+    if (ds->halt) return;
     const float fb = params[ds->param_index].factor_b;
     const float fa = 1.0f - fb;
     auto pos = [&](uint32_t v) {
@@ -53,7 +54,8 @@ __device__ __forceinline__ float angle3(float3 a, float3 b) {
 
 // Vertex normals: the angle-weighted sum of the normals of the vertex's
 // triangles, normalised (interpolate_input.rs:57-81).
-__global__ __launch_bounds__(256) void k_vertex_normals(MeshDev m) {
+__global__ __launch_bounds__(256) void k_vertex_normals(MeshDev m, const DeviceScalars *ds) {
+    if (ds->halt) return;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < m.nvert; v += gridDim.x * blockDim.x) {
         const float3 p = f3(m.vpos[v]);
         float3 sum = make_float3(0.f, 0.f, 0.f);
@@ -81,6 +83,7 @@ __device__ __forceinline__ bool compatible(uint32_t a, uint32_t b) {
 // options.  The other table, read by the gather that has run, is cleared.
 __global__ __launch_bounds__(256) void k_twin_meld(TwinTables t, const float4 *__restrict__ grid, GridGeom G,
                                                    const SubstepParams *params, DeviceScalars *ds) {
+    if (ds->halt) return;
     const SubstepParams &S = params[ds->grid_param];
     TwinEntry *cur = t.tab[S.epoch & 1u];
     TwinEntry *other = t.tab[(S.epoch + 1u) & 1u];
@@ -149,7 +152,7 @@ uint32_t groups_for(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, std::mi
 hipError_t launch_mesh_interp(const MeshDev &m, const SubstepParams *params, DeviceScalars *ds, hipStream_t stream) {
     if (m.ntri == 0) return hipSuccess;
     hipLaunchKernelGGL(k_mesh_interp, dim3(groups_for((uint64_t)m.nvert + m.ntri)), dim3(256), 0, stream, m, params, ds);
-    hipLaunchKernelGGL(k_vertex_normals, dim3(groups_for(m.nvert)), dim3(256), 0, stream, m);
+    hipLaunchKernelGGL(k_vertex_normals, dim3(groups_for(m.nvert)), dim3(256), 0, stream, m, ds);
     return hipGetLastError();
 }
 

@@ -142,6 +142,8 @@ struct mpm_engine {
     uint32_t judge_stride = 0, judge_phase = 0;
     unsigned long long *pair_count = nullptr;  // mpm_closure_pairs
     float4 *limit_part = nullptr, *limit_out = nullptr;  // mpm_time_step_limits
+    DeviceAdaptive *adapt_dev = nullptr;    // adaptive steps: the device state
+    DeviceAdaptive *adapt_host = nullptr;   // and its pinned host copy
     // mpm_read_particles staging in original order (allocated on first use)
     float *rb_x = nullptr, *rb_v = nullptr, *rb_F = nullptr, *rb_C = nullptr;
     uint32_t *rb_flags = nullptr;
@@ -189,6 +191,7 @@ struct mpm_engine {
         if (param_table) (void)hipFree(param_table);
         if (param_host) (void)hipHostFree(param_host);
         if (ds_host) (void)hipHostFree(ds_host);
+        if (adapt_host) (void)hipHostFree(adapt_host);
         for (auto &r : records) {
             (void)hipEventDestroy(r.a);
             (void)hipEventDestroy(r.b);
@@ -729,6 +732,7 @@ int run_closure(mpm_engine *e) {
     // the factors of the last gather, by the same slots (a sort since then
     // only adds the order that c.order applies)
     c.svd = e->svd_fresh ? e->closure_svd : nullptr;
+    c.ds = e->ds;
     e->begin(ST_CLOSURE, r);
     HIP_TRY(launch_closure(c, e->stream));
     e->end(r);
@@ -1021,19 +1025,9 @@ int upload_mesh_frames(mpm_engine *e, const mpm_frame_input *in) {
     return MPM_OK;
 }
 
-// Entry k of the parameter table rewritten (pinned staging slot k; the
-// caller has synchronized since the slot's previous copy).
-int upload_param_entry(mpm_engine *e, uint32_t k, SubstepParams p) {
-    p.index = k;
-    std::memcpy(e->param_host + k, &p, sizeof(SubstepParams));
-    HIP_TRY(hipMemcpyAsync(e->param_table + k, e->param_host + k, sizeof(SubstepParams), hipMemcpyHostToDevice,
-                           e->stream));
-    return MPM_OK;
-}
-
-// A limit of the device reductions: +inf (no particle, or none finite) is the
-// reference's None, which the adaptive state counts as f32::MAX.
-float as_limit(float v) { return std::isinf(v) || std::isnan(v) ? FLT_MAX : v; }
+// run_adaptive's result for an overflow of the collider twins' table: the
+// call runs again from its start with a larger one.
+constexpr int RERUN_LARGER_TWINS = 1000;
 
 // The substep loop with adaptive time steps, following the phase order of
 // CpuState::run_phase (rust/crates/cpu/src/phase/mod.rs:27-75) with
@@ -1042,117 +1036,142 @@ float as_limit(float v) { return std::isinf(v) || std::isnan(v) ? FLT_MAX : v; }
 // (limit_time_step_before_force) set the step of the scatter, and the
 // velocity and deformation limits of the gathered v and C
 // (limit_time_step_before_integrate) the step of the advance and of the time
-// (limit_time_step.rs:25-223).  The limits are device reductions; the
-// AdaptiveTimeStepState is host code, read after two synchronizations per
-// substep.  Substep t uses parameter entries 0 (its scatter) and 1 (its
-// gather).  A particle that fails stops the call after the substep in which
-// it failed, with the state its gather wrote; `stop` receives the failure's
-// status bits.
+// (limit_time_step.rs:25-223).  The limits are device reductions and the
+// AdaptiveTimeStepState lives on the device with the call's time: three
+// one-thread kernels per substep push the limits and write the steps, the
+// frame factor and the gravity into parameter entries 0 (the scatter) and 1
+// (the gather).  The host launches the substeps a sort cycle at a time
+// without waiting; the substeps past the end of the loop return at once
+// (DeviceScalars::halt), and the host reads the device state between cycles
+// (for the cancel flag and the sort it then launches).  A particle that
+// fails stops the call after the substep in which it failed, with the state
+// its gather wrote; `stop` receives the failure's status bits.
 int run_adaptive(mpm_engine *e, double target_time, const mpm_frame_input *input, bool goals,
                  const volatile int32_t *cancel, LoopReport &rep, uint32_t &stop) {
     int rc;
     stop = 0;
+    if (!(e->time < target_time)) return MPM_OK;
     const size_t K = (size_t)std::max(1, e->resort_interval);
-    const float h = e->consts.grid_node_size;
     const uint32_t fail_bits = MPM_STATUS_ENERGY_ERROR | MPM_STATUS_PARTICLE_CLOSE_TO_INVERTED;
     if (!e->limit_part &&
         (e->alloc(e->limit_part, step_limit_groups(e->n)) != hipSuccess || e->alloc(e->limit_out, 1) != hipSuccess))
         return MPM_ERR_OUT_OF_MEMORY;
-    float4 *lim_host = nullptr;
-    HIP_TRY(hipHostMalloc((void **)&lim_host, sizeof(float4), hipHostMallocDefault));
-    struct PinnedFree {
-        float4 *p;
-        ~PinnedFree() { (void)hipHostFree(p); }
-    } pinned_guard{lim_host};
-    auto sync_read = [&]() -> int {
-        HIP_TRY(hipMemcpyAsync(lim_host, e->limit_out, sizeof(float4), hipMemcpyDeviceToHost, e->stream));
+    if (!e->adapt_dev) {
+        if (e->alloc(e->adapt_dev, 1) != hipSuccess) return MPM_ERR_OUT_OF_MEMORY;
+        HIP_TRY(hipHostMalloc((void **)&e->adapt_host, sizeof(DeviceAdaptive), hipHostMallocDefault));
+    }
+    // The device state from the engine's AdaptiveTimeStepState.
+    DeviceAdaptive &A = *e->adapt_host;
+    std::memset(&A, 0, sizeof(A));
+    A.time = e->time;
+    A.target = target_time;
+    A.fps = (double)e->consts.frames_per_second;
+    A.frame = input ? input->frame : 0;
+    for (int a = 0; a < 3; ++a) {
+        A.gravity_a[a] = input ? input->gravity_a[a] : 0.f;
+        A.gravity_b[a] = input ? (input->has_b ? input->gravity_b[a] : input->gravity_a[a]) : 0.f;
+    }
+    A.h = e->consts.grid_node_size;
+    A.max_time_step = e->ats.max_time_step;
+    A.by_velocity = e->ats.by_velocity;
+    A.by_deformation = e->ats.by_deformation;
+    A.by_sound = e->ats.by_sound;
+    A.by_isolated = e->ats.by_isolated;
+    for (float v : e->ats.prior) A.prior[A.nprior++] = v;
+    const uint32_t epoch0 = e->epoch;
+    const uint32_t grid_epoch0 = e->grid_epoch;
+    const bool grid_valid0 = e->grid_valid;
+    A.epoch0 = epoch0;
+    A.stop = ADAPT_RUNNING;
+    HIP_TRY(hipMemcpyAsync(e->adapt_dev, &A, sizeof(A), hipMemcpyHostToDevice, e->stream));
+    // Template entries: the fields that stay for the call; the device writes the rest.
+    float g0[3] = {0.f, 0.f, 0.f};
+    const SubstepParams tmpl = make_params(e, g0, 0.f, goals, input && input->has_b);
+    std::vector<SubstepParams> two{tmpl, tmpl};
+    if ((rc = upload_params(e, two)) != MPM_OK) return rc;
+    HIP_TRY(hipMemsetAsync(&e->ds->halt, 0, 4, e->stream));
+    // The host's view of the device state after a synchronization.
+    auto read_state = [&]() -> int {
+        HIP_TRY(hipMemcpyAsync(e->adapt_host, e->adapt_dev, sizeof(DeviceAdaptive), hipMemcpyDeviceToHost, e->stream));
         HIP_TRY(hipMemcpyAsync(e->ds_host, e->ds, sizeof(DeviceScalars), hipMemcpyDeviceToHost, e->stream));
         HIP_TRY(hipStreamSynchronize(e->stream));
         return MPM_OK;
     };
-    bool force_sort = false;
-    float last_advance = 0.f;
-    for (size_t t = 0; e->time < target_time;) {
-        if (rep.done >= MPM_MAX_SUBSTEPS_PER_CALL) return MPM_ERR_INVALID_ARGUMENT;
-        if ((cancel && *cancel) || (e->cancel_after >= 0 && t >= (uint64_t)e->cancel_after)) {
+    // One substep's launches; want_keys: its gather writes the keys of the next sort.
+    auto substep = [&](bool want_keys) -> int {
+        int r;
+        HIP_TRY(launch_adapt_begin(e->adapt_dev, e->param_table, e->ds, e->stream));
+        HIP_TRY(launch_step_limits(e->P[e->cur], e->n, e->consts.grid_node_size, e->limit_part, e->limit_out, e->ds,
+                                   e->stream));
+        HIP_TRY(launch_adapt_force(e->adapt_dev, e->param_table, e->ds, e->limit_out, e->stream));
+        if (e->closure_on && (r = run_closure(e)) != MPM_OK) return r;
+        if ((r = run_transfer(e, TransferKind::P2G, goals, false)) != MPM_OK) return r;
+        if ((r = run_grid(e)) != MPM_OK) return r;
+        if ((r = run_transfer(e, TransferKind::LIMITS, false, false)) != MPM_OK) return r;
+        HIP_TRY(launch_adapt_integrate(e->adapt_dev, e->param_table, e->ds, e->stream));
+        return run_transfer(e, TransferKind::G2P, false, want_keys);
+    };
+    size_t t = 0;
+    uint32_t done_before = 0;
+    int cur_before = e->cur;
+    bool sorted = false;
+    for (;;) {
+        if (t > 0) {
+            if ((rc = read_state()) != MPM_OK) return rc;
+            // The first substep of a cycle runs unless the call ends with an
+            // error; then its permuting transfer did not run either.
+            if (sorted && A.substeps == done_before && A.stop != ADAPT_RUNNING) {
+                e->cur = cur_before;
+                e->pending_permute = false;
+            }
+            if (A.stop != ADAPT_RUNNING || (e->ds_host->status & fail_bits)) break;
+        }
+        if (A.substeps >= MPM_MAX_SUBSTEPS_PER_CALL) return MPM_ERR_INVALID_ARGUMENT;
+        if ((cancel && *cancel) || (e->cancel_after >= 0 && (int64_t)A.substeps >= e->cancel_after)) {
             rep.canceled = true;
             break;
         }
-        if (e->ats.allowed() == 0.f) return MPM_ERR_ZERO_TIME_STEP;
-        const AdaptiveStep saved = e->ats;
-        const uint32_t epoch0 = e->epoch;
-        const float dt_force = e->ats.allowed();
-        // limit_time_step_before_force, over the state the last gather wrote
-        HIP_TRY(launch_step_limits(e->P[e->cur], e->n, h, e->limit_part, e->limit_out, e->stream));
-        if ((rc = sync_read()) != MPM_OK) return rc;
-        if (e->ds_host->status & fail_bits) {
-            // the last gather failed a particle: its substep is the last
-            stop = e->ds_host->status;
-            if ((stop & MPM_STATUS_ENERGY_ERROR) && rep.done > 0) e->time -= (double)last_advance;
-            break;
-        }
-        e->ats.by_sound = as_limit(lim_host->x);
-        e->ats.by_isolated = as_limit(lim_host->y);
-        e->ats.push_current_limit();
-        if (e->ats.allowed() == 0.f) return MPM_ERR_ZERO_TIME_STEP;
-        const float dt_scatter = e->ats.allowed();
-        float g[3], fb;
-        if ((rc = frame_params(e, input, e->time, g, fb)) != MPM_OK) return rc;
-        SubstepParams s0 = make_params(e, g, fb, goals, input && input->has_b);
-        s0.dt = dt_scatter;
-        s0.dt_force = dt_force;
-        std::vector<SubstepParams> two{s0, s0};
-        two[1].dt_force = dt_scatter;  // rewritten with the step of the advance below
-        if ((rc = upload_params(e, two)) != MPM_OK) return rc;
-        if ((t % K == 0 || force_sort) && (rc = run_sort(e)) != MPM_OK) return rc;
-        force_sort = false;
-        if (e->closure_on && (rc = run_closure(e)) != MPM_OK) return rc;
-        if ((rc = run_transfer(e, TransferKind::P2G, goals, false)) != MPM_OK) return rc;
-        if ((rc = run_grid(e)) != MPM_OK) return rc;
-        // limit_time_step_before_integrate, over the gathered v and C
-        HIP_TRY(hipMemsetAsync(e->ds->limit_bits, 0, sizeof(e->ds->limit_bits), e->stream));
-        if ((rc = run_transfer(e, TransferKind::LIMITS, false, false)) != MPM_OK) return rc;
-        if ((rc = sync_read()) != MPM_OK) return rc;
-        if (e->launch.colliders && (e->ds_host->status & MPM_STATUS_TABLE_TRIES_EXCEEDED)) {
-            // the substep's twin table overflowed: again with a larger table
-            if (e->twin_capacity >= (1u << 28)) return MPM_ERR_CAPACITY;
-            if ((rc = set_twin_capacity(e, e->twin_capacity * 4)) != MPM_OK) return rc;
-            HIP_TRY(hipMemsetAsync(&e->ds->status, 0, 4, e->stream));
-            e->ats = saved;
-            e->epoch = epoch0;
-            force_sort = true;
-            e->keys_valid = false;
-            continue;
-        }
-        if (e->ds_host->status & fail_bits) {
-            // the stored state failed before the scatter: nothing advanced
-            stop = e->ds_host->status;
-            break;
-        }
-        float vmax, cmax;
-        std::memcpy(&vmax, &e->ds_host->limit_bits[0], 4);
-        std::memcpy(&cmax, &e->ds_host->limit_bits[1], 4);
-        const bool any = e->ds_host->limit_bits[2] != 0;
-        e->ats.by_velocity = any && vmax != 0.f ? 0.5f * h / vmax : FLT_MAX;
-        e->ats.by_deformation = any ? 0.2f / std::max(cmax, 1e-8f) : FLT_MAX;
-        e->ats.push_current_limit();
-        if (e->ats.allowed() == 0.f) return MPM_ERR_ZERO_TIME_STEP;
-        const float dt_advance = e->ats.allowed();
-        SubstepParams s1 = two[1];
-        s1.epoch = epoch0 + 1;
-        s1.dt_force = dt_advance;
-        if ((rc = upload_param_entry(e, 1, s1)) != MPM_OK) return rc;
-        if ((rc = run_transfer(e, TransferKind::G2P, false, (t + 1) % K == 0)) != MPM_OK) return rc;
-        e->time += (double)dt_advance;  // cpu_state.rs:187-190
-        last_advance = dt_advance;
-        ++rep.done;
-        ++t;
+        // A cycle of substeps up to the next sort (or to the test hook's cancel).
+        size_t chunk = K - (t % K);
+        if (e->cancel_after >= 0) chunk = std::min<size_t>(chunk, (size_t)(e->cancel_after - (int64_t)A.substeps));
+        cur_before = e->cur;
+        done_before = A.substeps;
+        sorted = t % K == 0;
+        if (sorted && (rc = run_sort(e)) != MPM_OK) return rc;
+        for (size_t u = 0; u < chunk; ++u)
+            if ((rc = substep(u + 1 == chunk || (t + u + 1) % K == 0)) != MPM_OK) return rc;
+        t += chunk;
     }
-    if (!stop) {
-        if ((rc = sync_read()) != MPM_OK) return rc;
-        if (e->ds_host->status & fail_bits) {
-            stop = e->ds_host->status;
-            if ((stop & MPM_STATUS_ENERGY_ERROR) && rep.done > 0) e->time -= (double)last_advance;
+    if ((rc = read_state()) != MPM_OK) return rc;
+    HIP_TRY(hipMemsetAsync(&e->ds->halt, 0, 4, e->stream));
+    // The engine's state from the device's.
+    e->time = A.time;
+    e->epoch = epoch0 + A.substeps;
+    e->keys_valid = false;
+    e->pending_permute = false;
+    e->ats.by_velocity = A.by_velocity;
+    e->ats.by_deformation = A.by_deformation;
+    e->ats.by_sound = A.by_sound;
+    e->ats.by_isolated = A.by_isolated;
+    e->ats.prior.clear();
+    for (uint32_t k = 0; k < A.nprior; ++k) e->ats.prior.push_back(A.prior[(A.head + k) % 11u]);
+    rep.done = A.substeps;
+    if (A.substeps > 0) {
+        e->grid_valid = true;
+        e->grid_epoch = epoch0 + A.substeps - 1u;
+    } else {
+        e->grid_valid = grid_valid0;
+        e->grid_epoch = grid_epoch0;
+    }
+    if (A.stop == ADAPT_ZERO_STEP) return MPM_ERR_ZERO_TIME_STEP;
+    if (A.stop == ADAPT_WRONG_FRAME) return MPM_ERR_WRONG_FRAME;
+    if (A.stop == ADAPT_TWIN_OVERFLOW) return RERUN_LARGER_TWINS;
+    if (e->ds_host->status & fail_bits) {
+        stop = e->ds_host->status;
+        // a failure in the call's last gather (the device has not seen it)
+        if (A.stop != ADAPT_FAILED && (stop & MPM_STATUS_ENERGY_ERROR) && A.substeps > 0) {
+            e->time -= (double)A.last_advance;
+            e->epoch = epoch0 + A.substeps;
         }
     }
     return MPM_OK;
@@ -1498,12 +1517,25 @@ int step_call(mpm_engine *e, double target_time, const mpm_frame_input *input, c
             float g[3], fb;
             if (e->time < target_time && (rc = frame_params(e, input, e->time, g, fb)) != MPM_OK) return rc;
         }
-        e->start = e->cur;
+        const CallStart cs = begin_call(e);
+        const AdaptiveStep ats0 = e->ats;
         HIP_TRY(hipMemsetAsync(&e->ds->status, 0, 4, e->stream));
         HIP_TRY(hipMemsetAsync(&e->ds->fail_index, 0xff, 4, e->stream));
+        if (e->jp_start && e->n > 0)
+            HIP_TRY(hipMemcpyAsync(e->jp_start, e->X.jp, e->n * 4, hipMemcpyDeviceToDevice, e->stream));
         LoopReport rep;
         uint32_t stop = 0;
         rc = run_adaptive(e, target_time, input, goals, cancel, rep, stop);
+        // The table of collider twins overflowed: the call runs again from its
+        // start with a larger table (gpu_state.rs:562-599).
+        for (int redo = 0; rc == RERUN_LARGER_TWINS; ++redo) {
+            if (redo >= 6 || e->twin_capacity >= (1u << 28)) return MPM_ERR_CAPACITY;
+            if ((rc = set_twin_capacity(e, e->twin_capacity * 4)) != MPM_OK) return rc;
+            if ((rc = restore_call_start(e, cs)) != MPM_OK) return rc;
+            e->ats = ats0;
+            rep = LoopReport{};
+            rc = run_adaptive(e, target_time, input, goals, cancel, rep, stop);
+        }
         if (rc != MPM_OK) return rc;
         if (rep.done > 0) e->stepped = true;
         if ((rc = fetch_scalars(e)) != MPM_OK) return rc;
@@ -1909,7 +1941,8 @@ int mpm_read_time_step_limits(mpm_engine *e, mpm_time_step_limits *out) {
     if (!e->limit_part &&
         (e->alloc(e->limit_part, step_limit_groups(e->n)) != hipSuccess || e->alloc(e->limit_out, 1) != hipSuccess))
         return MPM_ERR_OUT_OF_MEMORY;
-    HIP_TRY(launch_step_limits(e->P[e->cur], e->n, e->consts.grid_node_size, e->limit_part, e->limit_out, e->stream));
+    HIP_TRY(launch_step_limits(e->P[e->cur], e->n, e->consts.grid_node_size, e->limit_part, e->limit_out, e->ds,
+                               e->stream));
     float4 r;
     HIP_TRY(hipMemcpyAsync(&r, e->limit_out, sizeof(r), hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));

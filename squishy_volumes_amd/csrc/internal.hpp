@@ -131,6 +131,9 @@ struct DeviceScalars {
                             // flagged a particle FAILED in the call (all ones: none)
     uint32_t limit_bits[3]; // the LIMITS transfer's largest |v| and |C_ij| (float bits), and 1 when
                             // it saw a live particle
+    uint32_t halt;          // adaptive steps on the device: nonzero once the call's loop has
+                            // ended, so that the kernels of the substeps launched past that
+                            // point return at once (zero otherwise)
 };
 
 // A particle whose deformation gradient has det F below this before its
@@ -393,14 +396,43 @@ struct ClosureLaunch {
     const float *svd = nullptr;  // U (rows 0-8) and s (rows 9-11) of each snow particle's F by
                                  // P slot (stride rows), written by the gather; null: the
                                  // closure takes the SVD itself
+    const DeviceScalars *ds = nullptr;  // its halt flag (adaptive steps on the device), or null
 };
 hipError_t launch_closure(const ClosureLaunch &c, hipStream_t stream);
 uint32_t step_limit_groups(uint64_t n);
+// AdaptiveTimeStepState (rust/crates/cpu/src/adaptive_time_step_state.rs:13-70)
+// kept on the device for a call with adaptive steps, with the call's time:
+// three one-thread kernels per substep update it from the limit reductions
+// and write the steps into the two parameter-table entries of the substep
+// (0: scatter and grid update, 1: gather and advance).
+enum : uint32_t { ADAPT_RUNNING = 0, ADAPT_DONE = 1, ADAPT_FAILED = 2, ADAPT_ZERO_STEP = 3, ADAPT_WRONG_FRAME = 4,
+                  ADAPT_TWIN_OVERFLOW = 5 };
+struct DeviceAdaptive {
+    double time, target;      // the call's time (advanced on the device) and target
+    double fps;               // frames per second
+    uint64_t frame;           // frame a of the call's input
+    float gravity_a[3], gravity_b[3];
+    float h;
+    float max_time_step, by_velocity, by_deformation, by_sound, by_isolated;
+    float prior[11];          // the last pushes, oldest at head
+    uint32_t nprior, head;
+    uint32_t epoch0;          // the grid epoch at the call's start
+    uint32_t substeps;        // substeps advanced by the call
+    uint32_t stop;            // ADAPT_*: why the loop ended (RUNNING while it runs)
+    float last_advance;       // the step of the last advance
+    float dt_force;           // the allowed step at the start of the current substep
+};
+hipError_t launch_adapt_begin(DeviceAdaptive *A, SubstepParams *table, DeviceScalars *ds, hipStream_t stream);
+hipError_t launch_adapt_force(DeviceAdaptive *A, SubstepParams *table, DeviceScalars *ds, const float4 *limits,
+                              hipStream_t stream);
+hipError_t launch_adapt_integrate(DeviceAdaptive *A, SubstepParams *table, DeviceScalars *ds, hipStream_t stream);
+
 // The live particles' bounding box and largest speed into out[0..7]
 // ({lo xyz, hi xyz, max |v|, 1 if any live particle}); part holds
 // 2 step_limit_groups(n) float4.
 hipError_t launch_particle_bounds(const ParticleArrays &P, uint64_t n, float4 *part, float *out, hipStream_t stream);
 hipError_t launch_step_limits(const ParticleArrays &P, uint64_t n, float h, float4 *part, float4 *out,
+                              const DeviceScalars *ds,
                               hipStream_t stream);
 hipError_t launch_closure_pairs(const ParticleArrays &P, const AuxArrays &X, uint64_t n, float hardening,
                                 uint32_t stride, uint32_t phase, float *x, float *t, uint64_t capacity,

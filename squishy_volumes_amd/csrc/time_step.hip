@@ -10,6 +10,8 @@
 // reference does for its two parameter variants (the engine's fixed-corotated
 // and snow solids are Solid there too).  A limit with no particle is +inf
 // (the reference's None).
+#include <cfloat>
+
 #include "device_math.hpp"
 #include "internal.hpp"
 
@@ -100,7 +102,9 @@ __device__ float isolated_limit(bool fluid, float p0, float p1, float m, float v
 }
 
 // Per work-group: {min by_sound, min by_isolated, max |v|, min by_deformation}.
-__global__ __launch_bounds__(256) void k_step_limits(ParticleArrays P, uint64_t n, float h, float4 *__restrict__ part) {
+__global__ __launch_bounds__(256) void k_step_limits(ParticleArrays P, uint64_t n, float h, float4 *__restrict__ part,
+                                                    const DeviceScalars *ds) {
+    if (ds && ds->halt) return;
     float snd = INFINITY, iso = INFINITY, vmax = 0.f, def = INFINITY;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t flags = P.flags(i);
@@ -140,7 +144,8 @@ __global__ __launch_bounds__(256) void k_step_limits(ParticleArrays P, uint64_t 
 }
 
 __global__ __launch_bounds__(256) void k_step_limits_final(const float4 *__restrict__ part, uint32_t groups,
-                                                           float4 *__restrict__ out) {
+                                                           float4 *__restrict__ out, const DeviceScalars *ds) {
+    if (ds && ds->halt) return;
     float snd = INFINITY, iso = INFINITY, vmax = 0.f, def = INFINITY;
     for (uint32_t i = threadIdx.x; i < groups; i += blockDim.x) {
         const float4 r = part[i];
@@ -260,10 +265,158 @@ hipError_t launch_particle_bounds(const ParticleArrays &P, uint64_t n, float4 *p
 uint32_t step_limit_groups(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 2048)); }
 
 hipError_t launch_step_limits(const ParticleArrays &P, uint64_t n, float h, float4 *part, float4 *out,
-                              hipStream_t stream) {
+                              const DeviceScalars *ds, hipStream_t stream) {
     const uint32_t groups = step_limit_groups(n);
-    hipLaunchKernelGGL(k_step_limits, dim3(groups), dim3(256), 0, stream, P, n, h, part);
-    hipLaunchKernelGGL(k_step_limits_final, dim3(1), dim3(256), 0, stream, part, groups, out);
+    hipLaunchKernelGGL(k_step_limits, dim3(groups), dim3(256), 0, stream, P, n, h, part, ds);
+    hipLaunchKernelGGL(k_step_limits_final, dim3(1), dim3(256), 0, stream, part, groups, out, ds);
+    return hipGetLastError();
+}
+
+namespace {
+
+// ---- AdaptiveTimeStepState on the device (adaptive_time_step_state.rs:13-70)
+
+constexpr uint32_t FAIL_BITS = MPM_STATUS_ENERGY_ERROR | MPM_STATUS_PARTICLE_CLOSE_TO_INVERTED;
+
+// A limit of the reductions: +inf (no particle) or NaN is the reference's
+// None, which the state counts as f32::MAX.
+__device__ __forceinline__ float as_limit(float v) { return isinf(v) || isnan(v) ? FLT_MAX : v; }
+
+__device__ __forceinline__ float without_prior(const DeviceAdaptive &A) {
+    return fminf(fminf(fminf(A.max_time_step, A.by_velocity), fminf(A.by_deformation, A.by_sound)), A.by_isolated);
+}
+__device__ __forceinline__ float allowed(const DeviceAdaptive &A) {
+    float a = without_prior(A);
+    for (uint32_t k = 0; k < A.nprior; ++k) a = fminf(a, A.prior[k]);
+    return a;
+}
+// push_current_limit: the history keeps the last 11 pushes (pop_front past 10).
+__device__ __forceinline__ void push(DeviceAdaptive &A) {
+    const float v = without_prior(A);
+    if (A.nprior < 11u) {
+        A.prior[(A.head + A.nprior) % 11u] = v;
+        ++A.nprior;
+    } else {
+        A.prior[A.head] = v;
+        A.head = (A.head + 1u) % 11u;
+    }
+}
+__device__ __forceinline__ void halt(DeviceAdaptive &A, DeviceScalars *ds, uint32_t why) {
+    A.stop = why;
+    ds->halt = 1u;
+}
+
+// The start of a substep: the call ends when the last gather failed a
+// particle (an energy error leaves the time at the start of the failing
+// substep, cpu_state.rs:178-190), when the target time is reached, or with
+// a zero step or a time outside frame a; else entry 0 gets the frame
+// factor, the interpolated gravity (interpolate_input.rs:24-28, in the same
+// f64 -> f32 order as the host) and the step of the external force.
+__global__ void k_adapt_begin(DeviceAdaptive *__restrict__ Ap, SubstepParams *__restrict__ table,
+                              DeviceScalars *__restrict__ ds) {
+    DeviceAdaptive &A = *Ap;
+    if (ds->halt) return;
+    if (ds->status & FAIL_BITS) {
+        if ((ds->status & MPM_STATUS_ENERGY_ERROR) && A.substeps > 0) A.time -= (double)A.last_advance;
+        halt(A, ds, ADAPT_FAILED);
+        return;
+    }
+    if (!(A.time < A.target)) {
+        halt(A, ds, ADAPT_DONE);
+        return;
+    }
+    const float dt = allowed(A);
+    if (dt == 0.f) {
+        halt(A, ds, ADAPT_ZERO_STEP);
+        return;
+    }
+    const double ft = A.time * A.fps;
+    if ((uint64_t)floor(ft) != A.frame) {
+        halt(A, ds, ADAPT_WRONG_FRAME);
+        return;
+    }
+    SubstepParams &P = table[0];
+    P.factor_b = (float)fmod(ft, 1.0);
+    const float fa = 1.0f - P.factor_b;
+    for (int a = 0; a < 3; ++a) P.gravity[a] = fa * A.gravity_a[a] + P.factor_b * A.gravity_b[a];
+    P.dt_force = dt;
+    P.epoch = A.epoch0 + A.substeps;
+    P.index = 0u;
+    A.dt_force = dt;
+    ds->param_index = 0u;
+    ds->grid_param = 0u;
+}
+
+// limit_time_step_before_force (limit_time_step.rs:25-34): the sound-speed
+// and isolated-particle limits of the state, pushed; the allowed step is the
+// step of the scatter and the grid update (entry 0) and, until the second
+// push, of the advance (entry 1, a copy of entry 0 otherwise).
+__global__ void k_adapt_force(DeviceAdaptive *__restrict__ Ap, SubstepParams *__restrict__ table,
+                              DeviceScalars *__restrict__ ds, const float4 *__restrict__ limits) {
+    DeviceAdaptive &A = *Ap;
+    if (ds->halt) return;
+    A.by_sound = as_limit(limits->x);
+    A.by_isolated = as_limit(limits->y);
+    push(A);
+    const float dt = allowed(A);
+    if (dt == 0.f) {
+        halt(A, ds, ADAPT_ZERO_STEP);
+        return;
+    }
+    table[0].dt = dt;
+    table[1] = table[0];
+    table[1].index = 1u;
+    table[1].epoch = table[0].epoch + 1u;
+    for (int k = 0; k < 3; ++k) ds->limit_bits[k] = 0u;
+}
+
+// limit_time_step_before_integrate (limit_time_step.rs:187-222): the
+// velocity and deformation limits of the gathered v and C, pushed; the
+// allowed step is the step of the advance and of the time (cpu_state.rs:
+// 187-190).  A particle that failed in the scatter (a stored state too close
+// to inversion) ends the call before the advance; so does an overflow of the
+// collider twins, after which the call runs again with a larger table.
+__global__ void k_adapt_integrate(DeviceAdaptive *__restrict__ Ap, SubstepParams *__restrict__ table,
+                                  DeviceScalars *__restrict__ ds) {
+    DeviceAdaptive &A = *Ap;
+    if (ds->halt) return;
+    if (ds->status & FAIL_BITS) {
+        halt(A, ds, ADAPT_FAILED);
+        return;
+    }
+    if (ds->status & MPM_STATUS_TABLE_TRIES_EXCEEDED) {
+        halt(A, ds, ADAPT_TWIN_OVERFLOW);
+        return;
+    }
+    const float vmax = __uint_as_float(ds->limit_bits[0]), cmax = __uint_as_float(ds->limit_bits[1]);
+    const bool any = ds->limit_bits[2] != 0u;
+    A.by_velocity = any && vmax != 0.f ? as_limit(0.5f * A.h / vmax) : FLT_MAX;
+    A.by_deformation = any ? as_limit(0.2f / fmaxf(cmax, 1e-8f)) : FLT_MAX;
+    push(A);
+    const float dt = allowed(A);
+    if (dt == 0.f) {
+        halt(A, ds, ADAPT_ZERO_STEP);
+        return;
+    }
+    table[1].dt_force = dt;
+    A.time += (double)dt;
+    A.last_advance = dt;
+    A.substeps += 1u;
+}
+
+}  // namespace
+
+hipError_t launch_adapt_begin(DeviceAdaptive *A, SubstepParams *table, DeviceScalars *ds, hipStream_t stream) {
+    hipLaunchKernelGGL(k_adapt_begin, dim3(1), dim3(1), 0, stream, A, table, ds);
+    return hipGetLastError();
+}
+hipError_t launch_adapt_force(DeviceAdaptive *A, SubstepParams *table, DeviceScalars *ds, const float4 *limits,
+                              hipStream_t stream) {
+    hipLaunchKernelGGL(k_adapt_force, dim3(1), dim3(1), 0, stream, A, table, ds, limits);
+    return hipGetLastError();
+}
+hipError_t launch_adapt_integrate(DeviceAdaptive *A, SubstepParams *table, DeviceScalars *ds, hipStream_t stream) {
+    hipLaunchKernelGGL(k_adapt_integrate, dim3(1), dim3(1), 0, stream, A, table, ds);
     return hipGetLastError();
 }
 

@@ -534,6 +534,8 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
     __shared__ uint32_t nx;
     __shared__ uint32_t blk_touch;  // blocks reached by the block's particles (touched_blocks)
 
+    // a substep launched past the end of an adaptive call's loop
+    if (a.ds->halt) return;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
     const uint32_t wave = tid >> 6;
@@ -1177,6 +1179,7 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
 // and the accumulation grid is cleared for the next scatter.
 __global__ __launch_bounds__(256) void k_grid_update(GridBuffers B, GridGeom G, const SubstepParams *params,
                                                      uint32_t *__restrict__ partial, DeviceScalars *ds) {
+    if (ds->halt) return;
     const SubstepParams S = load_invariant(params + ds->grid_param);
     const uint32_t *ovf = B.ovf[S.epoch & 1u];
     // The next transfer reads the next substep's entry.
@@ -1258,6 +1261,7 @@ __global__ __launch_bounds__(256) void k_sum_partials(const uint32_t *__restrict
 __global__ __launch_bounds__(256) void k_copy_dead(ParticleArrays Pin, ParticleArrays Pout,
                                                    const uint32_t *__restrict__ order, uint64_t n,
                                                    uint32_t *__restrict__ keys_out, uint32_t tomb, DeviceScalars *ds) {
+    if (ds->halt) return;
     const uint64_t live = ds->live_count;
     const bool copy = Pin.base != Pout.base;
     for (uint64_t i = live + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;

@@ -192,3 +192,28 @@ def test_adaptive_arguments():
             g.set_adaptive_time_steps(True, bad)
     g.set_adaptive_time_steps(True, float("inf"))
     g.set_adaptive_time_steps(False)
+
+
+def test_a_call_past_its_frame_stops_where_the_oracle_stops():
+    """The thrown-up cube at 24 frames per second with frame 0's input and a
+    target past the frame's end: the substep that would start in frame 1
+    fails with MPM_ERR_WRONG_FRAME (FrameInput::frame_factor,
+    rust/crates/xpu/src/frame_input.rs:266-278), after the same substeps and
+    at the same time as the oracle."""
+    sc = scene("thrown_up")
+    from squishy_volumes_amd.io_state import InputConsts
+
+    c = sc.consts
+    consts24 = InputConsts(grid_node_size=c.grid_node_size, frames_per_second=24, domain_min=c.domain_min,
+                           domain_max=c.domain_max)
+    g = GpuState.from_io_state(0, consts24, sc.time_step, sc.io_state, sc.options)
+    with pytest.raises(GpuError) as e:
+        g.produce_next_state(sc.frame_input, params(0.08, 0.01))
+    assert e.value.code == abi.ERR_WRONG_FRAME
+    o = pyoracle.Oracle(consts24, sc.time_step, sc.io_state, sc.options, 8)
+    o.set_adaptive(True, 0.01)
+    s = o.step_to(0.08, sc.frame_input)
+    assert s.result == abi.ERR_WRONG_FRAME
+    print(f"stopped at t = {g.time:.6g} (oracle {o.time:.6g}) after frame 0 ends at {1 / 24:.6g}")
+    assert g.time == pytest.approx(o.time, rel=1e-5) and g.time >= 1 / 24 > g.time - 0.01
+    o.close()
