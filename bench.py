@@ -186,8 +186,11 @@ def learned_form(args) -> dict:
     model = net.model(data.x)
     log(f"closure: trained {args.epochs} epochs in {train_s:.1f} s, mse {tlog.epochs[0]:.3e} -> {tlog.epochs[-1]:.3e}")
 
-    def timed(learned: bool):
-        g = GpuState.from_io_state(0, sc.consts, sc.time_step, sc.io_state, sc.options, tuning)
+    def timed(learned: bool, fused: bool = False):
+        tun = dict(tuning)
+        if learned and fused:
+            tun["learned_fused"] = 1  # the network inside the fused gather-scatter
+        g = GpuState.from_io_state(0, sc.consts, sc.time_step, sc.io_state, sc.options, tun)
         if learned:
             g.set_closure(model, judge_stride=64)
         adv(g, args.closure_start)           # up to the impact
@@ -213,6 +216,7 @@ def learned_form(args) -> dict:
     log(f"closure: MLP kernels {mlp_rates}")
     el_plain, _, parts_plain, stages_plain = timed(False)
     el_learned, stats, parts_learned, stages_learned = timed(True)
+    el_fused, _, _, stages_fused = timed(True, fused=True)
 
     # Offline: the network against the analytic stress in FP64 on the final
     # learned state (the judge above compares with the FP32 analytic stress).
@@ -226,6 +230,7 @@ def learned_form(args) -> dict:
     w = args.closure_window
     return {
         "metric": "particle-substeps/s (learned closure vs analytic stress)",
+        "learned_path": "the network in its own pass between the gather and a scatter from the state",
         "value": n * w / el_learned,
         "unit": "particle-substeps/s",
         "n_gpus": 1,
@@ -240,6 +245,8 @@ def learned_form(args) -> dict:
                   "stage_ms_per_substep": stages_plain},
         "learned_stage_ms_per_substep": stages_learned,
         "learned_vs_plain": el_plain / el_learned,
+        "learned_in_fused_kernel": {"value": n * w / el_fused, "ms_per_step": el_fused / w * 1e3,
+                                    "vs_plain": el_plain / el_fused, "stage_ms_per_substep": stages_fused},
         "stress_error_vs_analytic_f32": stats["relative_stress_error"],
         "stress_error_vs_analytic_f64": err64,
         "network_evaluations": stats["evaluated"],

@@ -247,6 +247,10 @@ typedef struct mpm_tuning {
                                             pass of a block, 1..3 (default 3: passes of 768) */
     int32_t grid_update_threads;         /* work-group size of the grid update kernel: 64, 128 or
                                             256 threads (0: 256); its wave count stays 16 per CU */
+    int32_t learned_fused;               /* 1: the learned closure evaluated after each advance
+                                            inside the fused gather-scatter (not with colliders);
+                                            otherwise (default) in its own pass between the gather
+                                            and a scatter from the state, which measures faster */
     int32_t grid_window;                 /* 1: the block grid is a window fitted to the particles
                                             at the start of every call (grown and the call run
                                             again when a particle leaves it), so that its memory

@@ -178,6 +178,7 @@ class MpmTuning(C.Structure):
         ("graph", C.c_int32),
         ("particles_per_lane", C.c_int32),
         ("grid_update_threads", C.c_int32),
+        ("learned_fused", C.c_int32),
         ("grid_window", C.c_int32),
     ]
 

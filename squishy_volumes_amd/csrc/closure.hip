@@ -20,41 +20,11 @@
 // inputs, hidden activations and back-propagated errors are rounded to BF16
 // (nearest even) where they enter a product; biases, accumulations and the
 // outputs are FP32.
+#include "closure_net.hpp"
 #include "internal.hpp"
 
 namespace svamd {
 namespace {
-
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ short to_bf16(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
-    return (short)(u >> 16);
-}
-__device__ __forceinline__ float bf16_round(float f) { return __uint_as_float(((uint32_t)(uint16_t)to_bf16(f)) << 16); }
-__device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
-}
-// p[0], p[s], p[2 s], p[3 s] as one BF16 operand.
-__device__ __forceinline__ bf16x4 op4(const float *p, int s) {
-    bf16x4 r;
-    r.x = to_bf16(p[0]);
-    r.y = to_bf16(p[s]);
-    r.z = to_bf16(p[2 * s]);
-    r.w = to_bf16(p[3 * s]);
-    return r;
-}
-__device__ __forceinline__ bf16x4 pack(const f32x4 v) {
-    bf16x4 r;
-    r.x = to_bf16(v.x);
-    r.y = to_bf16(v.y);
-    r.z = to_bf16(v.z);
-    r.w = to_bf16(v.w);
-    return r;
-}
-__device__ __forceinline__ f32x4 zero4() { return f32x4{0.f, 0.f, 0.f, 0.f}; }
 
 // One layer tile set: tile t holds rows 16 t + 4 (l >> 4) + i, column l & 15.
 struct Tiles {
@@ -86,67 +56,6 @@ __device__ __forceinline__ void forward16(const float *__restrict__ P, bf16x4 xo
     for (int kk = 0; kk < 4; ++kk) acc = mfma(op4(P + MLP_OFF_W3 + c * MLP_H + 16 * kk + r0, 1), pack(h2.t[kk]), acc);
 #pragma unroll
     for (int i = 0; i < 4; ++i) y[i] = acc[i] + P[MLP_OFF_B3 + r0 + i];
-}
-
-// The network's operands staged in LDS in the order the lanes read them:
-// op[o][l] is lane l's BF16 operand o (o = t for W1 tile t, 4 + 4 t + kk for
-// W2 tile (t, kk), 20 + kk for W3 tile kk), so that every operand is one
-// conflict-free 8-byte read; the biases follow as floats.
-struct StagedNet {
-    bf16x4 op[24][64];
-    float b1[MLP_H], b2[MLP_H], b3[16];
-};
-
-__device__ __forceinline__ void stage_net(const float *__restrict__ P, StagedNet &S) {
-    for (int idx = (int)threadIdx.x; idx < 24 * 64; idx += (int)blockDim.x) {
-        const int o = idx >> 6, ln = idx & 63, r0 = 4 * (ln >> 4), c = ln & 15;
-        const float *src;
-        if (o < 4)
-            src = P + MLP_OFF_W1 + (16 * o + c) * MLP_KIN + r0;
-        else if (o < 20)
-            src = P + MLP_OFF_W2 + (16 * ((o - 4) >> 2) + c) * MLP_H + 16 * ((o - 4) & 3) + r0;
-        else
-            src = P + MLP_OFF_W3 + c * MLP_H + 16 * (o - 20) + r0;
-        S.op[o][ln] = op4(src, 1);
-    }
-    for (int idx = (int)threadIdx.x; idx < 2 * MLP_H + 16; idx += (int)blockDim.x) {
-        if (idx < MLP_H)
-            S.b1[idx] = P[MLP_OFF_B1 + idx];
-        else if (idx < 2 * MLP_H)
-            S.b2[idx - MLP_H] = P[MLP_OFF_B2 + idx - MLP_H];
-        else
-            S.b3[idx - 2 * MLP_H] = P[MLP_OFF_B3 + idx - 2 * MLP_H];
-    }
-}
-
-// forward16 with the operands from LDS.
-__device__ __forceinline__ void forward16_staged(const StagedNet &S, bf16x4 xop, f32x4 &y) {
-    const int l = (int)(threadIdx.x & 63u), r0 = 4 * (l >> 4);
-    bf16x4 h1[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        const f32x4 acc = mfma(S.op[t][l], xop, zero4());
-        f32x4 h;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) h[i] = fmaxf(acc[i] + S.b1[16 * t + r0 + i], 0.f);
-        h1[t] = pack(h);
-    }
-    bf16x4 h2[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        f32x4 acc = zero4();
-#pragma unroll
-        for (int kk = 0; kk < 4; ++kk) acc = mfma(S.op[4 + 4 * t + kk][l], h1[kk], acc);
-        f32x4 h;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) h[i] = fmaxf(acc[i] + S.b2[16 * t + r0 + i], 0.f);
-        h2[t] = pack(h);
-    }
-    f32x4 acc = zero4();
-#pragma unroll
-    for (int kk = 0; kk < 4; ++kk) acc = mfma(S.op[20 + kk][l], h2[kk], acc);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) y[i] = acc[i] + S.b3[r0 + i];
 }
 
 // The B operand of 16 samples' features, from row-major x[n][MLP_IN].
@@ -393,13 +302,7 @@ __global__ __launch_bounds__(256) void k_closure(ClosureArgs a) {
                 id = a.P.id(src);
                 const float jp = a.X.jp[id];
                 hard = expf(a.hardening * (1.0f - jp));
-                E = mu * (3.f * lam + 2.f * mu) / (lam + mu);
-                const float nu = lam / (2.f * (lam + mu));
-#pragma unroll
-                for (int q = 0; q < 3; ++q) x[q] = (s[q] - 1.0f) * (1.0f / CLOSURE_SIGMA_SCALE);
-                x[3] = log10f(E * 1e-5f);
-                x[4] = (nu - 0.3f) * 10.0f;
-                x[5] = (jp - 1.0f) * (1.0f / CLOSURE_JP_SCALE);
+                closure_features(s, mu, lam, jp, x, E);
                 inside = true;
 #pragma unroll
                 for (int q = 0; q < MLP_IN; ++q) inside = inside && x[q] >= a.hull_lo[q] && x[q] <= a.hull_hi[q];
@@ -433,12 +336,8 @@ __global__ __launch_bounds__(256) void k_closure(ClosureArgs a) {
             float tau[9];
             if (inside) {
                 // linear-elastic part (FP32, exact in s) plus the learned remainder
-                const float unit = E * hard * CLOSURE_STRESS_SCALE * closure_strain_scale(s);
-                float lin[3], rem[3], d[3];
-                fcr_principal_parts(mu * hard, lam * hard, s, lin, rem);
-#pragma unroll
-                for (int q = 0; q < 3; ++q) d[q] = lin[q] + outs[w][l * 4 + q] * unit;
-                recompose(U, d, U, tau);
+                const float y[3] = {outs[w][l * 4], outs[w][l * 4 + 1], outs[w][l * 4 + 2]};
+                closure_tau(U, s, mu, lam, hard, E, y, tau);
                 ++evaluated;
             } else {
                 tau_fixed_corotated_svd(mu * hard, lam * hard, U, s, tau);
@@ -544,7 +443,16 @@ __global__ __launch_bounds__(256) void k_closure_pairs(ParticleArrays P, AuxArra
     }
 }
 
+__global__ __launch_bounds__(256) void k_pack_net(const float *__restrict__ P, StagedNet *__restrict__ out) {
+    stage_net(P, *out);
+}
+
 }  // namespace
+
+hipError_t launch_pack_net(const float *params, StagedNet *out, hipStream_t stream) {
+    hipLaunchKernelGGL(k_pack_net, dim3(1), dim3(256), 0, stream, params, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_closure_pairs(const ParticleArrays &P, const AuxArrays &X, uint64_t n, float hardening,
                                 uint32_t stride, uint32_t phase, float *x, float *t, uint64_t capacity,

@@ -274,6 +274,7 @@ enum : uint32_t { M_NH = 1, M_FLUID = 2, M_FCR = 4, M_SNOW = 8, M_SAND = 16, M_V
 // own copy of the domain).
 constexpr uint32_t M_BATCH = 64;
 constexpr uint32_t M_COLL = 128;  // mesh colliders (collide phase, twin nodes)
+constexpr uint32_t M_LEARN = 256; // the learned snow stress inside the fused gather-scatter
 constexpr uint32_t SET_POLAR = M_NH | M_FLUID | M_FCR | M_VISC;  // no SVD in the hot path
 constexpr uint32_t SET_ALL = M_NH | M_FLUID | M_FCR | M_SNOW | M_SAND | M_VISC;
 

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "device_math.hpp"
+#include "closure_net.hpp"
 #include "internal.hpp"
 
 using namespace svamd;
@@ -140,6 +141,8 @@ struct mpm_engine {
     ClosureStats *closure_stats = nullptr;
     float hull_lo[MLP_IN] = {0}, hull_hi[MLP_IN] = {0};
     uint32_t judge_stride = 0, judge_phase = 0;
+    StagedNet *closure_net = nullptr;  // the parameters packed for the fused kernel
+    bool learned_fused = false;         // mpm_tuning.learned_fused
     unsigned long long *pair_count = nullptr;  // mpm_closure_pairs
     float4 *limit_part = nullptr, *limit_out = nullptr;  // mpm_time_step_limits
     DeviceAdaptive *adapt_dev = nullptr;    // adaptive steps: the device state
@@ -670,6 +673,21 @@ int run_transfer(mpm_engine *e, TransferKind kind, bool goals, bool want_keys) {
         t.svd_out = e->closure_svd;
         t.tau_stride = e->n;
     }
+    // The learned snow stress inside the fused gather-scatter.
+    LearnedCall lc{};
+    const bool learn = kind == TransferKind::G2P2G && e->closure_on;
+    if (learn) {
+        if (!e->learned_fused || e->launch.colliders) return MPM_ERR_INVALID_ARGUMENT;
+        lc.net = e->closure_net;
+        for (int q = 0; q < MLP_IN; ++q) {
+            lc.hull_lo[q] = e->hull_lo[q];
+            lc.hull_hi[q] = e->hull_hi[q];
+        }
+        lc.judge_stride = e->judge_stride;
+        lc.judge_phase = e->judge_phase;
+        lc.stats = e->closure_stats;
+        t.learn = &lc;
+    }
     const int stage = kind == TransferKind::P2G ? ST_P2G : (kind == TransferKind::G2P2G ? ST_G2P2G : ST_G2P);
     if (kind == TransferKind::LIMITS && t.permute) return MPM_ERR_INVALID_ARGUMENT;
     if (e->launch.colliders) {
@@ -680,6 +698,7 @@ int run_transfer(mpm_engine *e, TransferKind kind, bool goals, bool want_keys) {
     e->begin(stage, r);
     HIP_TRY(launch_transfer(e->launch, t, e->stream));
     e->end(r);
+    if (learn && e->judge_stride) e->judge_phase = (e->judge_phase + 1) % e->judge_stride;
     if (kind == TransferKind::LIMITS) return MPM_OK;  // reads the state, writes only the reduction
     if (t.permute || want_keys) {
         // Slots past the live particles: copied when permuting, tombstone keys.
@@ -861,10 +880,14 @@ int run_loop(mpm_engine *e, const std::vector<double> &times, size_t k, bool goa
         ++rep.done;
         rep.transfers = t + 1;
     };
-    // With the learned closure, substep t is a gather (of substep t - 1, it
-    // writes the state), the sort when due, the closure, a scatter from the
-    // state and the grid update.
-    for (size_t t = 0; e->closure_on && k > 0 && t <= k; ++t) {
+    // With the learned closure in its own pass (the default), substep t is a
+    // gather (of substep t - 1, it writes the state), the sort when due, the
+    // closure, a scatter from the state and the grid update.  With
+    // mpm_tuning.learned_fused = 1 (and no colliders) the learned loop is the
+    // plain one: the closure gives the first scatter's snow stress, and the
+    // fused kernels evaluate the network after each advance.
+    const bool separate = e->closure_on && (!e->learned_fused || e->launch.colliders);
+    for (size_t t = 0; separate && k > 0 && t <= k; ++t) {
         if (t > 0) {
             if ((rc = run_transfer(e, TransferKind::G2P, false, t == k || sorts_before(t))) != MPM_OK) return rc;
             advanced(t);
@@ -885,6 +908,7 @@ int run_loop(mpm_engine *e, const std::vector<double> &times, size_t k, bool goa
         if (sorts_before(t) && (r = run_sort(e)) != MPM_OK) return r;
         const bool want_keys = t == k || sorts_before(t + 1);
         const TransferKind kind = t == 0 ? TransferKind::P2G : (t == k ? TransferKind::G2P : TransferKind::G2P2G);
+        if (kind == TransferKind::P2G && e->closure_on && (r = run_closure(e)) != MPM_OK) return r;
         if ((r = run_transfer(e, kind, goals, want_keys)) != MPM_OK) return r;
         if (t < k && (r = run_grid(e)) != MPM_OK) return r;
         return MPM_OK;
@@ -892,8 +916,9 @@ int run_loop(mpm_engine *e, const std::vector<double> &times, size_t k, bool goa
     // A sort cycle t .. t + K - 1 that starts after a transfer which wrote
     // the keys and ends with one that writes the next cycle's keys has the
     // same launches every time: it runs as one HIP graph launch.
-    const bool graphs = e->use_graph && !e->profiling;
-    for (size_t t = 0; !e->closure_on && k > 0 && t <= k;) {
+    // (not with the learned closure: its judge samples other particles every substep)
+    const bool graphs = e->use_graph && !e->profiling && !e->closure_on;
+    for (size_t t = 0; !separate && k > 0 && t <= k;) {
         if (canceled(t)) {
             if (t > 0) {
                 if ((rc = run_transfer(e, TransferKind::G2P, false, true)) != MPM_OK) return rc;
@@ -1399,6 +1424,7 @@ int mpm_create(const mpm_consts *consts, const mpm_particles_view *pv, const mpm
         if (tuning->graph < 0) e->use_graph = false;
         if (tuning->particles_per_lane < 0 || tuning->particles_per_lane > 3) return fail(MPM_ERR_INVALID_ARGUMENT);
         if (tuning->particles_per_lane > 0) e->launch.particles_per_lane = tuning->particles_per_lane;
+        if (tuning->learned_fused > 0) e->learned_fused = true;
         const int gut = tuning->grid_update_threads;
         if (gut != 0 && gut != 64 && gut != 128 && gut != 256) return fail(MPM_ERR_INVALID_ARGUMENT);
         if (gut) e->launch.grid_update_threads = gut;
@@ -1908,6 +1934,8 @@ int mpm_set_closure(mpm_engine *e, const mpm_closure *c) {
     }
     // Host or device memory (the parameters of a network trained on the GPU).
     HIP_TRY(hipMemcpyAsync(e->closure_params, c->params, MLP_NPARAM * sizeof(float), hipMemcpyDefault, e->stream));
+    if (!e->closure_net && e->alloc(e->closure_net, 1) != hipSuccess) return MPM_ERR_OUT_OF_MEMORY;
+    HIP_TRY(launch_pack_net(e->closure_params, e->closure_net, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     for (int q = 0; q < MLP_IN; ++q) {
         e->hull_lo[q] = c->hull_lo[q];

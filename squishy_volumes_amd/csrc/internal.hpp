@@ -295,6 +295,9 @@ struct ColliderCall {
     TwinTables twins;
 };
 
+struct LearnedCall;
+struct StagedNet;
+
 struct TransferCall {
     TransferKind kind;
     bool permute;               // slot i takes Pin[order[i]] and is written to Pout; else in place in Pin
@@ -315,6 +318,7 @@ struct TransferCall {
     uint64_t tau_stride = 0;
     float *svd_out = nullptr;        // G2P: U and s of each snow particle's new F (12 rows of tau_stride)
     const ColliderCall *coll = nullptr;  // mesh colliders, or null
+    const LearnedCall *learn = nullptr;  // G2P2G: the learned snow stress evaluated in the kernel
 };
 hipError_t launch_transfer(const TransferLaunch &L, const TransferCall &t, hipStream_t stream);
 uint32_t grid_update_waves(const TransferLaunch &L);
@@ -399,6 +403,14 @@ struct ClosureLaunch {
     const DeviceScalars *ds = nullptr;  // its halt flag (adaptive steps on the device), or null
 };
 hipError_t launch_closure(const ClosureLaunch &c, hipStream_t stream);
+// The network and its judge for the fused gather-scatter (M_LEARN kernels).
+struct LearnedCall {
+    const StagedNet *net;   // the parameters packed as the lanes read them (launch_pack_net)
+    float hull_lo[MLP_IN], hull_hi[MLP_IN];
+    uint32_t judge_stride, judge_phase;
+    ClosureStats *stats;    // CLOSURE_STATS_SLOTS slots, by work-group
+};
+hipError_t launch_pack_net(const float *params, StagedNet *out, hipStream_t stream);
 uint32_t step_limit_groups(uint64_t n);
 // AdaptiveTimeStepState (rust/crates/cpu/src/adaptive_time_step_state.rs:13-70)
 // kept on the device for a call with adaptive steps, with the call's time:

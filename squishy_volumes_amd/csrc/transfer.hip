@@ -38,6 +38,7 @@
 // polar decomposition; snow and sand need the full SVD.
 #include <algorithm>
 
+#include "closure_net.hpp"
 #include "collide.hpp"
 #include "device_math.hpp"
 #include "internal.hpp"
@@ -324,6 +325,7 @@ struct XferArgs {
                                  // tau_stride floats), for the learned closure; or null
     uint32_t pass_cap;           // particles per pass of a block: XF_THREADS x 1..3
     ColliderCall coll;           // M_COLL kernels: the mesh and the twin tables
+    LearnedCall learn;           // M_LEARN kernels: the network and its judge
 };
 
 // The substep parameters do not change while a kernel runs: read through the
@@ -497,13 +499,15 @@ __device__ __forceinline__ void accumulate_plane(int i, uint32_t cnt, int cx, in
     }
 }
 
+// The learned kernel asks for two waves per SIMD, which its network's
+// registers would otherwise cost it (a few values then live in scratch).
 // The gather-only mode has no scatter arenas (10 KB of LDS): asking for three
 // waves per SIMD fits it into 168 registers (no spills without the permute),
 // which takes the learned substep's gather from 0.148 to 0.124 ms per 1M
 // particles.  The other modes stay at the compiler's choice (2 waves, bound
 // by their 78 KB of LDS per work-group anyway).
 template <int MODE, bool PERMUTE, bool GOALS, uint32_t SET>
-__global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE == XM_G2P || MODE == XM_LIMITS ? 3 : 1))) void k_transfer(XferArgs a) {
+__global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE == XM_G2P || MODE == XM_LIMITS ? 3 : ((SET & M_LEARN) && MODE == XM_G2P2G ? 2 : 1)))) void k_transfer(XferArgs a) {
     constexpr bool HAS_G2P = MODE != XM_P2G;
     constexpr bool HAS_P2G = MODE == XM_P2G || MODE == XM_G2P2G;
     constexpr bool LAST = MODE == XM_G2P;
@@ -511,6 +515,12 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
     float lim_v = 0.f, lim_c = 0.f;  // LIMITS: largest |v| and |C_ij| of this thread's particles
     uint32_t lim_any = 0;
     constexpr bool COLL = (SET & M_COLL) != 0;
+    // The learned snow stress: the network runs on a wave's 64 particles at
+    // once, so a wave keeps all its lanes in every iteration of the particle
+    // loop (a lane with nothing to do skips its work instead of leaving).
+    constexpr bool LEARN = (SET & M_LEARN) != 0 && MODE == XM_G2P2G;
+    uint32_t n_eval = 0, n_fallback = 0, n_judged = 0;
+    double err2 = 0.0, ref2 = 0.0;
     // The region's node velocities (gather) and the arenas (scatter: one
     // 256-slot arena per core plane wave, one 512-slot arena for the overflow
     // wave) share LDS; a pass restages the velocities after the previous
@@ -628,7 +638,9 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
             // load counter then lets the staging wait for them alone).
             __builtin_amdgcn_sched_barrier(0);
         }
-        load_particle<PERMUTE>(a.Pin, a.order, min(w.cs + (tid ^ flip), ce - 1u), nxt);
+        // (the learned kernel loads each particle when it starts it: the network
+        // needs the registers a prefetched particle would hold)
+        if constexpr (!LEARN) load_particle<PERMUTE>(a.Pin, a.order, min(w.cs + (tid ^ flip), ce - 1u), nxt);
     };
 
     load_meta(0);
@@ -662,22 +674,36 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
 #pragma unroll 1
             for (int kk = 0; kk < XF_PER_THREAD; ++kk) {
                 const uint32_t i = cs + (tid ^ flip) + (uint32_t)(kk * XF_THREADS);
-                if (i >= ce) break;
-                cur = nxt;
-                if (kk + 1 < XF_PER_THREAD && i + XF_THREADS < ce)
-                    load_particle<PERMUTE>(a.Pin, a.order, i + XF_THREADS, nxt);
+                bool skip = false;
+                if constexpr (LEARN) {
+                    if (__ballot(i < ce) == 0ull) break;
+                    skip = i >= ce;
+                } else {
+                    if (i >= ce) break;
+                }
+                if constexpr (LEARN) {
+                    load_particle<PERMUTE>(a.Pin, a.order, min(i, ce - 1u), cur);
+                } else {
+                    cur = nxt;
+                    if (kk + 1 < XF_PER_THREAD && i + XF_THREADS < ce)
+                        load_particle<PERMUTE>(a.Pin, a.order, i + XF_THREADS, nxt);
+                }
                 const uint32_t src = cur.src;
                 const ParticleArrays &Pin = a.Pin;
                 const ParticleArrays &Pout = a.Pout;
                 const uint32_t flags0 = cur.flags;
                 uint32_t flags = flags0;
-                if (flags & F_TOMB) {
+                if (!skip && (flags & F_TOMB)) {
                     // Culled since the last sort: keeps its slot, takes no part.
                     if (PERMUTE) {
                         for (int f = 0; f < NFIELDS; ++f) Pout.at(i)[f * TILE] = Pin.at(src)[f * TILE];
                     }
                     if (a.keys_out) a.keys_out[i] = G.tomb_key;
-                    continue;
+                    if constexpr (LEARN) {
+                        skip = true;
+                    } else {
+                        continue;
+                    }
                 }
                 float xp[3], F[9];
 #pragma unroll
@@ -697,8 +723,14 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                 bool inside = true;
                 uint32_t failed = 0;  // status bits of a failure detected in this transfer
                 float A[9];
+                float jp_new = 1.f;                  // snow: Jp after the advance
+                float xnet[MLP_IN] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                float E_net = 0.f;
+                bool want_net = false;               // the network gives this particle's stress
+                float tau_l[LEARN ? 9 : 1];          // the learned kernel's stress of the new F
 
                 if constexpr (HAS_G2P) {
+                  if (!skip) {
                     // collect_velocity: v = sum w v_i, C = 4/h^2 sum w v_i (x_i - x_p)^T.
                     float f[3], w[3][3];
                     int c[3];
@@ -795,6 +827,7 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                             sv[q] = cl;
                         }
                         a.X.jp[id] = jp;
+                        jp_new = jp;
                         recompose(U, sv, V, F);
                         J = sv[0] * sv[1] * sv[2];
                         if constexpr (LAST) {
@@ -892,17 +925,72 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                         Pout.p1(i) = p1;
                         Pout.id(i) = id;
                     }
-                    if (LAST || !inside || failed) continue;
-                    // Stress of the new F for the next substep's scatter.
-                    float tau[9];
-                    kirchhoff<SET>(flags, p0, p1, hard, F, J, R, U, sv, tau);
-                    const bool visc = flags & F_VISC;
-                    float vd = 0.f, vb = 0.f;
-                    if ((SET & M_VISC) && visc) {
-                        vd = a.X.visc_dyn[id];
-                        vb = a.X.visc_bulk[id];
+                    if (LAST || !inside || failed) {
+                        if constexpr (LEARN) {
+                            skip = true;
+                        } else {
+                            continue;
+                        }
                     }
-                    fused_affine<SET>(m, vol, scaling, C, tau, J, visc, vd, vb, A);
+                    if constexpr (!LEARN) {
+                        // Stress of the new F for the next substep's scatter.
+                        float tau[9];
+                        kirchhoff<SET>(flags, p0, p1, hard, F, J, R, U, sv, tau);
+                        const bool visc = flags & F_VISC;
+                        float vd = 0.f, vb = 0.f;
+                        if ((SET & M_VISC) && visc) {
+                            vd = a.X.visc_dyn[id];
+                            vb = a.X.visc_bulk[id];
+                        }
+                        fused_affine<SET>(m, vol, scaling, C, tau, J, visc, vd, vb, A);
+                    } else if (!skip) {
+                        // the network's inputs; inside the training hull it gives the
+                        // stress, else the analytic model does (before the network runs,
+                        // so that F need not live through it)
+                        if (flags & F_SNOW) {
+                            closure_features(sv, p0, p1, jp_new, xnet, E_net);
+                            want_net = true;
+#pragma unroll
+                            for (int q = 0; q < MLP_IN; ++q)
+                                want_net = want_net && xnet[q] >= a.learn.hull_lo[q] && xnet[q] <= a.learn.hull_hi[q];
+                        }
+                        if (!want_net) kirchhoff<SET>(flags, p0, p1, hard, F, J, R, U, sv, tau_l);
+                    }
+                  }
+                  if constexpr (LEARN) {
+                    // The network on the wave's particles (every lane takes part), then
+                    // the stress: learned inside the hull, analytic outside it, both
+                    // for the particles the judge samples (closure.hip, k_closure).
+                    float ynet[3];
+                    closure_outputs(*a.learn.net, xnet, want_net, ynet);
+                    if (!skip) {
+                        float *tau = tau_l;
+                        const bool snow = (flags & F_SNOW) != 0;
+                        if (want_net) {
+                            closure_tau(U, sv, p0, p1, hard, E_net, ynet, tau);
+                            ++n_eval;
+                        } else if (snow) {
+                            ++n_fallback;
+                        }
+                        if (snow && a.learn.judge_stride && id % a.learn.judge_stride == a.learn.judge_phase) {
+                            float ref[9];
+                            tau_fixed_corotated_svd(p0 * hard, p1 * hard, U, sv, ref);
+#pragma unroll
+                            for (int k = 0; k < 9; ++k) {
+                                err2 += (double)(tau[k] - ref[k]) * (double)(tau[k] - ref[k]);
+                                ref2 += (double)ref[k] * (double)ref[k];
+                            }
+                            ++n_judged;
+                        }
+                        const bool visc = flags & F_VISC;
+                        float vd = 0.f, vb = 0.f;
+                        if ((SET & M_VISC) && visc) {
+                            vd = a.X.visc_dyn[id];
+                            vb = a.X.visc_bulk[id];
+                        }
+                        fused_affine<SET>(m, vol, scaling, C, tau, J, visc, vd, vb, A);
+                    }
+                  }
                 } else {
                     // First substep of a call: the state holds v and C.
 #pragma unroll
@@ -922,7 +1010,7 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                     }
                 }
 
-                if constexpr (HAS_P2G) {
+                if constexpr (HAS_P2G) if (!skip) {
                     // external_force of the substep that starts, then the payload
                     //   (f, m | q | h A columns): contribution at stencil node (i,j,k)
                     //   w_ijk (m v + A d) = w_ijk (q + i c0 + j c1 + k c2).
@@ -1155,6 +1243,25 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
         w = wn;
         have = have_next;
         ++npass;
+    }
+    if constexpr (LEARN) {
+        // the closure's statistics of this work-group's particles, into its slot
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            n_eval += (uint32_t)__shfl_xor((int)n_eval, o, 64);
+            n_fallback += (uint32_t)__shfl_xor((int)n_fallback, o, 64);
+            n_judged += (uint32_t)__shfl_xor((int)n_judged, o, 64);
+            err2 += __shfl_xor(err2, o, 64);
+            ref2 += __shfl_xor(ref2, o, 64);
+        }
+        if (lane == 0 && (n_eval | n_fallback | n_judged)) {
+            ClosureStats &S = a.learn.stats[blockIdx.x % CLOSURE_STATS_SLOTS];
+            atomicAdd(&S.evaluated, (unsigned long long)n_eval);
+            atomicAdd(&S.fallbacks, (unsigned long long)n_fallback);
+            atomicAdd(&S.judged, (unsigned long long)n_judged);
+            unsafeAtomicAdd(&S.err2, err2);
+            unsafeAtomicAdd(&S.ref2, ref2);
+        }
     }
     if constexpr (LIMITS) {
         // non-negative floats order as their bit patterns
@@ -1407,7 +1514,17 @@ hipError_t dispatch_flags(const TransferLaunch &L, bool permute, bool goals, con
 }
 
 template <int MODE>
-hipError_t dispatch_set(const TransferLaunch &L, bool permute, bool goals, const XferArgs &a, hipStream_t stream) {
+hipError_t dispatch_set(const TransferLaunch &L, bool permute, bool goals, bool learn, const XferArgs &a,
+                        hipStream_t stream) {
+    if constexpr (MODE == XM_G2P2G) {
+        // the learned snow stress inside the fused kernel (no colliders: the learned loop
+        // runs its closure pass between gather and scatter then)
+        if (learn && !L.colliders) {
+            if (L.batched) return dispatch_flags<MODE, SET_ALL | M_BATCH | M_LEARN>(L, permute, goals, a, stream);
+            return dispatch_flags<MODE, SET_ALL | M_LEARN>(L, permute, goals, a, stream);
+        }
+    }
+    if (learn) return hipErrorInvalidValue;
     if (L.colliders) {
         if (L.model_set == SET_POLAR) return dispatch_flags<MODE, SET_POLAR | M_COLL>(L, permute, goals, a, stream);
         return dispatch_flags<MODE, SET_ALL | M_COLL>(L, permute, goals, a, stream);
@@ -1444,12 +1561,13 @@ hipError_t launch_transfer(const TransferLaunch &L, const TransferCall &t, hipSt
         if (!t.coll) return hipErrorInvalidValue;
         a.coll = *t.coll;
     }
+    if (t.learn) a.learn = *t.learn;
     const bool goals = t.goals;
     switch (t.kind) {
-        case TransferKind::P2G: return dispatch_set<XM_P2G>(L, t.permute, goals, a, stream);
-        case TransferKind::G2P2G: return dispatch_set<XM_G2P2G>(L, t.permute, goals, a, stream);
-        case TransferKind::G2P: return dispatch_set<XM_G2P>(L, t.permute, false, a, stream);
-        case TransferKind::LIMITS: return dispatch_set<XM_LIMITS>(L, t.permute, false, a, stream);
+        case TransferKind::P2G: return dispatch_set<XM_P2G>(L, t.permute, goals, false, a, stream);
+        case TransferKind::G2P2G: return dispatch_set<XM_G2P2G>(L, t.permute, goals, t.learn != nullptr, a, stream);
+        case TransferKind::G2P: return dispatch_set<XM_G2P>(L, t.permute, false, false, a, stream);
+        case TransferKind::LIMITS: return dispatch_set<XM_LIMITS>(L, t.permute, false, false, a, stream);
     }
     return hipErrorInvalidValue;
 }

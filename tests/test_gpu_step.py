@@ -396,6 +396,13 @@ def test_empty_and_all_culled_states():
     assert err is None and dead.last_status.substeps == 12
     assert np.array_equal(st.particles.positions, parts.positions)
     assert np.array_equal(st.particles.velocities, parts.velocities)
+    # adaptive steps with no limit (every one f32::MAX, adaptive_time_step_state.rs:36-47):
+    # the step is max_time_step, or f32::MAX without one (one substep reaches any target)
+    st, err = dead.produce_next_state(fi, GpuRunParameters(target_time=T + 0.01, max_time_step=2e-3,
+                                                           adaptive_time_steps=True))
+    assert err is None and dead.last_status.substeps == 5
+    st, err = empty.produce_next_state(fi, GpuRunParameters(target_time=T + 0.02, adaptive_time_steps=True))
+    assert err is None and empty.last_status.substeps == 1 and empty.time > 1e30
 
 
 def test_cancellation_before_the_first_substep_is_fatal_and_harmless():
