@@ -1174,6 +1174,7 @@ int run_adaptive(mpm_engine *e, double target_time, const mpm_frame_input *input
     e->epoch = epoch0 + A.substeps;
     e->keys_valid = false;
     e->pending_permute = false;
+    e->svd_fresh = false;  // the last gather launched may not have run
     e->ats.by_velocity = A.by_velocity;
     e->ats.by_deformation = A.by_deformation;
     e->ats.by_sound = A.by_sound;
@@ -1193,11 +1194,10 @@ int run_adaptive(mpm_engine *e, double target_time, const mpm_frame_input *input
     if (A.stop == ADAPT_TWIN_OVERFLOW) return RERUN_LARGER_TWINS;
     if (e->ds_host->status & fail_bits) {
         stop = e->ds_host->status;
-        // a failure in the call's last gather (the device has not seen it)
-        if (A.stop != ADAPT_FAILED && (stop & MPM_STATUS_ENERGY_ERROR) && A.substeps > 0) {
+        // a failure in the last gather before a cycle's end (the device stops
+        // at the next substep's start and adjusts the time itself)
+        if (A.stop != ADAPT_FAILED && (stop & MPM_STATUS_ENERGY_ERROR) && A.substeps > 0)
             e->time -= (double)A.last_advance;
-            e->epoch = epoch0 + A.substeps;
-        }
     }
     return MPM_OK;
 }
