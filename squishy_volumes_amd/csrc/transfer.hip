@@ -1280,10 +1280,24 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
 }
 
 // ------------------------------------------------------------ grid update
-// One wave per touched block T (lane = node): the candidate slots written by
-// the transfer kernel (NCAND per block-list entry, then the overflow list)
-// are read 64 at a time, and each flagged block is processed once: v = p / m,
-// and the accumulation grid is cleared for the next scatter.
+// Grids of up to this many blocks are updated from their block flags (4 bytes
+// per block, read once); larger ones from the candidate slots.
+constexpr uint32_t DENSE_FLAG_SCAN_BLOCKS = 1u << 22;
+// A prime above every block count: i -> (i * FLAG_SCAN_STRIDE) mod n is a
+// permutation of [0, n).
+constexpr uint64_t FLAG_SCAN_STRIDE = 4294967291ull;
+
+// One wave per touched block T (lane = node): each flagged block is processed
+// once: v = p / m, and the accumulation grid is cleared for the next scatter.
+// The flagged blocks are found in one of two ways:
+//   * grids of up to DENSE_FLAG_SCAN_BLOCKS blocks: the flags themselves, 64
+//     per wave step, at the permuted indices (64 q + lane) * FLAG_SCAN_STRIDE
+//     mod n, so that every wave meets about the same number of flagged blocks
+//     however the particles cluster;
+//   * larger grids: the candidate slots written by the transfer kernel (NCAND
+//     per block-list entry, then the overflow list), 64 at a time; a block
+//     listed by several entries is processed by the one whose exchange finds
+//     its flag set.
 __global__ __launch_bounds__(256) void k_grid_update(GridBuffers B, GridGeom G, const SubstepParams *params,
                                                      uint32_t *__restrict__ partial, DeviceScalars *ds) {
     if (ds->halt) return;
@@ -1300,12 +1314,9 @@ __global__ __launch_bounds__(256) void k_grid_update(GridBuffers B, GridGeom G, 
     const uint32_t ncand = nlist + novf;
     const uint32_t x = lane & 3u, y = (lane >> 2) & 3u, z = lane >> 4;
     uint32_t active = 0;
-    for (uint32_t chunk = gw; chunk * 64u < ncand; chunk += nw) {
-        const uint32_t c = chunk * 64u + lane;
-        uint32_t mine = c < nlist ? B.cand[c] : (c < ncand ? ovf[c - nlist] : NO_BLOCK);
-        // A block may be listed by several entries: the one whose exchange
-        // finds the flag set processes it (and clears the flag).
-        if (mine != NO_BLOCK && atomicExch(&B.blk_flag[mine], 0u) == 0u) mine = NO_BLOCK;
+    // The blocks that lane l found flagged (mine = its index, else NO_BLOCK),
+    // processed two at a time.
+    auto run = [&](uint32_t mine) {
         uint64_t todo = __ballot(mine != NO_BLOCK);
         // One block's nodes from the node values g (lane = node).
         auto update = [&](uint32_t T, const float4 g) {
@@ -1342,6 +1353,28 @@ __global__ __launch_bounds__(256) void k_grid_update(GridBuffers B, GridGeom G, 
             const float4 g1 = T1 != NO_BLOCK ? B.grid[T1 * 64u + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
             update(T0, g0);
             if (T1 != NO_BLOCK) update(T1, g1);
+        }
+    };
+    const uint32_t n = G.num_blocks_total;
+    if (n <= DENSE_FLAG_SCAN_BLOCKS) {
+        for (uint32_t q = gw; q * 64u < n; q += nw) {
+            const uint32_t i = q * 64u + lane;
+            uint32_t mine = NO_BLOCK;
+            if (i < n) {
+                const uint32_t T = (uint32_t)(((uint64_t)i * FLAG_SCAN_STRIDE) % n);
+                if (B.blk_flag[T]) {
+                    B.blk_flag[T] = 0u;
+                    mine = T;
+                }
+            }
+            run(mine);
+        }
+    } else {
+        for (uint32_t chunk = gw; chunk * 64u < ncand; chunk += nw) {
+            const uint32_t c = chunk * 64u + lane;
+            uint32_t mine = c < nlist ? B.cand[c] : (c < ncand ? ovf[c - nlist] : NO_BLOCK);
+            if (mine != NO_BLOCK && atomicExch(&B.blk_flag[mine], 0u) == 0u) mine = NO_BLOCK;
+            run(mine);
         }
     }
     // Per-wave count of active nodes (summed once per step_to call).
