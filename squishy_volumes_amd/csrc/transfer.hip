@@ -1280,9 +1280,11 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
 }
 
 // ------------------------------------------------------------ grid update
-// Grids of up to this many blocks are updated from their block flags (4 bytes
-// per block, read once); larger ones from the candidate slots.
+// Grids of up to this many blocks, and at most DENSE_FLAG_SCAN_RATIO times
+// as many as the candidate slots of the launch, are updated from their block
+// flags (4 bytes per block, read once); other ones from the candidate slots.
 constexpr uint32_t DENSE_FLAG_SCAN_BLOCKS = 1u << 22;
+constexpr uint64_t DENSE_FLAG_SCAN_RATIO = 4;
 // A prime above every block count: i -> (i * FLAG_SCAN_STRIDE) mod n is a
 // permutation of [0, n).
 constexpr uint64_t FLAG_SCAN_STRIDE = 4294967291ull;
@@ -1290,11 +1292,13 @@ constexpr uint64_t FLAG_SCAN_STRIDE = 4294967291ull;
 // One wave per touched block T (lane = node): each flagged block is processed
 // once: v = p / m, and the accumulation grid is cleared for the next scatter.
 // The flagged blocks are found in one of two ways:
-//   * grids of up to DENSE_FLAG_SCAN_BLOCKS blocks: the flags themselves, 64
-//     per wave step, at the permuted indices (64 q + lane) * FLAG_SCAN_STRIDE
-//     mod n, so that every wave meets about the same number of flagged blocks
-//     however the particles cluster;
-//   * larger grids: the candidate slots written by the transfer kernel (NCAND
+//   * grids of up to DENSE_FLAG_SCAN_BLOCKS blocks, and up to
+//     DENSE_FLAG_SCAN_RATIO times the candidate slots: the flags themselves,
+//     64 per wave step, at the permuted indices (64 q + lane) *
+//     FLAG_SCAN_STRIDE mod n, so that every wave meets about the same number
+//     of flagged blocks however the particles cluster;
+//   * otherwise (large grids, small scenes): the candidate slots written by
+//     the transfer kernel (NCAND
 //     per block-list entry, then the overflow list), 64 at a time; a block
 //     listed by several entries is processed by the one whose exchange finds
 //     its flag set.
@@ -1356,7 +1360,7 @@ __global__ __launch_bounds__(256) void k_grid_update(GridBuffers B, GridGeom G, 
         }
     };
     const uint32_t n = G.num_blocks_total;
-    if (n <= DENSE_FLAG_SCAN_BLOCKS) {
+    if (n <= DENSE_FLAG_SCAN_BLOCKS && (uint64_t)n <= DENSE_FLAG_SCAN_RATIO * ((uint64_t)ds->num_blocks * NCAND + novf)) {
         for (uint32_t q = gw; q * 64u < n; q += nw) {
             const uint32_t i = q * 64u + lane;
             uint32_t mine = NO_BLOCK;
