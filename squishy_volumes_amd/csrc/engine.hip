@@ -25,7 +25,7 @@ using namespace svamd;
 namespace {
 
 enum Stage { ST_KEYS = 0, ST_SORT, ST_BLOCKS, ST_P2G, ST_GRID, ST_G2P2G, ST_G2P, ST_COPY, ST_CLOSURE, ST_COUNT };
-const char *kStageNames[ST_COUNT] = {"keys",        "radix_sort", "block_list", "p2g",    "grid_update",
+const char *kStageNames[ST_COUNT] = {"keys",        "sort",       "block_list", "p2g",    "grid_update",
                                      "g2p2g",       "g2p",        "copy_dead",  "closure"};
 
 thread_local std::string g_last_error;
@@ -94,6 +94,9 @@ struct mpm_engine {
 
     uint32_t *keys = nullptr, *keys_sorted = nullptr, *order = nullptr;
     void *sort_temp = nullptr;
+    // The counting sort's buffers, for 2^bucket_bits keys (0: none held).
+    BucketBuffers bucket{};
+    int bucket_bits = 0;
     uint32_t *blk_start = nullptr, *blk_key = nullptr;
     uint32_t blk_capacity = 0;
     void *blk_temp = nullptr;
@@ -482,21 +485,54 @@ int run_sort(mpm_engine *e) {
     EventRecord r{};
     ParticleArrays &P = e->P[e->cur];
     HIP_TRY(launch_reset_scalars(e->ds, e->stream));
+    // keys that a transfer wrote (stayers in key order) take the counting sort
+    const bool counting = e->keys_valid && e->key_bits <= e->bucket_bits;
     if (!e->keys_valid) {
         e->begin(ST_KEYS, r);
         HIP_TRY(launch_keys(P, e->geom, e->n, e->keys, e->ds, e->stream));
         e->end(r);
     }
-    e->begin(ST_SORT, r);
-    HIP_TRY(radix_sort(e->keys, nullptr, e->keys_sorted, e->order, nullptr, nullptr, e->n, e->key_bits, e->sort_temp,
-                       e->stream));
-    e->end(r);
-    e->begin(ST_BLOCKS, r);
-    HIP_TRY(launch_block_list(e->keys_sorted, e->n, e->geom.tomb_key, e->blk_start, e->blk_key, e->blk_capacity,
-                              e->ds, e->blk_temp, e->stream));
-    e->end(r);
+    if (counting) {
+        // the counting sort writes the block list too
+        e->begin(ST_SORT, r);
+        HIP_TRY(bucket_sort(e->keys, e->n, e->key_bits, e->bucket, e->order, e->blk_start, e->blk_key,
+                            e->blk_capacity, e->ds, e->stream));
+        e->end(r);
+    } else {
+        e->begin(ST_SORT, r);
+        HIP_TRY(radix_sort(e->keys, nullptr, e->keys_sorted, e->order, nullptr, nullptr, e->n, e->key_bits,
+                           e->sort_temp, e->stream));
+        e->end(r);
+        e->begin(ST_BLOCKS, r);
+        HIP_TRY(launch_block_list(e->keys_sorted, e->n, e->geom.tomb_key, e->blk_start, e->blk_key, e->blk_capacity,
+                                  e->ds, e->blk_temp, e->stream));
+        e->end(r);
+    }
     e->keys_valid = false;
     e->pending_permute = true;
+    return MPM_OK;
+}
+
+// The counting sort's buffers for the geometry's key range, when it has at
+// most 2^BUCKET_SORT_MAX_BITS keys (larger ranges take the radix sort).  The
+// per-key counts start at zero; every sort leaves them so.
+int ensure_buckets(mpm_engine *e) {
+    if (e->key_bits > BUCKET_SORT_MAX_BITS || e->key_bits <= e->bucket_bits) return MPM_OK;
+    BucketBuffers &B = e->bucket;
+    e->release(B.counts);
+    e->release(B.stay);
+    e->release(B.cursor);
+    e->release(B.stay_base);
+    e->release(B.tiles);
+    e->bucket_bits = 0;
+    const size_t nkeys = (size_t)1 << e->key_bits;
+    if (e->alloc(B.counts, nkeys) || e->alloc(B.stay, nkeys) || e->alloc(B.cursor, nkeys) ||
+        e->alloc(B.stay_base, nkeys) || e->alloc(B.tiles, bucket_tile_sums(e->key_bits, e->n)))
+        return MPM_ERR_OUT_OF_MEMORY;
+    if (!B.group_stay && e->alloc(B.group_stay, bucket_group_count(e->n))) return MPM_ERR_OUT_OF_MEMORY;
+    HIP_TRY(hipMemsetAsync(B.counts, 0, nkeys * sizeof(uint32_t), e->stream));
+    HIP_TRY(hipMemsetAsync(B.stay, 0, nkeys * sizeof(uint32_t), e->stream));
+    e->bucket_bits = e->key_bits;
     return MPM_OK;
 }
 
@@ -570,6 +606,7 @@ int set_window(mpm_engine *e, const long long lo[3], const long long hi[3]) {
         e->win_lo[a] = lo[a];
         e->win_hi[a] = hi[a];
     }
+    if ((rc = ensure_buckets(e)) != MPM_OK) return rc;
     if ((rc = clear_grid(e)) != MPM_OK) return rc;
     e->keys_valid = false;
     e->grid_valid = false;
@@ -662,6 +699,7 @@ int run_transfer(mpm_engine *e, TransferKind kind, bool goals, bool want_keys) {
     t.goals = goals;
     t.B = grid_buffers(e);
     t.keys_out = want_keys ? e->keys : nullptr;
+    t.mover_bit = e->key_bits <= e->bucket_bits ? BUCKET_MOVER : 0u;
     t.ds = e->ds;
     if (kind == TransferKind::P2G && e->closure_on) {
         t.tau_ext = e->closure_tau;
@@ -1456,6 +1494,7 @@ int mpm_create(const mpm_consts *consts, const mpm_particles_view *pv, const mpm
     if ((rc = alloc_grid(e, e->geom.num_blocks_total)) != MPM_OK) return fail(rc);
     e->sort_temp = st;
     e->blk_temp = bt;
+    if ((rc = ensure_buckets(e)) != MPM_OK) return fail(rc);
 
     if (hipHostMalloc((void **)&e->ds_host, sizeof(DeviceScalars), hipHostMallocDefault) != hipSuccess)
         return fail(MPM_ERR_OUT_OF_MEMORY);

@@ -260,6 +260,28 @@ hipError_t launch_keys(const ParticleArrays &P, const GridGeom &G, uint64_t n, u
 size_t block_list_temp_bytes(uint64_t n);
 hipError_t launch_block_list(const uint32_t *keys, uint64_t n, uint32_t tomb_key, uint32_t *blk_start,
                              uint32_t *blk_key, uint32_t capacity, DeviceScalars *ds, void *temp, hipStream_t stream);
+// The re-sort of keys that a transfer wrote, for key ranges of up to
+// 2^BUCKET_SORT_MAX_BITS keys: a counting sort over every key.  The transfer
+// marks with BUCKET_MOVER a particle whose key differs from that of the block
+// it was processed in; the others (stayers) come in key order, since the
+// slots are grouped by block in key order.  Stayers keep their input order
+// within a key (they take their slots from a prefix count); movers follow
+// their key's stayers in the order of the atomics that place them.  It writes
+// order[] (sorted slot -> input slot), the occupied-block list and the live
+// count, as radix_sort and launch_block_list together do.
+constexpr int BUCKET_SORT_MAX_BITS = 22;
+constexpr uint32_t BUCKET_MOVER = 0x80000000u;
+struct BucketBuffers {
+    uint32_t *counts = nullptr, *stay = nullptr;       // per key: particles, stayers (zero between sorts)
+    uint32_t *cursor = nullptr, *stay_base = nullptr;  // per key: the movers' next slot, the stayers' base
+    uint32_t *tiles = nullptr;                         // bucket_tile_sums(key_bits, n) entries
+    uint32_t *group_stay = nullptr;                    // per counting work-group: bucket_group_count(n)
+};
+uint64_t bucket_group_count(uint64_t n);
+size_t bucket_tile_sums(int key_bits, uint64_t n);
+hipError_t bucket_sort(const uint32_t *keys, uint64_t n, int key_bits, const BucketBuffers &B, uint32_t *order,
+                       uint32_t *blk_start, uint32_t *blk_key, uint32_t capacity, DeviceScalars *ds,
+                       hipStream_t stream);
 hipError_t launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t stream);
 hipError_t launch_reset_scalars(DeviceScalars *ds, hipStream_t stream);
 
@@ -313,6 +335,7 @@ struct TransferCall {
     bool goals;                 // that substep has goal positions
     GridBuffers B;
     uint32_t *keys_out;         // null: keys not needed
+    uint32_t mover_bit = 0;     // BUCKET_MOVER: keys_out marks particles that left their block
     DeviceScalars *ds;
     const float *tau_ext = nullptr;  // P2G: snow stress from the learned closure (rows of tau_stride)
     uint64_t tau_stride = 0;

@@ -318,6 +318,8 @@ struct XferArgs {
                                  // scatter runs, P2G modes)
     GridBuffers B;               // B.ovf[epoch & 1]: the overflow list for that substep's grid
     uint32_t *keys_out;          // block keys of the new positions, or null
+    uint32_t mover_bit;          // or'ed into a key that differs from the key of the block
+                                 // the particle was processed in (BUCKET_MOVER or 0)
     DeviceScalars *ds;
     const float *tau_ext;        // P2G mode: Kirchhoff stress of snow particles by slot (9 rows
     uint64_t tau_stride;         // of tau_stride floats), from the learned closure; or null
@@ -656,6 +658,7 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
         const uint32_t ce = min(end, cs + a.pass_cap);
         const uint32_t flip = npass & 1u ? 255u : 0u;
         const uint32_t b[3] = {w.b[0], w.b[1], w.b[2]};
+        const uint32_t bkey = morton3(b[0], b[1], b[2]);
         int cell0[3];  // global cell of the block's first cell
 #pragma unroll
         for (int q = 0; q < 3; ++q) cell0[q] = G.origin[q] + 4 * (int)b[q];
@@ -698,7 +701,7 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                     if (PERMUTE) {
                         for (int f = 0; f < NFIELDS; ++f) Pout.at(i)[f * TILE] = Pin.at(src)[f * TILE];
                     }
-                    if (a.keys_out) a.keys_out[i] = G.tomb_key;
+                    if (a.keys_out) a.keys_out[i] = G.tomb_key | a.mover_bit;
                     if constexpr (LEARN) {
                         skip = true;
                     } else {
@@ -904,7 +907,10 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                         report_failure(ds, failed | (KID_G2P << 16), pidx);
                     }
 
-                    if (a.keys_out) a.keys_out[i] = inside ? particle_block_key(xp, G, ds) : G.tomb_key;
+                    if (a.keys_out) {
+                        const uint32_t key = inside ? particle_block_key(xp, G, ds) : G.tomb_key;
+                        a.keys_out[i] = key | (key != bkey ? a.mover_bit : 0u);
+                    }
 #pragma unroll
                     for (int q = 0; q < 3; ++q) Pout.x(q, i) = xp[q];
 #pragma unroll
@@ -1589,6 +1595,7 @@ hipError_t launch_transfer(const TransferLaunch &L, const TransferCall &t, hipSt
     a.params = t.params;
     a.B = t.B;
     a.keys_out = t.keys_out;
+    a.mover_bit = t.mover_bit;
     a.ds = t.ds;
     a.tau_ext = t.tau_ext;
     a.tau_stride = t.tau_stride;

@@ -167,7 +167,7 @@ class GpuState:
         fi, keep = frame_input.to_c()
         _check(self._lib, self._lib.mpm_run_stage(self._h, stage, C.byref(fi)))
 
-    STAGES = ("keys", "radix_sort", "block_list", "p2g", "grid_update", "g2p2g", "g2p", "copy_dead")
+    STAGES = ("keys", "sort", "block_list", "p2g", "grid_update", "g2p2g", "g2p", "copy_dead")
 
     def set_profiling(self, enabled: bool, stages=None):
         """HIP-event timing of every stage, or of the named `stages` only."""

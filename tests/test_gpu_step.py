@@ -359,7 +359,7 @@ def test_profiling_times_the_selected_stages_only():
     g.set_profiling(True)
     step(4)
     every = g.kernel_times(reset=True)
-    assert {"radix_sort", "block_list", "p2g", "grid_update", "g2p2g", "g2p"} <= {k for k, v in every.items() if v[1]}
+    assert {"sort", "p2g", "grid_update", "g2p2g", "g2p"} <= {k for k, v in every.items() if v[1]}
     assert every["g2p2g"][1] == 3 and every["grid_update"][1] == 4 and every["p2g"][1] == 1
     g.set_profiling(True, stages=("g2p2g", "grid_update"))
     step(4)
