@@ -1208,7 +1208,9 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                 // memory (the float atomics are carried out beyond the L2)
                 // carries whole nodes.  Zero components are skipped.
                 // Thread t takes component t & 3 of region slot 64 k + (t >> 2)
-                // in each of the 8 region sub-blocks k.
+                // in each of the 8 region sub-blocks k, four sub-blocks at a
+                // time (their 16 LDS reads in flight together; all 8 at once
+                // take the kernel to 256 VGPRs and measured far slower).
                 static_assert(4 * 64 == XF_THREADS, "one quad per slot of a sub-block");
                 {
                     const uint32_t lo = tid >> 2, comp = tid & 3u;
@@ -1216,7 +1218,7 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                     const float *shellf = reinterpret_cast<const float *>(shell) + comp;
                     const float *planef = reinterpret_cast<const float *>(planes) + comp;
                     float *gridf = reinterpret_cast<float *>(a.B.grid) + comp;
-#pragma unroll 1
+#pragma unroll 4
                     for (int k = 0; k < 8; ++k) {
                         const int rx = lx | ((k & 1) << 2), ry = ly | ((k & 2) << 1), rz = lz | (k & 4);
                         // All four reads are issued together (a plane that does
