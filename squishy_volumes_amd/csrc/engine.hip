@@ -529,7 +529,7 @@ int ensure_buckets(mpm_engine *e) {
     if (e->alloc(B.counts, nkeys) || e->alloc(B.stay, nkeys) || e->alloc(B.cursor, nkeys) ||
         e->alloc(B.stay_base, nkeys) || e->alloc(B.tiles, bucket_tile_sums(e->key_bits, e->n)))
         return MPM_ERR_OUT_OF_MEMORY;
-    if (!B.group_stay && e->alloc(B.group_stay, bucket_group_count(e->n))) return MPM_ERR_OUT_OF_MEMORY;
+    if (!B.span_stay && e->alloc(B.span_stay, bucket_span_count(e->n))) return MPM_ERR_OUT_OF_MEMORY;
     HIP_TRY(hipMemsetAsync(B.counts, 0, nkeys * sizeof(uint32_t), e->stream));
     HIP_TRY(hipMemsetAsync(B.stay, 0, nkeys * sizeof(uint32_t), e->stream));
     e->bucket_bits = e->key_bits;

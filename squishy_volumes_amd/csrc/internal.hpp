@@ -275,9 +275,9 @@ struct BucketBuffers {
     uint32_t *counts = nullptr, *stay = nullptr;       // per key: particles, stayers (zero between sorts)
     uint32_t *cursor = nullptr, *stay_base = nullptr;  // per key: the movers' next slot, the stayers' base
     uint32_t *tiles = nullptr;                         // bucket_tile_sums(key_bits, n) entries
-    uint32_t *group_stay = nullptr;                    // per counting work-group: bucket_group_count(n)
+    uint32_t *span_stay = nullptr;                     // per span of input slots: bucket_span_count(n)
 };
-uint64_t bucket_group_count(uint64_t n);
+uint64_t bucket_span_count(uint64_t n);
 size_t bucket_tile_sums(int key_bits, uint64_t n);
 hipError_t bucket_sort(const uint32_t *keys, uint64_t n, int key_bits, const BucketBuffers &B, uint32_t *order,
                        uint32_t *blk_start, uint32_t *blk_key, uint32_t capacity, DeviceScalars *ds,

@@ -333,19 +333,24 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_heads_write(const uint32_t *__
 // after their key's stayers, a run of equal keys in a wave with one atomic.
 // Five launches:
 //   k_bucket_count   counts[key] and stay[key] += run lengths; the stayers of
-//                    each work-group of BUCKET_GROUP input slots;
-//   k_bucket_reduce  per tile of SCAN_TILE keys: its particles, occupied
+//                    each span of BUCKET_SPAN input slots;
+//   k_bucket_reduce  per tile of BUCKET_TILE keys: its particles, occupied
 //                    blocks (the tomb key excluded) and stayers; per tile of
-//                    SCAN_TILE work-groups: their stayers;
+//                    BUCKET_TILE spans: their stayers;
 //   k_bucket_top     the tiles' exclusive offsets (four sums), and the number
 //                    of occupied blocks;
 //   k_bucket_down    per key: the stayers' slot base, the movers' first slot,
 //                    the block list (start, key) in key order and the live
 //                    count (the tomb key is the largest: culled particles go
-//                    last), the counts cleared for the next sort; per
-//                    work-group: the stayers before it;
+//                    last), the counts cleared for the next sort; per span:
+//                    the stayers before it;
 //   k_bucket_place   order[slot] = input slot.
+// A span's slot base + j BUCKET_GROUP + thread is its item j (j < BUCKET_ITEMS).
 constexpr int BUCKET_GROUP = 256;
+constexpr int BUCKET_ITEMS = 8;
+constexpr int BUCKET_SPAN = BUCKET_GROUP * BUCKET_ITEMS;
+constexpr int BUCKET_TILE_ITEMS = 4;  // consecutive keys per thread in the key scans
+constexpr int BUCKET_TILE = SCAN_THREADS * BUCKET_TILE_ITEMS;
 
 // The runs of equal keys among the lanes of the wave that take part: f(leader
 // lane, key, run mask) once per run.
@@ -366,45 +371,53 @@ __device__ __forceinline__ uint64_t lanes_below(uint32_t lane) { return (1ull <<
 __global__ __launch_bounds__(BUCKET_GROUP) void k_bucket_count(const uint32_t *__restrict__ keys, uint64_t n,
                                                                uint32_t mask, uint32_t *__restrict__ counts,
                                                                uint32_t *__restrict__ stay,
-                                                               uint32_t *__restrict__ group_stay) {
+                                                               uint32_t *__restrict__ span_stay) {
     __shared__ uint32_t ws[BUCKET_GROUP / 64];
-    const uint64_t i = (uint64_t)blockIdx.x * BUCKET_GROUP + threadIdx.x;
-    const bool valid = i < n;
-    const uint32_t raw = valid ? keys[i] : 0u;
-    const uint32_t key = raw & mask;
-    const bool stayer = valid && !(raw & BUCKET_MOVER);
+    const uint64_t base = (uint64_t)blockIdx.x * BUCKET_SPAN + threadIdx.x;
     const uint32_t lane = lane_id();
-    for_each_run(valid, key, [&](uint32_t l, uint32_t k, uint64_t m) {
-        if (lane == l) atomicAdd(&counts[k], (uint32_t)__popcll(m));
-    });
-    for_each_run(stayer, key, [&](uint32_t l, uint32_t k, uint64_t m) {
-        if (lane == l) atomicAdd(&stay[k], (uint32_t)__popcll(m));
-    });
-    const uint64_t sm = __ballot(stayer);
-    if (lane == 0) ws[threadIdx.x / 64] = (uint32_t)__popcll(sm);
+    uint32_t raw[BUCKET_ITEMS];
+#pragma unroll
+    for (int j = 0; j < BUCKET_ITEMS; ++j) {
+        const uint64_t i = base + (uint64_t)j * BUCKET_GROUP;
+        raw[j] = i < n ? keys[i] : 0u;
+    }
+    uint32_t stayers = 0;
+#pragma unroll
+    for (int j = 0; j < BUCKET_ITEMS; ++j) {
+        const bool valid = base + (uint64_t)j * BUCKET_GROUP < n;
+        const uint32_t key = raw[j] & mask;
+        const bool stayer = valid && !(raw[j] & BUCKET_MOVER);
+        for_each_run(valid, key, [&](uint32_t l, uint32_t k, uint64_t m) {
+            if (lane == l) atomicAdd(&counts[k], (uint32_t)__popcll(m));
+        });
+        for_each_run(stayer, key, [&](uint32_t l, uint32_t k, uint64_t m) {
+            if (lane == l) atomicAdd(&stay[k], (uint32_t)__popcll(m));
+        });
+        stayers += (uint32_t)__popcll(__ballot(stayer));
+    }
+    if (lane == 0) ws[threadIdx.x / 64] = stayers;
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t t = 0;
 #pragma unroll
         for (int w = 0; w < BUCKET_GROUP / 64; ++w) t += ws[w];
-        group_stay[blockIdx.x] = t;
+        span_stay[blockIdx.x] = t;
     }
 }
 
-// Tiles [0, nt) over the keys, tiles [nt, nt + ng) over the work-groups of
-// k_bucket_count.
+// Tiles [0, nt) over the keys, tiles [nt, nt + nst) over the spans.
 __global__ __launch_bounds__(SCAN_THREADS) void k_bucket_reduce(const uint32_t *__restrict__ counts,
                                                                  const uint32_t *__restrict__ stay, uint32_t nkeys,
                                                                  uint32_t tomb, uint32_t nt,
-                                                                 const uint32_t *__restrict__ group_stay,
-                                                                 uint32_t ngroups, uint32_t *__restrict__ tiles) {
+                                                                 const uint32_t *__restrict__ span_stay,
+                                                                 uint32_t nspans, uint32_t *__restrict__ tiles) {
     __shared__ uint32_t ws[SCAN_THREADS / 64 + 1];
     uint32_t total;
     if (blockIdx.x < nt) {
-        const uint32_t base = blockIdx.x * (uint32_t)SCAN_TILE + threadIdx.x;
+        const uint32_t base = blockIdx.x * (uint32_t)BUCKET_TILE + threadIdx.x;
         uint32_t parts = 0, blocks = 0, stayers = 0;
 #pragma unroll
-        for (int k = 0; k < SCAN_ITEMS; ++k) {
+        for (int k = 0; k < BUCKET_TILE_ITEMS; ++k) {
             const uint32_t b = base + (uint32_t)(k * SCAN_THREADS);
             const uint32_t c = b < nkeys ? counts[b] : 0u;
             parts += c;
@@ -419,12 +432,12 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_bucket_reduce(const uint32_t *
         if (threadIdx.x == 0) tiles[2 * nt + blockIdx.x] = total;
     } else {
         const uint32_t t = blockIdx.x - nt;
-        const uint32_t base = t * (uint32_t)SCAN_TILE + threadIdx.x;
+        const uint32_t base = t * (uint32_t)BUCKET_TILE + threadIdx.x;
         uint32_t stayers = 0;
 #pragma unroll
-        for (int k = 0; k < SCAN_ITEMS; ++k) {
+        for (int k = 0; k < BUCKET_TILE_ITEMS; ++k) {
             const uint32_t g = base + (uint32_t)(k * SCAN_THREADS);
-            stayers += g < ngroups ? group_stay[g] : 0u;
+            stayers += g < nspans ? span_stay[g] : 0u;
         }
         block_exclusive_scan<SCAN_THREADS>(stayers, ws, total);
         if (threadIdx.x == 0) tiles[3 * nt + t] = total;
@@ -432,13 +445,13 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_bucket_reduce(const uint32_t *
 }
 
 // Exclusive scans, in place, of the four tile sums: particles, blocks and
-// stayers over nt key tiles, stayers over ngt work-group tiles.
-__global__ __launch_bounds__(TOP_THREADS) void k_bucket_top(uint32_t *__restrict__ tiles, uint32_t nt, uint32_t ngt,
+// stayers over nt key tiles, stayers over nst span tiles.
+__global__ __launch_bounds__(TOP_THREADS) void k_bucket_top(uint32_t *__restrict__ tiles, uint32_t nt, uint32_t nst,
                                                             DeviceScalars *ds) {
     __shared__ uint32_t ws[TOP_THREADS / 64 + 1];
     for (int a = 0; a < 4; ++a) {
         uint32_t *sums = tiles + (uint32_t)a * nt;
-        const uint32_t len = a < 3 ? nt : ngt;
+        const uint32_t len = a < 3 ? nt : nst;
         uint32_t carry = 0;
         for (uint32_t base = 0; base < len; base += TOP_THREADS) {
             const uint32_t i = base + threadIdx.x;
@@ -452,100 +465,130 @@ __global__ __launch_bounds__(TOP_THREADS) void k_bucket_top(uint32_t *__restrict
     }
 }
 
+// Key tiles: thread t takes keys 4t .. 4t + 3 of its tile (nkeys is a power
+// of two of at least 2, and a tile's keys either all exist or start past the
+// end).
 __global__ __launch_bounds__(SCAN_THREADS) void k_bucket_down(
     uint32_t *__restrict__ counts, uint32_t *__restrict__ stay, uint32_t nkeys, uint32_t tomb, uint32_t nt,
-    uint32_t *__restrict__ group_stay, uint32_t ngroups, const uint32_t *__restrict__ tiles,
+    uint32_t *__restrict__ span_stay, uint32_t nspans, const uint32_t *__restrict__ tiles,
     uint32_t *__restrict__ cursor, uint32_t *__restrict__ stay_base, uint32_t *__restrict__ blk_start,
     uint32_t *__restrict__ blk_key, uint32_t capacity, DeviceScalars *ds) {
     __shared__ uint32_t ws[SCAN_THREADS / 64 + 1];
     uint32_t total;
     if (blockIdx.x >= nt) {
-        // work-group tile: the stayers before each work-group
+        // span tile: the stayers before each span
         const uint32_t t = blockIdx.x - nt;
-        const uint32_t base = t * (uint32_t)SCAN_TILE + threadIdx.x * (uint32_t)SCAN_ITEMS;
-        uint32_t v[SCAN_ITEMS], sum = 0;
+        const uint32_t base = t * (uint32_t)BUCKET_TILE + threadIdx.x * (uint32_t)BUCKET_TILE_ITEMS;
+        uint32_t v[BUCKET_TILE_ITEMS], sum = 0;
 #pragma unroll
-        for (int k = 0; k < SCAN_ITEMS; ++k) {
+        for (int k = 0; k < BUCKET_TILE_ITEMS; ++k) {
             const uint32_t g = base + (uint32_t)k;
-            v[k] = g < ngroups ? group_stay[g] : 0u;
+            v[k] = g < nspans ? span_stay[g] : 0u;
             sum += v[k];
         }
         uint32_t run = block_exclusive_scan<SCAN_THREADS>(sum, ws, total) + tiles[3 * nt + t];
 #pragma unroll
-        for (int k = 0; k < SCAN_ITEMS; ++k) {
+        for (int k = 0; k < BUCKET_TILE_ITEMS; ++k) {
             const uint32_t g = base + (uint32_t)k;
-            if (g < ngroups) group_stay[g] = run;
+            if (g < nspans) span_stay[g] = run;
             run += v[k];
         }
         return;
     }
-    const uint32_t base = blockIdx.x * (uint32_t)SCAN_TILE + threadIdx.x * (uint32_t)SCAN_ITEMS;
-    uint32_t c[SCAN_ITEMS], st[SCAN_ITEMS];
+    static_assert(BUCKET_TILE_ITEMS == 4, "one uint4 per thread");
+    const uint32_t base = blockIdx.x * (uint32_t)BUCKET_TILE + threadIdx.x * (uint32_t)BUCKET_TILE_ITEMS;
+    const bool here = base < nkeys;
+    uint4 c4 = make_uint4(0u, 0u, 0u, 0u), s4 = make_uint4(0u, 0u, 0u, 0u);
+    if (here) {
+        c4 = *reinterpret_cast<const uint4 *>(counts + base);
+        s4 = *reinterpret_cast<const uint4 *>(stay + base);
+    }
+    const uint32_t c[4] = {c4.x, c4.y, c4.z, c4.w}, st[4] = {s4.x, s4.y, s4.z, s4.w};
     uint32_t parts = 0, blocks = 0, stayers = 0;
 #pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; ++k) {
-        const uint32_t b = base + (uint32_t)k;
-        c[k] = b < nkeys ? counts[b] : 0u;
-        st[k] = b < nkeys ? stay[b] : 0u;
+    for (int k = 0; k < 4; ++k) {
         parts += c[k];
-        blocks += (c[k] != 0u && b != tomb) ? 1u : 0u;
+        blocks += (c[k] != 0u && base + (uint32_t)k != tomb) ? 1u : 0u;
         stayers += st[k];
     }
     uint32_t rp = block_exclusive_scan<SCAN_THREADS>(parts, ws, total) + tiles[blockIdx.x];
     uint32_t rb = block_exclusive_scan<SCAN_THREADS>(blocks, ws, total) + tiles[nt + blockIdx.x];
     uint32_t rs = block_exclusive_scan<SCAN_THREADS>(stayers, ws, total) + tiles[2 * nt + blockIdx.x];
+    if (!here) return;
+    uint32_t sb[4], cu[4];
 #pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; ++k) {
+    for (int k = 0; k < 4; ++k) {
         const uint32_t b = base + (uint32_t)k;
-        if (b < nkeys) {
-            // slots of key b: [rp, rp + st) its stayers, then its movers
-            stay_base[b] = rp - rs;
-            cursor[b] = rp + st[k];
-            if (b == tomb) {
-                ds->live_count = rp;
-            } else if (c[k] != 0u) {
-                if (rb < capacity) {
-                    blk_start[rb] = rp;
-                    blk_key[rb] = b;
-                } else {
-                    atomicOr(&ds->status, MPM_STATUS_INDIRECT_LIMIT_EXCEEDED | (KID_BLOCKS << 16));
-                }
-                ++rb;
+        // slots of key b: [rp, rp + st) its stayers, then its movers
+        sb[k] = rp - rs;
+        cu[k] = rp + st[k];
+        if (b == tomb) {
+            ds->live_count = rp;
+        } else if (c[k] != 0u) {
+            if (rb < capacity) {
+                blk_start[rb] = rp;
+                blk_key[rb] = b;
+            } else {
+                atomicOr(&ds->status, MPM_STATUS_INDIRECT_LIMIT_EXCEEDED | (KID_BLOCKS << 16));
             }
-            if (c[k] != 0u) counts[b] = 0u;
-            if (st[k] != 0u) stay[b] = 0u;
+            ++rb;
         }
         rp += c[k];
         rs += st[k];
     }
+    *reinterpret_cast<uint4 *>(stay_base + base) = make_uint4(sb[0], sb[1], sb[2], sb[3]);
+    *reinterpret_cast<uint4 *>(cursor + base) = make_uint4(cu[0], cu[1], cu[2], cu[3]);
+    if (c4.x | c4.y | c4.z | c4.w) *reinterpret_cast<uint4 *>(counts + base) = make_uint4(0u, 0u, 0u, 0u);
+    if (s4.x | s4.y | s4.z | s4.w) *reinterpret_cast<uint4 *>(stay + base) = make_uint4(0u, 0u, 0u, 0u);
 }
 
 __global__ __launch_bounds__(BUCKET_GROUP) void k_bucket_place(const uint32_t *__restrict__ keys, uint64_t n,
                                                                uint32_t mask, const uint32_t *__restrict__ stay_base,
-                                                               const uint32_t *__restrict__ group_stay_before,
+                                                               const uint32_t *__restrict__ span_stay_before,
                                                                uint32_t *__restrict__ cursor,
                                                                uint32_t *__restrict__ order) {
-    __shared__ uint32_t ws[BUCKET_GROUP / 64];
-    const uint64_t i = (uint64_t)blockIdx.x * BUCKET_GROUP + threadIdx.x;
-    const bool valid = i < n;
-    const uint32_t raw = valid ? keys[i] : 0u;
-    const uint32_t key = raw & mask;
-    const bool stayer = valid && !(raw & BUCKET_MOVER);
+    constexpr int NW = BUCKET_GROUP / 64;
+    __shared__ uint32_t ws[BUCKET_ITEMS * NW];
+    const uint64_t base = (uint64_t)blockIdx.x * BUCKET_SPAN + threadIdx.x;
     const uint32_t lane = lane_id(), wave = threadIdx.x / 64;
-    const uint64_t sm = __ballot(stayer);
-    if (lane == 0) ws[wave] = (uint32_t)__popcll(sm);
+    uint32_t raw[BUCKET_ITEMS];
+    uint64_t sm[BUCKET_ITEMS];
+#pragma unroll
+    for (int j = 0; j < BUCKET_ITEMS; ++j) {
+        const uint64_t i = base + (uint64_t)j * BUCKET_GROUP;
+        raw[j] = i < n ? keys[i] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < BUCKET_ITEMS; ++j) {
+        const bool stayer = base + (uint64_t)j * BUCKET_GROUP < n && !(raw[j] & BUCKET_MOVER);
+        sm[j] = __ballot(stayer);
+        if (lane == 0) ws[j * NW + (int)wave] = (uint32_t)__popcll(sm[j]);
+    }
     __syncthreads();
-    uint32_t before = group_stay_before[blockIdx.x];
-    for (uint32_t w = 0; w < wave; ++w) before += ws[w];
-    uint64_t slot = n;
-    if (stayer) slot = (uint64_t)stay_base[key] + before + (uint32_t)__popcll(sm & lanes_below(lane));
-    for_each_run(valid && !stayer, key, [&](uint32_t l, uint32_t k, uint64_t m) {
-        uint32_t first = 0;
-        if (lane == l) first = atomicAdd(&cursor[k], (uint32_t)__popcll(m));
-        first = (uint32_t)__builtin_amdgcn_readlane((int)first, (int)l);
-        if ((m >> lane) & 1ull) slot = (uint64_t)first + (uint32_t)__popcll(m & lanes_below(lane));
-    });
-    if (slot < n) order[slot] = (uint32_t)i;
+    // stayers of the span before (item j, this wave), in slot order
+    uint32_t before = span_stay_before[blockIdx.x];
+    for (uint32_t q = 0; q < wave; ++q) before += ws[q];
+#pragma unroll
+    for (int j = 0; j < BUCKET_ITEMS; ++j) {
+        const uint64_t i = base + (uint64_t)j * BUCKET_GROUP;
+        const bool valid = i < n;
+        const uint32_t key = raw[j] & mask;
+        const bool stayer = (sm[j] >> lane) & 1ull;
+        uint64_t slot = n;
+        if (stayer) slot = (uint64_t)stay_base[key] + before + (uint32_t)__popcll(sm[j] & lanes_below(lane));
+        for_each_run(valid && !stayer, key, [&](uint32_t l, uint32_t k, uint64_t m) {
+            uint32_t first = 0;
+            if (lane == l) first = atomicAdd(&cursor[k], (uint32_t)__popcll(m));
+            first = (uint32_t)__builtin_amdgcn_readlane((int)first, (int)l);
+            if ((m >> lane) & 1ull) slot = (uint64_t)first + (uint32_t)__popcll(m & lanes_below(lane));
+        });
+        if (slot < n) order[slot] = (uint32_t)i;
+        // the stayers of item j after this wave, and of item j + 1 before it
+        if (j + 1 < BUCKET_ITEMS) {
+            for (uint32_t q = wave; q < (uint32_t)NW; ++q) before += ws[j * NW + (int)q];
+            for (uint32_t q = 0; q < wave; ++q) before += ws[(j + 1) * NW + (int)q];
+        }
+    }
 }
 
 __global__ void k_copy_bytes(const float4 *__restrict__ src, float4 *__restrict__ dst, uint64_t n4) {
@@ -655,30 +698,30 @@ hipError_t launch_block_list(const uint32_t *keys, uint64_t n, uint32_t tomb_key
     return hipGetLastError();
 }
 
-uint64_t bucket_group_count(uint64_t n) { return std::max<uint64_t>((n + BUCKET_GROUP - 1) / BUCKET_GROUP, 1); }
+uint64_t bucket_span_count(uint64_t n) { return std::max<uint64_t>((n + BUCKET_SPAN - 1) / BUCKET_SPAN, 1); }
 size_t bucket_tile_sums(int key_bits, uint64_t n) {
-    return 3 * (size_t)tiles_of(1ull << key_bits, SCAN_TILE) + tiles_of(bucket_group_count(n), SCAN_TILE);
+    return 3 * (size_t)tiles_of(1ull << key_bits, BUCKET_TILE) + tiles_of(bucket_span_count(n), BUCKET_TILE);
 }
 
 hipError_t bucket_sort(const uint32_t *keys, uint64_t n, int key_bits, const BucketBuffers &B, uint32_t *order,
                        uint32_t *blk_start, uint32_t *blk_key, uint32_t capacity, DeviceScalars *ds,
                        hipStream_t stream) {
-    if (key_bits < 1 || key_bits > BUCKET_SORT_MAX_BITS) return hipErrorInvalidValue;
+    if (key_bits < 2 || key_bits > BUCKET_SORT_MAX_BITS) return hipErrorInvalidValue;
     const uint32_t nkeys = 1u << key_bits, tomb = nkeys - 1u;
-    const uint32_t nt = tiles_of(nkeys, SCAN_TILE);
-    const uint32_t groups = (uint32_t)((n + BUCKET_GROUP - 1) / BUCKET_GROUP);
-    const uint32_t ngt = tiles_of(groups, SCAN_TILE);
+    const uint32_t nt = tiles_of(nkeys, BUCKET_TILE);
+    const uint32_t spans = (uint32_t)((n + BUCKET_SPAN - 1) / BUCKET_SPAN);
+    const uint32_t nst = tiles_of(spans, BUCKET_TILE);
     if (n > 0)
-        hipLaunchKernelGGL(k_bucket_count, dim3(groups), dim3(BUCKET_GROUP), 0, stream, keys, n, tomb, B.counts,
-                           B.stay, B.group_stay);
-    hipLaunchKernelGGL(k_bucket_reduce, dim3(nt + ngt), dim3(SCAN_THREADS), 0, stream, B.counts, B.stay, nkeys, tomb,
-                       nt, B.group_stay, groups, B.tiles);
-    hipLaunchKernelGGL(k_bucket_top, dim3(1), dim3(TOP_THREADS), 0, stream, B.tiles, nt, ngt, ds);
-    hipLaunchKernelGGL(k_bucket_down, dim3(nt + ngt), dim3(SCAN_THREADS), 0, stream, B.counts, B.stay, nkeys, tomb,
-                       nt, B.group_stay, groups, B.tiles, B.cursor, B.stay_base, blk_start, blk_key, capacity, ds);
+        hipLaunchKernelGGL(k_bucket_count, dim3(spans), dim3(BUCKET_GROUP), 0, stream, keys, n, tomb, B.counts,
+                           B.stay, B.span_stay);
+    hipLaunchKernelGGL(k_bucket_reduce, dim3(nt + nst), dim3(SCAN_THREADS), 0, stream, B.counts, B.stay, nkeys, tomb,
+                       nt, B.span_stay, spans, B.tiles);
+    hipLaunchKernelGGL(k_bucket_top, dim3(1), dim3(TOP_THREADS), 0, stream, B.tiles, nt, nst, ds);
+    hipLaunchKernelGGL(k_bucket_down, dim3(nt + nst), dim3(SCAN_THREADS), 0, stream, B.counts, B.stay, nkeys, tomb,
+                       nt, B.span_stay, spans, B.tiles, B.cursor, B.stay_base, blk_start, blk_key, capacity, ds);
     if (n > 0)
-        hipLaunchKernelGGL(k_bucket_place, dim3(groups), dim3(BUCKET_GROUP), 0, stream, keys, n, tomb, B.stay_base,
-                           B.group_stay, B.cursor, order);
+        hipLaunchKernelGGL(k_bucket_place, dim3(spans), dim3(BUCKET_GROUP), 0, stream, keys, n, tomb, B.stay_base,
+                           B.span_stay, B.cursor, order);
     return hipGetLastError();
 }
 
