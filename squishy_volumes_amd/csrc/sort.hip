@@ -1,5 +1,5 @@
-// sort.hip - Morton block keys, stable LSD radix sort, device-wide scans and
-// the occupied-block list.
+// sort.hip - Morton block keys, stable LSD radix sort, the counting re-sort of
+// keys that a transfer wrote, device-wide scans and the occupied-block list.
 //
 // The radix sort is the reduce-then-scan form: per pass, a histogram kernel
 // counts the digits of each 4096-key tile, a device scan turns the digit-major
