@@ -208,6 +208,62 @@ SV_DEV void svd3_impl(const T F[3][3], T U[3][3], T s[3], T V[3][3]) {
         }
 }
 
+// The singular values of F alone, non-negative and in descending order: the
+// Jacobi sweeps of svd3_impl give V, and the singular values are the column
+// norms of F V (taken from F itself, not from the squared F^T F).
+template <class T, int SWEEPS>
+SV_DEV void singular_values_impl(const T F[3][3], T s[3]) {
+    T S[3][3], V[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            S[i][j] = F[0][i] * F[0][j] + F[1][i] * F[1][j] + F[2][i] * F[2][j];
+            V[i][j] = i == j ? T(1) : T(0);
+        }
+#pragma unroll 1
+    for (int it = 0; it < SWEEPS; ++it) {
+        jacobi_pq<T>(S, V, 0, 1);
+        jacobi_pq<T>(S, V, 0, 2);
+        jacobi_pq<T>(S, V, 1, 2);
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        T n = T(0);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const T b = F[i][0] * V[0][j] + F[i][1] * V[1][j] + F[i][2] * V[2][j];
+            n += b * b;
+        }
+        s[j] = sqrt(n);
+    }
+    T t;
+    if (s[0] < s[1]) { t = s[0]; s[0] = s[1]; s[1] = t; }
+    if (s[1] < s[2]) { t = s[1]; s[1] = s[2]; s[2] = t; }
+    if (s[0] < s[1]) { t = s[0]; s[0] = s[1]; s[1] = t; }
+}
+
+// Singular values of F (column-major), non-negative and descending, with the
+// FP64 recomputation of svd3 when the smallest is below 1e-3 of the largest.
+SV_DEV void singular_values3(const float Fc[9], float s[3]) {
+    float F[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) F[r][c] = Fc[3 * c + r];
+    singular_values_impl<float, 5>(F, s);
+    if (s[2] < 1e-3f * s[0] || !(s[0] == s[0])) {
+        double Fd[3][3], sd[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) Fd[i][j] = (double)F[i][j];
+        singular_values_impl<double, 12>(Fd, sd);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) s[i] = (float)sd[i];
+    }
+}
+
 // Float SVD with an FP64 recomputation when F is near degenerate: when the
 // smallest singular value is below 1e-3 of the largest, or when the float
 // factors reconstruct F with a relative error above 1e-4.  Returns true when

@@ -1164,8 +1164,8 @@ int run_adaptive(mpm_engine *e, double target_time, const mpm_frame_input *input
     auto substep = [&](bool want_keys) -> int {
         int r;
         HIP_TRY(launch_adapt_begin(e->adapt_dev, e->param_table, e->ds, e->stream));
-        HIP_TRY(launch_step_limits(e->P[e->cur], e->n, e->consts.grid_node_size, e->limit_part, e->limit_out, e->ds,
-                                   e->stream));
+        HIP_TRY(launch_step_limits(e->P[e->cur], e->n, e->consts.grid_node_size, false, e->limit_part, e->limit_out,
+                                   e->ds, e->stream));
         HIP_TRY(launch_adapt_force(e->adapt_dev, e->param_table, e->ds, e->limit_out, e->stream));
         if (e->closure_on && (r = run_closure(e)) != MPM_OK) return r;
         if ((r = run_transfer(e, TransferKind::P2G, goals, false)) != MPM_OK) return r;
@@ -2008,7 +2008,7 @@ int mpm_read_time_step_limits(mpm_engine *e, mpm_time_step_limits *out) {
     if (!e->limit_part &&
         (e->alloc(e->limit_part, step_limit_groups(e->n)) != hipSuccess || e->alloc(e->limit_out, 1) != hipSuccess))
         return MPM_ERR_OUT_OF_MEMORY;
-    HIP_TRY(launch_step_limits(e->P[e->cur], e->n, e->consts.grid_node_size, e->limit_part, e->limit_out, e->ds,
+    HIP_TRY(launch_step_limits(e->P[e->cur], e->n, e->consts.grid_node_size, true, e->limit_part, e->limit_out, e->ds,
                                e->stream));
     float4 r;
     HIP_TRY(hipMemcpyAsync(&r, e->limit_out, sizeof(r), hipMemcpyDeviceToHost, e->stream));

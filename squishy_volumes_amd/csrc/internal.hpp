@@ -466,9 +466,11 @@ hipError_t launch_adapt_integrate(DeviceAdaptive *A, SubstepParams *table, Devic
 // ({lo xyz, hi xyz, max |v|, 1 if any live particle}); part holds
 // 2 step_limit_groups(n) float4.
 hipError_t launch_particle_bounds(const ParticleArrays &P, uint64_t n, float4 *part, float *out, hipStream_t stream);
-hipError_t launch_step_limits(const ParticleArrays &P, uint64_t n, float h, float4 *part, float4 *out,
-                              const DeviceScalars *ds,
-                              hipStream_t stream);
+// The time-step limits of the stored state into out: {by_sound, by_isolated,
+// max |v|, by_deformation}; motion = false skips v and C (max |v| 0,
+// by_deformation +inf).
+hipError_t launch_step_limits(const ParticleArrays &P, uint64_t n, float h, bool motion, float4 *part, float4 *out,
+                              const DeviceScalars *ds, hipStream_t stream);
 hipError_t launch_closure_pairs(const ParticleArrays &P, const AuxArrays &X, uint64_t n, float hardening,
                                 uint32_t stride, uint32_t phase, float *x, float *t, uint64_t capacity,
                                 unsigned long long *count, hipStream_t stream);

@@ -22,14 +22,10 @@ constexpr float SEPARATION = 1e-5f;  // SINGULAR_VALUE_SEPARATION, rust/crates/u
 
 // h / c for one particle, or +inf when kappa / rho is not a positive number.
 __device__ float sound_limit(bool fluid, float p0, float p1, float m, float vol, const float F[9], float h) {
-    float U[9], s[3], V[9];
-    svd3(F, U, s, V);
     // nalgebra's singular values: non-negative, in descending order
-    float a = fabsf(s[0]), b = fabsf(s[1]), c = fabsf(s[2]);
-    if (a < b) { const float t = a; a = b; b = t; }
-    if (b < c) { const float t = b; b = c; c = t; }
-    if (a < b) { const float t = a; a = b; b = t; }
-    const float x = a, y = b, z = c;
+    float s[3];
+    singular_values3(F, s);
+    const float x = s[0], y = s[1], z = s[2];
     const float J = x * y * z;
     const float g[3] = {y * z, x * z, x * y};  // dJ/ds
     float d1, d2, diag;                          // dPsi/dJ, d2Psi/dJ2, the identity's coefficient
@@ -101,9 +97,10 @@ __device__ float isolated_limit(bool fluid, float p0, float p1, float m, float v
     return r == r ? r : INFINITY;
 }
 
-// Per work-group: {min by_sound, min by_isolated, max |v|, min by_deformation}.
-__global__ __launch_bounds__(256) void k_step_limits(ParticleArrays P, uint64_t n, float h, float4 *__restrict__ part,
-                                                    const DeviceScalars *ds) {
+// Per work-group: {min by_sound, min by_isolated, max |v|, min by_deformation};
+// without motion the stored v and C are not read (0 and +inf in their place).
+__global__ __launch_bounds__(256) void k_step_limits(ParticleArrays P, uint64_t n, float h, bool motion,
+                                                    float4 *__restrict__ part, const DeviceScalars *ds) {
     if (ds && ds->halt) return;
     float snd = INFINITY, iso = INFINITY, vmax = 0.f, def = INFINITY;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
@@ -116,10 +113,12 @@ __global__ __launch_bounds__(256) void k_step_limits(ParticleArrays P, uint64_t 
         const float m = P.m(i), vol = P.vol(i), p0 = P.p0(i), p1 = P.p1(i);
         snd = fminf(snd, sound_limit(fluid, p0, p1, m, vol, F, h));
         iso = fminf(iso, isolated_limit(fluid, p0, p1, m, vol, F, h));
-        const float v0 = P.v(0, i), v1 = P.v(1, i), v2 = P.v(2, i);
-        vmax = fmaxf(vmax, sqrtf(v0 * v0 + v1 * v1 + v2 * v2));
+        if (motion) {
+            const float v0 = P.v(0, i), v1 = P.v(1, i), v2 = P.v(2, i);
+            vmax = fmaxf(vmax, sqrtf(v0 * v0 + v1 * v1 + v2 * v2));
 #pragma unroll
-        for (int k = 0; k < 9; ++k) def = fminf(def, 0.2f / fmaxf(fabsf(P.at(i)[(FA + k) * TILE]), 1e-8f));
+            for (int k = 0; k < 9; ++k) def = fminf(def, 0.2f / fmaxf(fabsf(P.at(i)[(FA + k) * TILE]), 1e-8f));
+        }
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -264,10 +263,10 @@ hipError_t launch_particle_bounds(const ParticleArrays &P, uint64_t n, float4 *p
 
 uint32_t step_limit_groups(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 2048)); }
 
-hipError_t launch_step_limits(const ParticleArrays &P, uint64_t n, float h, float4 *part, float4 *out,
+hipError_t launch_step_limits(const ParticleArrays &P, uint64_t n, float h, bool motion, float4 *part, float4 *out,
                               const DeviceScalars *ds, hipStream_t stream) {
     const uint32_t groups = step_limit_groups(n);
-    hipLaunchKernelGGL(k_step_limits, dim3(groups), dim3(256), 0, stream, P, n, h, part, ds);
+    hipLaunchKernelGGL(k_step_limits, dim3(groups), dim3(256), 0, stream, P, n, h, motion, part, ds);
     hipLaunchKernelGGL(k_step_limits_final, dim3(1), dim3(256), 0, stream, part, groups, out, ds);
     return hipGetLastError();
 }
