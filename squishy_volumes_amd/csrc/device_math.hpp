@@ -210,10 +210,11 @@ SV_DEV void svd3_impl(const T F[3][3], T U[3][3], T s[3], T V[3][3]) {
 
 // The singular values of F alone, non-negative and in descending order: the
 // Jacobi sweeps of svd3_impl give V, and the singular values are the column
-// norms of F V (taken from F itself, not from the squared F^T F).  The float
-// form stops once every lane of the wave has converged: the off-diagonal of
-// the rotated F^T F below 1e-7 of its trace, which bounds the change of its
-// eigenvalues (Weyl) at that fraction.
+// norms of F V (taken from F itself, not from the squared F^T F).  In the
+// float form a lane stops sweeping once the off-diagonal of its rotated F^T F
+// is below 1e-7 of its smallest diagonal entry, which bounds the change of
+// every eigenvalue (Weyl) at about that fraction of the smallest; the wave
+// stops once all its lanes have.
 template <class T, int SWEEPS>
 SV_DEV void singular_values_impl(const T F[3][3], T s[3]) {
     T S[3][3], V[3][3];
@@ -228,8 +229,10 @@ SV_DEV void singular_values_impl(const T F[3][3], T s[3]) {
     for (int it = 0; it < SWEEPS; ++it) {
         if constexpr (sizeof(T) == 4) {
             const T off = S[0][1] * S[0][1] + S[0][2] * S[0][2] + S[1][2] * S[1][2];
-            const T tr = T(1e-7) * (S[0][0] + S[1][1] + S[2][2]);
-            if (__all(off <= tr * tr)) break;
+            const T small = T(1e-7) * fmin(fmin(S[0][0], S[1][1]), S[2][2]);
+            const bool done = off <= small * small;
+            if (__all(done)) break;
+            if (done) continue;
         }
         jacobi_pq<T>(S, V, 0, 1);
         jacobi_pq<T>(S, V, 0, 2);
