@@ -1203,41 +1203,50 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                 }
                 __syncthreads();
                 // Flush the arenas to the grid with global float atomics, one
-                // component per lane: the four lanes of a quad add the four
-                // components of one node, so that each atomic request to
-                // memory (the float atomics are carried out beyond the L2)
-                // carries whole nodes.  Zero components are skipped.
-                // Thread t takes component t & 3 of region slot 64 k + (t >> 2)
-                // in each of the 8 region sub-blocks k, four sub-blocks at a
-                // time (their 16 LDS reads in flight together; all 8 at once
-                // take the kernel to 256 VGPRs and measured far slower).
-                static_assert(4 * 64 == XF_THREADS, "one quad per slot of a sub-block");
+                // component per lane: the eight lanes of a group add the four
+                // components of two x-neighbouring nodes that share a 32-byte
+                // sector, so that each atomic request to memory (the float
+                // atomics are carried out beyond the L2) carries whole nodes.
+                // Zero components are skipped.  The region's x runs in classes,
+                // the pairs (1, 2), (3, 4), (5, 6) and (0, 7), and each class
+                // reads the plane arenas that can reach it only (the planes of
+                // waves 0-2 cover x in [1 + w, 4 + w]): 22 LDS reads per thread
+                // instead of 32, all in flight together.  Thread t takes
+                // component t & 3 of region x = class pair[(t >> 2) & 1] at
+                // (y, z) index 8 (4 h + wave) + ((t >> 3) & 7), h = 0, 1.
+                static_assert(4 * 64 == XF_THREADS, "four waves of eight (y, z) positions");
                 {
-                    const uint32_t lo = tid >> 2, comp = tid & 3u;
-                    const int lx = (int)(lo & 3u), ly = (int)((lo >> 2) & 3u), lz = (int)(lo >> 4);
+                    const uint32_t comp = tid & 3u, xsel = (tid >> 2) & 1u, pos = (tid >> 3) & 7u;
                     const float *shellf = reinterpret_cast<const float *>(shell) + comp;
                     const float *planef = reinterpret_cast<const float *>(planes) + comp;
                     float *gridf = reinterpret_cast<float *>(a.B.grid) + comp;
-#pragma unroll 4
-                    for (int k = 0; k < 8; ++k) {
-                        const int rx = lx | ((k & 1) << 2), ry = ly | ((k & 2) << 1), rz = lz | (k & 4);
-                        // All four reads are issued together (a plane that does
-                        // not reach the slot is read at a valid slot and dropped).
-                        float g = shellf[4 * (64 * k + (int)lo)];
+                    // region slot (rx, ry, rz); pmask: the planes that may reach its class
+                    auto flush_slot = [&](int rx, int ry, int rz, uint32_t pmask) {
+                        float g = shellf[4 * arena_slot(rx, ry, rz)];
 #pragma unroll
                         for (int w = 0; w < 3; ++w) {
+                            if (!((pmask >> w) & 1u)) continue;
                             const int u = rx - 1 - w;
                             const bool reach = u >= 0 && u <= 3;
                             const float o = planef[4 * (w * PSLOT + plane_slot(reach ? u : 0, ry, rz))];
                             g += reach ? o : 0.f;
                         }
-                        if (!(g != 0.f)) continue;
+                        if (!(g != 0.f)) return;
                         const int x = 4 * (int)b[0] + rx - 1, y = 4 * (int)b[1] + ry - 1, z = 4 * (int)b[2] + rz - 1;
                         if (!node_in_grid(G, x, y, z)) {
                             atomicOr(&ds->status, MPM_STATUS_OUT_OF_GRID | (KID_P2G << 16));
-                            continue;
+                            return;
                         }
                         unsafeAtomicAdd(gridf + 4u * node_index(G, (uint32_t)x, (uint32_t)y, (uint32_t)z), g);
+                    };
+#pragma unroll
+                    for (int hh = 0; hh < 2; ++hh) {
+                        const int idx = (hh * 4 + (int)wave) * 8 + (int)pos;
+                        const int ry = idx & 7, rz = idx >> 3;
+                        flush_slot(3 + (int)xsel, ry, rz, 7u);
+                        flush_slot(1 + (int)xsel, ry, rz, 3u);
+                        flush_slot(5 + (int)xsel, ry, rz, 6u);
+                        flush_slot(xsel ? 7 : 0, ry, rz, 0u);
                     }
                 }
                 if (tid < 64u) core_count[tid] = 0u;
