@@ -531,7 +531,7 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
     constexpr int SHARED4 = (HAS_P2G ? ARENA4 : 0) > (HAS_G2P ? VS_SLOTS : 0) ? (HAS_P2G ? ARENA4 : 0)
                                                                               : (HAS_G2P ? VS_SLOTS : 1);
     __shared__ float4 shared4[SHARED4];
-    float4 *vstage = shared4;                                    // node (x, y, z) at x + VS_Y y + VS_Z z
+    float4 *vstage = shared4;                                    // node (x, y, z) at x + VS_Y y + VS_Z z: (v, m)
     float4 *planes = shared4;                                    // [wave][plane_slot], waves 0-2
     float4 *shell = shared4 + 3 * PSLOT;                         // [arena_slot], wave 3
     __shared__ float4 payc[HAS_P2G ? 4 * XF_RANKS * 64 : 1];     // core payloads [quarter][rank][cell]
@@ -667,7 +667,11 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
         {
             if constexpr (HAS_G2P) {
 #pragma unroll
-                for (int t = 0; t < 2; ++t) vstage[so[t]] = sin[t] ? sv[t] : make_float4(0.f, 0.f, 0.f, 0.f);
+                // staged as (v, m): the gather then reads each node's v with one
+                // 12-byte LDS load from a 16-byte aligned address
+                for (int t = 0; t < 2; ++t)
+                    vstage[so[t]] =
+                        sin[t] ? make_float4(sv[t].y, sv[t].z, sv[t].w, sv[t].x) : make_float4(0.f, 0.f, 0.f, 0.f);
             }
             if constexpr (HAS_P2G) {
                 if (cs == start && tid == 0) blk_touch = 0u;
@@ -767,7 +771,12 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                     } else if (in_region) {
                         const float4 *base = vstage + (c[0] + 1) + VS_Y * (c[1] + 1) + VS_Z * (c[2] + 1);
                         gather_stencil(
-                            w, f, apic, [&](int i3, int j3, int k3) { return base[i3 + VS_Y * j3 + VS_Z * k3]; }, v, C);
+                            w, f, apic,
+                            [&](int i3, int j3, int k3) {
+                                const float4 q = base[i3 + VS_Y * j3 + VS_Z * k3];
+                                return make_float4(q.w, q.x, q.y, q.z);
+                            },
+                            v, C);
                     } else {
                         const int g0[3] = {c[0] + 4 * (int)b[0], c[1] + 4 * (int)b[1], c[2] + 4 * (int)b[2]};
                         const bool ok = node_in_grid(G, g0[0], g0[1], g0[2]) &&
