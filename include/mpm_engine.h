@@ -246,7 +246,7 @@ typedef struct mpm_tuning {
     int32_t particles_per_lane;          /* particles each thread of the transfer kernel takes per
                                             pass of a block, 1..3 (default 3: passes of 768) */
     int32_t grid_update_threads;         /* work-group size of the grid update kernel: 64, 128 or
-                                            256 threads (0: 256); its wave count stays 16 per CU */
+                                            256 threads (0: 256); its wave count stays 32 per CU */
     int32_t learned_fused;               /* 1: the learned closure evaluated after each advance
                                             inside the fused gather-scatter (not with colliders);
                                             otherwise (default) in its own pass between the gather

@@ -1320,7 +1320,7 @@ constexpr uint64_t FLAG_SCAN_STRIDE = 4294967291ull;
 // The flagged blocks are found in one of two ways:
 //   * grids of up to DENSE_FLAG_SCAN_BLOCKS blocks, and up to
 //     DENSE_FLAG_SCAN_RATIO times the candidate slots: the flags themselves,
-//     64 per wave step, at the permuted indices (64 q + lane) *
+//     32 per wave step, at the permuted indices (32 q + lane) *
 //     FLAG_SCAN_STRIDE mod n, so that every wave meets about the same number
 //     of flagged blocks however the particles cluster;
 //   * otherwise (large grids, small scenes): the candidate slots written by
@@ -1387,10 +1387,12 @@ __global__ __launch_bounds__(256) void k_grid_update(GridBuffers B, GridGeom G, 
     };
     const uint32_t n = G.num_blocks_total;
     if (n <= DENSE_FLAG_SCAN_BLOCKS && (uint64_t)n <= DENSE_FLAG_SCAN_RATIO * ((uint64_t)ds->num_blocks * NCAND + novf)) {
-        for (uint32_t q = gw; q * 64u < n; q += nw) {
-            const uint32_t i = q * 64u + lane;
+        // 32 flags per wave step (half the lanes): twice the waves share
+        // the flagged blocks, fewer of them per wave
+        for (uint32_t q = gw; q * 32u < n; q += nw) {
+            const uint32_t i = q * 32u + lane;
             uint32_t mine = NO_BLOCK;
-            if (i < n) {
+            if (lane < 32u && i < n) {
                 const uint32_t T = (uint32_t)(((uint64_t)i * FLAG_SCAN_STRIDE) % n);
                 if (B.blk_flag[T]) {
                     B.blk_flag[T] = 0u;
@@ -1636,10 +1638,10 @@ hipError_t launch_transfer(const TransferLaunch &L, const TransferCall &t, hipSt
     return hipErrorInvalidValue;
 }
 
-// The grid update runs on a fixed number of waves (16 per CU), so that its
+// The grid update runs on a fixed number of waves (32 per CU), so that its
 // per-wave partial counts have a fixed size, in work-groups of
 // L.grid_update_threads (a tuning knob: 1, 2 or 4 waves).
-uint32_t grid_update_waves(const TransferLaunch &L) { return (uint32_t)std::max(32, 16 * L.cu_count); }
+uint32_t grid_update_waves(const TransferLaunch &L) { return (uint32_t)std::max(32, 32 * L.cu_count); }
 uint32_t grid_update_groups(const TransferLaunch &L) {
     return grid_update_waves(L) / (uint32_t)(L.grid_update_threads / 64);
 }
