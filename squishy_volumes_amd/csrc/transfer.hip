@@ -682,6 +682,7 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
             for (int kk = 0; kk < XF_PER_THREAD; ++kk) {
                 const uint32_t i = cs + (tid ^ flip) + (uint32_t)(kk * XF_THREADS);
                 bool skip = false;
+                uint32_t early_rank = 0u;  // G2P2G: the core rank taken before the stress
                 if constexpr (LEARN) {
                     if (__ballot(i < ce) == 0ull) break;
                     skip = i >= ce;
@@ -948,6 +949,21 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                         }
                     }
                     if constexpr (!LEARN) {
+                        if constexpr (MODE == XM_G2P2G && !COLL) {
+                            // The scatter's core rank, taken now so that the LDS
+                            // atomic's latency hides behind the stress (the cell is
+                            // the payload's: nothing moves the particle before it).
+                            int ce[3];
+                            bool core_e = true;
+#pragma unroll
+                            for (int q = 0; q < 3; ++q) {
+                                const float xn = div_by(xp[q], h, G.inv_h);
+                                ce[q] = (int)floorf(xn - 0.5f) - cell0[q];
+                                core_e = core_e && ce[q] >= 0 && ce[q] <= 3;
+                            }
+                            if (core_e)
+                                early_rank = atomicAdd(&core_count[(uint32_t)(ce[0] | (ce[1] << 2) | (ce[2] << 4))], 1u);
+                        }
                         // Stress of the new F for the next substep's scatter.
                         float tau[9];
                         kirchhoff<SET>(flags, p0, p1, hard, F, J, R, U, sv, tau);
@@ -1066,7 +1082,8 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                         const bool core = c[0] >= 0 && c[0] <= 3 && c[1] >= 0 && c[1] <= 3 && c[2] >= 0 && c[2] <= 3;
                         if (core) {
                             const uint32_t l = (uint32_t)(c[0] | (c[1] << 2) | (c[2] << 4));
-                            const uint32_t r = atomicAdd(&core_count[l], 1u);
+                            const uint32_t r =
+                                (MODE == XM_G2P2G && !COLL && !LEARN) ? early_rank : atomicAdd(&core_count[l], 1u);
                             if (r < (uint32_t)XF_RANKS) {
                                 dst = payc + r * 64u + l;
                                 stride = XF_RANKS * 64;
