@@ -521,6 +521,8 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
     // once, so a wave keeps all its lanes in every iteration of the particle
     // loop (a lane with nothing to do skips its work instead of leaving).
     constexpr bool LEARN = (SET & M_LEARN) != 0 && MODE == XM_G2P2G;
+    // kernels without the SVD, mesh colliders or the network: registers to spare
+    constexpr bool LIGHT = (SET & (M_SNOW | M_SAND | M_COLL | M_LEARN)) == 0;
     uint32_t n_eval = 0, n_fallback = 0, n_judged = 0;
     double err2 = 0.0, ref2 = 0.0;
     // The region's node velocities (gather) and the arenas (scatter: one
@@ -949,7 +951,7 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                         }
                     }
                     if constexpr (!LEARN) {
-                        if constexpr (MODE == XM_G2P2G && !COLL) {
+                        if constexpr (MODE == XM_G2P2G && LIGHT) {
                             // The scatter's core rank, taken now so that the LDS
                             // atomic's latency hides behind the stress (the cell is
                             // the payload's: nothing moves the particle before it).
@@ -1083,7 +1085,7 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                         if (core) {
                             const uint32_t l = (uint32_t)(c[0] | (c[1] << 2) | (c[2] << 4));
                             const uint32_t r =
-                                (MODE == XM_G2P2G && !COLL && !LEARN) ? early_rank : atomicAdd(&core_count[l], 1u);
+                                (MODE == XM_G2P2G && LIGHT) ? early_rank : atomicAdd(&core_count[l], 1u);
                             if (r < (uint32_t)XF_RANKS) {
                                 dst = payc + r * 64u + l;
                                 stride = XF_RANKS * 64;
@@ -1241,7 +1243,7 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                 // component t & 3 of region x = class pair[(t >> 2) & 1] at
                 // (y, z) index 8 (4 h + wave) + ((t >> 3) & 7), h = 0, 1.
                 static_assert(4 * 64 == XF_THREADS, "four waves of eight (y, z) positions");
-                {
+                if constexpr (LIGHT) {
                     const uint32_t comp = tid & 3u, xsel = (tid >> 2) & 1u, pos = (tid >> 3) & 7u;
                     const float *shellf = reinterpret_cast<const float *>(shell) + comp;
                     const float *planef = reinterpret_cast<const float *>(planes) + comp;
@@ -1273,6 +1275,36 @@ __global__ __launch_bounds__(XF_THREADS) __attribute__((amdgpu_waves_per_eu(MODE
                         flush_slot(1 + (int)xsel, ry, rz, 3u);
                         flush_slot(5 + (int)xsel, ry, rz, 6u);
                         flush_slot(xsel ? 7 : 0, ry, rz, 0u);
+                    }
+                } else {
+                    // The kernels with the SVD, mesh colliders or the network (whose
+                    // registers leave less room): thread t takes component t & 3 of
+                    // region slot 64 k + (t >> 2) in each of the 8 region sub-blocks
+                    // k, four sub-blocks at a time, every plane read and dropped
+                    // where it does not reach.
+                    const uint32_t lo = tid >> 2, comp = tid & 3u;
+                    const int lx = (int)(lo & 3u), ly = (int)((lo >> 2) & 3u), lz = (int)(lo >> 4);
+                    const float *shellf = reinterpret_cast<const float *>(shell) + comp;
+                    const float *planef = reinterpret_cast<const float *>(planes) + comp;
+                    float *gridf = reinterpret_cast<float *>(a.B.grid) + comp;
+#pragma unroll 4
+                    for (int k = 0; k < 8; ++k) {
+                        const int rx = lx | ((k & 1) << 2), ry = ly | ((k & 2) << 1), rz = lz | (k & 4);
+                        float g = shellf[4 * (64 * k + (int)lo)];
+#pragma unroll
+                        for (int w = 0; w < 3; ++w) {
+                            const int u = rx - 1 - w;
+                            const bool reach = u >= 0 && u <= 3;
+                            const float o = planef[4 * (w * PSLOT + plane_slot(reach ? u : 0, ry, rz))];
+                            g += reach ? o : 0.f;
+                        }
+                        if (!(g != 0.f)) continue;
+                        const int x = 4 * (int)b[0] + rx - 1, y = 4 * (int)b[1] + ry - 1, z = 4 * (int)b[2] + rz - 1;
+                        if (!node_in_grid(G, x, y, z)) {
+                            atomicOr(&ds->status, MPM_STATUS_OUT_OF_GRID | (KID_P2G << 16));
+                            continue;
+                        }
+                        unsafeAtomicAdd(gridf + 4u * node_index(G, (uint32_t)x, (uint32_t)y, (uint32_t)z), g);
                     }
                 }
                 if (tid < 64u) core_count[tid] = 0u;
