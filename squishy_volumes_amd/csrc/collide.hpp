@@ -95,8 +95,10 @@ __device__ __forceinline__ uint32_t block_of_position(const float x[3], const Gr
 // bits `bits` (both updated).  Candidates are the triangles listed for the
 // particle's block, ascending; per collider the first triangle of the
 // smallest distance wins, as in the reference's scan.
-__device__ __noinline__ void collide_particle(const MeshDev &M, const GridGeom &G, float dt, const float xp[3],
-                                              float v[3], uint32_t &bits) {
+// (Inlined: as a call it made the transfer kernel spill 736 bytes per lane
+// around it and run at one wave per SIMD.)
+__device__ __forceinline__ void collide_particle(const MeshDev &M, const GridGeom &G, float dt, const float xp[3],
+                                                 float v[3], uint32_t &bits) {
     const uint32_t B = block_of_position(xp, G);
     const uint32_t k0 = B == 0xffffffffu ? 0u : M.blk_off[B];
     const uint32_t k1 = B == 0xffffffffu ? 0u : M.blk_off[B + 1];
@@ -232,8 +234,8 @@ __device__ __forceinline__ float4 twin_velocity(const TwinEntry *tab, uint32_t m
 // Scatter of one payload (the transfer kernel's 16 floats: f, m | q | h A
 // columns) into the twins (node, bits) of its 27 stencil nodes, cell = the
 // global cell of the stencil's first node.
-__device__ __noinline__ void twin_scatter(TwinEntry *tab, uint32_t mask, const GridGeom &G, DeviceScalars *ds,
-                                          const int cell[3], const float pl[16], uint32_t bits) {
+__device__ __forceinline__ void twin_scatter(TwinEntry *tab, uint32_t mask, const GridGeom &G, DeviceScalars *ds,
+                                             const int cell[3], const float pl[16], uint32_t bits) {
     if (cell[0] < 0 || cell[1] < 0 || cell[2] < 0 || cell[0] + 2 >= 4 * G.nb[0] || cell[1] + 2 >= 4 * G.nb[1] ||
         cell[2] + 2 >= 4 * G.nb[2])
         return;  // the plain scatter reports OUT_OF_GRID
