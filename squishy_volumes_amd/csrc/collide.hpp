@@ -193,17 +193,22 @@ __device__ __forceinline__ uint32_t twin_hash(uint32_t node) {
 }
 
 // The accumulator of twin (node, bits), inserted when missing; null (and
-// TABLE_TRIES_EXCEEDED) when the table or the node's slots are full.
+// TABLE_TRIES_EXCEEDED) when the table or the node's slots are full.  A key
+// once set stays for the substep, so a plain load that already shows this
+// node (or these bits) settles the probe without an atomic; one that shows an
+// empty key (perhaps stale) falls back to the compare-and-swap.
 __device__ __forceinline__ float4 *twin_slot(TwinEntry *tab, uint32_t mask, uint32_t node, uint32_t bits,
                                              DeviceScalars *ds) {
     const uint32_t h = twin_hash(node);
     for (int probe = 0; probe < TWIN_PROBES; ++probe) {
         TwinEntry &e = tab[(h + (uint32_t)probe) & mask];
-        const uint32_t old = atomicCAS(&e.node, TWIN_EMPTY, node);
+        uint32_t old = e.node;
+        if (old == TWIN_EMPTY) old = atomicCAS(&e.node, TWIN_EMPTY, node);
         if (old != TWIN_EMPTY && old != node) continue;
 #pragma unroll
         for (int s = 0; s < TWIN_SLOTS; ++s) {
-            const uint32_t ob = atomicCAS(&e.bits[s], 0u, bits);
+            uint32_t ob = e.bits[s];
+            if (ob == 0u) ob = atomicCAS(&e.bits[s], 0u, bits);
             if (ob == 0u || ob == bits) return &e.mp[s];
         }
         break;
